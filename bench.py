@@ -1,0 +1,152 @@
+#!/usr/bin/env python3
+"""Benchmark of the leap-frog hot path on one MI355X.
+
+A "step" is one LeapFrog::step over the whole mesh: block-ELL SpMV + inverse
+mass + leap-frog update + point source + receiver sampling, fused in one
+kernel launch.  The workload is config 2 (4000 m x 2000 m, 160k triangles,
+p = 6, 8.96M DOF, three layers with lognormal moduli, one Ricker source,
+128 receivers), the largest single-GPU configuration; synthetic inputs.
+
+Prints ONE JSON line: value = DOF*steps/s of the plain FP64 path (whole job,
+inputs resident in HBM), plus the FP32-compensated variant, a roofline entry
+for the dominant kernel and a CPU baseline (the FP64 oracle, single thread,
+on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(prob, scalar_bytes, vec_bytes_per_dof):
+    """Bytes one step must move at minimum: every stored block once, the state
+    vectors once each, the inverse-mass scalar and the neighbour metadata."""
+    d = 2 * prob.nln
+    nblk = prob.ne + int((prob.neigh >= 0).sum())
+    return nblk * d * d * scalar_bytes + prob.ne * d * vec_bytes_per_dof + prob.ne * (8 + 16)
+
+
+def run_variant(prob, precision, steps, warmup):
+    from tspeed_amd import capi
+    eng = capi.Engine(precision)
+    t0 = time.time()
+    eng.set_tables(prob.N)
+    eng.set_mesh(prob.verts, prob.neigh, prob.neighedge, prob.lam, prob.mu, prob.rho, prob.penalty)
+    eng.set_sources(prob.src_elem, prob.src_phi, prob.wavelet(steps + warmup + 1))
+    eng.set_receivers(prob.rec_elem, prob.rec_phi, 1, steps + warmup + 1)
+    eng.set_dt(prob.dt)
+    eng.set_state()
+    eng.first_step()
+    eng.synchronize()
+    setup_s = time.time() - t0
+    if warmup:
+        eng.run(warmup)
+    eng.synchronize()
+    t1 = time.perf_counter()
+    ms = eng.run_timed(steps)          # HIP events on the engine's stream
+    eng.synchronize()
+    wall = time.perf_counter() - t1
+    status = eng.status()
+    dev_bytes = eng.device_bytes()
+    eng.close()
+    return dict(ms_total=ms, wall_s=wall, setup_s=setup_s, status=status, device_bytes=dev_bytes)
+
+
+def cpu_baseline(sample_steps):
+    """The FP64 oracle (single thread, reference Release flags) on a 40x20 sub-grid
+    of config 2 (same p, materials model and per-element work)."""
+    from oracle import oracle as O
+    from tspeed_amd import workloads as W
+    p = W.config2(nx=40, ny=20, nsteps=sample_steps)
+    op = O.Operator(p.N, p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+    t = time.perf_counter()
+    op.run(p.dt, p.nsteps, src=p.source_dict(0), rec_elem=p.rec_elem, rec_phi=p.rec_phi, stride=1)
+    dt = time.perf_counter() - t
+    return dict(value=p.ndof * sample_steps / dt, unit="DOF*steps/s", cores=1, kind="port",
+                sample=f"oracle (FP64 C restatement of LeapFrog::step, -O3 -msse4, 1 thread) on config 2's "
+                       f"model at 40x20 squares ({p.ne} triangles, p=6, {p.ndof} DOF), {sample_steps} steps, "
+                       f"{dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=600)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32c", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(the path would shard by element strips with one halo exchange per step).", file=sys.stderr)
+        sys.exit(2)
+
+    from tspeed_amd import capi, workloads as W
+    if capi.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    prob = W.config2(nsteps=args.steps + args.warmup)
+    r64 = run_variant(prob, "fp64", args.steps, args.warmup)
+    ms_step = r64["ms_total"] / args.steps
+    value = prob.ndof * args.steps / (r64["ms_total"] / 1e3)
+    bytes64 = algorithmic_bytes(prob, 8, 24)
+    achieved = bytes64 / (ms_step / 1e3) / 1e9
+
+    variants = {}
+    if not args.no_fp32c:
+        r32 = run_variant(prob, "fp32c", args.steps, args.warmup)
+        ms32 = r32["ms_total"] / args.steps
+        b32 = algorithmic_bytes(prob, 4, 24)
+        variants["fp32c"] = dict(value=prob.ndof * args.steps / (r32["ms_total"] / 1e3), ms_per_step=ms32,
+                                 achieved_GBs=b32 / (ms32 / 1e3) / 1e9, algorithmic_bytes=b32,
+                                 status=r32["status"])
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_step64_config2.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+
+    out = {
+        "metric": "DOF-steps/s batched FP32 vs plain FP64 @<=1e-4 seismogram err; p99 latency",
+        "value": value,
+        "unit": "DOF*steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "config2: 4000x2000 m, 400x200 split squares (160000 triangles), p=6, "
+                               "8.96M DOF, 3 layers + lognormal moduli, 1 Ricker source, 128 receivers; "
+                               "plain FP64 leap-frog",
+                   "ne": prob.ne, "p": prob.N, "dof": prob.ndof, "dt": prob.dt, "precision": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_step64<6,false>", "algorithmic_bytes": bytes64},
+        "variants": variants,
+        "setup_s": r64["setup_s"],
+        "status_flags": r64["status"],
+        "device_bytes": r64["device_bytes"],
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,173 @@
+/*
+ * tspeed_amd.h -- C-ABI of the MI355X (gfx950) leap-frog engine.
+ *
+ * This is the drop-in boundary under tspeed's `LeapFrog` class.  Each entry point
+ * names the reference interface it replaces (paths relative to the tspeed source
+ * tree).  Conventions:
+ *   - every function returns an int status (TSP_OK = 0) and never throws;
+ *     tsp_last_error() gives the message of the last failure on a context;
+ *   - host buffers belong to the caller, device buffers to the library;
+ *   - coefficient vectors cross the ABI in the reference's component-major
+ *     layout idx = c*ne*nln + e*nln + i (TimeAdvance.cpp:73-74), one vector per
+ *     right-hand-side column; the device keeps its own element-major layout;
+ *   - basis tables follow FESpace's layout (FESpace_imp.hpp:26-47), see
+ *     tsp_set_tables;
+ *   - a context is not thread-safe: one context per host thread, or lock it.
+ */
+#ifndef TSPEED_AMD_H
+#define TSPEED_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tsp_context tsp_context;
+
+enum tsp_status_code {
+    TSP_OK = 0,
+    TSP_ERR_ARG = 1,         /* bad argument or size                          */
+    TSP_ERR_STATE = 2,       /* call out of order (e.g. step before mesh)     */
+    TSP_ERR_HIP = 3,         /* HIP runtime error                             */
+    TSP_ERR_UNSUPPORTED = 4, /* degree / precision / column count not built   */
+    TSP_ERR_NOMEM = 5        /* device allocation failed                      */
+};
+
+enum tsp_precision {
+    TSP_FP64 = 0,  /* plain FP64 blocks and update: the correctness path             */
+    TSP_FP32C = 1  /* FP32 blocks, FP32 increments, Kahan-compensated displacement  */
+};
+
+/* Status bits reported by tsp_status (no counterpart in the reference, which has
+ * no failure detection: a blown-up run just prints Inf/NaN). */
+#define TSP_FLAG_NONFINITE 1u
+
+/* Number of visible HIP devices (0 when no GPU driver is present). */
+int tsp_device_count(int *n);
+
+/* Context lifetime.  Replaces the construction of TimeAdvance/LeapFrog
+ * (TimeAdvance_imp.hpp:26-40).  `device` is the HIP ordinal; `precision` one
+ * of tsp_precision; `ncols` the number of right-hand-side columns (1 = the
+ * reference's single simulation). */
+int  tsp_create(int device, int precision, int ncols, tsp_context **out);
+void tsp_destroy(tsp_context *ctx);
+const char *tsp_last_error(const tsp_context *ctx);
+
+/* Optional: launch on a caller-owned hipStream_t (default: a stream the
+ * context creates).  Pass NULL to return to the context's own stream. */
+int tsp_set_stream(tsp_context *ctx, void *hip_stream);
+
+/* Basis/quadrature tables of FESpace<N, Gauss<N+1>, Dubiner<N>>
+ * (FESpace_imp.hpp:26-47; FESpace.hpp:89,98,108), nln = (N+1)(N+2)/2,
+ * nq1 = N+1, nq2 = nq1^2:
+ *   B[k*nln+i]                  phi_i at area node k          (M_B)
+ *   G[(i*2+a)*nq2+k]            d phi_i / d xi_a at node k    (M_G)
+ *   Bedge[(f*nq1+k)*nln+i]      phi_i at node k of edge f     (M_Bedge)
+ *   Gedge[((i*3+f)*2+a)*nq1+k]  gradient on edge f            (M_Gedge)
+ *   w2[nq2], w1[nq1]            area / edge weights           (iweight, eweight)
+ * Degrees N = 1..8 are built. */
+int tsp_set_tables(tsp_context *ctx, int N, const double *B, const double *G, const double *Bedge,
+                   const double *Gedge, const double *w2, const double *w1);
+
+/* Mesh and materials; assembles B = penalty*N^2*S + I + A and the inverse mass
+ * on the device in FP64 (Matrices::Matrices, Matrices_imp.hpp:192-215;
+ * TimeAdvance_imp.hpp:32).  Unlike the reference, whose set_penalty is a no-op
+ * for B (TimeAdvance.hpp:165 vs TimeAdvance_imp.hpp:27,32), the penalty is an
+ * argument here; penalty = 2 reproduces the reference's B.
+ *   verts[e*6 + 2*v + {0,1}]   vertex v of triangle e (counter-clockwise)
+ *   neigh[e*3+f], neighedge[e*3+f]  neighbour across edge f and its edge index,
+ *                              -1 on the boundary (Mesh.cpp:33-128; edge f is
+ *                              opposite vertex f)
+ *   lambda, mu, rho[ne]        per-element Lame parameters and density
+ *                              (Parameters, FESpace.hpp:194-247) */
+int tsp_set_mesh(tsp_context *ctx, int ne, const double *verts, const int *neigh, const int *neighedge,
+                 const double *lambda, const double *mu, const double *rho, double penalty);
+
+/* Alternative operator input: already assembled blocks in the canonical form
+ * blocks[((e*4+s)*d + I)*d + J] (s = 0 own element, s = 1+f neighbour across
+ * edge f; I = m*nln+i, J = n*nln+j; d = 2*nln) and the per-element inverse-mass
+ * scalar invm[e] (Matrices_imp.hpp:187).  Replaces filling MyMatMultiDim by
+ * hand (MyMat.hpp:193-237). */
+int tsp_upload_blocks(tsp_context *ctx, int ne, const int *neigh, const double *blocks, const double *invm);
+/* Download the FP64 operator rows of elements [e_begin, e_begin+e_count) in the
+ * same canonical form (either pointer may be NULL). */
+int tsp_get_blocks(tsp_context *ctx, int e_begin, int e_count, double *blocks, double *invm);
+
+/* Initial state of column `col`: u0 (set_initial_u, TimeAdvance_imp.hpp:46-50)
+ * and v0 (set_initial_v, :41-45), component-major, 2*ne*nln each; NULL = zero. */
+int tsp_set_state(tsp_context *ctx, int col, const double *u0, const double *v0);
+
+/* Point sources (PointWiseForce, Force.hpp:56-86; Force_imp.hpp:24-31):
+ *   col[s], elem[s]            column and element of source s
+ *   phi[s*nln+i]               basis values at the source point
+ *   wavelet[(row*nsrc+s)*2+c]  force component c of source s for time row `row`:
+ *                              row 0 is F(0) used by first_step, row n >= 1 the
+ *                              value step n uses, F(t_{n+1}) with t accumulated by
+ *                              repeated addition (TimeAdvance.cpp:28,48-51).
+ * ntab rows are stored; tsp_set_wavelet rewrites rows later. */
+int tsp_set_sources(tsp_context *ctx, int nsrc, const int *col, const int *elem, const double *phi,
+                    int ntab, const double *wavelet);
+int tsp_set_wavelet(tsp_context *ctx, int row0, int nrows, const double *wavelet);
+
+/* Receivers (Receivers.hpp:77-121; PointWiseEntity::M_add, Receivers_imp.hpp:35-51):
+ * col[r], elem[r], phi[r*nln+i].  Samples are recorded on the device after
+ * first_step and after every step whose 1-based index is a multiple of
+ * `stride` (0 = only on tsp_eval_receivers), up to max_samples samples. */
+int tsp_set_receivers(tsp_context *ctx, int nrec, const int *col, const int *elem, const double *phi,
+                      int stride, int max_samples);
+
+/* Change the receiver sampling stride without touching recorded samples
+ * (the driver loop's `if(step%2 == 0) eval_receivers()`, Examples/src/Lamb.cpp:56-57,
+ * moved into the step kernels).  0 = sample only on tsp_eval_receivers. */
+int tsp_set_stride(tsp_context *ctx, int stride);
+
+/* Step size (TimeAdvance::set_dt, TimeAdvance.hpp:153). */
+int tsp_set_dt(tsp_context *ctx, double dt);
+
+/* LeapFrog::first_step (TimeAdvance.cpp:24-39):
+ *   u1 = u0 + dt*v0 - invM*(dt^2/2*(B*u0 - f(0))). */
+int tsp_first_step(tsp_context *ctx);
+
+/* nsteps calls of LeapFrog::step (TimeAdvance.cpp:41-59), fused with the point
+ * source and the receiver sampling:
+ *   u^{n+1} = 2u^n - u^{n-1} + invM*(-dt^2*(B*u^n - f(t_{n+1}))). */
+int tsp_run(tsp_context *ctx, int nsteps);
+
+/* tsp_run bracketed by HIP events recorded on the context's stream; *ms gets
+ * the elapsed device time of the nsteps launches (measurement helper, no
+ * counterpart in the reference). */
+int tsp_run_timed(tsp_context *ctx, int nsteps, float *ms);
+
+/* TimeAdvance::eval_receivers (TimeAdvance.cpp:68-79): record one sample of the
+ * current displacement now. */
+int tsp_eval_receivers(tsp_context *ctx);
+
+/* Number of steps taken so far (M_last_step, TimeAdvance.hpp:233). */
+int tsp_steps_done(const tsp_context *ctx);
+
+/* Current displacement of column `col` (get_uh, TimeAdvance.hpp:202),
+ * component-major 2*ne*nln doubles. */
+int tsp_get_u(tsp_context *ctx, int col, double *out);
+
+/* Seismograms recorded so far: out[(sample*nrec + r)*2 + c], up to max_samples
+ * samples (Receivers::M_val, Receivers.hpp:115).  *nsamples gets the count. */
+int tsp_get_seismograms(tsp_context *ctx, double *out, int max_samples, int *nsamples);
+
+/* Step-size estimate and stability check (no counterpart in the reference,
+ * which hard-codes dt, Examples/src/Lamb.cpp:18): lambda_max of invM*B by FP64
+ * power iteration on the device (`iters` iterations), and
+ * dt = factor * 2/sqrt(lambda_max), the leap-frog limit scaled by `factor`. */
+int tsp_cfl(tsp_context *ctx, double factor, int iters, double *dt_out, double *lambda_max_out);
+
+/* OR of TSP_FLAG_* seen on the current displacement. */
+int tsp_status(tsp_context *ctx, unsigned *flags);
+
+/* Wait for all queued work of the context. */
+int tsp_synchronize(tsp_context *ctx);
+
+/* Device bytes held by the context (operator + state + tables). */
+long long tsp_device_bytes(const tsp_context *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,294 @@
+"""Parity of the HIP path (through the C-ABI) with the FP64 CPU oracle.
+
+Error measure (stated once, used throughout): relative L2 = ||x - x_ref|| /
+||x_ref|| over the whole array -- the final displacement vector, or the full
+seismogram array [sample, receiver, component].  Per-receiver errors are
+normalised by the norm of the whole seismogram array, so that receivers the
+wave has not reached (tiny signal) do not turn roundoff into large ratios.
+
+Bars: FP64 <= 1e-12; assembled blocks <= 1e-13 (relative to the largest entry);
+FP32 compensated <= 1e-4 on seismograms."""
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tspeed_amd import capi, workloads as W
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def small_problem(N, nx=6, ny=5, nsteps=200, hetero=True, seed=4, nrec=7, stride=1):
+    p = W.config1(nx=nx, ny=ny, N=N, nsteps=nsteps, nrec=nrec, seed=seed, stride=stride)
+    if hetero:
+        rng = np.random.default_rng(seed)
+        f = rng.uniform(0.6, 1.4, p.ne)
+        p.lam = p.lam * f * rng.uniform(0.8, 1.2, p.ne)
+        p.mu = p.mu * f
+        p.rho = p.rho * rng.uniform(0.8, 1.2, p.ne)
+    # strong source early in the run so short runs see signal
+    p.t0 = np.array([10 * p.dt])
+    p.f0 = np.array([1.0 / (8 * p.dt)])
+    return p
+
+
+def oracle_op(p):
+    return O.Operator(p.N, p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+
+
+def oracle_run(p, op=None, u0=None, v0=None, stride=None):
+    op = op or oracle_op(p)
+    return op.run(p.dt, p.nsteps, src=p.source_dict(0), rec_elem=p.rec_elem, rec_phi=p.rec_phi,
+                  stride=p.stride if stride is None else stride, u0=u0, v0=v0)
+
+
+def seis_err(s, sref):
+    nrm = np.linalg.norm(sref)
+    per_rec = np.linalg.norm((s - sref).transpose(1, 0, 2).reshape(s.shape[1], -1), axis=1) / nrm
+    return rel_l2(s, sref), float(per_rec.max())
+
+
+# ------------------------------------------------------------------ assembly
+@pytest.mark.parametrize("N", range(1, 9))
+def test_assembly_matches_oracle(N):
+    p = small_problem(N, nx=4, ny=3)
+    say(f"assembly N={N} ne={p.ne}")
+    eng = capi.Engine("fp64")
+    eng.set_tables(N)
+    eng.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+    blk, invm = eng.get_blocks()
+    op = oracle_op(p)
+    ref = op.blocks()
+    err = np.abs(blk - ref).max() / np.abs(ref).max()
+    say(f"  max block diff / max entry = {err:.3e}; bitwise equal entries: {(blk == ref).mean():.4f}")
+    assert err <= 1e-13
+    np.testing.assert_array_equal(invm, op.invm())
+    # absent neighbours give zero blocks
+    for e, f in zip(*np.nonzero(p.neigh < 0)):
+        assert not blk[e, 1 + f].any()
+
+
+# ------------------------------------------------------------------ FP64 trajectories
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_fp64_trajectory_matches_oracle(N):
+    p = small_problem(N, nsteps=150)
+    rng = np.random.default_rng(N)
+    u0 = rng.standard_normal(p.ndof) * 1e-6
+    v0 = rng.standard_normal(p.ndof) * 1e-4
+    say(f"fp64 trajectory N={N} ne={p.ne} steps={p.nsteps}")
+    eng, u, s = W.run_engine(p, "fp64", u0=u0, v0=v0)
+    uref, sref = oracle_run(p, u0=u0, v0=v0)
+    e_u = rel_l2(u, uref)
+    e_s, e_r = seis_err(s, sref)
+    say(f"  u rel L2 {e_u:.3e}  seis rel L2 {e_s:.3e}  worst receiver {e_r:.3e}")
+    assert s.shape == sref.shape
+    assert e_u <= 1e-12 and e_s <= 1e-12 and e_r <= 1e-12
+    assert eng.status() == 0
+
+
+def test_config1_full_parity():
+    """Config 1 at full size (3200 triangles, p=4, 2000 steps, 32 receivers):
+    final displacement and every seismogram within 1e-12 of the oracle."""
+    p = W.config1()
+    say(f"config1 ne={p.ne} dof={p.ndof} dt={p.dt:.4e} steps={p.nsteps}")
+    t = time.time()
+    eng, u, s = W.run_engine(p, "fp64")
+    say(f"  gpu done in {time.time() - t:.2f}s")
+    t = time.time()
+    uref, sref = oracle_run(p)
+    say(f"  oracle done in {time.time() - t:.2f}s")
+    e_u = rel_l2(u, uref)
+    e_s, e_r = seis_err(s, sref)
+    say(f"  u rel L2 {e_u:.3e}  seis rel L2 {e_s:.3e}  worst receiver {e_r:.3e}  |s|max {np.abs(sref).max():.3e}")
+    assert np.abs(sref).max() > 0
+    assert e_u <= 1e-12 and e_s <= 1e-12 and e_r <= 1e-12
+    np.save(os.path.join(os.environ.get("TSP_OUT", "/tmp"), "config1_seis_fp64.npy"), s) if os.environ.get("TSP_OUT") else None
+
+
+def test_fp32c_error_vs_fp64_oracle_config1():
+    """FP32 blocks + compensated update stay within 1e-4 of the FP64 oracle on
+    config 1's seismograms over 2000 steps."""
+    p = W.config1()
+    say(f"fp32c config1 steps={p.nsteps}")
+    eng, u, s = W.run_engine(p, "fp32c")
+    uref, sref = oracle_run(p)
+    e_u = rel_l2(u, uref)
+    e_s, e_r = seis_err(s, sref)
+    say(f"  fp32c: u rel L2 {e_u:.3e}  seis rel L2 {e_s:.3e}  worst receiver {e_r:.3e}")
+    assert e_s <= 1e-4 and e_r <= 1e-4
+    assert eng.status() == 0
+
+
+def test_fp32c_small_many_degrees():
+    for N in (2, 5, 8):
+        p = small_problem(N, nsteps=400)
+        eng, u, s = W.run_engine(p, "fp32c")
+        uref, sref = oracle_run(p)
+        e_s, _ = seis_err(s, sref)
+        say(f"fp32c N={N}: seis rel L2 {e_s:.3e}")
+        assert e_s <= 1e-4
+
+
+# ------------------------------------------------------------------ API paths
+def test_upload_blocks_path_matches_assembled_path():
+    p = small_problem(4, nsteps=60)
+    op = oracle_op(p)
+    e1, u1, s1 = W.run_engine(p, "fp64")
+    eng = capi.Engine("fp64")
+    eng.set_tables(p.N)
+    eng.upload_blocks(p.neigh, op.blocks(), op.invm())
+    eng.set_sources(p.src_elem, p.src_phi, p.wavelet())
+    eng.set_receivers(p.rec_elem, p.rec_phi, 1, p.nsamples)
+    eng.set_dt(p.dt)
+    eng.set_state()
+    eng.first_step()
+    eng.run(p.nsteps)
+    u2, s2 = eng.get_u(), eng.seismograms()
+    assert rel_l2(u2, u1) <= 1e-13 and rel_l2(s2, s1) <= 1e-13
+
+
+def test_stride_and_eval_receivers():
+    """Fused sampling every 3 steps equals the stride-1 record decimated, and an
+    explicit eval_receivers() after the run equals the last fused sample."""
+    p = small_problem(3, nsteps=30)
+    _, _, s1 = W.run_engine(p, "fp64", stride=1)
+    eng, _, s3 = W.run_engine(p, "fp64", stride=3)
+    np.testing.assert_array_equal(s3[0], s1[0])
+    np.testing.assert_allclose(s3[1:], s1[3::3], rtol=0, atol=1e-14 * np.abs(s1).max())
+    # stepping in pieces equals stepping at once
+    eng2 = capi.Engine("fp64")
+    eng2.set_tables(p.N)
+    eng2.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+    eng2.set_sources(p.src_elem, p.src_phi, p.wavelet())
+    eng2.set_receivers(p.rec_elem, p.rec_phi, 0, 4)
+    eng2.set_dt(p.dt)
+    eng2.set_state()
+    eng2.first_step()
+    eng2.eval_receivers()
+    for chunk in (7, 13, 10):
+        eng2.run(chunk)
+    eng2.eval_receivers()
+    s = eng2.seismograms()
+    assert s.shape[0] == 2
+    np.testing.assert_array_equal(s[0], s1[0])
+    np.testing.assert_allclose(s[1], s1[-1], rtol=0, atol=1e-14 * np.abs(s1).max())
+
+
+def test_cfl_lambda_max_matches_oracle():
+    p = small_problem(3, nx=4, ny=4)
+    eng = capi.Engine("fp64")
+    eng.set_tables(p.N)
+    eng.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+    dt, lm = eng.cfl(1.0, 400)
+    ref = oracle_op(p).lambda_max(400)
+    say(f"lambda_max gpu {lm:.6e} oracle {ref:.6e}")
+    assert abs(lm - ref) <= 1e-6 * ref
+    assert abs(dt - 2 / np.sqrt(lm)) <= 1e-12 * dt
+    assert p.dt < dt            # the workload step size is stable
+
+
+def test_nonfinite_detected_and_errors_reported():
+    p = small_problem(2, nsteps=400)
+    p.dt *= 50                   # far above the stability limit
+    eng, u, s = W.run_engine(p, "fp64")
+    assert eng.status() & capi.TSP_FLAG_NONFINITE
+    e = capi.Engine("fp64")
+    with pytest.raises(capi.EngineError) as ex:
+        e.run(1)
+    assert ex.value.code == capi.TSP_ERR_STATE
+    e.set_tables(2)
+    e.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+    e.set_sources(p.src_elem, p.src_phi, p.wavelet(5))
+    e.set_dt(p.dt)
+    e.set_state()
+    e.first_step()
+    with pytest.raises(capi.EngineError) as ex:
+        e.run(10)                # wavelet table has 5 rows
+    assert ex.value.code == capi.TSP_ERR_ARG
+    bad = p.neigh.copy()
+    bad[0, 0] = p.ne + 5
+    with pytest.raises(capi.EngineError):
+        e.set_mesh(p.verts, bad, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+
+
+def test_single_element_mesh():
+    """A lone triangle: every face is a boundary face, B = A."""
+    pts = np.array([[0.0, 0.0], [100.0, 10.0], [20.0, 90.0]])
+    tri = np.array([[0, 1, 2]], np.int32)
+    p = W.Problem("one", 3, pts, tri, np.array([6e9]), np.array([5e9]), np.array([2000.0]), 10.0, 1e-5, 50,
+                  np.array([[40.0, 30.0]]), np.array([50.0]), np.array([0.01]), np.array([[1.0, 0.0]]),
+                  np.array([[30.0, 40.0]]))
+    eng, u, s = W.run_engine(p, "fp64")
+    uref, sref = oracle_run(p)
+    assert rel_l2(u, uref) <= 1e-12 and rel_l2(s, sref) <= 1e-12
+
+
+# ------------------------------------------------------------------ full-size config 2
+def _submesh(p, centre):
+    """Elements `centre` plus their neighbours, renumbered; returns the sub-problem
+    arrays and the local index of each centre element."""
+    keep = set(int(e) for e in centre)
+    for e in centre:
+        keep.update(int(x) for x in p.neigh[e] if x >= 0)
+    keep = np.array(sorted(keep))
+    tri = p.tri[keep]
+    used, inv = np.unique(tri, return_inverse=True)
+    pts = p.pts[used]
+    tri = inv.reshape(-1, 3).astype(np.int32)
+    nb, ng = capi.neighbors(tri)
+    loc = np.searchsorted(keep, centre)
+    return pts, tri, nb, ng, keep, loc
+
+
+def test_config2_full_size_blocks_and_linearity():
+    """Config 2 at full size (160k triangles, p=6, 16 GB of FP64 blocks):
+    (a) blocks of 64 sampled elements equal the oracle's blocks assembled on the
+        sub-mesh of those elements and their neighbours;
+    (b) doubling the force doubles every displacement and sample bit for bit
+        (every operation of the step is linear and scaling by 2 is exact)."""
+    p = W.config2(nsteps=20)
+    say(f"config2 ne={p.ne} dof={p.ndof}")
+    eng = capi.Engine("fp64")
+    t = time.time()
+    eng.set_tables(p.N)
+    eng.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+    say(f"  assembled in {time.time() - t:.2f}s, device bytes {eng.device_bytes() / 1e9:.2f} GB")
+    rng = np.random.default_rng(0)
+    centre = np.unique(np.concatenate([rng.integers(0, p.ne, 60), [0, p.ne - 1, int(p.src_elem[0]), 1000]]))
+    pts, tri, nb, ng, keep, loc = _submesh(p, centre)
+    op = O.Operator(p.N, pts[tri].reshape(-1, 6), nb, ng, p.lam[keep], p.mu[keep], p.rho[keep], p.penalty)
+    ref = op.blocks()
+    worst = 0.0
+    for e, l in zip(centre, loc):
+        blk, im = eng.get_blocks(int(e), 1)
+        worst = max(worst, np.abs(blk[0] - ref[l]).max() / np.abs(ref[l]).max())
+        assert im[0] == op.invm()[l]
+    say(f"  sampled block max rel diff {worst:.3e}")
+    assert worst <= 1e-13
+    # (b) linearity at full size
+    eng.set_sources(p.src_elem, p.src_phi, p.wavelet())
+    eng.set_receivers(p.rec_elem, p.rec_phi, 1, p.nsamples)
+    eng.set_dt(p.dt)
+    eng.set_state()
+    eng.first_step()
+    eng.run(p.nsteps)
+    u1, s1 = eng.get_u(), eng.seismograms()
+    eng.set_sources(p.src_elem, p.src_phi, 2.0 * p.wavelet())
+    eng.set_state()
+    eng.first_step()
+    eng.run(p.nsteps)
+    u2, s2 = eng.get_u(), eng.seismograms()
+    assert np.abs(u1).max() > 0
+    np.testing.assert_array_equal(u2, 2.0 * u1)
+    np.testing.assert_array_equal(s2, 2.0 * s1)
+    assert eng.status() == 0

@@ -1,0 +1,154 @@
+"""Host C++ layer (tspeed_amd/csrc/host) against the oracle and closed forms.
+CPU only: nothing here touches the GPU."""
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tspeed_amd import capi, workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_MESHES = "/root/reference/Examples/Meshes"
+
+
+@pytest.mark.parametrize("N", range(1, 9))
+def test_tables_equal_oracle(N):
+    a, b = capi.tables(N), O.tables(N)
+    for k in ("B", "G", "Bedge", "Gedge", "w2", "w1", "x1"):
+        np.testing.assert_array_equal(a[k].ravel(), b[k].ravel(), err_msg=k)
+    np.testing.assert_array_equal(a["xy2"].ravel(), b["xy2"].ravel())
+
+
+@pytest.mark.parametrize("N", [1, 4, 8])
+def test_dubiner_point_values_equal_oracle(N):
+    rng = np.random.default_rng(N)
+    for _ in range(10):
+        x, y = rng.uniform(0.0, 0.5, 2)
+        p, g = capi.dubiner(N, x, y)
+        po, go = O.dubiner(N, x, y)
+        np.testing.assert_array_equal(p, po)
+        np.testing.assert_array_equal(g, go)
+
+
+@pytest.mark.parametrize("nx,ny,jit,seed", [(3, 2, 0.0, 1), (7, 5, 0.2, 2), (12, 9, 0.2, 3)])
+def test_edge_hash_neighbors_equal_bruteforce(nx, ny, jit, seed):
+    pts, tri = capi.split_square(nx, ny, 0.0, 0.0, 1.0, 1.0, jit, seed)
+    nb, ng = capi.neighbors(tri)
+    nbo, ngo = O.neighbors(tri)
+    np.testing.assert_array_equal(nb, nbo)
+    np.testing.assert_array_equal(ng, ngo)
+
+
+def test_split_square_is_valid_at_config2_size():
+    """160k triangles, jitter 0.2h: every triangle counter-clockwise, boundary
+    vertices on the boundary, interior jitter within +-0.2h."""
+    pts, tri = capi.split_square(400, 200, 0.0, 0.0, 4000.0, 2000.0, 0.2, 2)
+    v = pts[tri]
+    area = 0.5 * ((v[:, 1, 0] - v[:, 0, 0]) * (v[:, 2, 1] - v[:, 0, 1]) - (v[:, 1, 1] - v[:, 0, 1]) * (v[:, 2, 0] - v[:, 0, 0]))
+    assert tri.shape == (160000, 3)
+    assert area.min() > 0
+    np.testing.assert_allclose(area.sum(), 4000.0 * 2000.0, rtol=1e-12)
+    grid = np.stack(np.meshgrid(np.arange(401) * 10.0, np.arange(201) * 10.0), -1).reshape(-1, 2)
+    d = np.abs(pts - grid)
+    assert d.max() <= 0.2 * 10.0 + 1e-12
+    onb = (grid[:, 0] == 0) | (grid[:, 0] == 4000) | (grid[:, 1] == 0) | (grid[:, 1] == 2000)
+    assert d[onb].max() == 0.0
+
+
+def test_locate_equals_linear_scan():
+    pts, tri = capi.split_square(9, 7, 0.0, 0.0, 900.0, 700.0, 0.2, 9)
+    verts = pts[tri].reshape(-1, 6)
+    rng = np.random.default_rng(0)
+    q = rng.uniform([0, 0], [900, 700], (300, 2))
+    # include vertices and edge midpoints, where several elements pass the tolerant test
+    q = np.concatenate([q, pts, 0.5 * (pts[tri[:, 0]] + pts[tri[:, 1]])])
+    el, xi, phi = capi.locate(pts, tri, q[:, 0], q[:, 1], 3)
+    for k, (x, y) in enumerate(q):
+        eo, xo = O.locate(verts, x, y)
+        assert el[k] == eo, (k, x, y)
+        if eo >= 0:
+            np.testing.assert_array_equal(xi[k], xo)
+    el2, _, _ = capi.locate(pts, tri, [-5.0, 950.0], [10.0, 10.0], 3)
+    assert (el2 == -1).all()
+
+
+def test_ricker_table_times():
+    """Row 0 at t = 0, row 1 at dt + dt, then repeated addition (TimeAdvance.cpp:48)."""
+    dt = 1.3e-4
+    tab = capi.ricker_table([10.0], [0.12], [0.0], [-1.0], dt, 50)
+    t, ts = 0.0, [0.0]
+    for r in range(1, 50):
+        t = dt + dt if r == 1 else t + dt
+        ts.append(t)
+    exp = np.array([-O.ricker(10.0, 0.12, t) for t in ts])
+    np.testing.assert_array_equal(tab[:, 0, 1], exp)
+    np.testing.assert_array_equal(tab[:, 0, 0], 0.0 * exp)
+
+
+def test_config_builders():
+    p = W.config1()
+    assert p.ne == 3200 and p.ndof == 96000 and p.N == 4
+    assert p.rec_xy.shape == (32, 2) and (p.rec_elem >= 0).all()
+    lam, mu = p.lam[0], p.mu[0]
+    np.testing.assert_allclose(np.sqrt((lam + 2 * mu) / 2000.0), 3000.0)
+    np.testing.assert_allclose(np.sqrt(mu / 2000.0), 1732.0)
+    assert 0 < p.dt < 1e-4
+    small = W.config2(nx=40, ny=20)
+    vs = np.sqrt(small.mu / small.rho)
+    assert vs.min() > 800 * np.exp(-0.6) and vs.max() < 2500 * np.exp(0.6)
+    np.testing.assert_allclose((small.lam / small.mu), 1.0, rtol=1e-12)   # vp = sqrt(3) vs
+
+
+# -------- reference mesh fixtures: known counts (read when the reference is mounted)
+MSH_FACTS = {
+    "Lamb_fullyunstruct3.msh": dict(nodes=822, tri=1532, bnd=110),
+    "wedge.msh": dict(nodes=1234, tri=2360, bnd=162),
+    "Lamb.msh": dict(nodes=3729, tri=7220, bnd=236),
+    "Lamb_fullyunstruct.msh": dict(tri=1178),
+    "Lamb_fullyunstruct2.msh": dict(tri=2298),
+    "Lamb_new_unstruct.msh": dict(tri=2686),
+    "wedge2.msh": dict(tri=2970),
+}
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_MESHES), reason="reference meshes not mounted")
+@pytest.mark.parametrize("name", sorted(MSH_FACTS))
+def test_msh_reader_and_neighbors_on_reference_meshes(name):
+    pts, tri, reg, bnd = capi.read_msh(os.path.join(REF_MESHES, name))
+    f = MSH_FACTS[name]
+    assert tri.shape[0] == f["tri"]
+    if "nodes" in f:
+        assert pts.shape[0] == f["nodes"]
+        assert bnd.shape[0] == f["bnd"]
+    nb, ng = capi.neighbors(tri)
+    # 2*interior + boundary = 3*ne; every boundary half-face is one of the mesh's
+    # el_type-1 lines (wedge.msh also carries lines on its internal interface)
+    nbnd = int((nb < 0).sum())
+    assert (3 * tri.shape[0] - nbnd) % 2 == 0
+    lines = {tuple(sorted(b)) for b in bnd.tolist()}
+    for e, fa in zip(*np.nonzero(nb < 0)):
+        assert tuple(sorted((tri[e, (fa + 1) % 3], tri[e, (fa + 2) % 3]))) in lines
+    if name.startswith("Lamb") and "bnd" in f:
+        assert nbnd == f["bnd"]
+    if tri.shape[0] <= 3000:
+        nbo, ngo = O.neighbors(tri)
+        np.testing.assert_array_equal(nb, nbo)
+        np.testing.assert_array_equal(ng, ngo)
+    v = pts[tri]
+    area = (v[:, 1, 0] - v[:, 0, 0]) * (v[:, 2, 1] - v[:, 0, 1]) - (v[:, 1, 1] - v[:, 0, 1]) * (v[:, 2, 0] - v[:, 0, 0])
+    assert (area > 0).all()                        # all counter-clockwise
+    if name == "wedge.msh":
+        assert sorted(set(reg.tolist())) == [1, 2] and (reg == 1).sum() == 548
+
+
+def test_config_receivers_file_format_roundtrip(tmp_path):
+    """Receivers::write format (Receivers.cpp:32-46): one 'x y' line per sample."""
+    from tspeed_amd import io
+    s = np.arange(24, dtype=np.float64).reshape(4, 3, 2) * 1.5e-7
+    io.write_receivers(str(tmp_path / "run"), s)
+    for r in range(3):
+        got = np.loadtxt(tmp_path / f"run_rcv_{r}.out")
+        np.testing.assert_array_equal(got, s[:, r, :])

@@ -1,0 +1,306 @@
+"""ctypes bindings of libtspeed_amd.so: the device engine (include/tspeed_amd.h)
+and the host-layer exports (include/tspeed_host.h).
+
+The library is built in-tree by ``tspeed_amd.build``.  Loading fails loudly when
+it is missing: there is no CPU fallback for the engine.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtspeed_amd.so")
+
+TSP_OK, TSP_ERR_ARG, TSP_ERR_STATE, TSP_ERR_HIP, TSP_ERR_UNSUPPORTED, TSP_ERR_NOMEM = range(6)
+TSP_FP64, TSP_FP32C = 0, 1
+TSP_FLAG_NONFINITE = 1
+PRECISIONS = {"fp64": TSP_FP64, "fp32c": TSP_FP32C}
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+
+# name -> (restype, argtypes) for every symbol of include/tspeed_amd.h and tspeed_host.h
+SIGNATURES = {
+    "tsp_device_count": (_i, [ctypes.POINTER(_i)]),
+    "tsp_create": (_i, [_i, _i, _i, ctypes.POINTER(_vp)]),
+    "tsp_destroy": (None, [_vp]),
+    "tsp_last_error": (ctypes.c_char_p, [_vp]),
+    "tsp_set_stream": (_i, [_vp, _vp]),
+    "tsp_set_tables": (_i, [_vp, _i, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "tsp_set_mesh": (_i, [_vp, _i, _dp, _ip, _ip, _dp, _dp, _dp, _d]),
+    "tsp_upload_blocks": (_i, [_vp, _i, _ip, _dp, _dp]),
+    "tsp_get_blocks": (_i, [_vp, _i, _i, _vp, _vp]),
+    "tsp_set_state": (_i, [_vp, _i, _vp, _vp]),
+    "tsp_set_sources": (_i, [_vp, _i, _ip, _ip, _dp, _i, _dp]),
+    "tsp_set_wavelet": (_i, [_vp, _i, _i, _dp]),
+    "tsp_set_receivers": (_i, [_vp, _i, _ip, _ip, _dp, _i, _i]),
+    "tsp_set_stride": (_i, [_vp, _i]),
+    "tsp_set_dt": (_i, [_vp, _d]),
+    "tsp_first_step": (_i, [_vp]),
+    "tsp_run": (_i, [_vp, _i]),
+    "tsp_run_timed": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_float)]),
+    "tsp_eval_receivers": (_i, [_vp]),
+    "tsp_steps_done": (_i, [_vp]),
+    "tsp_get_u": (_i, [_vp, _i, _dp]),
+    "tsp_get_seismograms": (_i, [_vp, _vp, _i, ctypes.POINTER(_i)]),
+    "tsp_cfl": (_i, [_vp, _d, _i, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
+    "tsp_status": (_i, [_vp, ctypes.POINTER(ctypes.c_uint)]),
+    "tsp_synchronize": (_i, [_vp]),
+    "tsp_device_bytes": (ctypes.c_longlong, [_vp]),
+    "tsh_gauleg": (_i, [_i, _d, _d, _dp, _dp]),
+    "tsh_dubiner": (_i, [_i, _d, _d, _dp, _vp]),
+    "tsh_tables": (_i, [_i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "tsh_neighbors": (_i, [_i, _ip, _ip, _ip]),
+    "tsh_split_square": (_i, [_i, _i, _d, _d, _d, _d, _d, ctypes.c_uint64, _dp, _ip]),
+    "tsh_msh_counts": (_i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "tsh_msh_read": (_i, [ctypes.c_char_p, _dp, _ip, _ip, _ip]),
+    "tsh_locate": (_i, [_i, _dp, _i, _ip, _i, _dp, _dp, _i, _ip, _dp, _dp]),
+    "tsh_ricker_table": (_i, [_i, _dp, _dp, _dp, _dp, _d, _i, _dp]),
+}
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libtspeed_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m tspeed_amd.build` "
+                          "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def device_count():
+    n = _i()
+    lib().tsp_device_count(ctypes.byref(n))
+    return n.value
+
+
+def _c64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c32i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def nln(N):
+    return (N + 1) * (N + 2) // 2
+
+
+# ---------------------------------------------------------------- host layer
+def gauleg(n, a=-1.0, b=1.0):
+    x, w = np.zeros(n), np.zeros(n)
+    assert lib().tsh_gauleg(n, a, b, x, w) == 0
+    return x, w
+
+
+def dubiner(N, xi, eta, grad=True):
+    phi = np.zeros(nln(N))
+    g = np.zeros(2 * nln(N)) if grad else None
+    assert lib().tsh_dubiner(N, float(xi), float(eta), phi, None if g is None else g.ctypes.data) == 0
+    return (phi, g.reshape(-1, 2)) if grad else phi
+
+
+def tables(N):
+    n, nq1 = nln(N), N + 1
+    nq2 = nq1 * nq1
+    B = np.zeros(nq2 * n); G = np.zeros(n * 2 * nq2); Be = np.zeros(3 * nq1 * n); Ge = np.zeros(n * 3 * 2 * nq1)
+    w2 = np.zeros(nq2); xy2 = np.zeros(2 * nq2); w1 = np.zeros(nq1); x1 = np.zeros(nq1)
+    assert lib().tsh_tables(N, B, G, Be, Ge, w2, xy2, w1, x1) == 0
+    return dict(B=B, G=G, Bedge=Be, Gedge=Ge, w2=w2, xy2=xy2.reshape(-1, 2), w1=w1, x1=x1)
+
+
+def neighbors(tri):
+    tri = _c32i(tri).reshape(-1, 3)
+    ne = tri.shape[0]
+    nb = np.zeros(3 * ne, np.int32); ng = np.zeros(3 * ne, np.int32)
+    if lib().tsh_neighbors(ne, tri.reshape(-1), nb, ng) != 0:
+        raise ValueError("non-manifold mesh")
+    return nb.reshape(ne, 3), ng.reshape(ne, 3)
+
+
+def split_square(nx, ny, x0, y0, Lx, Ly, jitter, seed):
+    pts = np.zeros(2 * (nx + 1) * (ny + 1)); tri = np.zeros(6 * nx * ny, np.int32)
+    assert lib().tsh_split_square(nx, ny, x0, y0, Lx, Ly, jitter, seed, pts, tri) == 0
+    return pts.reshape(-1, 2), tri.reshape(-1, 3)
+
+
+def read_msh(path):
+    np_, nt, nbd = _i(), _i(), _i()
+    if lib().tsh_msh_counts(path.encode(), ctypes.byref(np_), ctypes.byref(nt), ctypes.byref(nbd)) != 0:
+        raise FileNotFoundError(path)
+    pts = np.zeros(2 * np_.value); tri = np.zeros(3 * nt.value, np.int32)
+    reg = np.zeros(max(nt.value, 1), np.int32); bnd = np.zeros(max(2 * nbd.value, 1), np.int32)
+    assert lib().tsh_msh_read(path.encode(), pts, tri, reg, bnd) == 0
+    return pts.reshape(-1, 2), tri.reshape(-1, 3), reg[:nt.value], bnd[:2 * nbd.value].reshape(-1, 2)
+
+
+def locate(pts, tri, qx, qy, N=0):
+    pts = _c64(pts).reshape(-1); tri = _c32i(tri).reshape(-1)
+    qx = _c64(np.atleast_1d(qx)); qy = _c64(np.atleast_1d(qy))
+    nq = qx.size
+    el = np.zeros(nq, np.int32); xi = np.zeros(2 * nq); phi = np.zeros(max(1, nq * (nln(N) if N > 0 else 1)))
+    assert lib().tsh_locate(pts.size // 2, pts, tri.size // 3, tri, nq, qx, qy, N, el, xi, phi) == 0
+    return el, xi.reshape(-1, 2), (phi.reshape(nq, nln(N)) if N > 0 else None)
+
+
+def ricker_table(f0, t0, dirx, diry, dt, nrows):
+    f0, t0, dirx, diry = (_c64(np.atleast_1d(v)) for v in (f0, t0, dirx, diry))
+    ns = f0.size
+    out = np.zeros(max(1, nrows * ns * 2))
+    assert lib().tsh_ricker_table(ns, f0, t0, dirx, diry, float(dt), nrows, out) == 0
+    return out[: nrows * ns * 2].reshape(nrows, ns, 2)
+
+
+# ---------------------------------------------------------------- device engine
+class Engine:
+    """One tsp_context: a mesh, its operator and the time-stepping state."""
+
+    def __init__(self, precision="fp64", device=0, ncols=1):
+        self._L = lib()
+        self.precision = precision
+        h = _vp()
+        rc = self._L.tsp_create(device, PRECISIONS[precision], ncols, ctypes.byref(h))
+        if rc != TSP_OK:
+            raise EngineError(rc, f"tsp_create(device={device}, precision={precision}) failed")
+        self._h = h
+        self.N = None
+        self.ne = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.tsp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _ck(self, rc):
+        if rc != TSP_OK:
+            raise EngineError(rc, self._L.tsp_last_error(self._h).decode())
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream_ptr):
+        self._ck(self._L.tsp_set_stream(self._h, stream_ptr))
+
+    def set_tables(self, N, tab=None):
+        tab = tab or tables(N)
+        self._ck(self._L.tsp_set_tables(self._h, N, _c64(tab["B"]), _c64(tab["G"]), _c64(tab["Bedge"]),
+                                        _c64(tab["Gedge"]), _c64(tab["w2"]), _c64(tab["w1"])))
+        self.N = N
+
+    def set_mesh(self, verts, neigh, neighedge, lam, mu, rho, penalty):
+        verts = _c64(verts).reshape(-1)
+        self.ne = verts.size // 6
+        self._ck(self._L.tsp_set_mesh(self._h, self.ne, verts, _c32i(neigh).reshape(-1),
+                                      _c32i(neighedge).reshape(-1), _c64(lam), _c64(mu), _c64(rho), float(penalty)))
+
+    def upload_blocks(self, neigh, blocks, invm):
+        neigh = _c32i(neigh).reshape(-1)
+        self.ne = neigh.size // 3
+        self._ck(self._L.tsp_upload_blocks(self._h, self.ne, neigh, _c64(blocks).reshape(-1), _c64(invm)))
+
+    def get_blocks(self, e_begin=0, e_count=None):
+        e_count = self.ne - e_begin if e_count is None else e_count
+        d = 2 * nln(self.N)
+        b = np.zeros(max(1, e_count * 4 * d * d)); im = np.zeros(max(1, e_count))
+        self._ck(self._L.tsp_get_blocks(self._h, int(e_begin), int(e_count), b.ctypes.data, im.ctypes.data))
+        return b[: e_count * 4 * d * d].reshape(e_count, 4, d, d), im[:e_count]
+
+    def set_state(self, u0=None, v0=None, col=0):
+        a = None if u0 is None else _c64(u0)
+        b = None if v0 is None else _c64(v0)
+        self._ck(self._L.tsp_set_state(self._h, col, None if a is None else a.ctypes.data,
+                                       None if b is None else b.ctypes.data))
+
+    def set_sources(self, elem, phi, wavelet, col=None):
+        elem = _c32i(np.atleast_1d(elem))
+        ns = elem.size
+        col = _c32i(np.zeros(ns) if col is None else col)
+        wav = _c64(wavelet).reshape(-1)
+        ntab = wav.size // (2 * ns) if ns else 0
+        self._ck(self._L.tsp_set_sources(self._h, ns, col, elem, _c64(phi).reshape(-1), ntab,
+                                         wav if ns else np.zeros(1)))
+
+    def set_receivers(self, elem, phi, stride, max_samples, col=None):
+        elem = _c32i(np.atleast_1d(elem))
+        nr = elem.size
+        col = _c32i(np.zeros(nr) if col is None else col)
+        self._ck(self._L.tsp_set_receivers(self._h, nr, col, elem, _c64(phi).reshape(-1) if nr else np.zeros(1),
+                                           int(stride), int(max_samples)))
+        self.nrec = nr
+
+    def set_stride(self, stride):
+        self._ck(self._L.tsp_set_stride(self._h, int(stride)))
+
+    def set_dt(self, dt):
+        self._ck(self._L.tsp_set_dt(self._h, float(dt)))
+
+    def first_step(self):
+        self._ck(self._L.tsp_first_step(self._h))
+
+    def run(self, nsteps):
+        self._ck(self._L.tsp_run(self._h, int(nsteps)))
+
+    def run_timed(self, nsteps):
+        """Run nsteps and return the device time in milliseconds (HIP events)."""
+        ms = ctypes.c_float()
+        self._ck(self._L.tsp_run_timed(self._h, int(nsteps), ctypes.byref(ms)))
+        return ms.value
+
+    def eval_receivers(self):
+        self._ck(self._L.tsp_eval_receivers(self._h))
+
+    def steps_done(self):
+        return self._L.tsp_steps_done(self._h)
+
+    def get_u(self, col=0):
+        out = np.zeros(2 * self.ne * nln(self.N))
+        self._ck(self._L.tsp_get_u(self._h, col, out))
+        return out
+
+    def seismograms(self):
+        ns = _i()
+        self._ck(self._L.tsp_get_seismograms(self._h, None, 0, ctypes.byref(ns)))
+        out = np.zeros(max(1, ns.value * self.nrec * 2))
+        self._ck(self._L.tsp_get_seismograms(self._h, out.ctypes.data, ns.value, ctypes.byref(ns)))
+        return out[: ns.value * self.nrec * 2].reshape(ns.value, self.nrec, 2)
+
+    def cfl(self, factor=1.0, iters=100):
+        dt, lm = _d(), _d()
+        self._ck(self._L.tsp_cfl(self._h, float(factor), int(iters), ctypes.byref(dt), ctypes.byref(lm)))
+        return dt.value, lm.value
+
+    def status(self):
+        f = ctypes.c_uint()
+        self._ck(self._L.tsp_status(self._h, ctypes.byref(f)))
+        return f.value
+
+    def synchronize(self):
+        self._ck(self._L.tsp_synchronize(self._h))
+
+    def device_bytes(self):
+        return int(self._L.tsp_device_bytes(self._h))

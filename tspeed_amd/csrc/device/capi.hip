@@ -1,0 +1,787 @@
+// capi.hip -- implementation of the C-ABI declared in include/tspeed_amd.h.
+//
+// The context owns every device buffer of one simulation (or of ncols columns
+// sharing one mesh).  Nothing here computes on the host: set-up data is copied
+// in, assembled and stepped by the kernels of assemble.hip / step.hip, and
+// results are copied out on request.  Errors never throw across the ABI: each
+// entry point catches, records a message and returns a status code.
+#include "engine.hpp"
+#include "../../../include/tspeed_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace tspeed_dev;
+
+struct tsp_context {
+    int device = 0, precision = TSP_FP64, ncols = 1;
+    hipStream_t own_stream = nullptr, stream = nullptr;
+    std::string err;
+    long long bytes = 0;
+    std::vector<void *> allocs;
+
+    // tables
+    int P = 0, nln = 0, D = 0, nq1 = 0, nq2 = 0;
+    double *t_B = nullptr, *t_G = nullptr, *t_Be = nullptr, *t_Ge = nullptr, *t_w2 = nullptr, *t_w1 = nullptr;
+
+    // mesh / operator
+    int ne = 0;
+    std::vector<int> h_neigh;
+    int4 *meta = nullptr;
+    double *blk64 = nullptr, *invm = nullptr;
+    float *blk32 = nullptr, *sc_first = nullptr, *sc_step = nullptr;
+
+    // state: FP64 two vectors; FP32c hi ping-pong + compensation + increment
+    double *u64[2] = {nullptr, nullptr};
+    int cur = 0;
+    float *uhi[2] = {nullptr, nullptr}, *ucomp = nullptr, *winc = nullptr;
+    double *scratch64 = nullptr;  // 2*ne*nln staging for layout conversion
+    bool have_state = false;
+
+    // sources / receivers
+    int nsrc = 0, ntab = 0;
+    std::vector<int> h_src_elem, h_src_col;
+    double *src_phi = nullptr, *wav = nullptr;
+    int nrec = 0, stride = 0, max_samples = 0, nsamples = 0;
+    std::vector<int> h_rec_elem;
+    double *rec_phi = nullptr, *seis = nullptr;
+    int *rec_elem = nullptr;
+    int4 *extra = nullptr;
+    int *src_sorted = nullptr, *rec_sorted = nullptr;
+
+    unsigned *flags = nullptr;
+    double dt = 0;
+    int steps = 0;        // completed step() calls
+    bool first_done = false;
+};
+
+namespace {
+
+struct Fail {
+    int code;
+};
+
+void fail(tsp_context *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+    throw Fail{code};
+}
+
+void hipcheck(tsp_context *c, hipError_t e, const char *what)
+{
+    if (e != hipSuccess) fail(c, TSP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+void kcheck(tsp_context *c, int rc, const char *what)
+{
+    if (rc == -2) fail(c, TSP_ERR_UNSUPPORTED, "%s: degree %d not built", what, c->P);
+    if (rc != 0) fail(c, TSP_ERR_HIP, "%s: launch failed: %s", what, hipGetErrorString(hipGetLastError()));
+}
+
+template <typename T>
+T *dalloc(tsp_context *c, size_t n, const char *what)
+{
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(c, TSP_ERR_NOMEM, "%s: cannot allocate %zu bytes on device %d", what, n * sizeof(T), c->device);
+    }
+    c->allocs.push_back(p);
+    c->bytes += (long long)(n * sizeof(T));
+    return static_cast<T *>(p);
+}
+
+template <typename T>
+void dfree(tsp_context *c, T *&p)
+{
+    if (!p) return;
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), (void *)p);
+    if (it != c->allocs.end()) c->allocs.erase(it);
+    (void)hipFree(p);
+    p = nullptr;
+}
+
+template <typename T>
+void h2d(tsp_context *c, T *dst, const T *src, size_t n, const char *what)
+{
+    hipcheck(c, hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream), what);
+}
+
+template <typename T>
+void d2h(tsp_context *c, T *dst, const T *src, size_t n, const char *what)
+{
+    hipcheck(c, hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, c->stream), what);
+    hipcheck(c, hipStreamSynchronize(c->stream), what);
+}
+
+void set_device(tsp_context *c) { hipcheck(c, hipSetDevice(c->device), "hipSetDevice"); }
+
+template <typename F>
+int guarded(tsp_context *c, F &&f)
+{
+    if (!c) return TSP_ERR_ARG;
+    try {
+        set_device(c);
+        f();
+        c->err.clear();
+        return TSP_OK;
+    } catch (const Fail &x) {
+        return x.code;
+    } catch (const std::exception &x) {
+        c->err = x.what();
+        return TSP_ERR_ARG;
+    }
+}
+
+// Rebuild the per-element source/receiver lists and the aux column of meta.
+void rebuild_extra(tsp_context *c)
+{
+    if (!c->ne) return;
+    std::vector<std::vector<int>> s(c->ne), r(c->ne);
+    for (int k = 0; k < c->nsrc; ++k) s[c->h_src_elem[k]].push_back(k);
+    for (int k = 0; k < c->nrec; ++k) r[c->h_rec_elem[k]].push_back(k);
+    std::vector<int4> ex;
+    std::vector<int> ss, rs;
+    std::vector<int4> meta(c->ne);
+    for (int e = 0; e < c->ne; ++e) {
+        meta[e] = make_int4(c->h_neigh[3 * e], c->h_neigh[3 * e + 1], c->h_neigh[3 * e + 2], -1);
+        if (s[e].empty() && r[e].empty()) continue;
+        int4 x;
+        x.x = (int)ss.size();
+        ss.insert(ss.end(), s[e].begin(), s[e].end());
+        x.y = (int)ss.size();
+        x.z = (int)rs.size();
+        rs.insert(rs.end(), r[e].begin(), r[e].end());
+        x.w = (int)rs.size();
+        meta[e].w = (int)ex.size();
+        ex.push_back(x);
+    }
+    dfree(c, c->extra);
+    dfree(c, c->src_sorted);
+    dfree(c, c->rec_sorted);
+    c->extra = dalloc<int4>(c, ex.size(), "extra");
+    c->src_sorted = dalloc<int>(c, ss.size(), "src list");
+    c->rec_sorted = dalloc<int>(c, rs.size(), "rec list");
+    if (!ex.empty()) h2d(c, c->extra, ex.data(), ex.size(), "extra");
+    if (!ss.empty()) h2d(c, c->src_sorted, ss.data(), ss.size(), "src list");
+    if (!rs.empty()) h2d(c, c->rec_sorted, rs.data(), rs.size(), "rec list");
+    h2d(c, c->meta, meta.data(), meta.size(), "meta");
+    hipcheck(c, hipStreamSynchronize(c->stream), "rebuild_extra");
+}
+
+void require_mesh(tsp_context *c)
+{
+    if (!c->ne || !c->blk64) fail(c, TSP_ERR_STATE, "no operator: call tsp_set_mesh or tsp_upload_blocks first");
+}
+
+void alloc_state(tsp_context *c)
+{
+    const size_t n = (size_t)c->ne * c->D;
+    if (!c->scratch64) c->scratch64 = dalloc<double>(c, n, "scratch");
+    if (c->precision == TSP_FP64) {
+        for (int k = 0; k < 2; ++k)
+            if (!c->u64[k]) c->u64[k] = dalloc<double>(c, n, "state");
+    } else {
+        for (int k = 0; k < 2; ++k)
+            if (!c->uhi[k]) c->uhi[k] = dalloc<float>(c, n, "state hi");
+        if (!c->ucomp) c->ucomp = dalloc<float>(c, n, "state comp");
+        if (!c->winc) c->winc = dalloc<float>(c, n, "state inc");
+        if (!c->u64[0]) c->u64[0] = dalloc<double>(c, n, "state in");
+        if (!c->u64[1]) c->u64[1] = dalloc<double>(c, n, "state in");
+    }
+}
+
+void operator_ready(tsp_context *c)
+{
+    // FP32 copy of the operator for the compensated path
+    if (c->precision == TSP_FP32C) {
+        const size_t n = (size_t)c->ne * 4 * c->D * c->D;
+        if (!c->blk32) c->blk32 = dalloc<float>(c, n, "blocks fp32");
+        kcheck(c, launch_demote_blocks(n, c->blk64, c->blk32, c->stream), "demote");
+        if (!c->sc_first) c->sc_first = dalloc<float>(c, c->ne, "sc");
+        if (!c->sc_step) c->sc_step = dalloc<float>(c, c->ne, "sc");
+    }
+    alloc_state(c);
+    hipcheck(c, hipMemsetAsync(c->u64[0], 0, sizeof(double) * c->ne * c->D, c->stream), "memset");
+    hipcheck(c, hipMemsetAsync(c->u64[1], 0, sizeof(double) * c->ne * c->D, c->stream), "memset");
+    c->have_state = true;
+    c->steps = 0;
+    c->first_done = false;
+    c->nsamples = 0;
+    rebuild_extra(c);
+}
+
+StepArgs base_args(tsp_context *c)
+{
+    StepArgs a{};
+    a.ne = c->ne;
+    a.nsrc = c->nsrc;
+    a.nrec = c->nrec;
+    a.meta = c->meta;
+    a.extra = c->extra;
+    a.src_sorted = c->src_sorted;
+    a.rec_sorted = c->rec_sorted;
+    a.rec_elem = c->rec_elem;
+    a.src_phi = c->src_phi;
+    a.rec_phi = c->rec_phi;
+    a.seis = c->seis;
+    a.sample = -1;
+    a.flags = c->flags;
+    a.blk64 = c->blk64;
+    a.invm = c->invm;
+    a.blk32 = c->blk32;
+    return a;
+}
+
+const double *wav_row(tsp_context *c, int row)
+{
+    if (c->nsrc == 0) return nullptr;
+    if (row >= c->ntab)
+        fail(c, TSP_ERR_ARG, "wavelet table has %d rows; step %d needs row %d (extend it with tsp_set_wavelet)",
+             c->ntab, row, row);
+    return c->wav + (size_t)row * c->nsrc * 2;
+}
+
+int take_sample_slot(tsp_context *c)
+{
+    if (c->nrec == 0) return -1;
+    if (c->nsamples >= c->max_samples) return -1;
+    return c->nsamples++;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tsp_device_count(int *n)
+{
+    if (!n) return TSP_ERR_ARG;
+    *n = 0;
+    int k = 0;
+    if (hipGetDeviceCount(&k) != hipSuccess) {
+        (void)hipGetLastError();
+        return TSP_OK;
+    }
+    *n = k;
+    return TSP_OK;
+}
+
+int tsp_create(int device, int precision, int ncols, tsp_context **out)
+{
+    if (!out) return TSP_ERR_ARG;
+    *out = nullptr;
+    if (precision != TSP_FP64 && precision != TSP_FP32C) return TSP_ERR_UNSUPPORTED;
+    if (ncols != 1) return TSP_ERR_UNSUPPORTED;
+    tsp_context *c = new (std::nothrow) tsp_context;
+    if (!c) return TSP_ERR_NOMEM;
+    c->device = device;
+    c->precision = precision;
+    c->ncols = ncols;
+    int rc = guarded(c, [&] {
+        int n = 0;
+        hipcheck(c, hipGetDeviceCount(&n), "hipGetDeviceCount");
+        if (device < 0 || device >= n) fail(c, TSP_ERR_ARG, "device %d out of range (%d devices)", device, n);
+        set_device(c);
+        hipcheck(c, hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking), "hipStreamCreate");
+        c->stream = c->own_stream;
+        c->flags = dalloc<unsigned>(c, 1, "flags");
+        hipcheck(c, hipMemsetAsync(c->flags, 0, sizeof(unsigned), c->stream), "memset");
+    });
+    if (rc != TSP_OK) {
+        tsp_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return TSP_OK;
+}
+
+void tsp_destroy(tsp_context *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+const char *tsp_last_error(const tsp_context *c) { return c ? c->err.c_str() : "null context"; }
+
+int tsp_set_stream(tsp_context *c, void *s)
+{
+    return guarded(c, [&] { c->stream = s ? static_cast<hipStream_t>(s) : c->own_stream; });
+}
+
+int tsp_set_tables(tsp_context *c, int N, const double *B, const double *G, const double *Be, const double *Ge,
+                   const double *w2, const double *w1)
+{
+    return guarded(c, [&] {
+        if (N < 1 || N > kMaxDegree) fail(c, TSP_ERR_UNSUPPORTED, "degree %d outside 1..%d", N, kMaxDegree);
+        if (!B || !G || !Be || !Ge || !w2 || !w1) fail(c, TSP_ERR_ARG, "null table pointer");
+        if (c->P && c->P != N && c->ne) fail(c, TSP_ERR_STATE, "degree change after the mesh was set");
+        c->P = N;
+        c->nln = nln_of(N);
+        c->D = 2 * c->nln;
+        c->nq1 = N + 1;
+        c->nq2 = c->nq1 * c->nq1;
+        dfree(c, c->t_B); dfree(c, c->t_G); dfree(c, c->t_Be); dfree(c, c->t_Ge); dfree(c, c->t_w2); dfree(c, c->t_w1);
+        c->t_B = dalloc<double>(c, (size_t)c->nq2 * c->nln, "tables");
+        c->t_G = dalloc<double>(c, (size_t)c->nln * 2 * c->nq2, "tables");
+        c->t_Be = dalloc<double>(c, (size_t)3 * c->nq1 * c->nln, "tables");
+        c->t_Ge = dalloc<double>(c, (size_t)c->nln * 3 * 2 * c->nq1, "tables");
+        c->t_w2 = dalloc<double>(c, c->nq2, "tables");
+        c->t_w1 = dalloc<double>(c, c->nq1, "tables");
+        h2d(c, c->t_B, B, (size_t)c->nq2 * c->nln, "tables");
+        h2d(c, c->t_G, G, (size_t)c->nln * 2 * c->nq2, "tables");
+        h2d(c, c->t_Be, Be, (size_t)3 * c->nq1 * c->nln, "tables");
+        h2d(c, c->t_Ge, Ge, (size_t)c->nln * 3 * 2 * c->nq1, "tables");
+        h2d(c, c->t_w2, w2, c->nq2, "tables");
+        h2d(c, c->t_w1, w1, c->nq1, "tables");
+        hipcheck(c, hipStreamSynchronize(c->stream), "tables");
+    });
+}
+
+static void check_mesh_args(tsp_context *c, int ne, const int *neigh, const int *neighedge)
+{
+    if (ne <= 0) fail(c, TSP_ERR_ARG, "ne must be positive (got %d)", ne);
+    for (int q = 0; q < 3 * ne; ++q) {
+        const int nb = neigh[q];
+        if (nb < -1 || nb >= ne) fail(c, TSP_ERR_ARG, "neigh[%d] = %d out of range", q, nb);
+        if (nb == q / 3) fail(c, TSP_ERR_ARG, "element %d is its own neighbour", q / 3);
+        if (nb >= 0 && neighedge) {
+            const int g = neighedge[q];
+            if (g < 0 || g > 2) fail(c, TSP_ERR_ARG, "neighedge[%d] = %d out of range", q, g);
+            if (neigh[3 * nb + g] != q / 3)
+                fail(c, TSP_ERR_ARG, "neighbour relation not symmetric at element %d edge %d", q / 3, q % 3);
+        }
+    }
+}
+
+static void reset_mesh(tsp_context *c, int ne)
+{
+    dfree(c, c->meta); dfree(c, c->blk64); dfree(c, c->invm); dfree(c, c->blk32);
+    dfree(c, c->sc_first); dfree(c, c->sc_step);
+    for (int k = 0; k < 2; ++k) { dfree(c, c->u64[k]); dfree(c, c->uhi[k]); }
+    dfree(c, c->ucomp); dfree(c, c->winc); dfree(c, c->scratch64);
+    c->ne = ne;
+    c->nsrc = 0; c->h_src_elem.clear(); c->h_src_col.clear(); c->ntab = 0;
+    dfree(c, c->src_phi); dfree(c, c->wav);
+    c->nrec = 0; c->h_rec_elem.clear(); dfree(c, c->rec_phi); dfree(c, c->seis); dfree(c, c->rec_elem);
+    c->meta = dalloc<int4>(c, ne, "meta");
+    c->blk64 = dalloc<double>(c, (size_t)ne * 4 * c->D * c->D, "blocks");
+    c->invm = dalloc<double>(c, ne, "invm");
+}
+
+int tsp_set_mesh(tsp_context *c, int ne, const double *verts, const int *neigh, const int *neighedge,
+                 const double *lambda, const double *mu, const double *rho, double penalty)
+{
+    return guarded(c, [&] {
+        if (!c->P) fail(c, TSP_ERR_STATE, "call tsp_set_tables before tsp_set_mesh");
+        if (!verts || !neigh || !neighedge || !lambda || !mu || !rho) fail(c, TSP_ERR_ARG, "null mesh pointer");
+        check_mesh_args(c, ne, neigh, neighedge);
+        for (int e = 0; e < ne; ++e)
+            if (!(rho[e] > 0) || !std::isfinite(lambda[e]) || !std::isfinite(mu[e]))
+                fail(c, TSP_ERR_ARG, "bad material at element %d", e);
+        reset_mesh(c, ne);
+        c->h_neigh.assign(neigh, neigh + 3 * ne);
+        double *d_verts = dalloc<double>(c, (size_t)ne * 6, "verts");
+        int *d_nb = dalloc<int>(c, (size_t)ne * 3, "neigh");
+        int *d_ng = dalloc<int>(c, (size_t)ne * 3, "neighedge");
+        double *d_mat = dalloc<double>(c, (size_t)ne * 3, "materials");
+        ElemGeo *d_geo = dalloc<ElemGeo>(c, ne, "geometry");
+        FaceGeo *d_face = dalloc<FaceGeo>(c, (size_t)ne * 3, "faces");
+        h2d(c, d_verts, verts, (size_t)ne * 6, "verts");
+        h2d(c, d_nb, neigh, (size_t)ne * 3, "neigh");
+        h2d(c, d_ng, neighedge, (size_t)ne * 3, "neighedge");
+        h2d(c, d_mat, lambda, ne, "lambda");
+        h2d(c, d_mat + ne, mu, ne, "mu");
+        h2d(c, d_mat + 2 * ne, rho, ne, "rho");
+        AssembleArgs a{};
+        a.ne = ne;
+        a.verts = d_verts;
+        a.neigh = d_nb;
+        a.neighedge = d_ng;
+        a.lambda = d_mat;
+        a.mu = d_mat + ne;
+        a.rho = d_mat + 2 * ne;
+        a.cpen = penalty * c->P * c->P;
+        a.tab = Tables{c->t_B, c->t_G, c->t_Be, c->t_Ge, c->t_w2, c->t_w1};
+        a.geo = d_geo;
+        a.face = d_face;
+        a.blk = c->blk64;
+        a.invm = c->invm;
+        kcheck(c, launch_assemble(c->P, a, c->stream), "assemble");
+        hipcheck(c, hipStreamSynchronize(c->stream), "assemble");
+        dfree(c, d_verts); dfree(c, d_nb); dfree(c, d_ng); dfree(c, d_mat); dfree(c, d_geo); dfree(c, d_face);
+        operator_ready(c);
+    });
+}
+
+int tsp_upload_blocks(tsp_context *c, int ne, const int *neigh, const double *blocks, const double *invm)
+{
+    return guarded(c, [&] {
+        if (!c->P) fail(c, TSP_ERR_STATE, "call tsp_set_tables first (it fixes the degree)");
+        if (!neigh || !blocks || !invm) fail(c, TSP_ERR_ARG, "null pointer");
+        check_mesh_args(c, ne, neigh, nullptr);
+        reset_mesh(c, ne);
+        c->h_neigh.assign(neigh, neigh + 3 * ne);
+        const size_t n = (size_t)ne * 4 * c->D * c->D;
+        double *canon = dalloc<double>(c, n, "canonical blocks");
+        h2d(c, canon, blocks, n, "blocks");
+        kcheck(c, launch_canonical_to_blocks(c->P, ne, canon, c->blk64, c->stream), "layout");
+        h2d(c, c->invm, invm, ne, "invm");
+        hipcheck(c, hipStreamSynchronize(c->stream), "upload");
+        dfree(c, canon);
+        operator_ready(c);
+    });
+}
+
+int tsp_get_blocks(tsp_context *c, int e_begin, int e_count, double *blocks, double *invm)
+{
+    return guarded(c, [&] {
+        require_mesh(c);
+        if (e_begin < 0 || e_count < 0 || e_begin + e_count > c->ne)
+            fail(c, TSP_ERR_ARG, "element range [%d, %d) outside 0..%d", e_begin, e_begin + e_count, c->ne);
+        if (blocks && e_count) {
+            const size_t per = (size_t)4 * c->D * c->D;
+            double *canon = dalloc<double>(c, per * e_count, "canonical blocks");
+            kcheck(c, launch_blocks_to_canonical(c->P, e_count, c->blk64 + per * e_begin, canon, c->stream), "layout");
+            d2h(c, blocks, canon, per * e_count, "blocks");
+            dfree(c, canon);
+        }
+        if (invm && e_count) d2h(c, invm, c->invm + e_begin, e_count, "invm");
+    });
+}
+
+int tsp_set_state(tsp_context *c, int col, const double *u0, const double *v0)
+{
+    return guarded(c, [&] {
+        require_mesh(c);
+        if (col != 0) fail(c, TSP_ERR_ARG, "column %d out of range (ncols = %d)", col, c->ncols);
+        const size_t n = (size_t)c->ne * c->D;
+        // u64[0] <- u0, u64[1] <- v0, element-major
+        for (int k = 0; k < 2; ++k) {
+            const double *src = k == 0 ? u0 : v0;
+            if (src) {
+                h2d(c, c->scratch64, src, n, "state");
+                kcheck(c, launch_to_elem_major(c->ne, c->nln, c->scratch64, c->u64[k], c->stream), "layout");
+            } else {
+                hipcheck(c, hipMemsetAsync(c->u64[k], 0, n * sizeof(double), c->stream), "memset");
+            }
+        }
+        c->cur = 0;
+        c->steps = 0;
+        c->first_done = false;
+        c->nsamples = 0;
+        hipcheck(c, hipMemsetAsync(c->flags, 0, sizeof(unsigned), c->stream), "memset");
+        hipcheck(c, hipStreamSynchronize(c->stream), "state");
+    });
+}
+
+int tsp_set_sources(tsp_context *c, int nsrc, const int *col, const int *elem, const double *phi, int ntab,
+                    const double *wavelet)
+{
+    return guarded(c, [&] {
+        require_mesh(c);
+        if (nsrc < 0 || (nsrc > 0 && (!col || !elem || !phi))) fail(c, TSP_ERR_ARG, "bad source arrays");
+        if (nsrc > 0 && (ntab < 1 || !wavelet)) fail(c, TSP_ERR_ARG, "sources need a wavelet table");
+        for (int k = 0; k < nsrc; ++k) {
+            if (col[k] < 0 || col[k] >= c->ncols) fail(c, TSP_ERR_ARG, "source %d: column %d out of range", k, col[k]);
+            if (elem[k] < 0 || elem[k] >= c->ne) fail(c, TSP_ERR_ARG, "source %d: element %d out of range", k, elem[k]);
+        }
+        dfree(c, c->src_phi);
+        dfree(c, c->wav);
+        c->nsrc = nsrc;
+        c->ntab = nsrc ? ntab : 0;
+        c->h_src_elem.assign(elem, elem + nsrc);
+        c->h_src_col.assign(col, col + nsrc);
+        c->src_phi = dalloc<double>(c, (size_t)nsrc * c->nln, "source phi");
+        c->wav = dalloc<double>(c, (size_t)c->ntab * nsrc * 2, "wavelet");
+        if (nsrc) {
+            h2d(c, c->src_phi, phi, (size_t)nsrc * c->nln, "source phi");
+            h2d(c, c->wav, wavelet, (size_t)c->ntab * nsrc * 2, "wavelet");
+        }
+        rebuild_extra(c);
+    });
+}
+
+int tsp_set_wavelet(tsp_context *c, int row0, int nrows, const double *wavelet)
+{
+    return guarded(c, [&] {
+        if (!c->nsrc) fail(c, TSP_ERR_STATE, "no sources");
+        if (row0 < 0 || nrows < 0 || row0 + nrows > c->ntab || (nrows && !wavelet))
+            fail(c, TSP_ERR_ARG, "rows [%d, %d) outside the table of %d rows", row0, row0 + nrows, c->ntab);
+        if (nrows) {
+            h2d(c, c->wav + (size_t)row0 * c->nsrc * 2, wavelet, (size_t)nrows * c->nsrc * 2, "wavelet");
+            hipcheck(c, hipStreamSynchronize(c->stream), "wavelet");
+        }
+    });
+}
+
+int tsp_set_receivers(tsp_context *c, int nrec, const int *col, const int *elem, const double *phi, int stride,
+                      int max_samples)
+{
+    return guarded(c, [&] {
+        require_mesh(c);
+        if (nrec < 0 || (nrec > 0 && (!col || !elem || !phi))) fail(c, TSP_ERR_ARG, "bad receiver arrays");
+        if (stride < 0 || max_samples < 0) fail(c, TSP_ERR_ARG, "negative stride or sample count");
+        for (int k = 0; k < nrec; ++k) {
+            if (col[k] < 0 || col[k] >= c->ncols) fail(c, TSP_ERR_ARG, "receiver %d: column %d out of range", k, col[k]);
+            if (elem[k] < 0 || elem[k] >= c->ne) fail(c, TSP_ERR_ARG, "receiver %d: element %d out of range", k, elem[k]);
+        }
+        dfree(c, c->rec_phi);
+        dfree(c, c->seis);
+        dfree(c, c->rec_elem);
+        c->nrec = nrec;
+        c->stride = stride;
+        c->max_samples = max_samples;
+        c->nsamples = 0;
+        c->h_rec_elem.assign(elem, elem + nrec);
+        c->rec_phi = dalloc<double>(c, (size_t)nrec * c->nln, "receiver phi");
+        c->rec_elem = dalloc<int>(c, nrec, "receiver elem");
+        c->seis = dalloc<double>(c, (size_t)max_samples * nrec * 2, "seismograms");
+        if (nrec) {
+            h2d(c, c->rec_phi, phi, (size_t)nrec * c->nln, "receiver phi");
+            h2d(c, c->rec_elem, elem, nrec, "receiver elem");
+        }
+        rebuild_extra(c);
+    });
+}
+
+int tsp_set_stride(tsp_context *c, int stride)
+{
+    return guarded(c, [&] {
+        if (stride < 0) fail(c, TSP_ERR_ARG, "negative stride");
+        c->stride = stride;
+    });
+}
+
+int tsp_set_dt(tsp_context *c, double dt)
+{
+    return guarded(c, [&] {
+        if (!(dt > 0) || !std::isfinite(dt)) fail(c, TSP_ERR_ARG, "dt must be positive and finite");
+        c->dt = dt;
+    });
+}
+
+int tsp_first_step(tsp_context *c)
+{
+    return guarded(c, [&] {
+        require_mesh(c);
+        if (!(c->dt > 0)) fail(c, TSP_ERR_STATE, "call tsp_set_dt first");
+        if (c->first_done) fail(c, TSP_ERR_STATE, "first step already taken (reset with tsp_set_state)");
+        StepArgs a = base_args(c);
+        a.wav_row = wav_row(c, 0);
+        a.sample = take_sample_slot(c);
+        const double dt = c->dt;
+        if (c->precision == TSP_FP64) {
+            a.coef = dt * dt / 2;
+            a.dt = dt;
+            a.ucur = c->u64[0];   // u0
+            a.uprev = c->u64[1];  // v0 in, u1 out
+            kcheck(c, launch_step64(c->P, true, a, c->stream), "first step");
+            c->cur = 1;           // u^n = u1 in u64[1]; u^{n-1} = u0 in u64[0]
+        } else {
+            // hi/comp from u0; the increment starts as (float)(dt*v0)
+            const size_t n = (size_t)c->ne * c->D;
+            kcheck(c, launch_split_state(c->ne, c->D, c->u64[0], c->uhi[0], c->ucomp, c->stream), "split");
+            kcheck(c, launch_scale_demote(c->ne, c->invm, -(dt * dt / 2), c->sc_first, c->stream), "sc");
+            kcheck(c, launch_scale_demote(c->ne, c->invm, (-dt) * dt, c->sc_step, c->stream), "sc");
+            kcheck(c, launch_scale_demote(n, c->u64[1], dt, c->winc, c->stream), "winc");
+            a.sc32 = c->sc_first;
+            a.uhi = c->uhi[0];
+            a.uhi_out = c->uhi[1];
+            a.ucomp = c->ucomp;
+            a.winc = c->winc;
+            kcheck(c, launch_step32c(c->P, true, a, c->stream), "first step");
+            c->cur = 1;
+        }
+        c->first_done = true;
+    });
+}
+
+int tsp_run(tsp_context *c, int nsteps)
+{
+    return guarded(c, [&] {
+        require_mesh(c);
+        if (!c->first_done) fail(c, TSP_ERR_STATE, "call tsp_first_step before tsp_run");
+        if (nsteps < 0) fail(c, TSP_ERR_ARG, "negative step count");
+        if (c->nsrc && c->steps + nsteps >= c->ntab)
+            fail(c, TSP_ERR_ARG, "wavelet table has %d rows; %d more steps need %d", c->ntab, nsteps,
+                 c->steps + nsteps + 1);
+        if (c->nrec > 0 && c->stride > 0) {
+            const int need = (c->steps + nsteps) / c->stride - c->steps / c->stride;
+            if (c->nsamples + need > c->max_samples)
+                fail(c, TSP_ERR_ARG, "seismogram buffer of %d samples cannot hold %d more", c->max_samples, need);
+        }
+        StepArgs a = base_args(c);
+        const double dt = c->dt;
+        a.coef = (-dt) * dt;
+        a.sc32 = c->sc_step;
+        a.ucomp = c->ucomp;
+        a.winc = c->winc;
+        for (int k = 0; k < nsteps; ++k) {
+            const int n = c->steps + 1;  // 1-based index of this step() call
+            a.wav_row = c->nsrc ? c->wav + (size_t)n * c->nsrc * 2 : nullptr;
+            a.sample = (c->stride > 0 && n % c->stride == 0) ? take_sample_slot(c) : -1;
+            if (c->precision == TSP_FP64) {
+                a.ucur = c->u64[c->cur];
+                a.uprev = c->u64[1 - c->cur];
+                kcheck(c, launch_step64(c->P, false, a, c->stream), "step");
+            } else {
+                a.uhi = c->uhi[c->cur];
+                a.uhi_out = c->uhi[1 - c->cur];
+                kcheck(c, launch_step32c(c->P, false, a, c->stream), "step");
+            }
+            c->cur = 1 - c->cur;
+            c->steps = n;
+        }
+    });
+}
+
+int tsp_run_timed(tsp_context *c, int nsteps, float *ms)
+{
+    if (!c) return TSP_ERR_ARG;
+    hipEvent_t a = nullptr, b = nullptr;
+    int rc = guarded(c, [&] {
+        if (!ms) fail(c, TSP_ERR_ARG, "null output");
+        hipcheck(c, hipEventCreate(&a), "hipEventCreate");
+        hipcheck(c, hipEventCreate(&b), "hipEventCreate");
+        hipcheck(c, hipEventRecord(a, c->stream), "hipEventRecord");
+    });
+    if (rc == TSP_OK) rc = tsp_run(c, nsteps);
+    if (rc == TSP_OK)
+        rc = guarded(c, [&] {
+            hipcheck(c, hipEventRecord(b, c->stream), "hipEventRecord");
+            hipcheck(c, hipEventSynchronize(b), "hipEventSynchronize");
+            hipcheck(c, hipEventElapsedTime(ms, a, b), "hipEventElapsedTime");
+        });
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    return rc;
+}
+
+int tsp_eval_receivers(tsp_context *c)
+{
+    return guarded(c, [&] {
+        require_mesh(c);
+        if (!c->first_done) fail(c, TSP_ERR_STATE, "no displacement yet: call tsp_first_step");
+        if (c->nrec == 0) return;
+        StepArgs a = base_args(c);
+        a.sample = take_sample_slot(c);
+        if (a.sample < 0) fail(c, TSP_ERR_ARG, "seismogram buffer full (%d samples)", c->max_samples);
+        if (c->precision == TSP_FP64)
+            kcheck(c, launch_sample64(c->P, a, c->u64[c->cur], c->stream), "sample");
+        else
+            kcheck(c, launch_sample32(c->P, a, c->uhi[c->cur], c->ucomp, c->stream), "sample");
+    });
+}
+
+int tsp_steps_done(const tsp_context *c) { return c ? c->steps : -1; }
+
+int tsp_get_u(tsp_context *c, int col, double *out)
+{
+    return guarded(c, [&] {
+        require_mesh(c);
+        if (col != 0) fail(c, TSP_ERR_ARG, "column %d out of range", col);
+        if (!out) fail(c, TSP_ERR_ARG, "null output");
+        const size_t n = (size_t)c->ne * c->D;
+        const double *src;
+        if (!c->first_done) {
+            src = c->u64[0];
+        } else if (c->precision == TSP_FP64) {
+            src = c->u64[c->cur];
+        } else {
+            // u0/v0 staging is free once the first step has run: join hi - comp there
+            kcheck(c, launch_join_state(c->ne, c->D, c->uhi[c->cur], c->ucomp, c->u64[0], c->stream), "join");
+            src = c->u64[0];
+        }
+        kcheck(c, launch_to_comp_major(c->ne, c->nln, src, c->scratch64, c->stream), "layout");
+        d2h(c, out, c->scratch64, n, "get_u");
+    });
+}
+
+int tsp_get_seismograms(tsp_context *c, double *out, int max_samples, int *nsamples)
+{
+    return guarded(c, [&] {
+        const int ns = std::min(c->nsamples, std::max(0, max_samples));
+        if (nsamples) *nsamples = c->nsamples;
+        if (out && ns > 0 && c->nrec > 0) d2h(c, out, c->seis, (size_t)ns * c->nrec * 2, "seismograms");
+    });
+}
+
+int tsp_cfl(tsp_context *c, double factor, int iters, double *dt_out, double *lmax_out)
+{
+    return guarded(c, [&] {
+        require_mesh(c);
+        if (iters < 1) fail(c, TSP_ERR_ARG, "iters must be >= 1");
+        const size_t n = (size_t)c->ne * c->D;
+        double *x = dalloc<double>(c, n, "cfl"), *y = dalloc<double>(c, n, "cfl");
+        const int nparts = 256;
+        double *part = dalloc<double>(c, nparts, "cfl");
+        std::vector<double> hx(n), hp(nparts);
+        uint64_t st = 88172645463325252ull;
+        for (size_t q = 0; q < n; ++q) {
+            st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+            hx[q] = (double)(st >> 11) / 9007199254740992.0 - 0.5;
+        }
+        h2d(c, x, hx.data(), n, "cfl");
+        auto norm = [&](const double *v) {
+            kcheck(c, launch_dot(n, v, v, part, nparts, c->stream), "dot");
+            d2h(c, hp.data(), part, nparts, "dot");
+            double s = 0;
+            for (double p : hp) s += p;
+            return std::sqrt(s);
+        };
+        double lam = 0, nx = norm(x);
+        for (int it = 0; it < iters; ++it) {
+            kcheck(c, launch_apply_minvB(c->P, c->ne, c->meta, c->blk64, c->invm, x, y, c->stream), "apply");
+            const double ny = norm(y);
+            lam = ny / nx;
+            std::swap(x, y);
+            nx = ny;
+            if (!(nx > 0) || !std::isfinite(nx)) break;
+            if (nx > 1e100 || nx < 1e-100) {
+                // rescale to keep the iterate in range
+                std::vector<double> t(n);
+                d2h(c, t.data(), x, n, "cfl");
+                for (auto &v : t) v /= nx;
+                h2d(c, x, t.data(), n, "cfl");
+                nx = 1.0;
+            }
+        }
+        dfree(c, x); dfree(c, y); dfree(c, part);
+        if (lmax_out) *lmax_out = lam;
+        if (dt_out) *dt_out = lam > 0 ? factor * 2.0 / std::sqrt(lam) : 0.0;
+    });
+}
+
+int tsp_status(tsp_context *c, unsigned *flags)
+{
+    return guarded(c, [&] {
+        if (!flags) fail(c, TSP_ERR_ARG, "null output");
+        d2h(c, flags, c->flags, 1, "flags");
+    });
+}
+
+int tsp_synchronize(tsp_context *c)
+{
+    return guarded(c, [&] { hipcheck(c, hipStreamSynchronize(c->stream), "synchronize"); });
+}
+
+long long tsp_device_bytes(const tsp_context *c) { return c ? c->bytes : 0; }
+
+}  // extern "C"

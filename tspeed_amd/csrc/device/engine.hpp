@@ -1,0 +1,125 @@
+// engine.hpp -- shapes, device layouts and kernel-launch entry points of the
+// gfx950 leap-frog engine.  Host-visible declarations only; kernels live in
+// assemble.hip and step.hip.
+//
+// Device layouts (element-major, chosen for coalesced streaming):
+//   operator  blk[((e*4 + s)*DH + jp)*2D + row*2 + t] = B_{e,s}(row, 2*jp + t)
+//             s = 0 own element, s = 1+f neighbour across edge f; each d x d block
+//             is stored as column pairs so one lane (= one row) loads two columns
+//             with a single 16-byte (FP64) or 8-byte (FP32) load, and a wave's
+//             load of one column pair covers 2*d contiguous values.
+//   vectors   u[e*D + c*NLN + i]  (the ABI converts from the reference's
+//             component-major layout c*ne*nln + e*nln + i)
+//   meta      int4 {nb0, nb1, nb2, aux}  aux = row of `extra` for elements that
+//             hold a source or receiver, else -1
+//   extra     int4 {src_begin, src_end, rec_begin, rec_end} into the per-element
+//             sorted source / receiver index lists
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace tspeed_dev {
+
+constexpr int kMaxDegree = 8;
+
+template <int P>
+struct Geom {
+    static constexpr int NLN = (P + 1) * (P + 2) / 2;  // modes per component
+    static constexpr int D = 2 * NLN;                  // rows of a merged block
+    static constexpr int DH = D / 2;                   // column pairs per block
+    static constexpr int NQ1 = P + 1;                  // Gauss<N+1> edge nodes
+    static constexpr int NQ2 = NQ1 * NQ1;              // collapsed area nodes
+    // lanes of a wave that hold one column group (one row per lane)
+    static constexpr int RW = D <= 16 ? 16 : (D <= 32 ? 32 : 64);
+    static constexpr int NG = 64 / RW;                  // column groups per wave
+    static constexpr int RPL = (D + 63) / 64;           // rows per lane when D > 64
+    static constexpr int NIT = (DH + NG - 1) / NG;      // column pairs per group
+};
+
+inline int nln_of(int P) { return (P + 1) * (P + 2) / 2; }
+
+// Per-element FP64 geometry produced by the geometry kernel.
+struct ElemGeo {
+    double inv[4];   // invJ row-major
+    double det;      // detJ (signed, as the reference uses it)
+    double invm;     // ((1/rho)*1)/detJ
+};
+// Per half-face data: unit outward normal, 1/length * penalty coefficients.
+struct FaceGeo {
+    double n0, n1, len;
+    double ch00, ch11, ch01;  // c_mn * (1/len) with harmonic-mean Lame parameters
+};
+
+struct Tables {
+    const double *B, *G, *Be, *Ge, *w2, *w1;
+};
+
+struct AssembleArgs {
+    int ne;
+    const double *verts;
+    const int *neigh, *neighedge;
+    const double *lambda, *mu, *rho;
+    double cpen;        // penalty * N * N
+    Tables tab;
+    ElemGeo *geo;       // out (and in for the block kernel)
+    FaceGeo *face;      // out
+    double *blk;        // out: device operator layout, FP64
+    double *invm;       // out
+};
+
+int launch_assemble(int P, const AssembleArgs &a, hipStream_t s);
+
+// Canonical <-> device block layout conversion and FP32 demotion.
+int launch_blocks_to_canonical(int P, int ne, const double *blk, double *canon, hipStream_t s);
+int launch_canonical_to_blocks(int P, int ne, const double *canon, double *blk, hipStream_t s);
+int launch_demote_blocks(size_t n, const double *src, float *dst, hipStream_t s);
+
+struct StepArgs {
+    int ne;
+    int nsrc, nrec;
+    const int4 *meta;
+    const int4 *extra;
+    const int *src_sorted, *rec_sorted;
+    const int *rec_elem;      // element of each receiver
+    const double *src_phi;    // nsrc * NLN
+    const double *wav_row;    // wavelet row for this step: nsrc * 2
+    const double *rec_phi;    // nrec * NLN
+    double *seis;             // [sample][rec][2]
+    int sample;               // sample slot to write, -1 = none
+    unsigned *flags;
+    // FP64 plain
+    const double *blk64;
+    const double *invm;
+    double coef;              // first step: dt*dt/2 ; step: (-dt)*dt
+    double dt;                // first step only
+    const double *ucur;       // u^n (first step: u0)
+    double *uprev;            // u^{n-1} in, u^{n+1} out (first step: v0 in, u1 out)
+    // FP32 compensated
+    const float *blk32;
+    const float *sc32;        // per element (float)(invm * coef)
+    const float *uhi;         // gathered displacement (hi part)
+    float *uhi_out, *ucomp, *winc;
+};
+
+int launch_step64(int P, bool first, const StepArgs &a, hipStream_t s);
+int launch_step32c(int P, bool first, const StepArgs &a, hipStream_t s);
+
+// FP32-compensated state helpers
+int launch_split_state(int ne, int D, const double *u64, float *hi, float *comp, hipStream_t s);
+int launch_join_state(int ne, int D, const float *hi, const float *comp, double *u64, hipStream_t s);
+int launch_scale_demote(size_t n, const double *src, double coef, float *dst, hipStream_t s);
+
+// Receiver sample of the current displacement (eval_receivers).
+int launch_sample64(int P, const StepArgs &a, const double *u, hipStream_t s);
+int launch_sample32(int P, const StepArgs &a, const float *hi, const float *comp, hipStream_t s);
+
+// Power iteration pieces for the step-size check: y = invM * B * x (FP64).
+int launch_apply_minvB(int P, int ne, const int4 *meta, const double *blk, const double *invm,
+                       const double *x, double *y, hipStream_t s);
+int launch_dot(size_t n, const double *x, const double *y, double *partial, int nparts, hipStream_t s);
+
+// Layout helpers (element-major <-> component-major), column-agnostic.
+int launch_to_elem_major(int ne, int nln, const double *cm, double *em, hipStream_t s);
+int launch_to_comp_major(int ne, int nln, const double *em, double *cm, hipStream_t s);
+
+}  // namespace tspeed_dev

@@ -1,0 +1,171 @@
+"""Synthetic workloads: the split-square elastic problems the engine is measured
+and checked on.  Everything here is input generation (meshes, materials,
+sources, receivers, step size); the numerics live in libtspeed_amd.so.
+
+Conventions (fixed and documented so runs are reproducible):
+  - meshes come from ``split_square`` (host C++ layer): squares cut along the
+    (i,j)-(i+1,j+1) diagonal, interior vertices jittered by U(-0.2h, 0.2h);
+  - step size dt = cfl * h_min / (vp_max * p^2) with cfl = 0.2, h_min the
+    shortest edge; this sits below the leap-frog limit 2/sqrt(lambda_max(M^-1 B)),
+    which measures at 0.26-0.38 h_min/(vp p^2) for penalty 10 on these meshes;
+  - penalty 10 (the SIPG penalty 10 p^2/h);
+  - Ricker sources F = dir * ricker(f0, t0, t), t0 = 1.2/f0; the "vertical
+    point force" has dir = (0, -1) as in Examples/src/Lamb.cpp:37.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+
+CFL = 0.2
+PENALTY = 10.0
+
+
+@dataclass
+class Problem:
+    name: str
+    N: int
+    pts: np.ndarray
+    tri: np.ndarray
+    lam: np.ndarray
+    mu: np.ndarray
+    rho: np.ndarray
+    penalty: float
+    dt: float
+    nsteps: int
+    src_xy: np.ndarray          # (nsrc, 2)
+    f0: np.ndarray
+    t0: np.ndarray
+    dirs: np.ndarray            # (nsrc, 2)
+    rec_xy: np.ndarray          # (nrec, 2)
+    stride: int = 1
+    meta: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        self.verts = self.pts[self.tri].reshape(-1, 6)
+        self.neigh, self.neighedge = capi.neighbors(self.tri)
+        self.src_elem, _, self.src_phi = capi.locate(self.pts, self.tri, self.src_xy[:, 0], self.src_xy[:, 1], self.N)
+        self.rec_elem, _, self.rec_phi = capi.locate(self.pts, self.tri, self.rec_xy[:, 0], self.rec_xy[:, 1], self.N)
+        if (self.src_elem < 0).any() or (self.rec_elem < 0).any():
+            raise ValueError(f"{self.name}: a source or receiver lies outside the mesh")
+
+    @property
+    def ne(self):
+        return self.tri.shape[0]
+
+    @property
+    def nln(self):
+        return capi.nln(self.N)
+
+    @property
+    def ndof(self):
+        """DOF per right-hand-side column: ne * 2 * nln."""
+        return self.ne * 2 * self.nln
+
+    @property
+    def nsamples(self):
+        return 1 + self.nsteps // self.stride
+
+    def wavelet(self, nrows=None, src=None):
+        """Force table rows 0..nrows-1 (row 0 = t=0, row n = t_{n+1})."""
+        nrows = self.nsteps + 1 if nrows is None else nrows
+        idx = slice(None) if src is None else [src]
+        return capi.ricker_table(self.f0[idx], self.t0[idx], self.dirs[idx, 0], self.dirs[idx, 1], self.dt, nrows)
+
+    def source_dict(self, k=0):
+        """Source k in the oracle's form."""
+        return dict(elem=int(self.src_elem[k]), phi=self.src_phi[k], f0=float(self.f0[k]), t0=float(self.t0[k]),
+                    dirx=float(self.dirs[k, 0]), diry=float(self.dirs[k, 1]))
+
+    def min_edge(self):
+        v = self.verts.reshape(-1, 3, 2)
+        return float(min(np.hypot(*(v[:, a] - v[:, b]).T).min() for a, b in ((0, 1), (1, 2), (2, 0))))
+
+
+def cfl_dt(pts, tri, N, vp_max, cfl=CFL):
+    v = pts[tri]
+    hmin = min(np.hypot(*(v[:, a] - v[:, b]).T).min() for a, b in ((0, 1), (1, 2), (2, 0)))
+    return cfl * hmin / (vp_max * N * N)
+
+
+def lame(rho, vp, vs):
+    mu = rho * vs * vs
+    return rho * vp * vp - 2 * mu, mu
+
+
+def config1(nx=40, ny=40, N=4, nsteps=2000, nrec=32, seed=1, stride=1):
+    """Plain method: 1000 m x 1000 m, 40x40 split squares (3200 triangles), p=4,
+    homogeneous rho=2000, vp=3000, vs=1732, vertical Ricker f0=10 Hz at
+    (500, 950), 32 receivers along y=990, 2000 steps.  nx/ny/nsteps shrink it
+    for tests (the domain stays 1000 m x 1000 m)."""
+    L = 1000.0
+    pts, tri = capi.split_square(nx, ny, 0.0, 0.0, L, L, 0.2, seed)
+    ne = tri.shape[0]
+    lam, mu = lame(2000.0, 3000.0, 1732.0)
+    rx = (np.arange(nrec) + 0.5) * L / nrec
+    return Problem("config1", N, pts, tri, np.full(ne, lam), np.full(ne, mu), np.full(ne, 2000.0), PENALTY,
+                   cfl_dt(pts, tri, N, 3000.0), nsteps, np.array([[500.0, 950.0]]), np.array([10.0]),
+                   np.array([1.2 / 10.0]), np.array([[0.0, -1.0]]), np.stack([rx, np.full(nrec, 990.0)], 1),
+                   stride, dict(nx=nx, ny=ny, seed=seed))
+
+
+def config2(nx=400, ny=200, N=6, nsteps=5000, nrec=128, seed=2, stride=1):
+    """Larger single run: 4000 m x 2000 m, 400x200 split squares (160k triangles),
+    p=6; three horizontal layers (equal thirds, top to bottom vs = 800, 1500,
+    2500 m/s, vp = sqrt(3) vs, rho = 1800, 2100, 2400) with every element's
+    lambda and mu scaled by one lognormal factor exp(0.1 Z); one vertical Ricker
+    f0 = 8 Hz at 10% depth (2000, 1800); 128 receivers half an element below the
+    surface; 5000 steps."""
+    Lx, Ly = 4000.0, 2000.0
+    pts, tri = capi.split_square(nx, ny, 0.0, 0.0, Lx, Ly, 0.2, seed)
+    ne = tri.shape[0]
+    yc = pts[tri][:, :, 1].mean(1)
+    layer = np.where(yc > 2 * Ly / 3, 0, np.where(yc > Ly / 3, 1, 2))
+    vs = np.array([800.0, 1500.0, 2500.0])[layer]
+    rho = np.array([1800.0, 2100.0, 2400.0])[layer]
+    lam, mu = lame(rho, np.sqrt(3.0) * vs, vs)
+    fac = np.exp(0.1 * np.random.default_rng(seed).standard_normal(ne))
+    lam, mu = lam * fac, mu * fac
+    vp_max = float(np.sqrt((lam + 2 * mu) / rho).max())
+    rx = (np.arange(nrec) + 0.5) * Lx / nrec
+    return Problem("config2", N, pts, tri, lam, mu, rho, PENALTY, cfl_dt(pts, tri, N, vp_max), nsteps,
+                   np.array([[Lx / 2, 0.9 * Ly]]), np.array([8.0]), np.array([1.2 / 8.0]), np.array([[0.0, -1.0]]),
+                   np.stack([rx, np.full(nrec, Ly - 0.5 * Ly / ny)], 1), stride, dict(nx=nx, ny=ny, seed=seed))
+
+
+def config3_sources(nsrc=64, seed=3, L=1000.0):
+    """Sources of the multi-source workload: positions uniform in the top 10% of
+    the domain, f0 ~ U(5, 15) Hz, direction uniform on the unit circle."""
+    rng = np.random.default_rng(seed)
+    xy = np.stack([rng.uniform(0.01 * L, 0.99 * L, nsrc), rng.uniform(0.9 * L, 0.99 * L, nsrc)], 1)
+    f0 = rng.uniform(5.0, 15.0, nsrc)
+    ang = rng.uniform(0.0, 2 * np.pi, nsrc)
+    return xy, f0, 1.2 / f0, np.stack([np.cos(ang), np.sin(ang)], 1)
+
+
+def config3_column(k, nx=40, ny=40, N=4, nsteps=3000, nrec=128, seed=3, stride=1):
+    """Column k of the 64-source workload as a single-source problem."""
+    base = config1(nx, ny, N, nsteps, nrec=nrec, stride=stride)
+    xy, f0, t0, dirs = config3_sources(64, seed)
+    base.name = f"config3[{k}]"
+    base.src_xy, base.f0, base.t0, base.dirs = xy[k:k + 1], f0[k:k + 1], t0[k:k + 1], dirs[k:k + 1]
+    base.__post_init__()
+    return base
+
+
+def run_engine(prob: Problem, precision="fp64", engine=None, u0=None, v0=None, stride=None):
+    """Assemble and run `prob` on the device; returns (engine, u_final, seismograms)."""
+    eng = engine or capi.Engine(precision)
+    eng.set_tables(prob.N)
+    eng.set_mesh(prob.verts, prob.neigh, prob.neighedge, prob.lam, prob.mu, prob.rho, prob.penalty)
+    stride = prob.stride if stride is None else stride
+    eng.set_sources(prob.src_elem, prob.src_phi, prob.wavelet())
+    eng.set_receivers(prob.rec_elem, prob.rec_phi, stride, 1 + prob.nsteps // max(stride, 1))
+    eng.set_dt(prob.dt)
+    eng.set_state(u0, v0)
+    eng.first_step()
+    eng.run(prob.nsteps)
+    return eng, eng.get_u(), eng.seismograms()
