@@ -78,7 +78,26 @@ def cpu_baseline(sample_steps):
                        f"{dt:.1f} s")
 
 
+def copy_calibration(gib=2, reps=5):
+    """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE
+    counters (run in its own process: torch carries its own HIP runtime, so it
+    never shares a process with the engine library)."""
+    import torch
+    n = gib << 30
+    x = torch.empty(n, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "reps": reps}), flush=True)
+
+
 def main():
+    if "--copy-calibration" in sys.argv:
+        copy_calibration()
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)

@@ -292,3 +292,24 @@ def test_config2_full_size_blocks_and_linearity():
     np.testing.assert_array_equal(u2, 2.0 * u1)
     np.testing.assert_array_equal(s2, 2.0 * s1)
     assert eng.status() == 0
+
+
+def test_cpp_facade_driver_matches_oracle(tmp_path):
+    """examples/config1_driver (the C++ LeapFrog facade, reference-style loop with
+    one step() and one eval_receivers() per iteration) against the oracle."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "examples", "config1_driver")
+    assert os.path.exists(exe), "build the example with python -m tspeed_amd.build"
+    steps = 150
+    out = str(tmp_path / "c1")
+    r = subprocess.run([exe, str(steps), out, "0"], capture_output=True, text=True, timeout=240)
+    say(r.stdout.strip(), r.stderr.strip()[-500:])
+    assert r.returncode == 0
+    from tspeed_amd import io
+    s = io.read_receivers(out, 32)
+    p = W.config1(nsteps=steps)
+    _, sref = oracle_run(p)
+    assert s.shape == sref.shape
+    e_s, e_r = seis_err(s, sref)
+    say(f"  facade driver: seis rel L2 {e_s:.3e} worst receiver {e_r:.3e}")
+    assert e_s <= 1e-12 and e_r <= 1e-12

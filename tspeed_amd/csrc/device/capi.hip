@@ -585,7 +585,9 @@ int tsp_first_step(tsp_context *c)
         if (c->first_done) fail(c, TSP_ERR_STATE, "first step already taken (reset with tsp_set_state)");
         StepArgs a = base_args(c);
         a.wav_row = wav_row(c, 0);
-        a.sample = take_sample_slot(c);
+        // with a sampling stride the first sample is u1 (the driver's eval_receivers
+        // after first_step); stride 0 leaves every sample to tsp_eval_receivers
+        a.sample = c->stride > 0 ? take_sample_slot(c) : -1;
         const double dt = c->dt;
         if (c->precision == TSP_FP64) {
             a.coef = dt * dt / 2;
