@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtspeed_amd.so")
+LIB_PATH = os.environ.get("TSPEED_AMD_LIB", os.path.join(HERE, "libtspeed_amd.so"))
 
 TSP_OK, TSP_ERR_ARG, TSP_ERR_STATE, TSP_ERR_HIP, TSP_ERR_UNSUPPORTED, TSP_ERR_NOMEM = range(6)
 TSP_FP64, TSP_FP32C = 0, 1

@@ -219,10 +219,26 @@ __global__ void k_from_canonical(int ne, const double *canon, double *blk)
     blk[q] = canon[b * D * D + (size_t)row * D + 2 * jp + tt];
 }
 
-__global__ void k_demote(size_t n, const double *src, float *dst)
+// FP32 copy of the operator in the column-quad layout (engine.hpp).
+template <int P>
+__global__ void k_demote_quads(int ne, const double *src, float *dst)
 {
+    constexpr int D = Geom<P>::D, NFULL = D / 4;
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) dst[q] = (float)src[q];
+    if (q >= (size_t)ne * 4 * D * D) return;
+    const size_t b = q / (D * D);
+    const int r = q % (D * D);
+    int row, col;
+    if (r < NFULL * D * 4) {
+        const int jq = r / (4 * D), rem = r % (4 * D);
+        row = rem / 4;
+        col = 4 * jq + rem % 4;
+    } else {
+        const int rr = r - NFULL * D * 4;
+        row = rr / 2;
+        col = 4 * NFULL + rr % 2;
+    }
+    dst[q] = (float)src[b * D * D + (size_t)(col / 2) * 2 * D + row * 2 + (col & 1)];
 }
 
 template <int P>
@@ -277,10 +293,20 @@ int launch_canonical_to_blocks(int P, int ne, const double *canon, double *blk, 
 #undef CALL
 }
 
-int launch_demote_blocks(size_t n, const double *src, float *dst, hipStream_t s)
+template <int P>
+int demote_p(int ne, const double *src, float *dst, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_demote, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, src, dst);
+    constexpr int D = Geom<P>::D;
+    const size_t n = (size_t)ne * 4 * D * D;
+    hipLaunchKernelGGL(k_demote_quads<P>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ne, src, dst);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_demote_blocks(int P, int ne, const double *src, float *dst, hipStream_t s)
+{
+#define CALL(p) demote_p<p>(ne, src, dst, s)
+    TSP_DISPATCH_P(P, CALL)
+#undef CALL
 }
 
 }  // namespace tspeed_dev

@@ -210,7 +210,7 @@ void operator_ready(tsp_context *c)
     if (c->precision == TSP_FP32C) {
         const size_t n = (size_t)c->ne * 4 * c->D * c->D;
         if (!c->blk32) c->blk32 = dalloc<float>(c, n, "blocks fp32");
-        kcheck(c, launch_demote_blocks(n, c->blk64, c->blk32, c->stream), "demote");
+        kcheck(c, launch_demote_blocks(c->P, c->ne, c->blk64, c->blk32, c->stream), "demote");
         if (!c->sc_first) c->sc_first = dalloc<float>(c, c->ne, "sc");
         if (!c->sc_step) c->sc_step = dalloc<float>(c, c->ne, "sc");
     }

@@ -3,11 +3,14 @@
 // assemble.hip and step.hip.
 //
 // Device layouts (element-major, chosen for coalesced streaming):
-//   operator  blk[((e*4 + s)*DH + jp)*2D + row*2 + t] = B_{e,s}(row, 2*jp + t)
+//   operator  FP64: blk[((e*4 + s)*DH + jp)*2D + row*2 + t] = B_{e,s}(row, 2*jp + t)
 //             s = 0 own element, s = 1+f neighbour across edge f; each d x d block
 //             is stored as column pairs so one lane (= one row) loads two columns
-//             with a single 16-byte (FP64) or 8-byte (FP32) load, and a wave's
-//             load of one column pair covers 2*d contiguous values.
+//             with a single 16-byte load, and a wave's load of one column pair
+//             covers 2*d contiguous values.
+//             FP32: the same blocks as column quads, [jq][row][4] for the first
+//             4*(d/4) columns, followed by [row][2] for the last two columns when
+//             d % 4 == 2: again one 16-byte load per lane per group.
 //   vectors   u[e*D + c*NLN + i]  (the ABI converts from the reference's
 //             component-major layout c*ne*nln + e*nln + i)
 //   meta      int4 {nb0, nb1, nb2, aux}  aux = row of `extra` for elements that
@@ -33,7 +36,7 @@ struct Geom {
     static constexpr int RW = D <= 16 ? 16 : (D <= 32 ? 32 : 64);
     static constexpr int NG = 64 / RW;                  // column groups per wave
     static constexpr int RPL = (D + 63) / 64;           // rows per lane when D > 64
-    static constexpr int NIT = (DH + NG - 1) / NG;      // column pairs per group
+    static constexpr int DP = (D + 3) / 4 * 4;          // LDS row stride (16-byte aligned groups)
 };
 
 inline int nln_of(int P) { return (P + 1) * (P + 2) / 2; }
@@ -72,7 +75,7 @@ int launch_assemble(int P, const AssembleArgs &a, hipStream_t s);
 // Canonical <-> device block layout conversion and FP32 demotion.
 int launch_blocks_to_canonical(int P, int ne, const double *blk, double *canon, hipStream_t s);
 int launch_canonical_to_blocks(int P, int ne, const double *canon, double *blk, hipStream_t s);
-int launch_demote_blocks(size_t n, const double *src, float *dst, hipStream_t s);
+int launch_demote_blocks(int P, int ne, const double *src, float *dst, hipStream_t s);
 
 struct StepArgs {
     int ne;

@@ -13,12 +13,16 @@
 // reference's four copies per step (update_variables, TimeAdvance.hpp:230)
 // become a pointer swap.
 //
-// SpMV lane mapping: lane = block row; the wave walks the block's column pairs,
-// each lane loading its row's two entries with one 16-byte (FP64) / 8-byte (FP32)
-// load, so one wave instruction reads 2*d contiguous values.  For d <= 32 the
-// wave is split into 2 or 4 column groups that walk interleaved column pairs
-// and are summed with cross-lane shuffles; for d > 64 each lane owns two rows.
-// The neighbour coefficients are staged in LDS and read as broadcasts.
+// SpMV lane mapping: lane = block row; the wave walks the block's column
+// groups (pairs of FP64 columns, quads of FP32 columns -- see engine.hpp), each
+// lane fetching its row's entries of one group with a single 16-byte
+// non-temporal load (the operator is streamed once per step; keeping it out of
+// the caches leaves them to the neighbour gathers).  For d <= 32 the wave is
+// split into 2 or 4 column groups reduced with cross-lane shuffles; for d > 64
+// each lane owns two rows.  The element's and its neighbours' coefficients are
+// gathered into LDS (all loads issued before the first wait) and read as
+// broadcasts; the row state the update needs is fetched before the barrier so
+// its latency hides behind the block stream.
 //
 // Elements are dealt to work-groups so that each XCD (work-groups are
 // dispatched round-robin over the 8 XCDs) processes one contiguous range of
@@ -33,6 +37,7 @@ namespace tspeed_dev {
 namespace {
 
 constexpr int kWaves = 4;  // waves (elements) per work-group
+constexpr int kThreads = 64 * kWaves;
 
 __device__ inline int xcd_remap(int bid, int nwg)
 {
@@ -86,26 +91,44 @@ __device__ inline void record(const StepArgs &a, int4 ex, const V *unew, const i
     }
 }
 
-// Gather u^n of the element and its neighbours into the wave's LDS slice.
+// Gather u^n of the element and its three neighbours into the wave's LDS slice;
+// every load is issued before the first store so the four fetches overlap.
 template <int P, typename T>
-__device__ inline void gather(T (*xs)[Geom<P>::D], const T *u, int e, int4 m, int lane, bool act)
+__device__ inline void gather(T (*xs)[Geom<P>::DP], const T *u, int e, int4 m, int lane, bool act)
 {
     using G = Geom<P>;
     const int nbs[4] = {act ? e : -1, m.x, m.y, m.z};
+    T v[4][G::RPL];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        if (nbs[s] < 0) continue;
-        for (int idx = lane; idx < G::D; idx += 64) xs[s][idx] = u[(size_t)nbs[s] * G::D + idx];
-    }
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            const int idx = lane + 64 * q;
+            v[s][q] = (nbs[s] >= 0 && idx < G::D) ? u[(size_t)nbs[s] * G::D + idx] : T(0);
+        }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            const int idx = lane + 64 * q;
+            if (idx < G::D) xs[s][idx] = v[s][q];
+        }
 }
 
-// r = sum_s B_{e,s} x_s for the lane's rows.  Returns partial sums per row slot;
-// for NG > 1 the column groups are reduced into group 0.
-template <int P, typename T, typename T2>
-__device__ inline void spmv(const T *blk, int e, int4 m, const T (*xs)[Geom<P>::D], int lane,
+template <typename T, int W>
+using vec_t = T __attribute__((ext_vector_type(W)));
+
+// r = sum_s B_{e,s} x_s for the lane's rows, blocks stored as column groups of
+// width W (+ a trailing pair when d % W == 2).  For NG > 1 the column groups of
+// the wave are reduced into group 0.
+template <int P, typename T, int W>
+__device__ inline void spmv(const T *blk, int e, int4 m, const T (*xs)[Geom<P>::DP], int lane,
                             T (&acc)[Geom<P>::RPL])
 {
     using G = Geom<P>;
+    constexpr int NFULL = G::D / W, TAIL = G::D % W;
+    constexpr int NIT = (NFULL + G::NG - 1) / G::NG;
+    constexpr int TAILG = NFULL % G::NG;  // the column group with one iteration less takes the tail
     const int g = lane / G::RW, r = lane % G::RW;
     const int nbs[4] = {e, m.x, m.y, m.z};
 #pragma unroll
@@ -113,19 +136,35 @@ __device__ inline void spmv(const T *blk, int e, int4 m, const T (*xs)[Geom<P>::
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         if (nbs[s] < 0) continue;
-        const T2 *Bs = reinterpret_cast<const T2 *>(blk + ((size_t)e * 4 + s) * G::D * G::D);
+        const T *Bs = blk + ((size_t)e * 4 + s) * G::D * G::D;
 #pragma unroll
-        for (int it = 0; it < G::NIT; ++it) {
-            const int jp = it * G::NG + g;
-            if (G::NG > 1 && jp >= G::DH) continue;
-            const T x0 = xs[s][2 * jp], x1 = xs[s][2 * jp + 1];
+        for (int it = 0; it < NIT; ++it) {
+            const int jg = it * G::NG + g;
+            if (G::NG > 1 && jg >= NFULL) continue;
+            const vec_t<T, W> x = *reinterpret_cast<const vec_t<T, W> *>(&xs[s][W * jg]);
 #pragma unroll
             for (int q = 0; q < G::RPL; ++q) {
                 const int row = r + 64 * q;
                 if (row < G::D) {
-                    const T2 b = Bs[jp * G::D + row];
-                    acc[q] = fma(b.x, x0, acc[q]);
-                    acc[q] = fma(b.y, x1, acc[q]);
+                    const vec_t<T, W> b = __builtin_nontemporal_load(
+                        reinterpret_cast<const vec_t<T, W> *>(Bs + ((size_t)jg * G::D + row) * W));
+#pragma unroll
+                    for (int t = 0; t < W; ++t) acc[q] = fma(b[t], x[t], acc[q]);
+                }
+            }
+        }
+        if constexpr (TAIL == 2) {
+            if (g == TAILG) {
+                const vec_t<T, 2> x = *reinterpret_cast<const vec_t<T, 2> *>(&xs[s][W * NFULL]);
+#pragma unroll
+                for (int q = 0; q < G::RPL; ++q) {
+                    const int row = r + 64 * q;
+                    if (row < G::D) {
+                        const vec_t<T, 2> b = __builtin_nontemporal_load(
+                            reinterpret_cast<const vec_t<T, 2> *>(Bs + (size_t)NFULL * G::D * W + row * 2));
+                        acc[q] = fma(b[0], x[0], acc[q]);
+                        acc[q] = fma(b[1], x[1], acc[q]);
+                    }
                 }
             }
         }
@@ -140,45 +179,51 @@ __device__ inline void spmv(const T *blk, int e, int4 m, const T (*xs)[Geom<P>::
 
 // ---------------------------------------------------------------- FP64 plain
 template <int P, bool FIRST>
-__global__ __launch_bounds__(256) void k_step64(StepArgs a)
+__global__ __launch_bounds__(kThreads) void k_step64(StepArgs a)
 {
     using G = Geom<P>;
-    __shared__ double xs_all[kWaves][4][G::D];
+    __shared__ __attribute__((aligned(16))) double xs_all[kWaves][4][G::DP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int e = xcd_remap(blockIdx.x, gridDim.x) * kWaves + wave;
     const bool act = e < a.ne;
     const int4 m = act ? a.meta[e] : make_int4(-1, -1, -1, -1);
-    double(*xs)[G::D] = xs_all[wave];
+    const int g = lane / G::RW, r = lane % G::RW;
+    // row state of the update, fetched before the barrier
+    double up[G::RPL];
+    int rows[G::RPL];
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const int row = r + 64 * q;
+        rows[q] = (act && g == 0 && row < G::D) ? row : -1;
+        up[q] = rows[q] >= 0 ? a.uprev[(size_t)e * G::D + row] : 0.0;
+    }
+    const double invm = act ? a.invm[e] : 0.0;
+    double(*xs)[G::DP] = xs_all[wave];
     gather<P, double>(xs, a.ucur, e, m, lane, act);
     __syncthreads();
     if (!act) return;
 
     double acc[G::RPL];
-    spmv<P, double, double2>(a.blk64, e, m, xs, lane, acc);
+    spmv<P, double, 2>(a.blk64, e, m, xs, lane, acc);
 
-    const int g = lane / G::RW, r = lane % G::RW;
     const int4 ex = m.w >= 0 ? a.extra[m.w] : make_int4(0, 0, 0, 0);
-    const double invm = a.invm[e];
     double unew[G::RPL];
-    int rows[G::RPL];
     bool bad = false;
 #pragma unroll
     for (int q = 0; q < G::RPL; ++q) {
-        const int row = r + 64 * q;
-        rows[q] = (g == 0 && row < G::D) ? row : -1;
         unew[q] = 0.0;
-        if (rows[q] < 0) continue;
-        const size_t idx = (size_t)e * G::D + row;
+        const int row = rows[q];
+        if (row < 0) continue;
         const double f = (ex.y > ex.x) ? source_term(a, ex, row, G::NLN) : 0.0;
         const double w = a.coef * (acc[q] - f);
         double un;
         if (FIRST) {
             // u1 = u0 + dt*v0 - invM*(dt*dt/2*(B*u0 - f(0)))  (TimeAdvance.cpp:30)
-            un = xs[0][row] + a.dt * a.uprev[idx] - invm * w;
+            un = xs[0][row] + a.dt * up[q] - invm * w;
         } else {
-            un = 2 * xs[0][row] - a.uprev[idx] + invm * w;
+            un = 2 * xs[0][row] - up[q] + invm * w;
         }
-        a.uprev[idx] = un;
+        a.uprev[(size_t)e * G::D + row] = un;
         unew[q] = un;
         bad |= !isfinite(un);
     }
@@ -191,44 +236,50 @@ __global__ __launch_bounds__(256) void k_step64(StepArgs a)
 // with sc_e = invm_e*(-dt^2), and u^{n+1} = u^n + w^{n+1/2} accumulated by Kahan
 // summation (hi part uhi, compensation ucomp, true value = uhi - ucomp).  The
 // block products and the increment are FP32; the compensation keeps the
-// displacement's roundoff from drifting over thousands of steps.
-template <int P, bool FIRST>
-__global__ __launch_bounds__(256) void k_step32c(StepArgs a)
+// displacement's roundoff from drifting over thousands of steps.  The first step
+// uses the same kernel with winc = dt*v0 and sc_e = invm_e*(-dt^2/2).
+template <int P>
+__global__ __launch_bounds__(kThreads) void k_step32c(StepArgs a)
 {
     using G = Geom<P>;
-    __shared__ float xs_all[kWaves][4][G::D];
+    __shared__ __attribute__((aligned(16))) float xs_all[kWaves][4][G::DP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int e = xcd_remap(blockIdx.x, gridDim.x) * kWaves + wave;
     const bool act = e < a.ne;
     const int4 m = act ? a.meta[e] : make_int4(-1, -1, -1, -1);
-    float(*xs)[G::D] = xs_all[wave];
+    const int g = lane / G::RW, r = lane % G::RW;
+    float wi[G::RPL], cp[G::RPL];
+    int rows[G::RPL];
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const int row = r + 64 * q;
+        rows[q] = (act && g == 0 && row < G::D) ? row : -1;
+        const size_t idx = (size_t)e * G::D + row;
+        wi[q] = rows[q] >= 0 ? a.winc[idx] : 0.f;
+        cp[q] = rows[q] >= 0 ? a.ucomp[idx] : 0.f;
+    }
+    const float sc = act ? a.sc32[e] : 0.f;
+    float(*xs)[G::DP] = xs_all[wave];
     gather<P, float>(xs, a.uhi, e, m, lane, act);
     __syncthreads();
     if (!act) return;
 
     float acc[G::RPL];
-    spmv<P, float, float2>(a.blk32, e, m, xs, lane, acc);
+    spmv<P, float, 4>(a.blk32, e, m, xs, lane, acc);
 
-    const int g = lane / G::RW, r = lane % G::RW;
     const int4 ex = m.w >= 0 ? a.extra[m.w] : make_int4(0, 0, 0, 0);
-    const float sc = a.sc32[e];
     float unew[G::RPL];
-    int rows[G::RPL];
     bool bad = false;
 #pragma unroll
     for (int q = 0; q < G::RPL; ++q) {
-        const int row = r + 64 * q;
-        rows[q] = (g == 0 && row < G::D) ? row : -1;
         unew[q] = 0.f;
-        if (rows[q] < 0) continue;
+        const int row = rows[q];
+        if (row < 0) continue;
         const size_t idx = (size_t)e * G::D + row;
         const float f = (ex.y > ex.x) ? (float)source_term(a, ex, row, G::NLN) : 0.f;
-        const float acl = sc * (acc[q] - f);
-        // FIRST: winc holds dt*v0 (prepared on the host side of the ABI) and
-        // sc is invm*(-dt^2/2): w^{1/2} = dt*v0 - invm*dt^2/2*(B u0 - f(0)).
-        const float w = a.winc[idx] + acl;
+        const float w = wi[q] + sc * (acc[q] - f);
         const float uh = xs[0][row];
-        const float y = w - a.ucomp[idx];
+        const float y = w - cp[q];
         const float t = uh + y;
         a.ucomp[idx] = (t - uh) - y;
         a.uhi_out[idx] = t;
@@ -289,21 +340,21 @@ __global__ void k_scale_demote(size_t n, const double *src, double coef, float *
 
 // y = invM * B * x for the power iteration (FP64).
 template <int P>
-__global__ __launch_bounds__(256) void k_apply_minvB(int ne, const int4 *meta, const double *blk,
-                                                     const double *invm, const double *x, double *y)
+__global__ __launch_bounds__(kThreads) void k_apply_minvB(int ne, const int4 *meta, const double *blk,
+                                                          const double *invm, const double *x, double *y)
 {
     using G = Geom<P>;
-    __shared__ double xs_all[kWaves][4][G::D];
+    __shared__ __attribute__((aligned(16))) double xs_all[kWaves][4][G::DP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int e = xcd_remap(blockIdx.x, gridDim.x) * kWaves + wave;
     const bool act = e < ne;
     const int4 m = act ? meta[e] : make_int4(-1, -1, -1, -1);
-    double(*xs)[G::D] = xs_all[wave];
+    double(*xs)[G::DP] = xs_all[wave];
     gather<P, double>(xs, x, e, m, lane, act);
     __syncthreads();
     if (!act) return;
     double acc[G::RPL];
-    spmv<P, double, double2>(blk, e, m, xs, lane, acc);
+    spmv<P, double, 2>(blk, e, m, xs, lane, acc);
     const int g = lane / G::RW, r = lane % G::RW;
     const double iv = invm[e];
 #pragma unroll
@@ -358,15 +409,15 @@ inline unsigned nwg_for(int ne) { return (unsigned)((ne + kWaves - 1) / kWaves);
 template <int P>
 int step64_p(bool first, const StepArgs &a, hipStream_t s)
 {
-    if (first) hipLaunchKernelGGL((k_step64<P, true>), dim3(nwg_for(a.ne)), dim3(256), 0, s, a);
-    else       hipLaunchKernelGGL((k_step64<P, false>), dim3(nwg_for(a.ne)), dim3(256), 0, s, a);
+    if (first) hipLaunchKernelGGL((k_step64<P, true>), dim3(nwg_for(a.ne)), dim3(kThreads), 0, s, a);
+    else       hipLaunchKernelGGL((k_step64<P, false>), dim3(nwg_for(a.ne)), dim3(kThreads), 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 template <int P>
 int step32c_p(bool first, const StepArgs &a, hipStream_t s)
 {
-    if (first) hipLaunchKernelGGL((k_step32c<P, true>), dim3(nwg_for(a.ne)), dim3(256), 0, s, a);
-    else       hipLaunchKernelGGL((k_step32c<P, false>), dim3(nwg_for(a.ne)), dim3(256), 0, s, a);
+    (void)first;  // the first step differs only in the data the context prepares
+    hipLaunchKernelGGL(k_step32c<P>, dim3(nwg_for(a.ne)), dim3(kThreads), 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 template <int P>
@@ -385,7 +436,7 @@ template <int P>
 int minvb_p(int ne, const int4 *meta, const double *blk, const double *invm, const double *x, double *y,
             hipStream_t s)
 {
-    hipLaunchKernelGGL(k_apply_minvB<P>, dim3(nwg_for(ne)), dim3(256), 0, s, ne, meta, blk, invm, x, y);
+    hipLaunchKernelGGL(k_apply_minvB<P>, dim3(nwg_for(ne)), dim3(kThreads), 0, s, ne, meta, blk, invm, x, y);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 }  // namespace
