@@ -62,6 +62,39 @@ def run_variant(prob, precision, steps, warmup):
     return dict(ms_total=ms, wall_s=wall, setup_s=setup_s, status=status, device_bytes=dev_bytes)
 
 
+def run_batched(steps, warmup, ncols=64):
+    """Config 3: config 1's mesh at p=4 with 64 sources stepped as 64 columns of one
+    block SpMM on the MFMA cores (FP32, compensated update)."""
+    from tspeed_amd import capi, workloads as W
+    prob = W.config3(nsrc=ncols, nsteps=steps + warmup)
+    eng = capi.Engine("fp32c", ncols=ncols)
+    eng.set_tables(prob.N)
+    eng.set_mesh(prob.verts, prob.neigh, prob.neighedge, prob.lam, prob.mu, prob.rho, prob.penalty)
+    eng.set_sources(prob.src_elem, prob.src_phi, prob.wavelet(), col=np.arange(ncols))
+    eng.set_receivers(prob.rec_elem, prob.rec_phi, 1, steps + warmup + 1)
+    eng.set_dt(prob.dt)
+    for c in range(ncols):
+        eng.set_state(None, None, col=c)
+    eng.first_step()
+    if warmup:
+        eng.run(warmup)
+    eng.synchronize()
+    ms = eng.run_timed(steps)
+    status = eng.status()
+    eng.close()
+    ms_step = ms / steps
+    d = 2 * prob.nln
+    kp = 32 * ((d + 31) // 32)
+    nblk = prob.ne + int((prob.neigh >= 0).sum())
+    flops_alg = 2.0 * nblk * d * d * ncols          # algorithmic block products per step
+    flops_mfma = 2.0 * nblk * kp * kp * ncols       # as issued on the padded 32x32 tiles
+    return dict(value=prob.ndof * ncols * steps / (ms / 1e3), ms_per_step=ms_step, columns=ncols,
+                workload="config3: config-1 mesh, p=4, 64 Ricker sources as 64 columns, 128 receivers",
+                tflops_algorithmic=flops_alg / (ms_step / 1e3) / 1e12,
+                tflops_mfma_issued=flops_mfma / (ms_step / 1e3) / 1e12, mfma_peak_tflops=157.3,
+                status=status)
+
+
 def cpu_baseline(sample_steps):
     """The FP64 oracle (single thread, reference Release flags) on a 40x20 sub-grid
     of config 2 (same p, materials model and per-element work)."""
@@ -105,6 +138,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32c", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -131,6 +165,9 @@ def main():
         variants["fp32c"] = dict(value=prob.ndof * args.steps / (r32["ms_total"] / 1e3), ms_per_step=ms32,
                                  achieved_GBs=b32 / (ms32 / 1e3) / 1e9, algorithmic_bytes=b32,
                                  status=r32["status"])
+
+    if not args.no_batched:
+        variants["batched_fp32_64col"] = run_batched(args.steps, args.warmup)
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_step64_config2.json")

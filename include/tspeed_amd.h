@@ -46,8 +46,10 @@ int tsp_device_count(int *n);
 
 /* Context lifetime.  Replaces the construction of TimeAdvance/LeapFrog
  * (TimeAdvance_imp.hpp:26-40).  `device` is the HIP ordinal; `precision` one
- * of tsp_precision; `ncols` the number of right-hand-side columns (1 = the
- * reference's single simulation). */
+ * of tsp_precision; `ncols` the number of right-hand-side columns: 1 = the
+ * reference's single simulation; 32 or 64 = that many simulations on one mesh
+ * (one point source each), stepped together as a block SpMM on the MFMA cores
+ * (TSP_FP32C only). */
 int  tsp_create(int device, int precision, int ncols, tsp_context **out);
 void tsp_destroy(tsp_context *ctx);
 const char *tsp_last_error(const tsp_context *ctx);
@@ -109,11 +111,12 @@ int tsp_set_sources(tsp_context *ctx, int nsrc, const int *col, const int *elem,
 int tsp_set_wavelet(tsp_context *ctx, int row0, int nrows, const double *wavelet);
 
 /* Receivers (Receivers.hpp:77-121; PointWiseEntity::M_add, Receivers_imp.hpp:35-51):
- * col[r], elem[r], phi[r*nln+i].  Samples are recorded on the device after
- * first_step and after every step whose 1-based index is a multiple of
- * `stride` (0 = only on tsp_eval_receivers), up to max_samples samples. */
-int tsp_set_receivers(tsp_context *ctx, int nrec, const int *col, const int *elem, const double *phi,
-                      int stride, int max_samples);
+ * elem[r], phi[r*nln+i]; every receiver records every column.  Samples are
+ * recorded on the device after first_step and after every step whose 1-based
+ * index is a multiple of `stride` (0 = only on tsp_eval_receivers), up to
+ * max_samples samples. */
+int tsp_set_receivers(tsp_context *ctx, int nrec, const int *elem, const double *phi, int stride,
+                      int max_samples);
 
 /* Change the receiver sampling stride without touching recorded samples
  * (the driver loop's `if(step%2 == 0) eval_receivers()`, Examples/src/Lamb.cpp:56-57,
@@ -148,8 +151,9 @@ int tsp_steps_done(const tsp_context *ctx);
  * component-major 2*ne*nln doubles. */
 int tsp_get_u(tsp_context *ctx, int col, double *out);
 
-/* Seismograms recorded so far: out[(sample*nrec + r)*2 + c], up to max_samples
- * samples (Receivers::M_val, Receivers.hpp:115).  *nsamples gets the count. */
+/* Seismograms recorded so far: out[((sample*nrec + r)*ncols + col)*2 + c], up to
+ * max_samples samples (Receivers::M_val, Receivers.hpp:115).  *nsamples gets
+ * the count. */
 int tsp_get_seismograms(tsp_context *ctx, double *out, int max_samples, int *nsamples);
 
 /* Step-size estimate and stability check (no counterpart in the reference,

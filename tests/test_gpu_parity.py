@@ -313,3 +313,64 @@ def test_cpp_facade_driver_matches_oracle(tmp_path):
     e_s, e_r = seis_err(s, sref)
     say(f"  facade driver: seis rel L2 {e_s:.3e} worst receiver {e_r:.3e}")
     assert e_s <= 1e-12 and e_r <= 1e-12
+
+
+# ------------------------------------------------------------------ many columns (MFMA)
+def test_batched_columns_match_oracle_and_single_column_path():
+    """64 sources as 64 columns of one block SpMM (FP32 on MFMA, compensated
+    update): every column's seismograms within 1e-4 of the FP64 oracle run of
+    that source alone, and close to the single-column FP32 path."""
+    p = W.config3(nsteps=400)
+    p.t0 = np.full(64, 20 * p.dt)               # early wavelets: signal within 400 steps
+    p.f0 = 1.0 / (10 * p.dt) * np.linspace(0.6, 1.0, 64)
+    say(f"batched config3 ne={p.ne} cols=64 steps={p.nsteps}")
+    eng, s = W.run_batched(p, 64)
+    assert s.shape == (p.nsamples, 128, 64, 2)
+    assert eng.status() == 0
+    worst = 0.0
+    for k in (0, 17, 40, 63):
+        q = W.config3_column(k, nsteps=p.nsteps)
+        q.t0, q.f0 = p.t0[k:k + 1], p.f0[k:k + 1]
+        _, sref = oracle_run(q)
+        e_s, e_r = seis_err(s[:, :, k, :], sref)
+        _, _, s1 = W.run_engine(q, "fp32c")
+        e_1 = rel_l2(s[:, :, k, :], s1)
+        say(f"  column {k}: vs oracle {e_s:.3e} (worst receiver {e_r:.3e}); vs single-column fp32c {e_1:.3e}")
+        worst = max(worst, e_s, e_r)
+        assert e_1 <= 1e-4
+    assert worst <= 1e-4
+
+
+def test_batched_state_roundtrip_and_eval_receivers():
+    """set_state / get_u per column, eval_receivers, and linearity across columns:
+    column 1 carries twice column 0's source and initial state."""
+    p = W.config3(nsrc=2, nx=6, ny=5, nsteps=40)
+    p.src_xy[1] = p.src_xy[0]
+    p.f0[1], p.t0[1] = p.f0[0], p.t0[0]
+    p.dirs[1] = 2 * p.dirs[0]
+    p.__post_init__()
+    rng = np.random.default_rng(2)
+    u0 = rng.standard_normal(p.ndof) * 1e-6
+    eng = capi.Engine("fp32c", ncols=32)
+    eng.set_tables(p.N)
+    eng.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+    eng.set_sources(p.src_elem, p.src_phi, p.wavelet(), col=np.array([0, 1]))
+    eng.set_receivers(p.rec_elem, p.rec_phi, 0, 2)
+    eng.set_dt(p.dt)
+    for c in range(32):
+        eng.set_state(u0 if c == 0 else (2 * u0 if c == 1 else None), None, col=c)
+    eng.first_step()
+    eng.eval_receivers()
+    eng.run(p.nsteps)
+    eng.eval_receivers()
+    s = eng.seismograms()
+    u_0, u_1, u_5 = eng.get_u(0), eng.get_u(1), eng.get_u(5)
+    assert not u_5.any()                                    # no source, zero state
+    np.testing.assert_allclose(u_1, 2 * u_0, rtol=0, atol=1e-6 * np.abs(u_0).max())
+    np.testing.assert_allclose(s[:, :, 1], 2 * s[:, :, 0], rtol=0, atol=1e-6 * np.abs(s[:, :, 0]).max())
+    # against the FP64 oracle for column 0
+    q = W.config3_column(0, nx=6, ny=5, nsteps=p.nsteps, nrec=p.rec_xy.shape[0])
+    q.src_xy, q.f0, q.t0, q.dirs = p.src_xy[:1], p.f0[:1], p.t0[:1], p.dirs[:1]
+    q.__post_init__()
+    uref, _ = oracle_run(q, u0=u0)
+    assert rel_l2(u_0, uref) <= 1e-4

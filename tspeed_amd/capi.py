@@ -40,7 +40,7 @@ SIGNATURES = {
     "tsp_set_state": (_i, [_vp, _i, _vp, _vp]),
     "tsp_set_sources": (_i, [_vp, _i, _ip, _ip, _dp, _i, _dp]),
     "tsp_set_wavelet": (_i, [_vp, _i, _i, _dp]),
-    "tsp_set_receivers": (_i, [_vp, _i, _ip, _ip, _dp, _i, _i]),
+    "tsp_set_receivers": (_i, [_vp, _i, _ip, _dp, _i, _i]),
     "tsp_set_stride": (_i, [_vp, _i]),
     "tsp_set_dt": (_i, [_vp, _d]),
     "tsp_first_step": (_i, [_vp]),
@@ -179,6 +179,7 @@ class Engine:
     def __init__(self, precision="fp64", device=0, ncols=1):
         self._L = lib()
         self.precision = precision
+        self.ncols = ncols
         h = _vp()
         rc = self._L.tsp_create(device, PRECISIONS[precision], ncols, ctypes.byref(h))
         if rc != TSP_OK:
@@ -245,11 +246,10 @@ class Engine:
         self._ck(self._L.tsp_set_sources(self._h, ns, col, elem, _c64(phi).reshape(-1), ntab,
                                          wav if ns else np.zeros(1)))
 
-    def set_receivers(self, elem, phi, stride, max_samples, col=None):
+    def set_receivers(self, elem, phi, stride, max_samples):
         elem = _c32i(np.atleast_1d(elem))
         nr = elem.size
-        col = _c32i(np.zeros(nr) if col is None else col)
-        self._ck(self._L.tsp_set_receivers(self._h, nr, col, elem, _c64(phi).reshape(-1) if nr else np.zeros(1),
+        self._ck(self._L.tsp_set_receivers(self._h, nr, elem, _c64(phi).reshape(-1) if nr else np.zeros(1),
                                            int(stride), int(max_samples)))
         self.nrec = nr
 
@@ -285,9 +285,10 @@ class Engine:
     def seismograms(self):
         ns = _i()
         self._ck(self._L.tsp_get_seismograms(self._h, None, 0, ctypes.byref(ns)))
-        out = np.zeros(max(1, ns.value * self.nrec * 2))
+        out = np.zeros(max(1, ns.value * self.nrec * self.ncols * 2))
         self._ck(self._L.tsp_get_seismograms(self._h, out.ctypes.data, ns.value, ctypes.byref(ns)))
-        return out[: ns.value * self.nrec * 2].reshape(ns.value, self.nrec, 2)
+        s = out[: ns.value * self.nrec * self.ncols * 2].reshape(ns.value, self.nrec, self.ncols, 2)
+        return s[:, :, 0, :] if self.ncols == 1 else s
 
     def cfl(self, factor=1.0, iters=100):
         dt, lm = _d(), _d()

@@ -53,7 +53,13 @@ struct tsp_context {
     double *rec_phi = nullptr, *seis = nullptr;
     int *rec_elem = nullptr;
     int4 *extra = nullptr;
-    int *src_sorted = nullptr, *rec_sorted = nullptr;
+    int *src_sorted = nullptr, *rec_sorted = nullptr, *src_col = nullptr;
+
+    // many columns (ncols > 1): packed MFMA operator, state rows padded to KP
+    float *blkA = nullptr;
+    int KP = 0;
+    bool batched() const { return ncols > 1; }
+    size_t state_len() const { return batched() ? (size_t)ne * KP * ncols : (size_t)ne * D; }
 
     unsigned *flags = nullptr;
     double dt = 0;
@@ -172,6 +178,9 @@ void rebuild_extra(tsp_context *c)
     dfree(c, c->extra);
     dfree(c, c->src_sorted);
     dfree(c, c->rec_sorted);
+    dfree(c, c->src_col);
+    c->src_col = dalloc<int>(c, c->nsrc, "src columns");
+    if (c->nsrc) h2d(c, c->src_col, c->h_src_col.data(), c->nsrc, "src columns");
     c->extra = dalloc<int4>(c, ex.size(), "extra");
     c->src_sorted = dalloc<int>(c, ss.size(), "src list");
     c->rec_sorted = dalloc<int>(c, rs.size(), "rec list");
@@ -189,8 +198,8 @@ void require_mesh(tsp_context *c)
 
 void alloc_state(tsp_context *c)
 {
-    const size_t n = (size_t)c->ne * c->D;
-    if (!c->scratch64) c->scratch64 = dalloc<double>(c, n, "scratch");
+    const size_t n = c->state_len();
+    if (!c->scratch64) c->scratch64 = dalloc<double>(c, (size_t)c->ne * c->D, "scratch");
     if (c->precision == TSP_FP64) {
         for (int k = 0; k < 2; ++k)
             if (!c->u64[k]) c->u64[k] = dalloc<double>(c, n, "state");
@@ -207,16 +216,22 @@ void alloc_state(tsp_context *c)
 void operator_ready(tsp_context *c)
 {
     // FP32 copy of the operator for the compensated path
-    if (c->precision == TSP_FP32C) {
+    if (c->batched()) {
+        c->KP = batch_kp(c->P);
+        if (!c->blkA) c->blkA = dalloc<float>(c, (size_t)c->ne * batch_block_floats(c->P), "blocks mfma");
+        kcheck(c, launch_pack_mfma(c->P, c->ne, c->blk64, c->blkA, c->stream), "pack");
+    } else if (c->precision == TSP_FP32C) {
         const size_t n = (size_t)c->ne * 4 * c->D * c->D;
         if (!c->blk32) c->blk32 = dalloc<float>(c, n, "blocks fp32");
         kcheck(c, launch_demote_blocks(c->P, c->ne, c->blk64, c->blk32, c->stream), "demote");
+    }
+    if (c->precision == TSP_FP32C) {
         if (!c->sc_first) c->sc_first = dalloc<float>(c, c->ne, "sc");
         if (!c->sc_step) c->sc_step = dalloc<float>(c, c->ne, "sc");
     }
     alloc_state(c);
-    hipcheck(c, hipMemsetAsync(c->u64[0], 0, sizeof(double) * c->ne * c->D, c->stream), "memset");
-    hipcheck(c, hipMemsetAsync(c->u64[1], 0, sizeof(double) * c->ne * c->D, c->stream), "memset");
+    hipcheck(c, hipMemsetAsync(c->u64[0], 0, sizeof(double) * c->state_len(), c->stream), "memset");
+    hipcheck(c, hipMemsetAsync(c->u64[1], 0, sizeof(double) * c->state_len(), c->stream), "memset");
     c->have_state = true;
     c->steps = 0;
     c->first_done = false;
@@ -243,6 +258,29 @@ StepArgs base_args(tsp_context *c)
     a.blk64 = c->blk64;
     a.invm = c->invm;
     a.blk32 = c->blk32;
+    return a;
+}
+
+BatchArgs batch_args(tsp_context *c)
+{
+    BatchArgs a{};
+    a.ne = c->ne;
+    a.nsrc = c->nsrc;
+    a.nrec = c->nrec;
+    a.meta = c->meta;
+    a.extra = c->extra;
+    a.src_sorted = c->src_sorted;
+    a.rec_sorted = c->rec_sorted;
+    a.src_col = c->src_col;
+    a.rec_elem = c->rec_elem;
+    a.src_phi = c->src_phi;
+    a.rec_phi = c->rec_phi;
+    a.seis = c->seis;
+    a.sample = -1;
+    a.flags = c->flags;
+    a.blk = c->blkA;
+    a.ucomp = c->ucomp;
+    a.winc = c->winc;
     return a;
 }
 
@@ -284,7 +322,8 @@ int tsp_create(int device, int precision, int ncols, tsp_context **out)
     if (!out) return TSP_ERR_ARG;
     *out = nullptr;
     if (precision != TSP_FP64 && precision != TSP_FP32C) return TSP_ERR_UNSUPPORTED;
-    if (ncols != 1) return TSP_ERR_UNSUPPORTED;
+    if (ncols != 1 && ncols != 32 && ncols != 64) return TSP_ERR_UNSUPPORTED;
+    if (ncols > 1 && precision != TSP_FP32C) return TSP_ERR_UNSUPPORTED;
     tsp_context *c = new (std::nothrow) tsp_context;
     if (!c) return TSP_ERR_NOMEM;
     c->device = device;
@@ -372,7 +411,7 @@ static void check_mesh_args(tsp_context *c, int ne, const int *neigh, const int 
 
 static void reset_mesh(tsp_context *c, int ne)
 {
-    dfree(c, c->meta); dfree(c, c->blk64); dfree(c, c->invm); dfree(c, c->blk32);
+    dfree(c, c->meta); dfree(c, c->blk64); dfree(c, c->invm); dfree(c, c->blk32); dfree(c, c->blkA);
     dfree(c, c->sc_first); dfree(c, c->sc_step);
     for (int k = 0; k < 2; ++k) { dfree(c, c->u64[k]); dfree(c, c->uhi[k]); }
     dfree(c, c->ucomp); dfree(c, c->winc); dfree(c, c->scratch64);
@@ -470,17 +509,19 @@ int tsp_set_state(tsp_context *c, int col, const double *u0, const double *v0)
 {
     return guarded(c, [&] {
         require_mesh(c);
-        if (col != 0) fail(c, TSP_ERR_ARG, "column %d out of range (ncols = %d)", col, c->ncols);
+        if (col < 0 || col >= c->ncols) fail(c, TSP_ERR_ARG, "column %d out of range (ncols = %d)", col, c->ncols);
         const size_t n = (size_t)c->ne * c->D;
-        // u64[0] <- u0, u64[1] <- v0, element-major
+        // u64[0] <- u0, u64[1] <- v0 (element-major, or the padded multi-column layout)
         for (int k = 0; k < 2; ++k) {
             const double *src = k == 0 ? u0 : v0;
-            if (src) {
-                h2d(c, c->scratch64, src, n, "state");
+            if (src) h2d(c, c->scratch64, src, n, "state");
+            if (c->batched())
+                kcheck(c, launch_col_in(c->ne, c->nln, c->KP, c->ncols, col, src ? c->scratch64 : nullptr, c->u64[k],
+                                        c->stream), "layout");
+            else if (src)
                 kcheck(c, launch_to_elem_major(c->ne, c->nln, c->scratch64, c->u64[k], c->stream), "layout");
-            } else {
+            else
                 hipcheck(c, hipMemsetAsync(c->u64[k], 0, n * sizeof(double), c->stream), "memset");
-            }
         }
         c->cur = 0;
         c->steps = 0;
@@ -531,17 +572,14 @@ int tsp_set_wavelet(tsp_context *c, int row0, int nrows, const double *wavelet)
     });
 }
 
-int tsp_set_receivers(tsp_context *c, int nrec, const int *col, const int *elem, const double *phi, int stride,
-                      int max_samples)
+int tsp_set_receivers(tsp_context *c, int nrec, const int *elem, const double *phi, int stride, int max_samples)
 {
     return guarded(c, [&] {
         require_mesh(c);
-        if (nrec < 0 || (nrec > 0 && (!col || !elem || !phi))) fail(c, TSP_ERR_ARG, "bad receiver arrays");
+        if (nrec < 0 || (nrec > 0 && (!elem || !phi))) fail(c, TSP_ERR_ARG, "bad receiver arrays");
         if (stride < 0 || max_samples < 0) fail(c, TSP_ERR_ARG, "negative stride or sample count");
-        for (int k = 0; k < nrec; ++k) {
-            if (col[k] < 0 || col[k] >= c->ncols) fail(c, TSP_ERR_ARG, "receiver %d: column %d out of range", k, col[k]);
+        for (int k = 0; k < nrec; ++k)
             if (elem[k] < 0 || elem[k] >= c->ne) fail(c, TSP_ERR_ARG, "receiver %d: element %d out of range", k, elem[k]);
-        }
         dfree(c, c->rec_phi);
         dfree(c, c->seis);
         dfree(c, c->rec_elem);
@@ -552,7 +590,7 @@ int tsp_set_receivers(tsp_context *c, int nrec, const int *col, const int *elem,
         c->h_rec_elem.assign(elem, elem + nrec);
         c->rec_phi = dalloc<double>(c, (size_t)nrec * c->nln, "receiver phi");
         c->rec_elem = dalloc<int>(c, nrec, "receiver elem");
-        c->seis = dalloc<double>(c, (size_t)max_samples * nrec * 2, "seismograms");
+        c->seis = dalloc<double>(c, (size_t)max_samples * nrec * c->ncols * 2, "seismograms");
         if (nrec) {
             h2d(c, c->rec_phi, phi, (size_t)nrec * c->nln, "receiver phi");
             h2d(c, c->rec_elem, elem, nrec, "receiver elem");
@@ -596,10 +634,26 @@ int tsp_first_step(tsp_context *c)
             a.uprev = c->u64[1];  // v0 in, u1 out
             kcheck(c, launch_step64(c->P, true, a, c->stream), "first step");
             c->cur = 1;           // u^n = u1 in u64[1]; u^{n-1} = u0 in u64[0]
+        } else if (c->batched()) {
+            const size_t n = c->state_len();
+            kcheck(c, launch_split_state(n, c->u64[0], c->uhi[0], c->ucomp, c->stream), "split");
+            kcheck(c, launch_scale_demote(c->ne, c->invm, -(dt * dt / 2), c->sc_first, c->stream), "sc");
+            kcheck(c, launch_scale_demote(c->ne, c->invm, (-dt) * dt, c->sc_step, c->stream), "sc");
+            kcheck(c, launch_scale_demote(n, c->u64[1], dt, c->winc, c->stream), "winc");
+            // the step writes rows < d only; keep the padded rows of the other buffer zero
+            hipcheck(c, hipMemsetAsync(c->uhi[1], 0, n * sizeof(float), c->stream), "memset");
+            BatchArgs b = batch_args(c);
+            b.wav_row = a.wav_row;
+            b.sample = a.sample;
+            b.sc32 = c->sc_first;
+            b.uhi = c->uhi[0];
+            b.uhi_out = c->uhi[1];
+            kcheck(c, launch_step_b32(c->P, c->ncols, b, c->stream), "first step");
+            c->cur = 1;
         } else {
             // hi/comp from u0; the increment starts as (float)(dt*v0)
             const size_t n = (size_t)c->ne * c->D;
-            kcheck(c, launch_split_state(c->ne, c->D, c->u64[0], c->uhi[0], c->ucomp, c->stream), "split");
+            kcheck(c, launch_split_state(n, c->u64[0], c->uhi[0], c->ucomp, c->stream), "split");
             kcheck(c, launch_scale_demote(c->ne, c->invm, -(dt * dt / 2), c->sc_first, c->stream), "sc");
             kcheck(c, launch_scale_demote(c->ne, c->invm, (-dt) * dt, c->sc_step, c->stream), "sc");
             kcheck(c, launch_scale_demote(n, c->u64[1], dt, c->winc, c->stream), "winc");
@@ -643,6 +697,14 @@ int tsp_run(tsp_context *c, int nsteps)
                 a.ucur = c->u64[c->cur];
                 a.uprev = c->u64[1 - c->cur];
                 kcheck(c, launch_step64(c->P, false, a, c->stream), "step");
+            } else if (c->batched()) {
+                BatchArgs b = batch_args(c);
+                b.wav_row = a.wav_row;
+                b.sample = a.sample;
+                b.sc32 = c->sc_step;
+                b.uhi = c->uhi[c->cur];
+                b.uhi_out = c->uhi[1 - c->cur];
+                kcheck(c, launch_step_b32(c->P, c->ncols, b, c->stream), "step");
             } else {
                 a.uhi = c->uhi[c->cur];
                 a.uhi_out = c->uhi[1 - c->cur];
@@ -685,7 +747,12 @@ int tsp_eval_receivers(tsp_context *c)
         StepArgs a = base_args(c);
         a.sample = take_sample_slot(c);
         if (a.sample < 0) fail(c, TSP_ERR_ARG, "seismogram buffer full (%d samples)", c->max_samples);
-        if (c->precision == TSP_FP64)
+        if (c->batched()) {
+            BatchArgs b = batch_args(c);
+            b.sample = a.sample;
+            b.uhi = c->uhi[c->cur];
+            kcheck(c, launch_sample_b32(c->P, c->ncols, b, c->stream), "sample");
+        } else if (c->precision == TSP_FP64)
             kcheck(c, launch_sample64(c->P, a, c->u64[c->cur], c->stream), "sample");
         else
             kcheck(c, launch_sample32(c->P, a, c->uhi[c->cur], c->ucomp, c->stream), "sample");
@@ -698,9 +765,16 @@ int tsp_get_u(tsp_context *c, int col, double *out)
 {
     return guarded(c, [&] {
         require_mesh(c);
-        if (col != 0) fail(c, TSP_ERR_ARG, "column %d out of range", col);
+        if (col < 0 || col >= c->ncols) fail(c, TSP_ERR_ARG, "column %d out of range", col);
         if (!out) fail(c, TSP_ERR_ARG, "null output");
         const size_t n = (size_t)c->ne * c->D;
+        if (c->batched()) {
+            if (!c->first_done) fail(c, TSP_ERR_STATE, "no displacement yet: call tsp_first_step");
+            kcheck(c, launch_col_out(c->ne, c->nln, c->KP, c->ncols, col, c->uhi[c->cur], c->ucomp, c->scratch64,
+                                     c->stream), "layout");
+            d2h(c, out, c->scratch64, n, "get_u");
+            return;
+        }
         const double *src;
         if (!c->first_done) {
             src = c->u64[0];
@@ -708,7 +782,7 @@ int tsp_get_u(tsp_context *c, int col, double *out)
             src = c->u64[c->cur];
         } else {
             // u0/v0 staging is free once the first step has run: join hi - comp there
-            kcheck(c, launch_join_state(c->ne, c->D, c->uhi[c->cur], c->ucomp, c->u64[0], c->stream), "join");
+            kcheck(c, launch_join_state((size_t)c->ne * c->D, c->uhi[c->cur], c->ucomp, c->u64[0], c->stream), "join");
             src = c->u64[0];
         }
         kcheck(c, launch_to_comp_major(c->ne, c->nln, src, c->scratch64, c->stream), "layout");
@@ -721,7 +795,7 @@ int tsp_get_seismograms(tsp_context *c, double *out, int max_samples, int *nsamp
     return guarded(c, [&] {
         const int ns = std::min(c->nsamples, std::max(0, max_samples));
         if (nsamples) *nsamples = c->nsamples;
-        if (out && ns > 0 && c->nrec > 0) d2h(c, out, c->seis, (size_t)ns * c->nrec * 2, "seismograms");
+        if (out && ns > 0 && c->nrec > 0) d2h(c, out, c->seis, (size_t)ns * c->nrec * c->ncols * 2, "seismograms");
     });
 }
 

@@ -108,8 +108,8 @@ int launch_step64(int P, bool first, const StepArgs &a, hipStream_t s);
 int launch_step32c(int P, bool first, const StepArgs &a, hipStream_t s);
 
 // FP32-compensated state helpers
-int launch_split_state(int ne, int D, const double *u64, float *hi, float *comp, hipStream_t s);
-int launch_join_state(int ne, int D, const float *hi, const float *comp, double *u64, hipStream_t s);
+int launch_split_state(size_t n, const double *u64, float *hi, float *comp, hipStream_t s);
+int launch_join_state(size_t n, const float *hi, const float *comp, double *u64, hipStream_t s);
 int launch_scale_demote(size_t n, const double *src, double coef, float *dst, hipStream_t s);
 
 // Receiver sample of the current displacement (eval_receivers).
@@ -120,6 +120,31 @@ int launch_sample32(int P, const StepArgs &a, const float *hi, const float *comp
 int launch_apply_minvB(int P, int ne, const int4 *meta, const double *blk, const double *invm,
                        const double *x, double *y, hipStream_t s);
 int launch_dot(size_t n, const double *x, const double *y, double *partial, int nparts, hipStream_t s);
+
+// Many right-hand sides (spmm.hip): R = ncols columns, state u[(e*KP + k)*R + j].
+struct BatchArgs {
+    int ne, nsrc, nrec;
+    const int4 *meta;
+    const int4 *extra;
+    const int *src_sorted, *rec_sorted;
+    const int *src_col, *rec_elem;
+    const double *src_phi, *wav_row, *rec_phi;
+    double *seis;             // [sample][rec][col][2]
+    int sample;
+    unsigned *flags;
+    const float *blk;         // A-fragment operator
+    const float *sc32;
+    const float *uhi;
+    float *uhi_out, *ucomp, *winc;
+};
+int batch_kp(int P);                 // padded rows per element (multiple of 32)
+size_t batch_block_floats(int P);    // floats of one element's four packed blocks
+int launch_step_b32(int P, int ncols, const BatchArgs &a, hipStream_t s);
+int launch_pack_mfma(int P, int ne, const double *src, float *dst, hipStream_t s);
+int launch_sample_b32(int P, int ncols, const BatchArgs &a, hipStream_t s);
+int launch_col_in(int ne, int nln, int KP, int R, int col, const double *cm, double *st, hipStream_t s);
+int launch_col_out(int ne, int nln, int KP, int R, int col, const float *hi, const float *comp, double *cm,
+                   hipStream_t s);
 
 // Layout helpers (element-major <-> component-major), column-agnostic.
 int launch_to_elem_major(int ne, int nln, const double *cm, double *em, hipStream_t s);

@@ -482,15 +482,13 @@ int launch_apply_minvB(int P, int ne, const int4 *meta, const double *blk, const
 #undef CALL
 }
 
-int launch_split_state(int ne, int D, const double *u64, float *hi, float *comp, hipStream_t s)
+int launch_split_state(size_t n, const double *u64, float *hi, float *comp, hipStream_t s)
 {
-    const size_t n = (size_t)ne * D;
     hipLaunchKernelGGL(k_split, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, u64, hi, comp);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
-int launch_join_state(int ne, int D, const float *hi, const float *comp, double *u64, hipStream_t s)
+int launch_join_state(size_t n, const float *hi, const float *comp, double *u64, hipStream_t s)
 {
-    const size_t n = (size_t)ne * D;
     hipLaunchKernelGGL(k_join, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, hi, comp, u64);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -604,8 +604,7 @@ void LeapFrog::first_step()
     const int est = (tmax_ > 0 && dt_ > 0) ? (int)std::ceil(tmax_ / dt_) + 2 : 1024;
     max_samples_ = est + 1;
     if (nrec_ > 0) {
-        std::vector<int> col(nrec_, 0);
-        check(tsp_set_receivers(ctx_, nrec_, col.data(), rec_el_.data(), rec_phi_.data(), 0, max_samples_));
+        check(tsp_set_receivers(ctx_, nrec_, rec_el_.data(), rec_phi_.data(), 0, max_samples_));
     }
     ensure_wavelet(est + 1);
     check(tsp_first_step(ctx_));
