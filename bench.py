@@ -95,6 +95,34 @@ def run_batched(steps, warmup, ncols=64):
                 status=status)
 
 
+def run_multi(steps, warmup, n_inst=256):
+    """Config 4: n_inst independent meshes (500-20000 triangles, p in 2..8,
+    random homogeneous materials, one Ricker source each, 500-5000 steps) in one
+    persistent kernel per step, graph-captured, lane mapping autotuned per p."""
+    from tspeed_amd import capi, workloads as W
+    t0 = time.time()
+    m = capi.Multi(chunk=32)
+    ndof, ne_tot, degrees = 0, 0, {}
+    for k in range(n_inst):
+        prob = W.config4_instance(k)
+        eng = W.prepare_instance(prob, stride=10)
+        m.add(eng, prob.nsteps)
+        ndof += prob.ndof
+        ne_tot += prob.ne
+        degrees[prob.N] = degrees.get(prob.N, 0) + 1
+    setup = time.time() - t0
+    m.autotune(3)
+    if warmup:
+        m.run(warmup)
+    m.synchronize()
+    ms = m.run_timed(steps)
+    variants = {p: m.variant(p) for p in sorted(degrees)}
+    m.close()
+    return dict(value=ndof * steps / (ms / 1e3), ms_per_step=ms / steps, instances=n_inst, elements=ne_tot,
+                dof=ndof, degrees=degrees, autotuned_variant=variants, setup_s=setup,
+                workload="config4: uneven batch, one persistent kernel per step, graph chunks of 32")
+
+
 def cpu_baseline(sample_steps):
     """The FP64 oracle (single thread, reference Release flags) on a 40x20 sub-grid
     of config 2 (same p, materials model and per-element work)."""
@@ -139,6 +167,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32c", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--multi", type=int, default=256, help="instances in the uneven-batch variant (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -168,6 +197,8 @@ def main():
 
     if not args.no_batched:
         variants["batched_fp32_64col"] = run_batched(args.steps, args.warmup)
+    if args.multi > 0:
+        variants["multi_fp32c"] = run_multi(min(args.steps, 200), min(args.warmup, 10), args.multi)
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_step64_config2.json")

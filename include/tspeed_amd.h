@@ -171,6 +171,56 @@ int tsp_synchronize(tsp_context *ctx);
 /* Device bytes held by the context (operator + state + tables). */
 long long tsp_device_bytes(const tsp_context *ctx);
 
+/* A new context on the same mesh that borrows owner's assembled operator (no
+ * second assembly, no second copy) and has its own sources, receivers and
+ * state: several simulations on one Mesh/Parameters, as a driver would build
+ * several LeapFrog objects from one FESpace (TimeAdvance_imp.hpp:30).  The
+ * owner must outlive every context that shares its operator. */
+int tsp_share_operator(tsp_context *owner, tsp_context **out);
+
+/* Free the FP64 operator of a TSP_FP32C context once its FP32 copy exists
+ * (tsp_get_blocks and tsp_cfl then report TSP_ERR_STATE). */
+int tsp_release_fp64(tsp_context *ctx);
+
+/* ------------------------------------------------------------------------
+ * Sets of independent simulations advanced together (no counterpart in the
+ * reference, which runs one LeapFrog per process).  Instances are ordinary
+ * single-column TSP_FP32C contexts, built with the calls above (mesh or
+ * tsp_share_operator, sources, receivers with a stride, dt, state) and
+ * stopped after tsp_first_step; the set takes ownership of each.  A step of
+ * the set is one persistent-kernel launch over every element of every running
+ * instance, whatever its size and degree; chunks of steps replay a captured HIP
+ * graph.  Results are read through tsp_multi_context + tsp_get_u /
+ * tsp_get_seismograms.
+ * ------------------------------------------------------------------------ */
+typedef struct tsp_multi tsp_multi;
+
+int  tsp_multi_create(int device, tsp_multi **out);
+void tsp_multi_destroy(tsp_multi *m);
+const char *tsp_multi_last_error(const tsp_multi *m);
+/* Add a context that will run `nsteps` step() calls; *id gets its index. */
+int  tsp_multi_add(tsp_multi *m, tsp_context *ctx, int nsteps, int *id);
+/* Steps per captured graph (default 32; 0 = launch every step directly). */
+int  tsp_multi_set_chunk(tsp_multi *m, int chunk);
+/* Advance every running instance by up to nsteps steps; *active gets the number
+ * of instances still running. */
+int  tsp_multi_run(tsp_multi *m, int nsteps, int *active);
+/* tsp_multi_run timed with HIP events on the set's stream (milliseconds). */
+int  tsp_multi_run_timed(tsp_multi *m, int nsteps, float *ms);
+/* Choose, per degree, the fastest launch variant of the element step by timing
+ * each on `steps_per_trial` real steps of that degree's instances. */
+int  tsp_multi_autotune(tsp_multi *m, int steps_per_trial);
+/* The launch variant in use for a degree: bit 0 = one row per lane across the
+ * wave (else column groups of 16/32 lanes, d <= 32 only), bit 1 = one item per
+ * wave (else a persistent loop). */
+int  tsp_multi_variant(const tsp_multi *m, int degree);
+int  tsp_multi_info(tsp_multi *m, int id, int *steps_done, int *nsteps);
+/* The instance's context (owned by the set; do not destroy). */
+tsp_context *tsp_multi_context(tsp_multi *m, int id);
+/* Destroy an instance (remove instances that borrow an operator before its owner). */
+int  tsp_multi_remove(tsp_multi *m, int id);
+int  tsp_multi_synchronize(tsp_multi *m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -374,3 +374,87 @@ def test_batched_state_roundtrip_and_eval_receivers():
     q.__post_init__()
     uref, _ = oracle_run(q, u0=u0)
     assert rel_l2(u_0, uref) <= 1e-4
+
+
+# ------------------------------------------------------------------ sets of instances
+def _mixed_set(steps=(40, 25, 60, 35, 50, 30), degrees=(2, 3, 4, 5, 6, 8)):
+    probs = []
+    for k, (n, N) in enumerate(zip(steps, degrees)):
+        p = small_problem(N, nx=3 + k, ny=2 + k % 3, nsteps=n, seed=10 + k, nrec=5)
+        probs.append(p)
+    return probs
+
+
+def test_multi_set_equals_each_instance_alone():
+    """Six meshes of different size and degree stepped together in the persistent
+    kernel (graph chunks of 8): each instance's seismograms and final state equal
+    the same instance stepped alone by the single-context FP32 path, bit for bit,
+    and stay within 1e-4 of the FP64 oracle."""
+    probs = _mixed_set()
+    m = capi.Multi(chunk=8)
+    ids = [m.add(W.prepare_instance(p), p.nsteps) for p in probs]
+    total = 0
+    while m.run(7):
+        total += 7
+    for i, p in zip(ids, probs):
+        done, n = m.info(i)
+        assert done == n == p.nsteps
+        e = m.engine(i)
+        s, u = e.seismograms(), e.get_u()
+        _, u1, s1 = W.run_engine(p, "fp32c")
+        np.testing.assert_array_equal(s, s1)
+        np.testing.assert_array_equal(u, u1)
+        _, sref = oracle_run(p)
+        say(f"  instance p={p.N} ne={p.ne} steps={p.nsteps}: vs oracle {rel_l2(s, sref):.3e}")
+        assert rel_l2(s, sref) <= 1e-4
+    m.close()
+
+
+def test_multi_graph_and_plain_launches_agree_and_autotune():
+    probs = _mixed_set()
+    res = []
+    for chunk in (0, 16):
+        m = capi.Multi(chunk=chunk)
+        ids = [m.add(W.prepare_instance(p), p.nsteps) for p in probs]
+        while m.run(16):
+            pass
+        res.append([m.engine(i).seismograms() for i in ids])
+        m.close()
+    for a, b in zip(*res):
+        np.testing.assert_array_equal(a, b)
+    # autotuned mapping: results within FP32 roundoff of the default mapping
+    m = capi.Multi(chunk=8)
+    ids = [m.add(W.prepare_instance(p), p.nsteps) for p in probs]
+    m.autotune(3)
+    say("  autotuned variants: " + ", ".join(f"p{N}:{m.variant(N)}" for N in (2, 3, 4, 5, 6, 8)))
+    while m.run(9):
+        pass
+    for i, p, s0 in zip(ids, probs, res[0]):
+        s = m.engine(i).seismograms()
+        assert rel_l2(s, s0) <= 1e-5
+    m.close()
+
+
+def test_multi_shared_operator_and_remove():
+    """Three simulations on one mesh share one assembled operator; removing one
+    mid-run leaves the others unchanged."""
+    base = small_problem(4, nx=6, ny=5, nsteps=60, seed=21, nrec=6)
+    probs = []
+    for k, (x, y) in enumerate([(150.0, 700.0), (600.0, 300.0), (820.0, 880.0)]):
+        q = small_problem(4, nx=6, ny=5, nsteps=60, seed=21, nrec=6)
+        q.src_xy = np.array([[x, y]])
+        q.dirs = np.array([[np.cos(k), np.sin(k)]])
+        q.__post_init__()
+        probs.append(q)
+    owner = W.prepare_instance(probs[0])
+    others = [W.prepare_instance(q, share=owner) for q in probs[1:]]
+    m = capi.Multi(chunk=4)
+    ids = [m.add(owner, 60)] + [m.add(e, 60) for e in others]
+    m.run(20)
+    m.remove(ids[2])
+    while m.run(10):
+        pass
+    for i, q in zip(ids[:2], probs[:2]):
+        _, _, s1 = W.run_engine(q, "fp32c")
+        np.testing.assert_array_equal(m.engine(i).seismograms(), s1)
+    m.close()

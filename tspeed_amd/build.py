@@ -20,7 +20,7 @@ ARCH = os.environ.get("TSPEED_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
-DEVICE_SRC = ["device/assemble.hip", "device/step.hip", "device/spmm.hip", "device/capi.hip"]
+DEVICE_SRC = ["device/assemble.hip", "device/step.hip", "device/spmm.hip", "device/multi.hip", "device/capi.hip"]
 HOST_SRC = ["host/tspeed.cpp", "host/host_capi.cpp"]
 
 # Packed FP32 VALU forms are switched off for every device compile.

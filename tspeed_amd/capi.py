@@ -54,6 +54,21 @@ SIGNATURES = {
     "tsp_status": (_i, [_vp, ctypes.POINTER(ctypes.c_uint)]),
     "tsp_synchronize": (_i, [_vp]),
     "tsp_device_bytes": (ctypes.c_longlong, [_vp]),
+    "tsp_share_operator": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "tsp_release_fp64": (_i, [_vp]),
+    "tsp_multi_create": (_i, [_i, ctypes.POINTER(_vp)]),
+    "tsp_multi_destroy": (None, [_vp]),
+    "tsp_multi_last_error": (ctypes.c_char_p, [_vp]),
+    "tsp_multi_add": (_i, [_vp, _vp, _i, ctypes.POINTER(_i)]),
+    "tsp_multi_set_chunk": (_i, [_vp, _i]),
+    "tsp_multi_run": (_i, [_vp, _i, ctypes.POINTER(_i)]),
+    "tsp_multi_run_timed": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_float)]),
+    "tsp_multi_autotune": (_i, [_vp, _i]),
+    "tsp_multi_variant": (_i, [_vp, _i]),
+    "tsp_multi_info": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "tsp_multi_context": (_vp, [_vp, _i]),
+    "tsp_multi_remove": (_i, [_vp, _i]),
+    "tsp_multi_synchronize": (_i, [_vp]),
     "tsh_gauleg": (_i, [_i, _d, _d, _dp, _dp]),
     "tsh_dubiner": (_i, [_i, _d, _d, _dp, _vp]),
     "tsh_tables": (_i, [_i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -176,22 +191,41 @@ def ricker_table(f0, t0, dirx, diry, dt, nrows):
 class Engine:
     """One tsp_context: a mesh, its operator and the time-stepping state."""
 
-    def __init__(self, precision="fp64", device=0, ncols=1):
+    def __init__(self, precision="fp64", device=0, ncols=1, share=None, handle=None):
+        """New context; share=Engine borrows that engine's operator
+        (tsp_share_operator); handle wraps an existing context without owning it."""
         self._L = lib()
         self.precision = precision
         self.ncols = ncols
+        self.N = None
+        self.ne = 0
+        self._owned = handle is None
+        self._keep = share
+        if handle is not None:
+            self._h = handle
+            return
         h = _vp()
-        rc = self._L.tsp_create(device, PRECISIONS[precision], ncols, ctypes.byref(h))
+        if share is not None:
+            rc = self._L.tsp_share_operator(share._h, ctypes.byref(h))
+            self.N, self.ne, self.precision, self.ncols = share.N, share.ne, share.precision, share.ncols
+        else:
+            rc = self._L.tsp_create(device, PRECISIONS[precision], ncols, ctypes.byref(h))
         if rc != TSP_OK:
             raise EngineError(rc, f"tsp_create(device={device}, precision={precision}) failed")
         self._h = h
-        self.N = None
-        self.ne = 0
 
     def close(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and getattr(self, "_owned", True):
             self._L.tsp_destroy(self._h)
-            self._h = None
+        self._h = None
+
+    def release(self):
+        """Hand the context over (to a Multi set): it is no longer destroyed here."""
+        self._owned = False
+        return self._h
+
+    def release_fp64(self):
+        self._ck(self._L.tsp_release_fp64(self._h))
 
     def __del__(self):
         self.close()
@@ -305,3 +339,74 @@ class Engine:
 
     def device_bytes(self):
         return int(self._L.tsp_device_bytes(self._h))
+
+
+class Multi:
+    """A set of independent simulations stepped together (tsp_multi_*)."""
+
+    def __init__(self, device=0, chunk=32):
+        self._L = lib()
+        h = _vp()
+        rc = self._L.tsp_multi_create(device, ctypes.byref(h))
+        if rc != TSP_OK:
+            raise EngineError(rc, "tsp_multi_create failed")
+        self._h = h
+        self.engines = {}
+        self.set_chunk(chunk)
+
+    def _ck(self, rc):
+        if rc != TSP_OK:
+            raise EngineError(rc, self._L.tsp_multi_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for e in self.engines.values():
+                e._h = None
+            self._L.tsp_multi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_chunk(self, chunk):
+        self._ck(self._L.tsp_multi_set_chunk(self._h, int(chunk)))
+
+    def add(self, engine, nsteps):
+        """Hand over an Engine after its first step; returns the instance id."""
+        i = _i()
+        self._ck(self._L.tsp_multi_add(self._h, engine.handle, int(nsteps), ctypes.byref(i)))
+        engine.release()
+        self.engines[i.value] = engine
+        return i.value
+
+    def run(self, nsteps):
+        a = _i()
+        self._ck(self._L.tsp_multi_run(self._h, int(nsteps), ctypes.byref(a)))
+        return a.value
+
+    def run_timed(self, nsteps):
+        ms = ctypes.c_float()
+        self._ck(self._L.tsp_multi_run_timed(self._h, int(nsteps), ctypes.byref(ms)))
+        return ms.value
+
+    def autotune(self, steps_per_trial=5):
+        self._ck(self._L.tsp_multi_autotune(self._h, int(steps_per_trial)))
+
+    def variant(self, degree):
+        return self._L.tsp_multi_variant(self._h, int(degree))
+
+    def info(self, i):
+        a, b = _i(), _i()
+        self._ck(self._L.tsp_multi_info(self._h, int(i), ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def engine(self, i):
+        return self.engines[i]
+
+    def remove(self, i):
+        self._ck(self._L.tsp_multi_remove(self._h, int(i)))
+        e = self.engines.pop(i)
+        e._h = None
+
+    def synchronize(self):
+        self._ck(self._L.tsp_multi_synchronize(self._h))

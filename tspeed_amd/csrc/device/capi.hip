@@ -6,6 +6,7 @@
 // results are copied out on request.  Errors never throw across the ABI: each
 // entry point catches, records a message and returns a status code.
 #include "engine.hpp"
+#include "internal.hpp"
 #include "../../../include/tspeed_amd.h"
 
 #include <algorithm>
@@ -114,9 +115,12 @@ template <typename T>
 void dfree(tsp_context *c, T *&p)
 {
     if (!p) return;
+    // pointers borrowed from another context (tsp_share_operator) are not ours to free
     auto it = std::find(c->allocs.begin(), c->allocs.end(), (void *)p);
-    if (it != c->allocs.end()) c->allocs.erase(it);
-    (void)hipFree(p);
+    if (it != c->allocs.end()) {
+        c->allocs.erase(it);
+        (void)hipFree(p);
+    }
     p = nullptr;
 }
 
@@ -193,7 +197,14 @@ void rebuild_extra(tsp_context *c)
 
 void require_mesh(tsp_context *c)
 {
-    if (!c->ne || !c->blk64) fail(c, TSP_ERR_STATE, "no operator: call tsp_set_mesh or tsp_upload_blocks first");
+    if (!c->ne || !(c->blk64 || c->blk32 || c->blkA))
+        fail(c, TSP_ERR_STATE, "no operator: call tsp_set_mesh or tsp_upload_blocks first");
+}
+
+void require_fp64_operator(tsp_context *c)
+{
+    require_mesh(c);
+    if (!c->blk64) fail(c, TSP_ERR_STATE, "the FP64 operator was released (tsp_release_fp64)");
 }
 
 void alloc_state(tsp_context *c)
@@ -491,7 +502,7 @@ int tsp_upload_blocks(tsp_context *c, int ne, const int *neigh, const double *bl
 int tsp_get_blocks(tsp_context *c, int e_begin, int e_count, double *blocks, double *invm)
 {
     return guarded(c, [&] {
-        require_mesh(c);
+        require_fp64_operator(c);
         if (e_begin < 0 || e_count < 0 || e_begin + e_count > c->ne)
             fail(c, TSP_ERR_ARG, "element range [%d, %d) outside 0..%d", e_begin, e_begin + e_count, c->ne);
         if (blocks && e_count) {
@@ -802,7 +813,7 @@ int tsp_get_seismograms(tsp_context *c, double *out, int max_samples, int *nsamp
 int tsp_cfl(tsp_context *c, double factor, int iters, double *dt_out, double *lmax_out)
 {
     return guarded(c, [&] {
-        require_mesh(c);
+        require_fp64_operator(c);
         if (iters < 1) fail(c, TSP_ERR_ARG, "iters must be >= 1");
         const size_t n = (size_t)c->ne * c->D;
         double *x = dalloc<double>(c, n, "cfl"), *y = dalloc<double>(c, n, "cfl");
@@ -860,4 +871,99 @@ int tsp_synchronize(tsp_context *c)
 
 long long tsp_device_bytes(const tsp_context *c) { return c ? c->bytes : 0; }
 
+int tsp_share_operator(tsp_context *owner, tsp_context **out)
+{
+    if (!owner || !out) return TSP_ERR_ARG;
+    *out = nullptr;
+    tsp_context *c = nullptr;
+    int rc = tsp_create(owner->device, owner->precision, owner->ncols, &c);
+    if (rc != TSP_OK) return rc;
+    rc = guarded(c, [&] {
+        if (!owner->ne) fail(c, TSP_ERR_STATE, "the owner has no operator");
+        c->P = owner->P; c->nln = owner->nln; c->D = owner->D; c->nq1 = owner->nq1; c->nq2 = owner->nq2;
+        c->ne = owner->ne;
+        c->h_neigh = owner->h_neigh;
+        // borrowed (not in c->allocs, never freed by c)
+        c->blk64 = owner->blk64;
+        c->blk32 = owner->blk32;
+        c->blkA = owner->blkA;
+        c->invm = owner->invm;
+        c->KP = owner->KP;
+        c->meta = dalloc<int4>(c, c->ne, "meta");
+        if (c->precision == TSP_FP32C) {
+            c->sc_first = dalloc<float>(c, c->ne, "sc");
+            c->sc_step = dalloc<float>(c, c->ne, "sc");
+        }
+        alloc_state(c);
+        hipcheck(c, hipMemsetAsync(c->u64[0], 0, sizeof(double) * c->state_len(), c->stream), "memset");
+        hipcheck(c, hipMemsetAsync(c->u64[1], 0, sizeof(double) * c->state_len(), c->stream), "memset");
+        c->have_state = true;
+        rebuild_extra(c);
+    });
+    if (rc != TSP_OK) {
+        if (owner) owner->err = c->err;
+        tsp_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return TSP_OK;
+}
+
+int tsp_release_fp64(tsp_context *c)
+{
+    return guarded(c, [&] {
+        require_mesh(c);
+        if (c->precision != TSP_FP32C) fail(c, TSP_ERR_STATE, "the FP64 context steps with its FP64 operator");
+        hipcheck(c, hipStreamSynchronize(c->stream), "synchronize");
+        dfree(c, c->blk64);
+    });
+}
+
 }  // extern "C"
+
+namespace tspeed_dev {
+
+bool ctx_view(tsp_context *c, InstView *v)
+{
+    if (!c || !v) return false;
+    v->device = c->device;
+    v->precision = c->precision;
+    v->ncols = c->ncols;
+    v->P = c->P;
+    v->ne = c->ne;
+    v->nsrc = c->nsrc;
+    v->nrec = c->nrec;
+    v->ntab = c->ntab;
+    v->stride = c->stride;
+    v->max_samples = c->max_samples;
+    v->nsamples = c->nsamples;
+    v->steps = c->steps;
+    v->cur = c->cur;
+    v->first_done = c->first_done;
+    v->stream = c->stream;
+    v->meta = c->meta;
+    v->extra = c->extra;
+    v->src_sorted = c->src_sorted;
+    v->rec_sorted = c->rec_sorted;
+    v->src_phi = c->src_phi;
+    v->wav = c->wav;
+    v->rec_phi = c->rec_phi;
+    v->seis = c->seis;
+    v->flags = c->flags;
+    v->blk32 = c->blk32;
+    v->sc_step = c->sc_step;
+    v->uhi[0] = c->uhi[0];
+    v->uhi[1] = c->uhi[1];
+    v->ucomp = c->ucomp;
+    v->winc = c->winc;
+    return true;
+}
+
+void ctx_set_progress(tsp_context *c, int steps, int cur, int nsamples)
+{
+    c->steps = steps;
+    c->cur = cur;
+    c->nsamples = nsamples;
+}
+
+}  // namespace tspeed_dev

@@ -25,7 +25,9 @@ namespace tspeed_dev {
 
 constexpr int kMaxDegree = 8;
 
-template <int P>
+// Shape of one element's work.  RWF forces the lanes per column group (0 = the
+// narrowest power of two >= d, capped at 64).
+template <int P, int RWF = 0>
 struct Geom {
     static constexpr int NLN = (P + 1) * (P + 2) / 2;  // modes per component
     static constexpr int D = 2 * NLN;                  // rows of a merged block
@@ -33,7 +35,7 @@ struct Geom {
     static constexpr int NQ1 = P + 1;                  // Gauss<N+1> edge nodes
     static constexpr int NQ2 = NQ1 * NQ1;              // collapsed area nodes
     // lanes of a wave that hold one column group (one row per lane)
-    static constexpr int RW = D <= 16 ? 16 : (D <= 32 ? 32 : 64);
+    static constexpr int RW = RWF ? RWF : (D <= 16 ? 16 : (D <= 32 ? 32 : 64));
     static constexpr int NG = 64 / RW;                  // column groups per wave
     static constexpr int RPL = (D + 63) / 64;           // rows per lane when D > 64
     static constexpr int DP = (D + 3) / 4 * 4;          // LDS row stride (16-byte aligned groups)

@@ -1,0 +1,28 @@
+// internal.hpp -- library-internal view of a context, used by the
+// multi-instance engine (multi.hip) to step many contexts in one kernel.
+#pragma once
+#include "engine.hpp"
+
+struct tsp_context;
+
+namespace tspeed_dev {
+
+struct InstView {
+    int device, precision, ncols;
+    int P, ne, nsrc, nrec, ntab, stride, max_samples, nsamples, steps, cur;
+    bool first_done;
+    hipStream_t stream;
+    const int4 *meta, *extra;
+    const int *src_sorted, *rec_sorted;
+    const double *src_phi, *wav, *rec_phi;
+    double *seis;
+    unsigned *flags;
+    const float *blk32, *sc_step;
+    float *uhi[2], *ucomp, *winc;
+};
+
+bool ctx_view(tsp_context *c, InstView *v);
+// record progress made by kernels launched outside the context
+void ctx_set_progress(tsp_context *c, int steps, int cur, int nsamples);
+
+}  // namespace tspeed_dev

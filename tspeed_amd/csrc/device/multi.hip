@@ -1,0 +1,637 @@
+// multi.hip -- many independent simulations (meshes of any size, degrees 1..8,
+// FP32 compensated) advanced together.
+//
+// Each instance is an ordinary context (its own operator, or one shared with
+// tsp_share_operator; its own sources, receivers and state).  The (instance,
+// element) items of all running instances form one list sorted by degree.  One
+// step of the whole set is one persistent kernel per degree present --
+// k_multi32c<Geom<P>> specialised for that degree, so each keeps the register
+// budget (and occupancy) of the single-mesh kernel -- whose resident waves walk
+// that degree's items, each running the same element step as k_step32c
+// (block-ELL SpMV over column quads + compensated update + source + receivers).
+// The per-degree kernels of a step are independent graph nodes, so they run
+// side by side; k_multi_advance, which bumps the step counter of every running
+// instance, depends on all of them.  Step counters live on the device, so a
+// chunk of steps is one graph, built once and replayed.  Instances stop at their
+// own step count; finished ones are dropped from the items at the next chunk.
+//
+// Autotuning: per degree, tsp_multi_autotune times the launch variants on real
+// steps of that degree's instances and keeps the fastest: persistent loop or
+// one item per wave, and for d <= 32 also the lane mapping (column groups of
+// 16/32 lanes reduced by shuffles, or one row per lane across the wave).
+#include "../../../include/tspeed_amd.h"
+#include "internal.hpp"
+#include "step_common.hpp"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace tspeed_dev {
+namespace {
+
+struct MInst {
+    int P, variant, ne, nsteps, stride, nrec, nsrc, max_samples;
+    int steps;  // completed step() calls (advanced on the device)
+    int live;   // 0 once removed
+    int sel;    // 1 when the instance's elements are in the current item list
+    const int4 *meta, *extra;
+    const int *src_sorted, *rec_sorted;
+    const double *src_phi, *wav, *rec_phi;
+    double *seis;
+    unsigned *flags;
+    const float *blk, *sc;
+    float *uhi0, *uhi1, *ucomp, *winc;
+};
+
+constexpr int kMWaves = 4;
+constexpr int kMaxDP = Geom<8>::DP;
+
+template <typename G>
+__device__ inline void multi_elem(const MInst &I, int e, int lane, float (*xs)[kMaxDP])
+{
+    const int n = I.steps + 1;               // 1-based index of this step() call
+    const int cur = (I.steps + 1) & 1;       // after the first step u^n sits in uhi1
+    const float *uin = cur ? I.uhi1 : I.uhi0;
+    float *uout = cur ? I.uhi0 : I.uhi1;
+    const int4 m = I.meta[e];
+    const int g = lane / G::RW, r = lane % G::RW;
+    float wi[G::RPL], cp[G::RPL];
+    int rows[G::RPL];
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const int row = r + 64 * q;
+        rows[q] = (g == 0 && row < G::D) ? row : -1;
+        const size_t idx = (size_t)e * G::D + row;
+        wi[q] = rows[q] >= 0 ? I.winc[idx] : 0.f;
+        cp[q] = rows[q] >= 0 ? I.ucomp[idx] : 0.f;
+    }
+    const float sc = I.sc[e];
+    float(*xv)[G::DP] = reinterpret_cast<float(*)[G::DP]>(&xs[0][0]);
+    gather<G, float>(xv, uin, e, m, lane, true);
+    // the wave reads only its own LDS slice: order its stores before its loads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float acc[G::RPL];
+    spmv<G, float, 4>(I.blk, e, m, xv, lane, acc);
+
+    StepArgs a{};
+    a.nrec = I.nrec;
+    a.src_sorted = I.src_sorted;
+    a.rec_sorted = I.rec_sorted;
+    a.src_phi = I.src_phi;
+    a.wav_row = I.nsrc ? I.wav + (size_t)n * I.nsrc * 2 : nullptr;
+    a.rec_phi = I.rec_phi;
+    a.seis = I.seis;
+    const int slot = (I.stride > 0 && n % I.stride == 0) ? n / I.stride : -1;
+    a.sample = (slot >= 0 && slot < I.max_samples) ? slot : -1;
+    const int4 ex = m.w >= 0 ? I.extra[m.w] : make_int4(0, 0, 0, 0);
+    float unew[G::RPL];
+    bool bad = false;
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        unew[q] = 0.f;
+        const int row = rows[q];
+        if (row < 0) continue;
+        const size_t idx = (size_t)e * G::D + row;
+        const float f = (ex.y > ex.x) ? (float)source_term(a, ex, row, G::NLN) : 0.f;
+        const float w = wi[q] + sc * (acc[q] - f);
+        const float uh = xv[0][row];
+        const float y = w - cp[q];
+        const float t = uh + y;
+        I.ucomp[idx] = (t - uh) - y;
+        uout[idx] = t;
+        I.winc[idx] = w;
+        unew[q] = t;
+        bad |= !isfinite(t);
+    }
+    if (bad) atomicOr(I.flags, 1u);
+    if (a.sample >= 0 && ex.w > ex.z) record<G, float>(a, ex, unew, rows, G::RPL);
+    // the next item's gather overwrites this slice
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Two launch shapes per degree: PERSIST = resident waves loop over the items
+// (at least 4 waves per SIMD where that fits without spilling, 2 for the
+// two-rows-per-lane degrees); otherwise one item per wave and the hardware
+// dispatcher balances.  The compiler schedules the looped form more
+// aggressively (more loads in flight, more registers), so which one is faster
+// depends on the degree -- the autotuner decides.
+template <typename G, bool PERSIST>
+__global__ __launch_bounds__(64 * kMWaves, PERSIST ? (G::D > 64 ? 2 : 4) : 1) void k_multi32c(
+    const MInst *insts, const int2 *items, int nitems)
+{
+    __shared__ __attribute__((aligned(16))) float xs_all[kMWaves][4][G::DP];
+    const int lane = threadIdx.x & 63;
+    // wave-uniform values held in scalar registers: the item, and through it the
+    // instance record, are then fetched with scalar loads
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    float(*xs)[kMaxDP] = reinterpret_cast<float(*)[kMaxDP]>(&xs_all[wave][0][0]);
+    if (PERSIST) {
+        for (int it = blockIdx.x * kMWaves + wave; it < nitems; it += gridDim.x * kMWaves) {
+            const int2 t = items[it];
+            const MInst I = insts[t.x];
+            if (!I.live || I.steps >= I.nsteps) continue;
+            multi_elem<G>(I, t.y, lane, xs);
+        }
+    } else {
+        const int it = blockIdx.x * kMWaves + wave;
+        if (it >= nitems) return;
+        const int2 t = items[it];
+        const MInst I = insts[t.x];
+        if (!I.live || I.steps >= I.nsteps) return;
+        multi_elem<G>(I, t.y, lane, xs);
+    }
+}
+
+// Launch variants: bit 0 = whole-wave rows (degrees with d <= 32 only),
+// bit 1 = one item per wave (else persistent).
+constexpr int kVarRows = 1, kVarOneShot = 2;
+
+template <int P>
+const void *kern_p(int v)
+{
+    constexpr bool narrow = 2 * ((P + 1) * (P + 2) / 2) <= 32;
+    if (narrow && (v & kVarRows))
+        return (v & kVarOneShot) ? reinterpret_cast<const void *>(&k_multi32c<Geom<P, 64>, false>)
+                                 : reinterpret_cast<const void *>(&k_multi32c<Geom<P, 64>, true>);
+    return (v & kVarOneShot) ? reinterpret_cast<const void *>(&k_multi32c<Geom<P>, false>)
+                             : reinterpret_cast<const void *>(&k_multi32c<Geom<P>, true>);
+}
+
+const void *multi_kernel(int P, int v)
+{
+    switch (P) {
+    case 1: return kern_p<1>(v);
+    case 2: return kern_p<2>(v);
+    case 3: return kern_p<3>(v);
+    case 4: return kern_p<4>(v);
+    case 5: return kern_p<5>(v);
+    case 6: return kern_p<6>(v);
+    case 7: return kern_p<7>(v);
+    case 8: return kern_p<8>(v);
+    default: return nullptr;
+    }
+}
+
+__global__ void k_multi_advance(MInst *insts, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && insts[i].sel && insts[i].live && insts[i].steps < insts[i].nsteps) insts[i].steps += 1;
+}
+
+}  // namespace
+}  // namespace tspeed_dev
+
+using namespace tspeed_dev;
+
+struct tsp_multi {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<tsp_context *> ctx;     // by instance id (nullptr once removed)
+    std::vector<MInst> host;            // host mirror of the instance table
+    MInst *d_inst = nullptr;
+    int d_cap = 0;
+    int2 *d_items = nullptr;
+    int nitems = 0, items_cap = 0;
+    bool dirty = true;                  // table / items must be rebuilt
+    int chunk = 32;                     // steps per captured graph (0 = plain launches)
+    hipGraphExec_t graph = nullptr;
+    int graph_chunk = 0;
+    struct Range {
+        int P, variant, off, count, grid;
+        const void *kern;
+    };
+    std::vector<Range> ranges;          // items of one degree: [off, off+count)
+    std::map<int, int> variant_of_p;    // autotuned lane mapping per degree
+    std::vector<int> only_p;            // restrict the item list (autotuning)
+};
+
+namespace {
+
+struct MFail {
+    int code;
+};
+
+void mfail(tsp_multi *m, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    m->err = buf;
+    throw MFail{code};
+}
+
+void mhip(tsp_multi *m, hipError_t e, const char *what)
+{
+    if (e != hipSuccess) mfail(m, TSP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+template <typename F>
+int mguard(tsp_multi *m, F &&f)
+{
+    if (!m) return TSP_ERR_ARG;
+    try {
+        mhip(m, hipSetDevice(m->device), "hipSetDevice");
+        f();
+        m->err.clear();
+        return TSP_OK;
+    } catch (const MFail &x) {
+        return x.code;
+    }
+}
+
+void drop_graph(tsp_multi *m)
+{
+    if (m->graph) (void)hipGraphExecDestroy(m->graph);
+    m->graph = nullptr;
+    m->graph_chunk = 0;
+}
+
+// pull the device step counters into the host mirror and the contexts
+void sync_progress(tsp_multi *m)
+{
+    if (!m->d_inst || m->host.empty()) return;
+    std::vector<MInst> dev(m->host.size());
+    mhip(m, hipMemcpyAsync(dev.data(), m->d_inst, sizeof(MInst) * dev.size(), hipMemcpyDeviceToHost, m->stream),
+         "progress");
+    mhip(m, hipStreamSynchronize(m->stream), "progress");
+    for (size_t i = 0; i < dev.size(); ++i) {
+        if (!m->ctx[i]) continue;
+        MInst &h = m->host[i];
+        h.steps = dev[i].steps;
+        const int ns = (h.stride > 0 ? 1 + h.steps / h.stride : 0);
+        ctx_set_progress(m->ctx[i], h.steps, (h.steps + 1) & 1, std::min(ns, h.max_samples));
+    }
+}
+
+void rebuild(tsp_multi *m)
+{
+    if (!m->dirty) return;
+    drop_graph(m);
+    const int n = (int)m->host.size();
+    for (int i = 0; i < n; ++i) {
+        MInst &h = m->host[i];
+        auto it = m->variant_of_p.find(h.P);
+        h.variant = it == m->variant_of_p.end() ? kVarOneShot : it->second;
+        h.sel = (h.live && h.steps < h.nsteps &&
+                 (m->only_p.empty() || std::find(m->only_p.begin(), m->only_p.end(), h.P) != m->only_p.end()))
+                    ? 1 : 0;
+    }
+    if (n > m->d_cap) {
+        if (m->d_inst) (void)hipFree(m->d_inst);
+        m->d_cap = std::max(n, 2 * m->d_cap);
+        mhip(m, hipMalloc(&m->d_inst, sizeof(MInst) * m->d_cap), "instance table");
+    }
+    if (n) mhip(m, hipMemcpyAsync(m->d_inst, m->host.data(), sizeof(MInst) * n, hipMemcpyHostToDevice, m->stream),
+                "instance table");
+    // items of running instances, grouped by degree
+    std::vector<int2> items;
+    std::vector<int> order;
+    for (int i = 0; i < n; ++i)
+        if (m->host[i].sel) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return m->host[a].P < m->host[b].P; });
+    for (int i : order)
+        for (int e = 0; e < m->host[i].ne; ++e) items.push_back(make_int2(i, e));
+    m->nitems = (int)items.size();
+    if (m->nitems > m->items_cap) {
+        if (m->d_items) (void)hipFree(m->d_items);
+        m->items_cap = std::max(m->nitems, 2 * m->items_cap);
+        mhip(m, hipMalloc(&m->d_items, sizeof(int2) * m->items_cap), "items");
+    }
+    if (m->nitems)
+        mhip(m, hipMemcpyAsync(m->d_items, items.data(), sizeof(int2) * m->nitems, hipMemcpyHostToDevice, m->stream),
+             "items");
+    mhip(m, hipStreamSynchronize(m->stream), "rebuild");
+    hipDeviceProp_t prop;
+    mhip(m, hipGetDeviceProperties(&prop, m->device), "properties");
+    const int cus = prop.multiProcessorCount;
+    m->ranges.clear();
+    for (int q = 0; q < m->nitems;) {
+        const int P = m->host[items[q].x].P;
+        int r = q;
+        while (r < m->nitems && m->host[items[r].x].P == P) ++r;
+        tsp_multi::Range g;
+        g.P = P;
+        g.variant = m->host[items[q].x].variant;
+        g.off = q;
+        g.count = r - q;
+        g.kern = multi_kernel(P, g.variant);
+        if (!g.kern) mfail(m, TSP_ERR_UNSUPPORTED, "degree %d not built", P);
+        const int want = (g.count + kMWaves - 1) / kMWaves;
+        if (g.variant & kVarOneShot) {
+            g.grid = want;
+        } else {
+            int per_cu = 0;
+            mhip(m, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, g.kern, 64 * kMWaves, 0), "occupancy");
+            g.grid = std::max(1, std::min(want, std::max(1, per_cu) * cus));
+        }
+        m->ranges.push_back(g);
+        q = r;
+    }
+    m->dirty = false;
+}
+
+void launch_steps(tsp_multi *m, int nsteps)
+{
+    int n = (int)m->host.size();
+    for (int k = 0; k < nsteps; ++k) {
+        for (auto &g : m->ranges) {
+            const int2 *items = m->d_items + g.off;
+            void *args[] = {&m->d_inst, &items, &g.count};
+            mhip(m, hipLaunchKernel(g.kern, dim3(g.grid), dim3(64 * kMWaves), args, 0, m->stream), "multi step");
+        }
+        hipLaunchKernelGGL(k_multi_advance, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->d_inst, n);
+    }
+    mhip(m, hipGetLastError(), "multi step launch");
+}
+
+// one graph for `chunk` steps: per step, the degree kernels side by side, then
+// the counter advance that depends on all of them
+void build_graph(tsp_multi *m)
+{
+    drop_graph(m);
+    hipGraph_t g = nullptr;
+    mhip(m, hipGraphCreate(&g, 0), "graph");
+    int n = (int)m->host.size();
+    hipGraphNode_t prev = nullptr;
+    for (int k = 0; k < m->chunk; ++k) {
+        std::vector<hipGraphNode_t> step_nodes;
+        for (auto &r : m->ranges) {
+            const int2 *items = m->d_items + r.off;
+            void *args[] = {&m->d_inst, &items, &r.count};
+            hipKernelNodeParams kp{};
+            kp.func = const_cast<void *>(r.kern);
+            kp.gridDim = dim3(r.grid);
+            kp.blockDim = dim3(64 * kMWaves);
+            kp.sharedMemBytes = 0;
+            kp.kernelParams = args;
+            kp.extra = nullptr;
+            hipGraphNode_t node;
+            mhip(m, hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &kp), "graph node");
+            step_nodes.push_back(node);
+        }
+        void *aargs[] = {&m->d_inst, &n};
+        hipKernelNodeParams ap{};
+        ap.func = reinterpret_cast<void *>(&k_multi_advance);
+        ap.gridDim = dim3((n + 255) / 256);
+        ap.blockDim = dim3(256);
+        ap.sharedMemBytes = 0;
+        ap.kernelParams = aargs;
+        ap.extra = nullptr;
+        hipGraphNode_t adv;
+        mhip(m, hipGraphAddKernelNode(&adv, g, step_nodes.data(), step_nodes.size(), &ap), "graph node");
+        prev = adv;
+    }
+    hipError_t e = hipGraphInstantiate(&m->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    mhip(m, e, "instantiate");
+    m->graph_chunk = m->chunk;
+}
+
+// run nsteps: whole chunks from the graph, the remainder as launches
+void advance(tsp_multi *m, int nsteps)
+{
+    rebuild(m);
+    if (!m->nitems) return;
+    int left = nsteps;
+    if (m->chunk > 0 && left >= m->chunk) {
+        if (!m->graph || m->graph_chunk != m->chunk) build_graph(m);
+        while (left >= m->chunk) {
+            mhip(m, hipGraphLaunch(m->graph, m->stream), "graph launch");
+            left -= m->chunk;
+        }
+    }
+    if (left > 0) launch_steps(m, left);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tsp_multi_create(int device, tsp_multi **out)
+{
+    if (!out) return TSP_ERR_ARG;
+    *out = nullptr;
+    tsp_multi *m = new (std::nothrow) tsp_multi;
+    if (!m) return TSP_ERR_NOMEM;
+    m->device = device;
+    int rc = mguard(m, [&] { mhip(m, hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "stream"); });
+    if (rc != TSP_OK) {
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return TSP_OK;
+}
+
+void tsp_multi_destroy(tsp_multi *m)
+{
+    if (!m) return;
+    (void)hipSetDevice(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    drop_graph(m);
+    // contexts that borrow an operator go before their owners: destroy newest first
+    for (size_t i = m->ctx.size(); i-- > 0;)
+        if (m->ctx[i]) tsp_destroy(m->ctx[i]);
+    if (m->d_inst) (void)hipFree(m->d_inst);
+    if (m->d_items) (void)hipFree(m->d_items);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+const char *tsp_multi_last_error(const tsp_multi *m) { return m ? m->err.c_str() : "null set"; }
+
+int tsp_multi_add(tsp_multi *m, tsp_context *c, int nsteps, int *id)
+{
+    return mguard(m, [&] {
+        InstView v{};
+        if (!ctx_view(c, &v)) mfail(m, TSP_ERR_ARG, "null context");
+        if (v.device != m->device) mfail(m, TSP_ERR_ARG, "context on device %d, set on device %d", v.device, m->device);
+        if (v.precision != TSP_FP32C || v.ncols != 1)
+            mfail(m, TSP_ERR_UNSUPPORTED, "instances must be single-column TSP_FP32C contexts");
+        if (!v.first_done || v.steps != 0) mfail(m, TSP_ERR_STATE, "take the first step, and no other, before adding");
+        if (nsteps < 0) mfail(m, TSP_ERR_ARG, "negative step count");
+        if (v.nsrc && nsteps >= v.ntab)
+            mfail(m, TSP_ERR_ARG, "wavelet table has %d rows, %d steps need %d", v.ntab, nsteps, nsteps + 1);
+        if (v.nrec && v.stride > 0 && 1 + nsteps / v.stride > v.max_samples)
+            mfail(m, TSP_ERR_ARG, "seismogram buffer of %d samples cannot hold %d", v.max_samples, 1 + nsteps / v.stride);
+        // the context's own stream must be done with the first step
+        mhip(m, hipStreamSynchronize(v.stream), "context sync");
+        sync_progress(m);
+        MInst I{};
+        I.P = v.P;
+        I.ne = v.ne;
+        I.nsteps = nsteps;
+        I.stride = v.stride;
+        I.nrec = v.nrec;
+        I.nsrc = v.nsrc;
+        I.max_samples = v.max_samples;
+        I.steps = 0;
+        I.live = 1;
+        I.meta = v.meta;
+        I.extra = v.extra;
+        I.src_sorted = v.src_sorted;
+        I.rec_sorted = v.rec_sorted;
+        I.src_phi = v.src_phi;
+        I.wav = v.wav;
+        I.rec_phi = v.rec_phi;
+        I.seis = v.seis;
+        I.flags = v.flags;
+        I.blk = v.blk32;
+        I.sc = v.sc_step;
+        I.uhi0 = v.uhi[0];
+        I.uhi1 = v.uhi[1];
+        I.ucomp = v.ucomp;
+        I.winc = v.winc;
+        m->host.push_back(I);
+        m->ctx.push_back(c);
+        m->dirty = true;
+        if (id) *id = (int)m->host.size() - 1;
+    });
+}
+
+int tsp_multi_set_chunk(tsp_multi *m, int chunk)
+{
+    return mguard(m, [&] {
+        if (chunk < 0) mfail(m, TSP_ERR_ARG, "negative chunk");
+        m->chunk = chunk;
+        drop_graph(m);
+    });
+}
+
+int tsp_multi_run(tsp_multi *m, int nsteps, int *active)
+{
+    return mguard(m, [&] {
+        if (nsteps < 0) mfail(m, TSP_ERR_ARG, "negative step count");
+        sync_progress(m);
+        bool any_done = false;
+        for (const auto &h : m->host)
+            if (h.live && h.steps >= h.nsteps && h.nsteps > 0) any_done = true;
+        if (any_done) m->dirty = true;  // drop finished instances from the items
+        advance(m, nsteps);
+        sync_progress(m);
+        int act = 0;
+        for (const auto &h : m->host) act += (h.live && h.steps < h.nsteps) ? 1 : 0;
+        if (active) *active = act;
+    });
+}
+
+int tsp_multi_run_timed(tsp_multi *m, int nsteps, float *ms)
+{
+    return mguard(m, [&] {
+        if (!ms) mfail(m, TSP_ERR_ARG, "null output");
+        sync_progress(m);
+        rebuild(m);
+        hipEvent_t a, b;
+        mhip(m, hipEventCreate(&a), "event");
+        mhip(m, hipEventCreate(&b), "event");
+        mhip(m, hipEventRecord(a, m->stream), "event");
+        advance(m, nsteps);
+        mhip(m, hipEventRecord(b, m->stream), "event");
+        mhip(m, hipEventSynchronize(b), "event");
+        mhip(m, hipEventElapsedTime(ms, a, b), "event");
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        sync_progress(m);
+    });
+}
+
+int tsp_multi_autotune(tsp_multi *m, int steps_per_trial)
+{
+    return mguard(m, [&] {
+        if (steps_per_trial < 1) mfail(m, TSP_ERR_ARG, "steps_per_trial must be >= 1");
+        sync_progress(m);
+        std::vector<int> degrees;
+        for (const auto &h : m->host)
+            if (h.live && h.steps + 4 * steps_per_trial <= h.nsteps) degrees.push_back(h.P);
+        std::sort(degrees.begin(), degrees.end());
+        degrees.erase(std::unique(degrees.begin(), degrees.end()), degrees.end());
+        const int saved_chunk = m->chunk;
+        m->chunk = 0;
+        for (int p : degrees) {
+            const std::vector<int> cand = 2 * nln_of(p) <= 32 ? std::vector<int>{0, 1, 2, 3} : std::vector<int>{0, 2};
+            float best = 0;
+            int bestv = kVarOneShot;
+            for (int var : cand) {
+                m->variant_of_p[p] = var;
+                m->only_p = {p};
+                m->dirty = true;
+                rebuild(m);
+                hipEvent_t a, b;
+                mhip(m, hipEventCreate(&a), "event");
+                mhip(m, hipEventCreate(&b), "event");
+                mhip(m, hipEventRecord(a, m->stream), "event");
+                advance(m, steps_per_trial);
+                mhip(m, hipEventRecord(b, m->stream), "event");
+                mhip(m, hipEventSynchronize(b), "event");
+                float t = 0;
+                mhip(m, hipEventElapsedTime(&t, a, b), "event");
+                (void)hipEventDestroy(a);
+                (void)hipEventDestroy(b);
+                sync_progress(m);
+                if (var == cand[0] || t < best) {
+                    best = t;
+                    bestv = var;
+                }
+            }
+            m->variant_of_p[p] = bestv;
+        }
+        m->only_p.clear();
+        m->chunk = saved_chunk;
+        m->dirty = true;
+    });
+}
+
+int tsp_multi_variant(const tsp_multi *m, int degree)
+{
+    if (!m) return -1;
+    auto it = m->variant_of_p.find(degree);
+    return it == m->variant_of_p.end() ? kVarOneShot : it->second;
+}
+
+int tsp_multi_info(tsp_multi *m, int id, int *steps_done, int *nsteps)
+{
+    return mguard(m, [&] {
+        if (id < 0 || id >= (int)m->host.size() || !m->ctx[id]) mfail(m, TSP_ERR_ARG, "no instance %d", id);
+        sync_progress(m);
+        if (steps_done) *steps_done = m->host[id].steps;
+        if (nsteps) *nsteps = m->host[id].nsteps;
+    });
+}
+
+tsp_context *tsp_multi_context(tsp_multi *m, int id)
+{
+    if (!m || id < 0 || id >= (int)m->ctx.size()) return nullptr;
+    return m->ctx[id];
+}
+
+int tsp_multi_remove(tsp_multi *m, int id)
+{
+    return mguard(m, [&] {
+        if (id < 0 || id >= (int)m->host.size() || !m->ctx[id]) mfail(m, TSP_ERR_ARG, "no instance %d", id);
+        mhip(m, hipStreamSynchronize(m->stream), "sync");
+        sync_progress(m);
+        tsp_destroy(m->ctx[id]);
+        m->ctx[id] = nullptr;
+        m->host[id].live = 0;
+        m->dirty = true;
+    });
+}
+
+int tsp_multi_synchronize(tsp_multi *m)
+{
+    return mguard(m, [&] { mhip(m, hipStreamSynchronize(m->stream), "synchronize"); });
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+// step_common.hpp -- device helpers shared by the single-mesh step kernels
+// (step.hip) and the multi-instance kernel (multi.hip): XCD-aware element
+// dealing, neighbour gather into LDS, the block-ELL SpMV over column groups,
+// source injection and receiver sampling.
+#pragma once
+#include "engine.hpp"
+
+// Products and sums round as written (no FMA contraction); the SpMV
+// accumulates with explicit fma().
+#pragma clang fp contract(off)
+
+namespace tspeed_dev {
+namespace {
+
+constexpr int kWaves = 4;  // waves (elements) per work-group
+constexpr int kThreads = 64 * kWaves;
+
+__device__ inline int xcd_remap(int bid, int nwg)
+{
+    const int per = nwg / 8, rem = nwg % 8;
+    const int x = bid % 8, k = bid / 8;
+    return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Sum of the source terms landing on row `row` of element e (Force.cpp:27-28).
+__device__ inline double source_term(const StepArgs &a, int4 ex, int row, int nln)
+{
+    double f = 0.0;
+    const int c = row / nln, i = row % nln;
+    for (int k = ex.x; k < ex.y; ++k) {
+        const int src = a.src_sorted[k];
+        f += a.wav_row[src * 2 + c] * a.src_phi[src * nln + i];
+    }
+    return f;
+}
+
+// Receiver dot products for element e on the new coefficients held one (or two)
+// rows per lane: seis[sample][rec] = (phi . u_x, phi . u_y) (TimeAdvance.cpp:73-74).
+template <typename G, typename V>
+__device__ inline void record(const StepArgs &a, int4 ex, const V *unew, const int *rows, int nrows_lane)
+{
+    for (int k = ex.z; k < ex.w; ++k) {
+        const int rec = a.rec_sorted[k];
+        double sx = 0.0, sy = 0.0;
+        for (int q = 0; q < nrows_lane; ++q) {
+            const int row = rows[q];
+            if (row < 0 || row >= G::D) continue;
+            const int c = row / G::NLN, i = row % G::NLN;
+            const double v = a.rec_phi[rec * G::NLN + i] * (double)unew[q];
+            if (c == 0) sx += v; else sy += v;
+        }
+        sx = wave_sum(sx);
+        sy = wave_sum(sy);
+        if ((threadIdx.x & 63) == 0) {
+            a.seis[((size_t)a.sample * a.nrec + rec) * 2 + 0] = sx;
+            a.seis[((size_t)a.sample * a.nrec + rec) * 2 + 1] = sy;
+        }
+    }
+}
+
+// Gather u^n of the element and its three neighbours into the wave's LDS slice;
+// every load is issued before the first store so the four fetches overlap.
+template <typename G, typename T>
+__device__ inline void gather(T (*xs)[G::DP], const T *u, int e, int4 m, int lane, bool act)
+{
+    const int nbs[4] = {act ? e : -1, m.x, m.y, m.z};
+    T v[4][G::RPL];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            const int idx = lane + 64 * q;
+            v[s][q] = (nbs[s] >= 0 && idx < G::D) ? u[(size_t)nbs[s] * G::D + idx] : T(0);
+        }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            const int idx = lane + 64 * q;
+            if (idx < G::D) xs[s][idx] = v[s][q];
+        }
+}
+
+template <typename T, int W>
+using vec_t = T __attribute__((ext_vector_type(W)));
+
+// r = sum_s B_{e,s} x_s for the lane's rows, blocks stored as column groups of
+// width W (+ a trailing pair when d % W == 2).  For NG > 1 the column groups of
+// the wave are reduced into group 0.
+template <typename G, typename T, int W>
+__device__ inline void spmv(const T *blk, int e, int4 m, const T (*xs)[G::DP], int lane, T (&acc)[G::RPL])
+{
+    constexpr int NFULL = G::D / W, TAIL = G::D % W;
+    constexpr int NIT = (NFULL + G::NG - 1) / G::NG;
+    constexpr int TAILG = NFULL % G::NG;  // the column group with one iteration less takes the tail
+    const int g = lane / G::RW, r = lane % G::RW;
+    const int nbs[4] = {e, m.x, m.y, m.z};
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) acc[q] = T(0);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (nbs[s] < 0) continue;
+        const T *Bs = blk + ((size_t)e * 4 + s) * G::D * G::D;
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int jg = it * G::NG + g;
+            if (G::NG > 1 && jg >= NFULL) continue;
+            const vec_t<T, W> x = *reinterpret_cast<const vec_t<T, W> *>(&xs[s][W * jg]);
+#pragma unroll
+            for (int q = 0; q < G::RPL; ++q) {
+                const int row = r + 64 * q;
+                if (row < G::D) {
+                    const vec_t<T, W> b = __builtin_nontemporal_load(
+                        reinterpret_cast<const vec_t<T, W> *>(Bs + ((size_t)jg * G::D + row) * W));
+#pragma unroll
+                    for (int t = 0; t < W; ++t) acc[q] = fma(b[t], x[t], acc[q]);
+                }
+            }
+        }
+        if constexpr (TAIL == 2) {
+            if (g == TAILG) {
+                const vec_t<T, 2> x = *reinterpret_cast<const vec_t<T, 2> *>(&xs[s][W * NFULL]);
+#pragma unroll
+                for (int q = 0; q < G::RPL; ++q) {
+                    const int row = r + 64 * q;
+                    if (row < G::D) {
+                        const vec_t<T, 2> b = __builtin_nontemporal_load(
+                            reinterpret_cast<const vec_t<T, 2> *>(Bs + (size_t)NFULL * G::D * W + row * 2));
+                        acc[q] = fma(b[0], x[0], acc[q]);
+                        acc[q] = fma(b[1], x[1], acc[q]);
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (G::NG > 1) {
+#pragma unroll
+        for (int o = 32; o >= G::RW; o >>= 1) acc[0] += __shfl_down(acc[0], o);
+    }
+}
+
+}  // namespace
+}  // namespace tspeed_dev

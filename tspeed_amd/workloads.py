@@ -187,6 +187,58 @@ def run_batched(prob: Problem, ncols=64, stride=None):
     return eng, eng.seismograms()
 
 
+def config4_instance(k, seed=4, ne_range=(500, 20000), p_range=(2, 8), steps_range=(500, 5000), nrec=8,
+                     h=10.0):
+    """Instance k of the uneven batch: ne log-uniform in ne_range (split squares of
+    side h with aspect ratio U(1,4)), p uniform in p_range, homogeneous
+    vs ~ U(500,3000), vp/vs ~ U(1.6,2.0), rho ~ U(1600,2800), one Ricker source
+    (f0 ~ U(5,20), random direction) inside the domain, `nrec` receivers along
+    the top, steps uniform in steps_range."""
+    rng = np.random.default_rng([seed, k])
+    ne = float(np.exp(rng.uniform(np.log(ne_range[0]), np.log(ne_range[1]))))
+    ar = rng.uniform(1.0, 4.0)
+    ny = max(2, int(round(np.sqrt(ne / (2 * ar)))))
+    nx = max(2, int(round(ne / (2 * ny))))
+    N = int(rng.integers(p_range[0], p_range[1] + 1))
+    vs = rng.uniform(500.0, 3000.0)
+    vp = vs * rng.uniform(1.6, 2.0)
+    rho = rng.uniform(1600.0, 2800.0)
+    f0 = rng.uniform(5.0, 20.0)
+    ang = rng.uniform(0, 2 * np.pi)
+    nsteps = int(rng.integers(steps_range[0], steps_range[1] + 1))
+    Lx, Ly = nx * h, ny * h
+    pts, tri = capi.split_square(nx, ny, 0.0, 0.0, Lx, Ly, 0.2, 1000 + k)
+    n = tri.shape[0]
+    lam, mu = lame(rho, vp, vs)
+    src = np.array([[rng.uniform(0.2, 0.8) * Lx, rng.uniform(0.2, 0.8) * Ly]])
+    rx = (np.arange(nrec) + 0.5) * Lx / nrec
+    return Problem(f"config4[{k}]", N, pts, tri, np.full(n, lam), np.full(n, mu), np.full(n, rho), PENALTY,
+                   cfl_dt(pts, tri, N, vp), nsteps, src, np.array([f0]), np.array([1.2 / f0]),
+                   np.array([[np.cos(ang), np.sin(ang)]]), np.stack([rx, np.full(nrec, Ly - 0.5 * h)], 1), 1,
+                   dict(nx=nx, ny=ny, seed=seed))
+
+
+def prepare_instance(prob: Problem, share=None, stride=None, nsteps=None):
+    """A single-column FP32-compensated Engine for `prob` (operator assembled, or
+    borrowed from `share`), sources/receivers set, first step taken: ready to be
+    added to a Multi set."""
+    nsteps = prob.nsteps if nsteps is None else nsteps
+    stride = prob.stride if stride is None else stride
+    if share is None:
+        eng = capi.Engine("fp32c")
+        eng.set_tables(prob.N)
+        eng.set_mesh(prob.verts, prob.neigh, prob.neighedge, prob.lam, prob.mu, prob.rho, prob.penalty)
+        eng.release_fp64()
+    else:
+        eng = capi.Engine(share=share)
+    eng.set_sources(prob.src_elem, prob.src_phi, prob.wavelet(nsteps + 1))
+    eng.set_receivers(prob.rec_elem, prob.rec_phi, stride, 1 + nsteps // max(stride, 1))
+    eng.set_dt(prob.dt)
+    eng.set_state()
+    eng.first_step()
+    return eng
+
+
 def run_engine(prob: Problem, precision="fp64", engine=None, u0=None, v0=None, stride=None):
     """Assemble and run `prob` on the device; returns (engine, u_final, seismograms)."""
     eng = engine or capi.Engine(precision)
