@@ -15,6 +15,7 @@ on a bounded sample of the same workload).
 from __future__ import annotations
 
 import argparse
+import copy
 import json
 import os
 import sys
@@ -123,6 +124,52 @@ def run_multi(steps, warmup, n_inst=256):
                 workload="config4: uneven batch, one persistent kernel per step, graph chunks of 32")
 
 
+def run_serving(n_req=32, loads=(0.5, 0.8), sample_frac=0.05, seed=11):
+    """Config 5: requests from the multi-source distribution (20-40 squares per
+    side, p in {3,4,5}, random homogeneous materials, 1-16 sources, 500-5000
+    steps, 16-128 receivers) arrive as a Poisson process; the loop admits them
+    against 80% of device memory and steps everything admitted together.  First
+    all requests arrive at once (capacity), then arrival rates at the given
+    fractions of that capacity.  A seeded 5% sample is re-run source by source
+    on the FP64 path and its seismogram error reported."""
+    from tspeed_amd import capi, workloads as W
+    reqs = [W.config5_request(k) for k in range(n_req)]
+    work = sum(p.ndof * len(p.src_elem) * p.nsteps for p in reqs)
+    out = dict(requests=n_req, dof_steps=work,
+               workload="config5: Poisson arrivals, HBM-footprint admission at 80%, continuous batching "
+                        "(chunks of 32 steps)")
+    srv, ids, lat, wall = W.serve(reqs, rate=0)
+    st = srv.stats()
+    cap = n_req / wall
+    out["saturated"] = dict(value=st["dof_steps"] / st["busy_s"], wall_s=wall, requests_per_s=cap,
+                            p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)))
+    seis = {i: srv.result(i)[0] for i in ids}
+    srv.close()
+    for f in loads:
+        srv, ids, lat, wall = W.serve(reqs, rate=f * cap, seed=seed)
+        st = srv.stats()
+        out[f"load_{f:g}"] = dict(rate_per_s=f * cap, value=st["dof_steps"] / st["busy_s"], wall_s=wall,
+                                  p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)))
+        same = all(np.array_equal(srv.result(i)[0], seis[j]) for i, j in zip(ids, seis))
+        out[f"load_{f:g}"]["same_as_saturated"] = bool(same)
+        srv.close()
+    rng = np.random.default_rng(seed)
+    pick = sorted(rng.choice(n_req, max(1, int(round(sample_frac * n_req))), replace=False).tolist())
+    errs = []
+    for k in pick:
+        p = reqs[k]
+        for j in range(len(p.src_elem)):
+            q = copy.copy(p)
+            q.src_xy, q.f0, q.t0, q.dirs = p.src_xy[j:j + 1], p.f0[j:j + 1], p.t0[j:j + 1], p.dirs[j:j + 1]
+            q.__post_init__()
+            eng, _, s64 = W.run_engine(q, "fp64")
+            eng.close()
+            errs.append(float(np.linalg.norm(seis[ids[k]][j] - s64) / np.linalg.norm(s64)))
+    out["fp64_sample"] = dict(requests=pick, max_rel_l2=max(errs), simulations=len(errs))
+    out["value"] = out["saturated"]["value"]
+    return out
+
+
 def cpu_baseline(sample_steps):
     """The FP64 oracle (single thread, reference Release flags) on a 40x20 sub-grid
     of config 2 (same p, materials model and per-element work)."""
@@ -168,6 +215,7 @@ def main():
     ap.add_argument("--no-fp32c", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--multi", type=int, default=256, help="instances in the uneven-batch variant (0 = skip)")
+    ap.add_argument("--serve", type=int, default=32, help="requests in the serving variant (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -199,6 +247,8 @@ def main():
         variants["batched_fp32_64col"] = run_batched(args.steps, args.warmup)
     if args.multi > 0:
         variants["multi_fp32c"] = run_multi(min(args.steps, 200), min(args.warmup, 10), args.multi)
+    if args.serve > 0:
+        variants["serving"] = run_serving(args.serve)
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_step64_config2.json")

@@ -44,6 +44,9 @@ enum tsp_precision {
 /* Number of visible HIP devices (0 when no GPU driver is present). */
 int tsp_device_count(int *n);
 
+/* Free and total device memory in bytes. */
+int tsp_device_memory(int device, long long *free_bytes, long long *total_bytes);
+
 /* Context lifetime.  Replaces the construction of TimeAdvance/LeapFrog
  * (TimeAdvance_imp.hpp:26-40).  `device` is the HIP ordinal; `precision` one
  * of tsp_precision; `ncols` the number of right-hand-side columns: 1 = the
@@ -198,7 +201,8 @@ typedef struct tsp_multi tsp_multi;
 int  tsp_multi_create(int device, tsp_multi **out);
 void tsp_multi_destroy(tsp_multi *m);
 const char *tsp_multi_last_error(const tsp_multi *m);
-/* Add a context that will run `nsteps` step() calls; *id gets its index. */
+/* Add a context that will run `nsteps` step() calls; *id gets its index (the
+ * index of a removed instance may be given again). */
 int  tsp_multi_add(tsp_multi *m, tsp_context *ctx, int nsteps, int *id);
 /* Steps per captured graph (default 32; 0 = launch every step directly). */
 int  tsp_multi_set_chunk(tsp_multi *m, int chunk);
@@ -217,9 +221,64 @@ int  tsp_multi_variant(const tsp_multi *m, int degree);
 int  tsp_multi_info(tsp_multi *m, int id, int *steps_done, int *nsteps);
 /* The instance's context (owned by the set; do not destroy). */
 tsp_context *tsp_multi_context(tsp_multi *m, int id);
-/* Destroy an instance (remove instances that borrow an operator before its owner). */
+/* Destroy an instance.  TSP_ERR_STATE if it owns an operator that another
+ * instance in the set still borrows (remove the borrowers first). */
 int  tsp_multi_remove(tsp_multi *m, int id);
 int  tsp_multi_synchronize(tsp_multi *m);
+
+/* ------------------------------------------------------------------------
+ * Request loop (no counterpart in the reference).  Simulations arrive over
+ * time; a request whose arrival time has passed is admitted as soon as its
+ * device-memory footprint fits mem_fraction of the device, then runs in the
+ * shared set together with every other admitted request (admission and
+ * retirement between chunks of `chunk` steps; fields stay resident meanwhile);
+ * on completion its seismograms are copied out and its memory released.  A
+ * request is one mesh with 1..16 point sources (one operator assembled, shared
+ * by one simulation per source).  Times are seconds of tsp_server_now().
+ * ------------------------------------------------------------------------ */
+typedef struct tsp_server tsp_server;
+
+typedef struct {
+    int N, ne;                         /* degree, triangles                         */
+    const double *verts;               /* ne*6, as tsp_set_mesh                     */
+    const int *neigh, *neighedge;      /* ne*3                                      */
+    const double *lambda, *mu, *rho;   /* ne                                        */
+    double penalty, dt;
+    int nsteps;                        /* step() calls after the first step         */
+    int nsrc;                          /* 1..16                                     */
+    const int *src_elem;               /* nsrc                                      */
+    const double *src_phi;             /* nsrc*nln                                  */
+    const double *wavelet;             /* (nsteps+1)*nsrc*2: [row][source][x|y]     */
+    int nrec;
+    const int *rec_elem;               /* nrec                                      */
+    const double *rec_phi;             /* nrec*nln                                  */
+    int stride;                        /* receiver sampling stride (>= 1)           */
+} tsp_request;
+
+int    tsp_server_create(int device, double mem_fraction, int chunk, tsp_server **out);
+void   tsp_server_destroy(tsp_server *s);
+const char *tsp_server_last_error(const tsp_server *s);
+double tsp_server_now(void);
+/* Device bytes the loop books for a request while it runs (resident) and the
+ * extra it needs while its operator is assembled (peak).  A request is
+ * admitted when used + resident + peak <= budget. */
+int    tsp_server_footprint(const tsp_request *req, long long *resident, long long *peak);
+/* Queue a request (its arrays are copied) that arrives at time `arrival`.
+ * TSP_ERR_NOMEM if resident + peak exceeds the whole budget. */
+int    tsp_server_submit(tsp_server *s, const tsp_request *req, double arrival, int *id);
+/* One pass of the loop: admit, run one chunk, retire.  Reports how many
+ * requests finished in this pass (completed, or failed at admission: then
+ * tsp_server_result returns TSP_ERR_STATE with the reason in
+ * tsp_server_last_error) and how many are running / queued. */
+int    tsp_server_tick(tsp_server *s, int *completed, int *running, int *queued);
+/* Result of a completed request: seis[((src*nsamples + sample)*nrec + r)*2 + c]
+ * and its arrival / admission / completion times. */
+int    tsp_server_result(tsp_server *s, int id, double *seis, int *nsamples, double *arrival, double *admitted,
+                         double *completed);
+/* Span from the first admission to the last completion, DOF*steps completed,
+ * device bytes held by running requests, and the budget. */
+int    tsp_server_stats(const tsp_server *s, double *busy_seconds, long long *dof_steps, long long *mem_used,
+                        long long *mem_budget);
 
 #ifdef __cplusplus
 }

@@ -451,6 +451,8 @@ def test_multi_shared_operator_and_remove():
     m = capi.Multi(chunk=4)
     ids = [m.add(owner, 60)] + [m.add(e, 60) for e in others]
     m.run(20)
+    with pytest.raises(capi.EngineError):
+        m.remove(ids[0])          # its operator is still borrowed
     m.remove(ids[2])
     while m.run(10):
         pass
@@ -458,3 +460,68 @@ def test_multi_shared_operator_and_remove():
         _, _, s1 = W.run_engine(q, "fp32c")
         np.testing.assert_array_equal(m.engine(i).seismograms(), s1)
     m.close()
+
+
+# ------------------------------------------------------------------ request loop
+def _req(k, nsteps=120, n=(8, 14), nsrc=(1, 4)):
+    return W.config5_request(k, n_range=n, nsrc_range=nsrc, steps_range=(nsteps, nsteps + 40), nrec_range=(6, 12))
+
+
+def test_server_result_is_the_same_alone_and_in_the_mix():
+    """A request's seismograms are identical whether it runs alone or admitted
+    among five others with staggered arrivals; they equal the single-context FP32
+    path source by source, and stay within 1e-4 of the FP64 path."""
+    import copy
+    target = _req(0, nsrc=(3, 3))
+    srv, ids, lat, _ = W.serve([target], rate=0)
+    alone, _ = srv.result(ids[0])
+    srv.close()
+    mix = [_req(k) for k in range(1, 6)]
+    srv, ids, lat, wall = W.serve(mix[:2] + [target] + mix[2:], rate=40.0, seed=3, chunk=8)
+    mixed, t = srv.result(ids[2])
+    say(f"  mix of 6: latencies {np.round(lat, 3).tolist()} s, wall {wall:.2f} s")
+    srv.close()
+    np.testing.assert_array_equal(mixed, alone)
+    for k in range(len(target.src_elem)):
+        q = copy.copy(target)
+        q.src_xy, q.f0, q.t0, q.dirs = target.src_xy[k:k + 1], target.f0[k:k + 1], target.t0[k:k + 1], \
+            target.dirs[k:k + 1]
+        q.__post_init__()
+        _, _, s32 = W.run_engine(q, "fp32c")
+        np.testing.assert_array_equal(alone[k], s32)
+        _, _, s64 = W.run_engine(q, "fp64")
+        err = rel_l2(alone[k], s64)
+        say(f"  source {k}: vs FP64 {err:.3e}")
+        assert err <= 1e-4
+
+
+def test_server_admission_respects_the_memory_budget():
+    """With a budget that holds one request at a time, three requests run one
+    after the other (each admitted no earlier than the previous completion), and
+    all three results equal the request run with an ample budget; a request
+    larger than the budget is refused at submission."""
+    from tspeed_amd import capi as C
+    _, total = ctypes_mem()
+    req = _req(7, nsteps=60, nsrc=(2, 2))
+    res, peak = C.Server.footprint(req)
+    frac = (res + peak + res // 2) / total
+    srv, ids, lat, _ = W.serve([req, req, req], rate=0, mem_fraction=frac, chunk=16)
+    times = sorted((srv.result(i)[1] for i in ids), key=lambda t: t["admitted"])
+    for a, b in zip(times, times[1:]):
+        assert b["admitted"] >= a["completed"]
+    outs = [srv.result(i)[0] for i in ids]
+    with pytest.raises(C.EngineError):
+        srv.submit(_req(8, nsteps=60, n=(30, 30), nsrc=(4, 4)), srv.now())
+    srv.close()
+    srv, ids, _, _ = W.serve([req], rate=0)
+    ref = srv.result(ids[0])[0]
+    srv.close()
+    for o in outs:
+        np.testing.assert_array_equal(o, ref)
+
+
+def ctypes_mem():
+    import ctypes as ct
+    f, t = ct.c_longlong(), ct.c_longlong()
+    assert capi.lib().tsp_device_memory(0, ct.byref(f), ct.byref(t)) == 0
+    return f.value, t.value

@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
 DEVICE_SRC = ["device/assemble.hip", "device/step.hip", "device/spmm.hip", "device/multi.hip", "device/capi.hip"]
-HOST_SRC = ["host/tspeed.cpp", "host/host_capi.cpp"]
+HOST_SRC = ["host/tspeed.cpp", "host/host_capi.cpp", "host/server.cpp"]
 
 # Packed FP32 VALU forms are switched off for every device compile.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

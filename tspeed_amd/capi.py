@@ -69,6 +69,18 @@ SIGNATURES = {
     "tsp_multi_context": (_vp, [_vp, _i]),
     "tsp_multi_remove": (_i, [_vp, _i]),
     "tsp_multi_synchronize": (_i, [_vp]),
+    "tsp_device_memory": (_i, [_i, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
+    "tsp_server_create": (_i, [_i, _d, _i, ctypes.POINTER(_vp)]),
+    "tsp_server_destroy": (None, [_vp]),
+    "tsp_server_last_error": (ctypes.c_char_p, [_vp]),
+    "tsp_server_now": (_d, []),
+    "tsp_server_footprint": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
+    "tsp_server_submit": (_i, [_vp, _vp, _d, ctypes.POINTER(_i)]),
+    "tsp_server_tick": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "tsp_server_result": (_i, [_vp, _i, _vp, ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d),
+                               ctypes.POINTER(_d)]),
+    "tsp_server_stats": (_i, [_vp, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_longlong),
+                              ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     "tsh_gauleg": (_i, [_i, _d, _d, _dp, _dp]),
     "tsh_dubiner": (_i, [_i, _d, _d, _dp, _vp]),
     "tsh_tables": (_i, [_i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -410,3 +422,87 @@ class Multi:
 
     def synchronize(self):
         self._ck(self._L.tsp_multi_synchronize(self._h))
+
+
+class Request(ctypes.Structure):
+    """tsp_request (include/tspeed_amd.h)."""
+    _fields_ = [("N", _i), ("ne", _i), ("verts", _vp), ("neigh", _vp), ("neighedge", _vp), ("lam", _vp),
+                ("mu", _vp), ("rho", _vp), ("penalty", _d), ("dt", _d), ("nsteps", _i), ("nsrc", _i),
+                ("src_elem", _vp), ("src_phi", _vp), ("wavelet", _vp), ("nrec", _i), ("rec_elem", _vp),
+                ("rec_phi", _vp), ("stride", _i)]
+
+
+class Server:
+    """The request loop (tsp_server_*)."""
+
+    def __init__(self, device=0, mem_fraction=0.8, chunk=32):
+        self._L = lib()
+        h = _vp()
+        rc = self._L.tsp_server_create(device, float(mem_fraction), int(chunk), ctypes.byref(h))
+        if rc != TSP_OK:
+            raise EngineError(rc, "tsp_server_create failed")
+        self._h = h
+        self.meta = {}
+
+    def _ck(self, rc):
+        if rc != TSP_OK:
+            raise EngineError(rc, self._L.tsp_server_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.tsp_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def now(self):
+        return self._L.tsp_server_now()
+
+    def submit(self, prob, arrival, stride=1):
+        """Queue Problem `prob` (all its sources: one request) arriving at `arrival`."""
+        arrs = dict(verts=_c64(prob.verts), neigh=_c32i(prob.neigh), neighedge=_c32i(prob.neighedge),
+                    lam=_c64(prob.lam), mu=_c64(prob.mu), rho=_c64(prob.rho), src_elem=_c32i(prob.src_elem),
+                    src_phi=_c64(prob.src_phi), wavelet=_c64(prob.wavelet()), rec_elem=_c32i(prob.rec_elem),
+                    rec_phi=_c64(prob.rec_phi))
+        q = Request(prob.N, prob.ne, arrs["verts"].ctypes.data, arrs["neigh"].ctypes.data,
+                    arrs["neighedge"].ctypes.data, arrs["lam"].ctypes.data, arrs["mu"].ctypes.data,
+                    arrs["rho"].ctypes.data, prob.penalty, prob.dt, prob.nsteps, len(prob.src_elem),
+                    arrs["src_elem"].ctypes.data, arrs["src_phi"].ctypes.data, arrs["wavelet"].ctypes.data,
+                    len(prob.rec_elem), arrs["rec_elem"].ctypes.data, arrs["rec_phi"].ctypes.data, int(stride))
+        i = _i()
+        self._ck(self._L.tsp_server_submit(self._h, ctypes.byref(q), float(arrival), ctypes.byref(i)))
+        self.meta[i.value] = (len(prob.src_elem), len(prob.rec_elem))
+        return i.value
+
+    @staticmethod
+    def footprint(prob, stride=1):
+        """(resident, peak) device bytes the loop books for `prob`."""
+        q = Request(prob.N, prob.ne, None, None, None, None, None, None, prob.penalty, prob.dt, prob.nsteps,
+                    len(prob.src_elem), None, None, None, len(prob.rec_elem), None, None, int(stride))
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        rc = lib().tsp_server_footprint(ctypes.byref(q), ctypes.byref(a), ctypes.byref(b))
+        if rc != TSP_OK:
+            raise EngineError(rc, "tsp_server_footprint: bad request")
+        return a.value, b.value
+
+    def tick(self):
+        c, r, q = _i(), _i(), _i()
+        self._ck(self._L.tsp_server_tick(self._h, ctypes.byref(c), ctypes.byref(r), ctypes.byref(q)))
+        return c.value, r.value, q.value
+
+    def result(self, i):
+        nsrc, nrec = self.meta[i]
+        ns, a, b, c = _i(), _d(), _d(), _d()
+        self._ck(self._L.tsp_server_result(self._h, int(i), None, ctypes.byref(ns), None, None, None))
+        out = np.zeros(max(1, nsrc * ns.value * nrec * 2))
+        self._ck(self._L.tsp_server_result(self._h, int(i), out.ctypes.data, ctypes.byref(ns), ctypes.byref(a),
+                                           ctypes.byref(b), ctypes.byref(c)))
+        return out[: nsrc * ns.value * nrec * 2].reshape(nsrc, ns.value, nrec, 2), dict(
+            arrival=a.value, admitted=b.value, completed=c.value)
+
+    def stats(self):
+        b, d, u, g = _d(), ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        self._ck(self._L.tsp_server_stats(self._h, ctypes.byref(b), ctypes.byref(d), ctypes.byref(u),
+                                          ctypes.byref(g)))
+        return dict(busy_s=b.value, dof_steps=d.value, mem_used=u.value, mem_budget=g.value)

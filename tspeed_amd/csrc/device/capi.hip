@@ -328,6 +328,23 @@ int tsp_device_count(int *n)
     return TSP_OK;
 }
 
+int tsp_device_memory(int device, long long *free_bytes, long long *total_bytes)
+{
+    if (!free_bytes || !total_bytes) return TSP_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) {
+        (void)hipGetLastError();
+        return TSP_ERR_ARG;
+    }
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) {
+        (void)hipGetLastError();
+        return TSP_ERR_HIP;
+    }
+    *free_bytes = (long long)f;
+    *total_bytes = (long long)t;
+    return TSP_OK;
+}
+
 int tsp_create(int device, int precision, int ncols, tsp_context **out)
 {
     if (!out) return TSP_ERR_ARG;
@@ -940,6 +957,7 @@ bool ctx_view(tsp_context *c, InstView *v)
     v->steps = c->steps;
     v->cur = c->cur;
     v->first_done = c->first_done;
+    v->owns_operator = c->blk32 && std::find(c->allocs.begin(), c->allocs.end(), (void *)c->blk32) != c->allocs.end();
     v->stream = c->stream;
     v->meta = c->meta;
     v->extra = c->extra;

@@ -200,6 +200,8 @@ struct tsp_multi {
     std::vector<MInst> host;            // host mirror of the instance table
     MInst *d_inst = nullptr;
     int d_cap = 0;
+    int d_n = 0;                        // entries uploaded by the last rebuild
+    bool stale = false;                 // steps launched since the host mirror was read back
     int2 *d_items = nullptr;
     int nitems = 0, items_cap = 0;
     bool dirty = true;                  // table / items must be rebuilt
@@ -259,13 +261,16 @@ void drop_graph(tsp_multi *m)
 }
 
 // pull the device step counters into the host mirror and the contexts
+// (only the entries the device table holds: slots added since the last
+// rebuild are not on the device yet, and their host record is the truth)
 void sync_progress(tsp_multi *m)
 {
-    if (!m->d_inst || m->host.empty()) return;
-    std::vector<MInst> dev(m->host.size());
+    if (!m->stale || !m->d_inst || m->d_n == 0) return;
+    std::vector<MInst> dev(m->d_n);
     mhip(m, hipMemcpyAsync(dev.data(), m->d_inst, sizeof(MInst) * dev.size(), hipMemcpyDeviceToHost, m->stream),
          "progress");
     mhip(m, hipStreamSynchronize(m->stream), "progress");
+    m->stale = false;
     for (size_t i = 0; i < dev.size(); ++i) {
         if (!m->ctx[i]) continue;
         MInst &h = m->host[i];
@@ -295,6 +300,7 @@ void rebuild(tsp_multi *m)
     }
     if (n) mhip(m, hipMemcpyAsync(m->d_inst, m->host.data(), sizeof(MInst) * n, hipMemcpyHostToDevice, m->stream),
                 "instance table");
+    m->d_n = n;
     // items of running instances, grouped by degree
     std::vector<int2> items;
     std::vector<int> order;
@@ -404,6 +410,7 @@ void advance(tsp_multi *m, int nsteps)
 {
     rebuild(m);
     if (!m->nitems) return;
+    m->stale = true;
     int left = nsteps;
     if (m->chunk > 0 && left >= m->chunk) {
         if (!m->graph || m->graph_chunk != m->chunk) build_graph(m);
@@ -441,7 +448,7 @@ void tsp_multi_destroy(tsp_multi *m)
     (void)hipSetDevice(m->device);
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     drop_graph(m);
-    // contexts that borrow an operator go before their owners: destroy newest first
+    // (a context that borrows an operator never frees it, so the order is free)
     for (size_t i = m->ctx.size(); i-- > 0;)
         if (m->ctx[i]) tsp_destroy(m->ctx[i]);
     if (m->d_inst) (void)hipFree(m->d_inst);
@@ -494,10 +501,19 @@ int tsp_multi_add(tsp_multi *m, tsp_context *c, int nsteps, int *id)
         I.uhi1 = v.uhi[1];
         I.ucomp = v.ucomp;
         I.winc = v.winc;
-        m->host.push_back(I);
-        m->ctx.push_back(c);
+        // reuse the slot of a removed instance, so the table stays as long as
+        // the most instances ever held at once
+        int slot = 0;
+        while (slot < (int)m->ctx.size() && m->ctx[slot]) ++slot;
+        if (slot == (int)m->ctx.size()) {
+            m->host.push_back(I);
+            m->ctx.push_back(c);
+        } else {
+            m->host[slot] = I;
+            m->ctx[slot] = c;
+        }
         m->dirty = true;
-        if (id) *id = (int)m->host.size() - 1;
+        if (id) *id = slot;
     });
 }
 
@@ -620,6 +636,13 @@ int tsp_multi_remove(tsp_multi *m, int id)
 {
     return mguard(m, [&] {
         if (id < 0 || id >= (int)m->host.size() || !m->ctx[id]) mfail(m, TSP_ERR_ARG, "no instance %d", id);
+        InstView v{};
+        ctx_view(m->ctx[id], &v);
+        if (v.owns_operator)
+            for (size_t j = 0; j < m->ctx.size(); ++j)
+                if ((int)j != id && m->ctx[j] && m->host[j].blk == v.blk32)
+                    mfail(m, TSP_ERR_STATE, "instance %zu still uses the operator of instance %d: remove it first", j,
+                          id);
         mhip(m, hipStreamSynchronize(m->stream), "sync");
         sync_progress(m);
         tsp_destroy(m->ctx[id]);

@@ -1,0 +1,371 @@
+// server.cpp -- a request loop over the multi-instance engine: simulations
+// arrive over time, are admitted while their device-memory footprint fits a
+// budget, run together with everything already on the device (continuous
+// batching: admissions and retirements happen between chunks of steps, while
+// the others keep their fields resident), and leave with their seismograms.
+//
+// A request is one mesh with 1..16 point sources; it becomes one owner context
+// that assembles the operator on the device and (nsrc-1) contexts sharing that
+// operator, all added to one tsp_multi set.  Each instance's arithmetic is the
+// same whatever else is in the set, so a request's result does not depend on
+// the mix it ran in.
+//
+// Uses only the public C-ABI (include/tspeed_amd.h, include/tspeed_host.h).
+#include "../../../include/tspeed_amd.h"
+#include "../../../include/tspeed_host.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace {
+
+double clock_s()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Tables {
+    std::vector<double> B, G, Be, Ge, w2, xy2, w1, x1;
+};
+
+struct Request {
+    int id = 0;
+    double arrival = 0, admitted = -1, completed = -1;
+    int N = 0, ne = 0, nsteps = 0, nsrc = 0, nrec = 0, stride = 1;
+    double penalty = 0, dt = 0;
+    std::vector<double> verts, lambda, mu, rho, src_phi, wavelet, rec_phi;
+    std::vector<int> neigh, neighedge, src_elem, rec_elem;
+    long long footprint = 0;
+    std::vector<int> inst;        // multi-set ids, owner first
+    std::vector<double> seis;     // [src][sample][rec][2]
+    int nsamples = 0;
+    int state = 0;                // 0 queued, 1 running, 2 done, 3 failed
+    std::string error;
+};
+
+long long footprint_of(const Request &r)
+{
+    const long long nln = (long long)(r.N + 1) * (r.N + 2) / 2, d = 2 * nln;
+    const long long op32 = (long long)r.ne * 4 * d * d * 4;       // FP32 column-quad operator
+    const long long per_col = (long long)r.ne * d * 4 * 4          // uhi x2, comp, increment
+                              + (long long)r.ne * d * 8 * 3        // staging + layout scratch
+                              + (long long)(r.nsteps + 1) * 16     // force rows
+                              + (long long)(1 + r.nsteps / std::max(r.stride, 1)) * r.nrec * 16;
+    return op32 + r.nsrc * per_col + (long long)r.ne * 64;
+}
+
+long long assembly_peak(const Request &r)
+{
+    const long long nln = (long long)(r.N + 1) * (r.N + 2) / 2, d = 2 * nln;
+    return (long long)r.ne * 4 * d * d * 8 + (long long)r.ne * 200;  // FP64 operator + geometry
+}
+
+}  // namespace
+
+struct tsp_server {
+    int device = 0;
+    tsp_multi *multi = nullptr;
+    long long budget = 0, used = 0;
+    int chunk = 32;
+    std::map<int, Tables> tables;
+    std::vector<Request> req;
+    std::deque<int> queue;   // ids waiting, arrival order
+    std::vector<int> running;
+    std::string err;
+    double t_first = -1, t_last = -1;
+    long long dof_steps = 0;
+    int failed_pending = 0;
+};
+
+namespace {
+
+const Tables &tables_for(tsp_server *s, int N)
+{
+    auto it = s->tables.find(N);
+    if (it != s->tables.end()) return it->second;
+    Tables t;
+    const int nln = (N + 1) * (N + 2) / 2, nq1 = N + 1, nq2 = nq1 * nq1;
+    t.B.resize((size_t)nq2 * nln);
+    t.G.resize((size_t)nln * 2 * nq2);
+    t.Be.resize((size_t)3 * nq1 * nln);
+    t.Ge.resize((size_t)nln * 3 * 2 * nq1);
+    t.w2.resize(nq2);
+    t.xy2.resize(2 * nq2);
+    t.w1.resize(nq1);
+    t.x1.resize(nq1);
+    tsh_tables(N, t.B.data(), t.G.data(), t.Be.data(), t.Ge.data(), t.w2.data(), t.xy2.data(), t.w1.data(),
+               t.x1.data());
+    return s->tables.emplace(N, std::move(t)).first->second;
+}
+
+// build the contexts of one request and add them to the set; on failure every
+// context made so far is destroyed and the request is marked failed
+bool admit(tsp_server *s, Request &r)
+{
+    const Tables &t = tables_for(s, r.N);
+    const int nln = (r.N + 1) * (r.N + 2) / 2;
+    std::vector<tsp_context *> made;
+    auto bail = [&](tsp_context *c, const char *what) {
+        r.error = std::string(what) + ": " + (c ? tsp_last_error(c) : "");
+        for (auto it = made.rbegin(); it != made.rend(); ++it) tsp_destroy(*it);
+        r.state = 3;
+        return false;
+    };
+    tsp_context *owner = nullptr;
+    if (tsp_create(s->device, TSP_FP32C, 1, &owner) != TSP_OK) return bail(nullptr, "create");
+    made.push_back(owner);
+    if (tsp_set_tables(owner, r.N, t.B.data(), t.G.data(), t.Be.data(), t.Ge.data(), t.w2.data(), t.w1.data()))
+        return bail(owner, "tables");
+    if (tsp_set_mesh(owner, r.ne, r.verts.data(), r.neigh.data(), r.neighedge.data(), r.lambda.data(), r.mu.data(),
+                     r.rho.data(), r.penalty))
+        return bail(owner, "mesh");
+    if (tsp_release_fp64(owner)) return bail(owner, "release");
+    for (int k = 1; k < r.nsrc; ++k) {
+        tsp_context *c = nullptr;
+        if (tsp_share_operator(owner, &c) != TSP_OK) return bail(owner, "share");
+        made.push_back(c);
+    }
+    const int rows = r.nsteps + 1;
+    std::vector<double> wav((size_t)rows * 2);
+    for (int k = 0; k < r.nsrc; ++k) {
+        tsp_context *c = made[k];
+        for (int row = 0; row < rows; ++row) {
+            wav[2 * (size_t)row] = r.wavelet[((size_t)row * r.nsrc + k) * 2];
+            wav[2 * (size_t)row + 1] = r.wavelet[((size_t)row * r.nsrc + k) * 2 + 1];
+        }
+        const int col = 0;
+        if (tsp_set_sources(c, 1, &col, &r.src_elem[k], r.src_phi.data() + (size_t)k * nln, rows, wav.data()))
+            return bail(c, "sources");
+        if (tsp_set_receivers(c, r.nrec, r.rec_elem.data(), r.rec_phi.data(), r.stride,
+                              1 + r.nsteps / std::max(r.stride, 1)))
+            return bail(c, "receivers");
+        if (tsp_set_dt(c, r.dt) || tsp_set_state(c, 0, nullptr, nullptr) || tsp_first_step(c))
+            return bail(c, "first step");
+    }
+    for (int k = 0; k < r.nsrc; ++k) {
+        int id = -1;
+        if (tsp_multi_add(s->multi, made[k], r.nsteps, &id) != TSP_OK) {
+            r.error = std::string("add: ") + tsp_multi_last_error(s->multi);
+            // contexts already handed over are removed, the rest destroyed here
+            for (int j = (int)r.inst.size(); j-- > 0;) tsp_multi_remove(s->multi, r.inst[j]);
+            for (int j = (int)made.size(); j-- > k;) tsp_destroy(made[j]);
+            r.inst.clear();
+            r.state = 3;
+            return false;
+        }
+        r.inst.push_back(id);
+    }
+    r.state = 1;
+    return true;
+}
+
+void retire(tsp_server *s, Request &r)
+{
+    r.nsamples = 1 + r.nsteps / std::max(r.stride, 1);
+    const size_t per = (size_t)r.nsamples * r.nrec * 2;
+    r.seis.assign(per * r.nsrc, 0.0);
+    for (int k = 0; k < r.nsrc; ++k) {
+        int ns = 0;
+        tsp_get_seismograms(tsp_multi_context(s->multi, r.inst[k]), r.seis.data() + per * k, r.nsamples, &ns);
+    }
+    // borrowers before the owner
+    for (int k = r.nsrc; k-- > 0;) tsp_multi_remove(s->multi, r.inst[k]);
+    r.inst.clear();
+    r.state = 2;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tsp_server_create(int device, double mem_fraction, int chunk, tsp_server **out)
+{
+    if (!out || !(mem_fraction > 0 && mem_fraction <= 1) || chunk < 1) return TSP_ERR_ARG;
+    *out = nullptr;
+    long long fr = 0, tot = 0;
+    int rc = tsp_device_memory(device, &fr, &tot);
+    if (rc != TSP_OK) return rc;
+    tsp_server *s = new tsp_server;
+    s->device = device;
+    s->budget = (long long)(mem_fraction * (double)tot);
+    s->chunk = chunk;
+    rc = tsp_multi_create(device, &s->multi);
+    if (rc != TSP_OK) {
+        delete s;
+        return rc;
+    }
+    tsp_multi_set_chunk(s->multi, chunk);
+    *out = s;
+    return TSP_OK;
+}
+
+void tsp_server_destroy(tsp_server *s)
+{
+    if (!s) return;
+    tsp_multi_destroy(s->multi);
+    delete s;
+}
+
+const char *tsp_server_last_error(const tsp_server *s) { return s ? s->err.c_str() : "null server"; }
+
+double tsp_server_now(void) { return clock_s(); }
+
+namespace {
+bool request_ok(const tsp_request *q)
+{
+    return q && q->N >= 1 && q->N <= 8 && q->ne >= 1 && q->nsteps >= 0 && q->nsrc >= 1 && q->nsrc <= 16 &&
+           q->nrec >= 0 && q->stride >= 1 && q->dt > 0;
+}
+Request sizes_of(const tsp_request *q)
+{
+    Request r;
+    r.N = q->N;
+    r.ne = q->ne;
+    r.nsteps = q->nsteps;
+    r.nsrc = q->nsrc;
+    r.nrec = q->nrec;
+    r.stride = q->stride;
+    return r;
+}
+}  // namespace
+
+int tsp_server_footprint(const tsp_request *q, long long *resident, long long *peak)
+{
+    if (!request_ok(q)) return TSP_ERR_ARG;
+    const Request r = sizes_of(q);
+    if (resident) *resident = footprint_of(r);
+    if (peak) *peak = assembly_peak(r);
+    return TSP_OK;
+}
+
+int tsp_server_submit(tsp_server *s, const tsp_request *q, double arrival, int *id)
+{
+    if (!s || !q) return TSP_ERR_ARG;
+    const int nln = (q->N + 1) * (q->N + 2) / 2;
+    if (!request_ok(q)) {
+        s->err = "bad request";
+        return TSP_ERR_ARG;
+    }
+    Request r = sizes_of(q);
+    r.id = (int)s->req.size();
+    r.arrival = arrival;
+    r.penalty = q->penalty;
+    r.dt = q->dt;
+    r.verts.assign(q->verts, q->verts + 6 * (size_t)q->ne);
+    r.neigh.assign(q->neigh, q->neigh + 3 * (size_t)q->ne);
+    r.neighedge.assign(q->neighedge, q->neighedge + 3 * (size_t)q->ne);
+    r.lambda.assign(q->lambda, q->lambda + q->ne);
+    r.mu.assign(q->mu, q->mu + q->ne);
+    r.rho.assign(q->rho, q->rho + q->ne);
+    r.src_elem.assign(q->src_elem, q->src_elem + q->nsrc);
+    r.src_phi.assign(q->src_phi, q->src_phi + (size_t)q->nsrc * nln);
+    r.wavelet.assign(q->wavelet, q->wavelet + (size_t)(q->nsteps + 1) * q->nsrc * 2);
+    r.rec_elem.assign(q->rec_elem, q->rec_elem + q->nrec);
+    r.rec_phi.assign(q->rec_phi, q->rec_phi + (size_t)q->nrec * nln);
+    r.footprint = footprint_of(r);
+    if (r.footprint + assembly_peak(r) > s->budget) {
+        s->err = "request does not fit the device-memory budget";
+        return TSP_ERR_NOMEM;
+    }
+    s->req.push_back(std::move(r));
+    // keep the queue in arrival order
+    auto pos = std::upper_bound(s->queue.begin(), s->queue.end(), arrival,
+                                [&](double a, int i) { return a < s->req[i].arrival; });
+    s->queue.insert(pos, (int)s->req.size() - 1);
+    if (id) *id = (int)s->req.size() - 1;
+    return TSP_OK;
+}
+
+int tsp_server_tick(tsp_server *s, int *completed, int *running, int *queued)
+{
+    if (!s) return TSP_ERR_ARG;
+    const double now = clock_s();
+    // admit every arrived request that fits (first fit in arrival order)
+    for (auto it = s->queue.begin(); it != s->queue.end();) {
+        Request &r = s->req[*it];
+        if (r.arrival > now) break;
+        if (s->used + r.footprint + assembly_peak(r) > s->budget) {
+            ++it;
+            continue;
+        }
+        int failed = 0;
+        if (admit(s, r)) {
+            s->used += r.footprint;
+            r.admitted = clock_s();
+            s->running.push_back(r.id);
+            if (s->t_first < 0) s->t_first = r.admitted;
+        } else {
+            r.completed = clock_s();   // leaves with its error (tsp_server_result reports it)
+            failed = 1;
+        }
+        it = s->queue.erase(it);
+        s->failed_pending += failed;
+    }
+    int done = s->failed_pending;
+    s->failed_pending = 0;
+    if (!s->running.empty()) {
+        int active = 0;
+        if (tsp_multi_run(s->multi, s->chunk, &active) != TSP_OK) {
+            s->err = tsp_multi_last_error(s->multi);
+            return TSP_ERR_HIP;
+        }
+        const double t = clock_s();
+        for (auto it = s->running.begin(); it != s->running.end();) {
+            Request &r = s->req[*it];
+            int steps = 0, n = 0;
+            tsp_multi_info(s->multi, r.inst[0], &steps, &n);
+            if (steps >= n) {
+                retire(s, r);
+                r.completed = t;
+                s->used -= r.footprint;
+                s->t_last = t;
+                s->dof_steps += (long long)r.ne * (r.N + 1) * (r.N + 2) * r.nsrc * r.nsteps;
+                ++done;
+                it = s->running.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    if (completed) *completed = done;
+    if (running) *running = (int)s->running.size();
+    if (queued) *queued = (int)s->queue.size();
+    return TSP_OK;
+}
+
+int tsp_server_result(tsp_server *s, int id, double *seis, int *nsamples, double *arrival, double *admitted,
+                      double *completed)
+{
+    if (!s || id < 0 || id >= (int)s->req.size()) return TSP_ERR_ARG;
+    const Request &r = s->req[id];
+    if (r.state == 3) {
+        s->err = r.error;
+        return TSP_ERR_STATE;
+    }
+    if (r.state != 2) return TSP_ERR_STATE;
+    if (nsamples) *nsamples = r.nsamples;
+    if (seis) std::memcpy(seis, r.seis.data(), sizeof(double) * r.seis.size());
+    if (arrival) *arrival = r.arrival;
+    if (admitted) *admitted = r.admitted;
+    if (completed) *completed = r.completed;
+    return TSP_OK;
+}
+
+int tsp_server_stats(const tsp_server *s, double *busy_seconds, long long *dof_steps, long long *mem_used,
+                     long long *mem_budget)
+{
+    if (!s) return TSP_ERR_ARG;
+    if (busy_seconds) *busy_seconds = (s->t_first >= 0 && s->t_last >= 0) ? s->t_last - s->t_first : 0.0;
+    if (dof_steps) *dof_steps = s->dof_steps;
+    if (mem_used) *mem_used = s->used;
+    if (mem_budget) *mem_budget = s->budget;
+    return TSP_OK;
+}
+
+}  // extern "C"

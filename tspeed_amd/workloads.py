@@ -16,6 +16,8 @@ from __future__ import annotations
 
 from dataclasses import dataclass, field
 
+import time
+
 import numpy as np
 
 from . import capi
@@ -216,6 +218,59 @@ def config4_instance(k, seed=4, ne_range=(500, 20000), p_range=(2, 8), steps_ran
                    cfl_dt(pts, tri, N, vp), nsteps, src, np.array([f0]), np.array([1.2 / f0]),
                    np.array([[np.cos(ang), np.sin(ang)]]), np.stack([rx, np.full(nrec, Ly - 0.5 * h)], 1), 1,
                    dict(nx=nx, ny=ny, seed=seed))
+
+
+def config5_request(k, seed=5, n_range=(20, 40), degrees=(3, 4, 5), nsrc_range=(1, 16), steps_range=(500, 5000),
+                    nrec_range=(16, 128)):
+    """Request k of the serving workload, drawn like the multi-source workload
+    (1000 m x 1000 m split squares, sources in the top 10%, receivers along
+    y = 990) with per-request mesh size n x n (n uniform in n_range), degree,
+    homogeneous materials (vs ~ U(500,3000), vp/vs ~ U(1.6,2.0),
+    rho ~ U(1600,2800)), 1..16 Ricker sources (f0 ~ U(5,15), random directions),
+    step count and 16..128 receivers."""
+    rng = np.random.default_rng([seed, k])
+    n = int(rng.integers(n_range[0], n_range[1] + 1))
+    N = int(rng.choice(degrees))
+    nsrc = int(rng.integers(nsrc_range[0], nsrc_range[1] + 1))
+    nsteps = int(rng.integers(steps_range[0], steps_range[1] + 1))
+    nrec = int(rng.integers(nrec_range[0], nrec_range[1] + 1))
+    vs = rng.uniform(500.0, 3000.0)
+    vp = vs * rng.uniform(1.6, 2.0)
+    rho = rng.uniform(1600.0, 2800.0)
+    L = 1000.0
+    pts, tri = capi.split_square(n, n, 0.0, 0.0, L, L, 0.2, 5000 + k)
+    ne = tri.shape[0]
+    lam, mu = lame(rho, vp, vs)
+    xy = np.stack([rng.uniform(0.01 * L, 0.99 * L, nsrc), rng.uniform(0.9 * L, 0.99 * L, nsrc)], 1)
+    f0 = rng.uniform(5.0, 15.0, nsrc)
+    ang = rng.uniform(0.0, 2 * np.pi, nsrc)
+    rx = (np.arange(nrec) + 0.5) * L / nrec
+    return Problem(f"config5[{k}]", N, pts, tri, np.full(ne, lam), np.full(ne, mu), np.full(ne, rho), PENALTY,
+                   cfl_dt(pts, tri, N, vp), nsteps, xy, f0, 1.2 / f0, np.stack([np.cos(ang), np.sin(ang)], 1),
+                   np.stack([rx, np.full(nrec, 990.0)], 1), 1, dict(n=n, seed=seed))
+
+
+def serve(requests, rate, seed=0, mem_fraction=0.8, chunk=32, stride=1, server=None):
+    """Submit `requests` (Problems) with Poisson arrivals at `rate` per second and
+    run the request loop until all complete.  Returns (server, ids, latencies [s],
+    wall seconds)."""
+    srv = server or capi.Server(mem_fraction=mem_fraction, chunk=chunk)
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate, len(requests)) if rate > 0 else np.zeros(len(requests))
+    t0 = srv.now() + 0.05
+    ids = [srv.submit(p, t0 + float(a), stride=stride) for p, a in zip(requests, np.cumsum(gaps))]
+    pending = len(ids)
+    while pending:
+        c, r, q = srv.tick()
+        pending -= c
+        if r == 0 and q == 0 and pending:
+            raise RuntimeError("request loop idle with requests outstanding")
+        if r == 0 and q > 0 and c == 0:
+            time.sleep(0.001)          # nothing running: wait for the next arrival
+    lat = []
+    for i in ids:
+        _, t = srv.result(i)
+        lat.append(t["completed"] - t["arrival"])
+    return srv, ids, np.array(lat), srv.now() - t0
 
 
 def prepare_instance(prob: Problem, share=None, stride=None, nsteps=None):
