@@ -215,22 +215,35 @@ void alloc_state(tsp_context *c)
         for (int k = 0; k < 2; ++k)
             if (!c->u64[k]) c->u64[k] = dalloc<double>(c, n, "state");
     } else {
+        const bool fresh = !c->uhi[0];
         for (int k = 0; k < 2; ++k)
             if (!c->uhi[k]) c->uhi[k] = dalloc<float>(c, n, "state hi");
         if (!c->ucomp) c->ucomp = dalloc<float>(c, n, "state comp");
         if (!c->winc) c->winc = dalloc<float>(c, n, "state inc");
         if (!c->u64[0]) c->u64[0] = dalloc<double>(c, n, "state in");
         if (!c->u64[1]) c->u64[1] = dalloc<double>(c, n, "state in");
+        if (fresh && c->batched()) {
+            // the multi-column step reads and writes whole row octets: the
+            // padded rows must start (and so stay) zero in every buffer
+            for (int k = 0; k < 2; ++k) {
+                hipcheck(c, hipMemsetAsync(c->uhi[k], 0, n * sizeof(float), c->stream), "memset");
+                hipcheck(c, hipMemsetAsync(c->u64[k], 0, n * sizeof(double), c->stream), "memset");
+            }
+            hipcheck(c, hipMemsetAsync(c->ucomp, 0, n * sizeof(float), c->stream), "memset");
+            hipcheck(c, hipMemsetAsync(c->winc, 0, n * sizeof(float), c->stream), "memset");
+        }
     }
 }
 
 void operator_ready(tsp_context *c)
 {
+    // neighbour table on the device first: the multi-column packing reads it
+    rebuild_extra(c);
     // FP32 copy of the operator for the compensated path
     if (c->batched()) {
         c->KP = batch_kp(c->P);
         if (!c->blkA) c->blkA = dalloc<float>(c, (size_t)c->ne * batch_block_floats(c->P), "blocks mfma");
-        kcheck(c, launch_pack_mfma(c->P, c->ne, c->blk64, c->blkA, c->stream), "pack");
+        kcheck(c, launch_pack_mfma(c->P, c->ne, c->meta, c->blk64, c->blkA, c->stream), "pack");
     } else if (c->precision == TSP_FP32C) {
         const size_t n = (size_t)c->ne * 4 * c->D * c->D;
         if (!c->blk32) c->blk32 = dalloc<float>(c, n, "blocks fp32");
@@ -247,7 +260,7 @@ void operator_ready(tsp_context *c)
     c->steps = 0;
     c->first_done = false;
     c->nsamples = 0;
-    rebuild_extra(c);
+    hipcheck(c, hipStreamSynchronize(c->stream), "operator");
 }
 
 StepArgs base_args(tsp_context *c)
@@ -668,8 +681,6 @@ int tsp_first_step(tsp_context *c)
             kcheck(c, launch_scale_demote(c->ne, c->invm, -(dt * dt / 2), c->sc_first, c->stream), "sc");
             kcheck(c, launch_scale_demote(c->ne, c->invm, (-dt) * dt, c->sc_step, c->stream), "sc");
             kcheck(c, launch_scale_demote(n, c->u64[1], dt, c->winc, c->stream), "winc");
-            // the step writes rows < d only; keep the padded rows of the other buffer zero
-            hipcheck(c, hipMemsetAsync(c->uhi[1], 0, n * sizeof(float), c->stream), "memset");
             BatchArgs b = batch_args(c);
             b.wav_row = a.wav_row;
             b.sample = a.sample;

@@ -1,23 +1,29 @@
 // spmm.hip -- many right-hand sides on one mesh: the leap-frog step as a block
 // SpMM on the MFMA cores (FP32, compensated update).
 //
-// R columns (R = 32*NT) share the operator; each column is an independent
+// R columns (R = 32 or 64) share the operator; each column is an independent
 // simulation with its own point source, i.e. the reference's LeapFrog::step
 // (TimeAdvance.cpp:41-59) applied to R vectors at once.  Per element the step is
 //   Y (d x R) = sum_s B_{e,s} (d x d) * U^n_{nb(e,s)} (d x R)
 // computed with v_mfma_f32_32x32x2f32 in 32x32 output tiles (MT row tiles of
-// the block, NT column tiles of R), followed by the same compensated update as
-// the single-column FP32 path and the receiver dot products of every column.
+// the block, NT column tiles per wave), followed by the same compensated update
+// as the single-column FP32 path and the receiver dot products of every column.
 //
-// Layouts (rows padded to KP = 32*MT with zeros):
-//   state     u[(e*KP + k)*R + j]            row k of element e, column j
-//   operator  A-fragment order: for block (e,s), row tile mt and k-block kb
-//             (8 columns = 4 MFMA k-steps) 64 lanes x 4 floats, where lane l,
-//             slot t holds B_{e,s}(32*mt + l%32, 8*kb + 2t + l/32); one 16-byte
-//             non-temporal load per lane feeds four MFMAs.
 // MFMA operand/result layout of v_mfma_f32_32x32x2f32 (wave64): A lane l holds
 // A[l%32][l/32], B lane l holds B[l/32][l%32], result register r of lane l
 // holds C[8*(r/4) + 4*(l/32) + r%4][l%32].
+//
+// Layouts (rows padded to KP = 32*MT with zeros; KB = KP/8 row octets):
+//   state     u[((e*KB + i/8)*R + j)*8 + i%8]    row i of element e, column j.
+//             A lane's four result registers 4g..4g+3 are rows 8g+4h..8g+4h+3
+//             (h = lane/32) of one column: 16 contiguous bytes.
+//   contraction order: in k-octet kb, MFMA step t pairs rows 8kb+t (lanes
+//             0-31) and 8kb+4+t (lanes 32-63), so the four B operands of a
+//             lane over t = 0..3 are again 16 contiguous bytes of the state.
+//   operator  A-fragment order: for block (e,s), row tile mt and k-octet kb,
+//             64 lanes x 4 floats, lane l / slot t holding
+//             B_{e,s}(32*mt + l%32, 8*kb + 4*(l/32) + t); one 16-byte
+//             non-temporal load per lane feeds four MFMAs.
 #include "engine.hpp"
 
 #pragma clang fp contract(off)
@@ -45,17 +51,26 @@ struct BGeom {
     static constexpr int KB = KP / 8;         // k-blocks per block
 };
 
-template <int P, int NT>
+// One wave computes one element's rows for NT column tiles (WPE waves share an
+// element, each with its own NT*32 columns).  A missing neighbour's block is
+// zero in the packed operator, so its slot multiplies the element's own
+// columns by zero and the slot loop needs no branch.  Padded rows stay zero:
+// their operator rows and columns are zero and they carry no source.
+template <int P, int NT, int WPE, int PD>
 __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 {
     using G = BGeom<P>;
-    constexpr int R = 32 * NT, MT = G::MT, KP = G::KP, KB = G::KB, D = G::D, NLN = G::NLN;
+    constexpr int MT = G::MT, KB = G::KB, D = G::D, NLN = G::NLN;
+    constexpr int R = 32 * NT * WPE;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int e = xcd_remap_b(blockIdx.x, gridDim.x) * kBWaves + wave;
+    const int gw = xcd_remap_b(blockIdx.x, gridDim.x) * kBWaves + wave;
+    const int e = gw / WPE, c0 = (gw % WPE) * NT * 32;
     if (e >= a.ne) return;  // no block-level barrier in this kernel
     const int half = lane >> 5, jj = lane & 31;
     const int4 m = a.meta[e];
-    const int nbs[4] = {e, m.x, m.y, m.z};
+    const int nbs[4] = {e, m.x >= 0 ? m.x : e, m.y >= 0 ? m.y : e, m.z >= 0 ? m.z : e};
+    // this lane's 16-byte piece of column c0 + nt*32 + jj in any row octet
+    const int lofs = (c0 + jj) * 8 + 4 * half;
 
     f32x16 acc[MT][NT];
 #pragma unroll
@@ -63,32 +78,65 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x16{};
 
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        if (nbs[s] < 0) continue;
+    // operands of PD slots ahead are in flight while a slot's products run
+    constexpr int NB = PD + 1;
+    f32x4 av[NB][KB][MT];
+    f32x4 bv[NB][KB][NT];
+    // the element's own columns (slot 0) are also u^n of the update: kept
+    f32x4 own[KB][NT];
+    // increment and compensation of the update, loaded with the last slot
+    f32x4 wv[KB][NT], cp[KB][NT];
+    const size_t ebase = (size_t)e * KB * R * 8 + lofs;
+    auto load = [&](int s, int buf) {
         const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * MT * KB * 256);
-        const float *X = a.uhi + (size_t)nbs[s] * KP * R + jj;
+        const float *X = a.uhi + (size_t)nbs[s] * KB * R * 8 + lofs;
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb) {
-            f32x4 av[MT];
+        for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-            for (int mt = 0; mt < MT; ++mt) av[mt] = __builtin_nontemporal_load(Af + (mt * KB + kb) * 64 + lane);
+            for (int mt = 0; mt < MT; ++mt) av[buf][kb][mt] = __builtin_nontemporal_load(Af + (mt * KB + kb) * 64 + lane);
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int k = kb * 8 + 2 * t + half;
-                float bv[NT];
+        for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt) bv[nt] = k < D ? X[k * R + nt * 32] : 0.f;
+            for (int nt = 0; nt < NT; ++nt) {
+                const f32x4 x = *reinterpret_cast<const f32x4 *>(X + (size_t)kb * R * 8 + nt * 256);
+                if (s == 0) own[kb][nt] = x; else bv[buf][kb][nt] = x;
+            }
+        if (s == 3) {
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const size_t idx = ebase + (size_t)kb * R * 8 + nt * 256;
+                    wv[kb][nt] = *reinterpret_cast<const f32x4 *>(a.winc + idx);
+                    cp[kb][nt] = *reinterpret_cast<const f32x4 *>(a.ucomp + idx);
+                }
+        }
+    };
+    auto product = [&](int s, int buf) {
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                     for (int nt = 0; nt < NT; ++nt)
-                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[mt][t], bv[nt], acc[mt][nt], 0, 0, 0);
-            }
-        }
+                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                            av[buf][kb][mt][t], s == 0 ? own[kb][nt][t] : bv[buf][kb][nt][t], acc[mt][nt], 0, 0, 0);
+    };
+#pragma unroll
+    for (int s = 0; s < PD && s < 4; ++s) load(s, s % NB);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (s + PD < 4) load(s + PD, (s + PD) % NB);
+        // keep the issue order: the scheduler would otherwise sink the loads
+        // next to their first use
+        if constexpr (PD > 0) __builtin_amdgcn_sched_barrier(0);
+        product(s, s % NB);
+        if constexpr (PD > 0) __builtin_amdgcn_sched_barrier(0);
     }
 
-    // compensated update of every (row, column) the lane holds
+    // compensated update, four rows of one column per 16-byte access
     const int4 ex = m.w >= 0 ? a.extra[m.w] : make_int4(0, 0, 0, 0);
     const float sc = a.sc32[e];
     bool bad = false;
@@ -97,30 +145,43 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int i = mt * 32 + 8 * (r / 4) + 4 * half + (r % 4);
-                const int j = nt * 32 + jj;
-                if (i >= D) continue;
-                const size_t idx = ((size_t)e * KP + i) * R + j;
-                double f = 0.0;
+            for (int g = 0; g < 4; ++g) {
+                const int i0 = mt * 32 + 8 * g + 4 * half;
+                if (mt * 32 + 8 * g >= D) continue;  // octet entirely padding
+                const int j = c0 + nt * 32 + jj;
+                const int kb = mt * 4 + g;
+                const size_t idx = ebase + (size_t)kb * R * 8 + nt * 256;
+                double fd[4] = {0.0, 0.0, 0.0, 0.0};
                 for (int k = ex.x; k < ex.y; ++k) {
                     const int src = a.src_sorted[k];
-                    if (a.src_col[src] == j)
-                        f += a.wav_row[src * 2 + i / NLN] * a.src_phi[src * NLN + i % NLN];
+                    if (a.src_col[src] != j) continue;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int i = i0 + q;
+                        if (i < D) fd[q] += a.wav_row[src * 2 + i / NLN] * a.src_phi[src * NLN + i % NLN];
+                    }
                 }
-                const float w = a.winc[idx] + sc * (acc[mt][nt][r] - (float)f);
-                const float uh = a.uhi[idx];
-                const float y = w - a.ucomp[idx];
-                const float t = uh + y;
-                a.ucomp[idx] = (t - uh) - y;
-                a.uhi_out[idx] = t;
-                a.winc[idx] = w;
-                acc[mt][nt][r] = t;
-                bad |= !isfinite(t);
+                const f32x4 f = {(float)fd[0], (float)fd[1], (float)fd[2], (float)fd[3]};
+                const f32x4 uh = own[kb][nt];
+                f32x4 wn, tn, cn;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float w = wv[kb][nt][q] + sc * (acc[mt][nt][4 * g + q] - f[q]);
+                    const float y = w - cp[kb][nt][q];
+                    const float t = uh[q] + y;
+                    cn[q] = (t - uh[q]) - y;
+                    tn[q] = t;
+                    wn[q] = w;
+                    acc[mt][nt][4 * g + q] = t;
+                    bad |= !isfinite(t);
+                }
+                *reinterpret_cast<f32x4 *>(a.ucomp + idx) = cn;
+                *reinterpret_cast<f32x4 *>(a.uhi_out + idx) = tn;
+                *reinterpret_cast<f32x4 *>(a.winc + idx) = wn;
             }
     if (bad) atomicOr(a.flags, 1u);
 
-    // receivers: every receiver records all R columns
+    // receivers: every receiver records all R columns (this wave's NT tiles)
     if (a.sample >= 0) {
         for (int k = ex.z; k < ex.w; ++k) {
             const int rec = a.rec_sorted[k];
@@ -139,7 +200,7 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
                 sx += __shfl_xor(sx, 32);
                 sy += __shfl_xor(sy, 32);
                 if (half == 0) {
-                    const size_t o = (((size_t)a.sample * a.nrec + rec) * R + nt * 32 + jj) * 2;
+                    const size_t o = (((size_t)a.sample * a.nrec + rec) * R + c0 + nt * 32 + jj) * 2;
                     a.seis[o] = sx;
                     a.seis[o + 1] = sy;
                 }
@@ -148,9 +209,10 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
     }
 }
 
-// FP64 column-pair operator -> FP32 A-fragment order with zero padding.
+// FP64 column-pair operator -> FP32 A-fragment order with zero padding; the
+// blocks of missing neighbours are written as zeros.
 template <int P>
-__global__ void k_pack_mfma(int ne, const double *src, float *dst)
+__global__ void k_pack_mfma(int ne, const int4 *meta, const double *src, float *dst)
 {
     using G = BGeom<P>;
     constexpr int D = G::D, MT = G::MT, KB = G::KB;
@@ -161,10 +223,18 @@ __global__ void k_pack_mfma(int ne, const double *src, float *dst)
     const int rem = (int)(q % per);
     const int frag = rem / 256, l = (rem % 256) / 4, t = rem % 4;
     const int mt = frag / KB, kb = frag % KB;
-    const int row = mt * 32 + l % 32, col = kb * 8 + 2 * t + l / 32;
+    const int row = mt * 32 + l % 32, col = kb * 8 + 4 * (l / 32) + t;
+    const int slot = (int)(b % 4);
+    const int4 mm = meta[b / 4];
+    const int nb = slot == 0 ? 0 : (slot == 1 ? mm.x : (slot == 2 ? mm.y : mm.z));
     float v = 0.f;
-    if (row < D && col < D) v = (float)src[b * D * D + (size_t)(col / 2) * 2 * D + row * 2 + (col & 1)];
+    if (row < D && col < D && nb >= 0) v = (float)src[b * D * D + (size_t)(col / 2) * 2 * D + row * 2 + (col & 1)];
     dst[q] = v;
+}
+
+__device__ inline size_t oct_index(int e, int i, int j, int KB, int R)
+{
+    return (((size_t)e * KB + i / 8) * R + j) * 8 + i % 8;
 }
 
 // Receiver sample of the stored state (eval_receivers): one wave per receiver.
@@ -178,7 +248,7 @@ __global__ __launch_bounds__(64) void k_sample_b32(BatchArgs a, int R)
     for (int j = threadIdx.x; j < R; j += 64) {
         double sx = 0.0, sy = 0.0;
         for (int i = 0; i < G::D; ++i) {
-            const size_t idx = ((size_t)e * G::KP + i) * R + j;
+            const size_t idx = oct_index(e, i, j, G::KB, R);
             const double u = (double)a.uhi[idx] - (double)a.ucomp[idx];
             const double v = a.rec_phi[rec * G::NLN + i % G::NLN] * u;
             if (i < G::NLN) sx += v; else sy += v;
@@ -199,7 +269,7 @@ __global__ void k_col_in(int ne, int nln, int KP, int R, int col, const double *
     const size_t rem = q % ((size_t)ne * nln);
     const size_t e = rem / nln;
     const int i = (int)(rem % nln);
-    st[(e * KP + c * nln + i) * R + col] = cm ? cm[q] : 0.0;
+    st[oct_index((int)e, c * nln + i, col, KP / 8, R)] = cm ? cm[q] : 0.0;
 }
 
 __global__ void k_col_out(int ne, int nln, int KP, int R, int col, const float *hi, const float *comp, double *cm)
@@ -211,24 +281,34 @@ __global__ void k_col_out(int ne, int nln, int KP, int R, int col, const float *
     const size_t rem = q % ((size_t)ne * nln);
     const size_t e = rem / nln;
     const int i = (int)(rem % nln);
-    const size_t idx = (e * KP + c * nln + i) * R + col;
+    const size_t idx = oct_index((int)e, c * nln + i, col, KP / 8, R);
     cm[q] = (double)hi[idx] - (double)comp[idx];
 }
 
-template <int P, int NT>
+// Launch shape per degree: one 32-column tile per wave (R/32 waves per
+// element); the operands of the next slot are prefetched while a slot's
+// products run when the registers allow it (up to two row tiles).
+#ifndef TSP_B32_PD
+#define TSP_B32_PD -1
+#endif
+
+template <int P, int R>
 int step_b32_p(const BatchArgs &a, hipStream_t s)
 {
-    const unsigned nwg = (unsigned)((a.ne + kBWaves - 1) / kBWaves);
-    hipLaunchKernelGGL((k_step_b32<P, NT>), dim3(nwg), dim3(64 * kBWaves), 0, s, a);
+    constexpr int WPE = R / 32, NT = 1;
+    constexpr int PD = TSP_B32_PD >= 0 ? TSP_B32_PD : (BGeom<P>::MT <= 2 ? 1 : 0);
+    const size_t waves = (size_t)a.ne * WPE;
+    const unsigned nwg = (unsigned)((waves + kBWaves - 1) / kBWaves);
+    hipLaunchKernelGGL((k_step_b32<P, NT, WPE, PD>), dim3(nwg), dim3(64 * kBWaves), 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int P>
-int pack_p(int ne, const double *src, float *dst, hipStream_t s)
+int pack_p(int ne, const int4 *meta, const double *src, float *dst, hipStream_t s)
 {
     using G = BGeom<P>;
     const size_t n = (size_t)ne * 4 * G::MT * G::KB * 256;
-    hipLaunchKernelGGL(k_pack_mfma<P>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ne, src, dst);
+    hipLaunchKernelGGL(k_pack_mfma<P>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ne, meta, src, dst);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -256,21 +336,21 @@ size_t batch_block_floats(int P) { return (size_t)4 * batch_kp(P) / 32 * (batch_
 int launch_step_b32(int P, int ncols, const BatchArgs &a, hipStream_t s)
 {
     if (ncols == 32) {
-#define CALL(p) step_b32_p<p, 1>(a, s)
+#define CALL(p) step_b32_p<p, 32>(a, s)
         TSP_DISPATCH_P(P, CALL)
 #undef CALL
     }
     if (ncols == 64) {
-#define CALL(p) step_b32_p<p, 2>(a, s)
+#define CALL(p) step_b32_p<p, 64>(a, s)
         TSP_DISPATCH_P(P, CALL)
 #undef CALL
     }
     return -2;
 }
 
-int launch_pack_mfma(int P, int ne, const double *src, float *dst, hipStream_t s)
+int launch_pack_mfma(int P, int ne, const int4 *meta, const double *src, float *dst, hipStream_t s)
 {
-#define CALL(p) pack_p<p>(ne, src, dst, s)
+#define CALL(p) pack_p<p>(ne, meta, src, dst, s)
     TSP_DISPATCH_P(P, CALL)
 #undef CALL
 }
