@@ -63,12 +63,12 @@ def run_variant(prob, precision, steps, warmup):
     return dict(ms_total=ms, wall_s=wall, setup_s=setup_s, status=status, device_bytes=dev_bytes)
 
 
-def run_batched(steps, warmup, ncols=64):
+def run_batched(steps, warmup, ncols=64, precision="fp32c"):
     """Config 3: config 1's mesh at p=4 with 64 sources stepped as 64 columns of one
-    block SpMM on the MFMA cores (FP32, compensated update)."""
+    block SpMM on the MFMA cores (FP32 or BF16x3 products, compensated update)."""
     from tspeed_amd import capi, workloads as W
     prob = W.config3(nsrc=ncols, nsteps=steps + warmup)
-    eng = capi.Engine("fp32c", ncols=ncols)
+    eng = capi.Engine(precision, ncols=ncols)
     eng.set_tables(prob.N)
     eng.set_mesh(prob.verts, prob.neigh, prob.neighedge, prob.lam, prob.mu, prob.rho, prob.penalty)
     eng.set_sources(prob.src_elem, prob.src_phi, prob.wavelet(), col=np.arange(ncols))
@@ -90,6 +90,7 @@ def run_batched(steps, warmup, ncols=64):
     flops_alg = 2.0 * nblk * d * d * ncols          # algorithmic block products per step
     flops_mfma = 2.0 * nblk * kp * kp * ncols       # as issued on the padded 32x32 tiles
     return dict(value=prob.ndof * ncols * steps / (ms / 1e3), ms_per_step=ms_step, columns=ncols,
+                precision=precision,
                 workload="config3: config-1 mesh, p=4, 64 Ricker sources as 64 columns, 128 receivers",
                 tflops_algorithmic=flops_alg / (ms_step / 1e3) / 1e12,
                 tflops_mfma_issued=flops_mfma / (ms_step / 1e3) / 1e12, mfma_peak_tflops=157.3,
@@ -245,6 +246,7 @@ def main():
 
     if not args.no_batched:
         variants["batched_fp32_64col"] = run_batched(args.steps, args.warmup)
+        variants["batched_bf16x3_64col"] = run_batched(args.steps, args.warmup, precision="bf16x3")
     if args.multi > 0:
         variants["multi_fp32c"] = run_multi(min(args.steps, 200), min(args.warmup, 10), args.multi)
     if args.serve > 0:

@@ -33,8 +33,10 @@ enum tsp_status_code {
 };
 
 enum tsp_precision {
-    TSP_FP64 = 0,  /* plain FP64 blocks and update: the correctness path             */
-    TSP_FP32C = 1  /* FP32 blocks, FP32 increments, Kahan-compensated displacement  */
+    TSP_FP64 = 0,   /* plain FP64 blocks and update: the correctness path            */
+    TSP_FP32C = 1,  /* FP32 blocks, FP32 increments, Kahan-compensated displacement */
+    TSP_BF16X3 = 2  /* as TSP_FP32C, with each block product split into three BF16
+                       products accumulated in FP32 (32/64 columns only)            */
 };
 
 /* Status bits reported by tsp_status (no counterpart in the reference, which has
@@ -52,7 +54,7 @@ int tsp_device_memory(int device, long long *free_bytes, long long *total_bytes)
  * of tsp_precision; `ncols` the number of right-hand-side columns: 1 = the
  * reference's single simulation; 32 or 64 = that many simulations on one mesh
  * (one point source each), stepped together as a block SpMM on the MFMA cores
- * (TSP_FP32C only). */
+ * (TSP_FP32C or TSP_BF16X3). */
 int  tsp_create(int device, int precision, int ncols, tsp_context **out);
 void tsp_destroy(tsp_context *ctx);
 const char *tsp_last_error(const tsp_context *ctx);

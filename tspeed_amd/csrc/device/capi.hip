@@ -235,6 +235,12 @@ void alloc_state(tsp_context *c)
     }
 }
 
+int launch_batched(tsp_context *c, const BatchArgs &b)
+{
+    return c->precision == TSP_BF16X3 ? launch_step_b16x3(c->P, c->ncols, b, c->stream)
+                                       : launch_step_b32(c->P, c->ncols, b, c->stream);
+}
+
 void operator_ready(tsp_context *c)
 {
     // neighbour table on the device first: the multi-column packing reads it
@@ -243,13 +249,16 @@ void operator_ready(tsp_context *c)
     if (c->batched()) {
         c->KP = batch_kp(c->P);
         if (!c->blkA) c->blkA = dalloc<float>(c, (size_t)c->ne * batch_block_floats(c->P), "blocks mfma");
-        kcheck(c, launch_pack_mfma(c->P, c->ne, c->meta, c->blk64, c->blkA, c->stream), "pack");
+        if (c->precision == TSP_BF16X3)
+            kcheck(c, launch_pack_bf16x3(c->P, c->ne, c->meta, c->blk64, c->blkA, c->stream), "pack");
+        else
+            kcheck(c, launch_pack_mfma(c->P, c->ne, c->meta, c->blk64, c->blkA, c->stream), "pack");
     } else if (c->precision == TSP_FP32C) {
         const size_t n = (size_t)c->ne * 4 * c->D * c->D;
         if (!c->blk32) c->blk32 = dalloc<float>(c, n, "blocks fp32");
         kcheck(c, launch_demote_blocks(c->P, c->ne, c->blk64, c->blk32, c->stream), "demote");
     }
-    if (c->precision == TSP_FP32C) {
+    if (c->precision != TSP_FP64) {
         if (!c->sc_first) c->sc_first = dalloc<float>(c, c->ne, "sc");
         if (!c->sc_step) c->sc_step = dalloc<float>(c, c->ne, "sc");
     }
@@ -362,9 +371,10 @@ int tsp_create(int device, int precision, int ncols, tsp_context **out)
 {
     if (!out) return TSP_ERR_ARG;
     *out = nullptr;
-    if (precision != TSP_FP64 && precision != TSP_FP32C) return TSP_ERR_UNSUPPORTED;
+    if (precision != TSP_FP64 && precision != TSP_FP32C && precision != TSP_BF16X3) return TSP_ERR_UNSUPPORTED;
     if (ncols != 1 && ncols != 32 && ncols != 64) return TSP_ERR_UNSUPPORTED;
-    if (ncols > 1 && precision != TSP_FP32C) return TSP_ERR_UNSUPPORTED;
+    if (ncols > 1 && precision == TSP_FP64) return TSP_ERR_UNSUPPORTED;
+    if (ncols == 1 && precision == TSP_BF16X3) return TSP_ERR_UNSUPPORTED;
     tsp_context *c = new (std::nothrow) tsp_context;
     if (!c) return TSP_ERR_NOMEM;
     c->device = device;
@@ -687,7 +697,7 @@ int tsp_first_step(tsp_context *c)
             b.sc32 = c->sc_first;
             b.uhi = c->uhi[0];
             b.uhi_out = c->uhi[1];
-            kcheck(c, launch_step_b32(c->P, c->ncols, b, c->stream), "first step");
+            kcheck(c, launch_batched(c, b), "first step");
             c->cur = 1;
         } else {
             // hi/comp from u0; the increment starts as (float)(dt*v0)
@@ -743,7 +753,7 @@ int tsp_run(tsp_context *c, int nsteps)
                 b.sc32 = c->sc_step;
                 b.uhi = c->uhi[c->cur];
                 b.uhi_out = c->uhi[1 - c->cur];
-                kcheck(c, launch_step_b32(c->P, c->ncols, b, c->stream), "step");
+                kcheck(c, launch_batched(c, b), "step");
             } else {
                 a.uhi = c->uhi[c->cur];
                 a.uhi_out = c->uhi[1 - c->cur];
@@ -918,7 +928,7 @@ int tsp_share_operator(tsp_context *owner, tsp_context **out)
         c->invm = owner->invm;
         c->KP = owner->KP;
         c->meta = dalloc<int4>(c, c->ne, "meta");
-        if (c->precision == TSP_FP32C) {
+        if (c->precision != TSP_FP64) {
             c->sc_first = dalloc<float>(c, c->ne, "sc");
             c->sc_step = dalloc<float>(c, c->ne, "sc");
         }
@@ -941,7 +951,7 @@ int tsp_release_fp64(tsp_context *c)
 {
     return guarded(c, [&] {
         require_mesh(c);
-        if (c->precision != TSP_FP32C) fail(c, TSP_ERR_STATE, "the FP64 context steps with its FP64 operator");
+        if (c->precision == TSP_FP64) fail(c, TSP_ERR_STATE, "the FP64 context steps with its FP64 operator");
         hipcheck(c, hipStreamSynchronize(c->stream), "synchronize");
         dfree(c, c->blk64);
     });

@@ -143,6 +143,8 @@ int batch_kp(int P);                 // padded rows per element (multiple of 32)
 size_t batch_block_floats(int P);    // floats of one element's four packed blocks
 int launch_step_b32(int P, int ncols, const BatchArgs &a, hipStream_t s);
 int launch_pack_mfma(int P, int ne, const int4 *meta, const double *src, float *dst, hipStream_t s);
+int launch_step_b16x3(int P, int ncols, const BatchArgs &a, hipStream_t s);
+int launch_pack_bf16x3(int P, int ne, const int4 *meta, const double *src, float *dst, hipStream_t s);
 int launch_sample_b32(int P, int ncols, const BatchArgs &a, hipStream_t s);
 int launch_col_in(int ne, int nln, int KP, int R, int col, const double *cm, double *st, hipStream_t s);
 int launch_col_out(int ne, int nln, int KP, int R, int col, const float *hi, const float *comp, double *cm,

@@ -33,6 +33,7 @@ namespace {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kBWaves = 4;
 
@@ -50,6 +51,88 @@ struct BGeom {
     static constexpr int KP = 32 * MT;        // padded rows / contraction length
     static constexpr int KB = KP / 8;         // k-blocks per block
 };
+
+// Compensated update of the rows and columns a lane holds (four rows of one
+// column per 16-byte access), the NaN/Inf flag, and the receiver dot products
+// of every column: shared by the FP32 and the BF16x3 products.
+template <int P, int NT, int R>
+__device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int half, int jj, size_t ebase,
+                                f32x16 (&acc)[BGeom<P>::MT][NT], const f32x4 (&uhv)[BGeom<P>::KB][NT],
+                                const f32x4 (&wv)[BGeom<P>::KB][NT], const f32x4 (&cp)[BGeom<P>::KB][NT])
+{
+    using G = BGeom<P>;
+    constexpr int MT = G::MT, D = G::D, NLN = G::NLN;
+    const int4 ex = m.w >= 0 ? a.extra[m.w] : make_int4(0, 0, 0, 0);
+    const float sc = a.sc32[e];
+    bool bad = false;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int i0 = mt * 32 + 8 * g + 4 * half;
+                if (mt * 32 + 8 * g >= D) continue;  // octet entirely padding
+                const int j = c0 + nt * 32 + jj;
+                const int kb = mt * 4 + g;
+                const size_t idx = ebase + (size_t)kb * R * 8 + nt * 256;
+                double fd[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int k = ex.x; k < ex.y; ++k) {
+                    const int src = a.src_sorted[k];
+                    if (a.src_col[src] != j) continue;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int i = i0 + q;
+                        if (i < D) fd[q] += a.wav_row[src * 2 + i / NLN] * a.src_phi[src * NLN + i % NLN];
+                    }
+                }
+                const f32x4 f = {(float)fd[0], (float)fd[1], (float)fd[2], (float)fd[3]};
+                const f32x4 uh = uhv[kb][nt];
+                f32x4 wn, tn, cn;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float w = wv[kb][nt][q] + sc * (acc[mt][nt][4 * g + q] - f[q]);
+                    const float y = w - cp[kb][nt][q];
+                    const float t = uh[q] + y;
+                    cn[q] = (t - uh[q]) - y;
+                    tn[q] = t;
+                    wn[q] = w;
+                    acc[mt][nt][4 * g + q] = t;
+                    bad |= !isfinite(t);
+                }
+                *reinterpret_cast<f32x4 *>(a.ucomp + idx) = cn;
+                *reinterpret_cast<f32x4 *>(a.uhi_out + idx) = tn;
+                *reinterpret_cast<f32x4 *>(a.winc + idx) = wn;
+            }
+    if (bad) atomicOr(a.flags, 1u);
+
+    // receivers: every receiver records all R columns (this wave's NT tiles)
+    if (a.sample >= 0) {
+        for (int k = ex.z; k < ex.w; ++k) {
+            const int rec = a.rec_sorted[k];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                float sx = 0.f, sy = 0.f;
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int i = mt * 32 + 8 * (r / 4) + 4 * half + (r % 4);
+                        if (i >= D) continue;
+                        const float v = (float)a.rec_phi[rec * NLN + i % NLN] * acc[mt][nt][r];
+                        if (i < NLN) sx += v; else sy += v;
+                    }
+                sx += __shfl_xor(sx, 32);
+                sy += __shfl_xor(sy, 32);
+                if (half == 0) {
+                    const size_t o = (((size_t)a.sample * a.nrec + rec) * R + c0 + nt * 32 + jj) * 2;
+                    a.seis[o] = sx;
+                    a.seis[o + 1] = sy;
+                }
+            }
+        }
+    }
+}
 
 // One wave computes one element's rows for NT column tiles (WPE waves share an
 // element, each with its own NT*32 columns).  A missing neighbour's block is
@@ -136,77 +219,128 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
         if constexpr (PD > 0) __builtin_amdgcn_sched_barrier(0);
     }
 
-    // compensated update, four rows of one column per 16-byte access
-    const int4 ex = m.w >= 0 ? a.extra[m.w] : make_int4(0, 0, 0, 0);
-    const float sc = a.sc32[e];
-    bool bad = false;
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const int i0 = mt * 32 + 8 * g + 4 * half;
-                if (mt * 32 + 8 * g >= D) continue;  // octet entirely padding
-                const int j = c0 + nt * 32 + jj;
-                const int kb = mt * 4 + g;
-                const size_t idx = ebase + (size_t)kb * R * 8 + nt * 256;
-                double fd[4] = {0.0, 0.0, 0.0, 0.0};
-                for (int k = ex.x; k < ex.y; ++k) {
-                    const int src = a.src_sorted[k];
-                    if (a.src_col[src] != j) continue;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int i = i0 + q;
-                        if (i < D) fd[q] += a.wav_row[src * 2 + i / NLN] * a.src_phi[src * NLN + i % NLN];
-                    }
-                }
-                const f32x4 f = {(float)fd[0], (float)fd[1], (float)fd[2], (float)fd[3]};
-                const f32x4 uh = own[kb][nt];
-                f32x4 wn, tn, cn;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float w = wv[kb][nt][q] + sc * (acc[mt][nt][4 * g + q] - f[q]);
-                    const float y = w - cp[kb][nt][q];
-                    const float t = uh[q] + y;
-                    cn[q] = (t - uh[q]) - y;
-                    tn[q] = t;
-                    wn[q] = w;
-                    acc[mt][nt][4 * g + q] = t;
-                    bad |= !isfinite(t);
-                }
-                *reinterpret_cast<f32x4 *>(a.ucomp + idx) = cn;
-                *reinterpret_cast<f32x4 *>(a.uhi_out + idx) = tn;
-                *reinterpret_cast<f32x4 *>(a.winc + idx) = wn;
-            }
-    if (bad) atomicOr(a.flags, 1u);
+    b_finish<P, NT, R>(a, e, m, c0, half, jj, ebase, acc, own, wv, cp);
+}
 
-    // receivers: every receiver records all R columns (this wave's NT tiles)
-    if (a.sample >= 0) {
-        for (int k = ex.z; k < ex.w; ++k) {
-            const int rec = a.rec_sorted[k];
+// The same step with the block products split into three BF16 products on
+// v_mfma_f32_32x32x16_bf16 (accumulating in FP32): with a = a_hi + a_lo and
+// x = x_hi + x_lo (each part rounded to BF16),
+//   a*x ~ a_lo*x_hi + a_hi*x_lo + a_hi*x_hi      (a_lo*x_lo dropped).
+// Operand layout of v_mfma_f32_32x32x16_bf16 (wave64): A lane l holds
+// A[l%32][8*(l/32) + 0..7], B lane l holds B[8*(l/32) + 0..7][l%32], results as
+// in v_mfma_f32_32x32x2f32.  K-chunk c (16 rows) is row octets 2c and 2c+1, so
+// lane l's B operand is octet 2c + l/32 of its column: 32 contiguous bytes.
+// Operator: per block, row tile mt and k-chunk c, 64 lanes x 8 BF16 of a_hi
+// then 64 lanes x 8 of a_lo (the same bytes as the FP32 operator).
+template <int P, int WPE, int PD>
+__global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3(BatchArgs a)
+{
+    using G = BGeom<P>;
+    constexpr int MT = G::MT, KB = G::KB, KC = KB / 2, NT = 1;
+    constexpr int R = 32 * WPE;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int gw = xcd_remap_b(blockIdx.x, gridDim.x) * kBWaves + wave;
+    const int e = gw / WPE, c0 = (gw % WPE) * 32;
+    if (e >= a.ne) return;  // no block-level barrier in this kernel
+    const int half = lane >> 5, jj = lane & 31;
+    const int4 m = a.meta[e];
+    const int nbs[4] = {e, m.x >= 0 ? m.x : e, m.y >= 0 ? m.y : e, m.z >= 0 ? m.z : e};
+    const int lofs = (c0 + jj) * 8 + 4 * half;   // update: rows 4h..4h+3 of an octet
+    const int bofs = (c0 + jj) * 8;              // products: a whole octet
+
+    f32x16 acc[MT][NT];
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                float sx = 0.f, sy = 0.f;
+    for (int mt = 0; mt < MT; ++mt) acc[mt][0] = f32x16{};
+
+    constexpr int NB = PD + 1;
+    f32x4 ah[NB][KC][MT], al[NB][KC][MT];
+    f32x4 xv[NB][KC][2];
+    f32x4 uhv[KB][NT], wv[KB][NT], cp[KB][NT];
+    const size_t ebase = (size_t)e * KB * R * 8 + lofs;
+    auto load = [&](int s, int buf) {
+        const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * MT * KB * 256);
+        const float *X = a.uhi + (size_t)nbs[s] * KB * R * 8 + bofs;
 #pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
+        for (int c = 0; c < KC; ++c)
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const int i = mt * 32 + 8 * (r / 4) + 4 * half + (r % 4);
-                        if (i >= D) continue;
-                        const float v = (float)a.rec_phi[rec * NLN + i % NLN] * acc[mt][nt][r];
-                        if (i < NLN) sx += v; else sy += v;
-                    }
-                sx += __shfl_xor(sx, 32);
-                sy += __shfl_xor(sy, 32);
-                if (half == 0) {
-                    const size_t o = (((size_t)a.sample * a.nrec + rec) * R + c0 + nt * 32 + jj) * 2;
-                    a.seis[o] = sx;
-                    a.seis[o + 1] = sy;
-                }
+            for (int mt = 0; mt < MT; ++mt) {
+                ah[buf][c][mt] = __builtin_nontemporal_load(Af + (mt * KC + c) * 128 + lane);
+                al[buf][c][mt] = __builtin_nontemporal_load(Af + (mt * KC + c) * 128 + 64 + lane);
+            }
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            const float *o = X + (size_t)(2 * c + half) * R * 8;
+            xv[buf][c][0] = *reinterpret_cast<const f32x4 *>(o);
+            xv[buf][c][1] = *reinterpret_cast<const f32x4 *>(o + 4);
+        }
+        if (s == 3) {
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb) {
+                const size_t idx = ebase + (size_t)kb * R * 8;
+                uhv[kb][0] = *reinterpret_cast<const f32x4 *>(a.uhi + idx);
+                wv[kb][0] = *reinterpret_cast<const f32x4 *>(a.winc + idx);
+                cp[kb][0] = *reinterpret_cast<const f32x4 *>(a.ucomp + idx);
             }
         }
+    };
+    auto product = [&](int buf) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            bf16x8 xh, xl;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float x = xv[buf][c][q / 4][q % 4];
+                const __bf16 h = (__bf16)x;
+                xh[q] = h;
+                xl[q] = (__bf16)(x - (float)h);
+            }
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const bf16x8 Ah = __builtin_bit_cast(bf16x8, ah[buf][c][mt]);
+                const bf16x8 Al = __builtin_bit_cast(bf16x8, al[buf][c][mt]);
+                acc[mt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, xh, acc[mt][0], 0, 0, 0);
+                acc[mt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, xl, acc[mt][0], 0, 0, 0);
+                acc[mt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, xh, acc[mt][0], 0, 0, 0);
+            }
+        }
+    };
+#pragma unroll
+    for (int s = 0; s < PD && s < 4; ++s) load(s, s % NB);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (s + PD < 4) load(s + PD, (s + PD) % NB);
+        if constexpr (PD > 0) __builtin_amdgcn_sched_barrier(0);
+        product(s % NB);
+        if constexpr (PD > 0) __builtin_amdgcn_sched_barrier(0);
     }
+    b_finish<P, NT, R>(a, e, m, c0, half, jj, ebase, acc, uhv, wv, cp);
+}
+
+// FP64 column-pair operator -> BF16 hi/lo fragments (layout above); blocks of
+// missing neighbours are zero.
+template <int P>
+__global__ void k_pack_bf16x3(int ne, const int4 *meta, const double *src, __bf16 *dst)
+{
+    using G = BGeom<P>;
+    constexpr int D = G::D, MT = G::MT, KC = G::KB / 2;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;   // one (lane, element-of-8) pair
+    const size_t per = (size_t)MT * KC * 512;
+    if (q >= (size_t)ne * 4 * per) return;
+    const size_t b = q / per;
+    const int rem = (int)(q % per);
+    const int frag = rem / 512, l = (rem % 512) / 8, i = rem % 8;
+    const int mt = frag / KC, c = frag % KC;
+    const int row = mt * 32 + l % 32, col = c * 16 + 8 * (l / 32) + i;
+    const int slot = (int)(b % 4);
+    const int4 mm = meta[b / 4];
+    const int nb = slot == 0 ? 0 : (slot == 1 ? mm.x : (slot == 2 ? mm.y : mm.z));
+    double v = 0.0;
+    if (row < D && col < D && nb >= 0) v = src[b * D * D + (size_t)(col / 2) * 2 * D + row * 2 + (col & 1)];
+    const __bf16 hi = (__bf16)(float)v;
+    const __bf16 lo = (__bf16)(float)(v - (double)(float)hi);
+    __bf16 *f = dst + b * (per * 2) + (size_t)frag * 1024;
+    f[l * 8 + i] = hi;
+    f[512 + l * 8 + i] = lo;
 }
 
 // FP64 column-pair operator -> FP32 A-fragment order with zero padding; the
@@ -303,6 +437,27 @@ int step_b32_p(const BatchArgs &a, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+template <int P, int R>
+int step_b16x3_p(const BatchArgs &a, hipStream_t s)
+{
+    constexpr int WPE = R / 32;
+    constexpr int PD = TSP_B32_PD >= 0 ? TSP_B32_PD : (BGeom<P>::MT <= 2 ? 1 : 0);
+    const size_t waves = (size_t)a.ne * WPE;
+    const unsigned nwg = (unsigned)((waves + kBWaves - 1) / kBWaves);
+    hipLaunchKernelGGL((k_step_b16x3<P, WPE, PD>), dim3(nwg), dim3(64 * kBWaves), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int P>
+int pack16_p(int ne, const int4 *meta, const double *src, float *dst, hipStream_t s)
+{
+    using G = BGeom<P>;
+    const size_t n = (size_t)ne * 4 * G::MT * (G::KB / 2) * 512;
+    hipLaunchKernelGGL(k_pack_bf16x3<P>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ne, meta, src,
+                       reinterpret_cast<__bf16 *>(dst));
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 template <int P>
 int pack_p(int ne, const int4 *meta, const double *src, float *dst, hipStream_t s)
 {
@@ -332,6 +487,28 @@ int sample_b32_p(const BatchArgs &a, int R, hipStream_t s)
 
 int batch_kp(int P) { return 32 * ((2 * nln_of(P) + 31) / 32); }
 size_t batch_block_floats(int P) { return (size_t)4 * batch_kp(P) / 32 * (batch_kp(P) / 8) * 256; }
+
+int launch_step_b16x3(int P, int ncols, const BatchArgs &a, hipStream_t s)
+{
+    if (ncols == 32) {
+#define CALL(p) step_b16x3_p<p, 32>(a, s)
+        TSP_DISPATCH_P(P, CALL)
+#undef CALL
+    }
+    if (ncols == 64) {
+#define CALL(p) step_b16x3_p<p, 64>(a, s)
+        TSP_DISPATCH_P(P, CALL)
+#undef CALL
+    }
+    return -2;
+}
+
+int launch_pack_bf16x3(int P, int ne, const int4 *meta, const double *src, float *dst, hipStream_t s)
+{
+#define CALL(p) pack16_p<p>(ne, meta, src, dst, s)
+    TSP_DISPATCH_P(P, CALL)
+#undef CALL
+}
 
 int launch_step_b32(int P, int ncols, const BatchArgs &a, hipStream_t s)
 {

@@ -171,11 +171,12 @@ def config3(nsrc=64, nx=40, ny=40, N=4, nsteps=3000, nrec=128, seed=3, stride=1)
     return base
 
 
-def run_batched(prob: Problem, ncols=64, stride=None):
+def run_batched(prob: Problem, ncols=64, stride=None, precision="fp32c"):
     """Run all sources of `prob` as ncols columns (source k -> column k) on the
-    MFMA path; returns (engine, seismograms [sample, rec, col, 2])."""
+    MFMA path (FP32 or BF16x3 products); returns (engine, seismograms
+    [sample, rec, col, 2])."""
     assert len(prob.src_elem) <= ncols
-    eng = capi.Engine("fp32c", ncols=ncols)
+    eng = capi.Engine(precision, ncols=ncols)
     eng.set_tables(prob.N)
     eng.set_mesh(prob.verts, prob.neigh, prob.neighedge, prob.lam, prob.mu, prob.rho, prob.penalty)
     stride = prob.stride if stride is None else stride
