@@ -88,13 +88,21 @@ def run_batched(steps, warmup, ncols=64, precision="fp32c"):
     kp = 32 * ((d + 31) // 32)
     nblk = prob.ne + int((prob.neigh >= 0).sum())
     flops_alg = 2.0 * nblk * d * d * ncols          # algorithmic block products per step
-    flops_mfma = 2.0 * nblk * kp * kp * ncols       # as issued on the padded 32x32 tiles
+    # as issued on the padded tiles (BF16x3: three products per block product)
+    nprod = 3 if precision == "bf16x3" else 1
+    flops_mfma = 2.0 * 4 * prob.ne * kp * kp * ncols * nprod
+    peak = 2500.0 if precision == "bf16x3" else 157.3
+    # operator once (4 B per entry either way) + per DOF and column: u^n read,
+    # u^{n+1} written, compensation and increment read and written (6 x 4 B)
+    nbytes = nblk * d * d * 4 + prob.ne * d * ncols * 24 + prob.ne * (8 + 16)
+    gbs = nbytes / (ms_step / 1e3) / 1e9
     return dict(value=prob.ndof * ncols * steps / (ms / 1e3), ms_per_step=ms_step, columns=ncols,
                 precision=precision,
                 workload="config3: config-1 mesh, p=4, 64 Ricker sources as 64 columns, 128 receivers",
                 tflops_algorithmic=flops_alg / (ms_step / 1e3) / 1e12,
-                tflops_mfma_issued=flops_mfma / (ms_step / 1e3) / 1e12, mfma_peak_tflops=157.3,
-                status=status)
+                tflops_mfma_issued=flops_mfma / (ms_step / 1e3) / 1e12, mfma_peak_tflops=peak,
+                mfma_frac=flops_mfma / (ms_step / 1e3) / 1e12 / peak,
+                algorithmic_bytes=nbytes, achieved_GBs=gbs, hbm_frac=gbs / HBM_PEAK_GBS, status=status)
 
 
 def run_multi(steps, warmup, n_inst=256):
@@ -104,13 +112,14 @@ def run_multi(steps, warmup, n_inst=256):
     from tspeed_amd import capi, workloads as W
     t0 = time.time()
     m = capi.Multi(chunk=32)
-    ndof, ne_tot, degrees = 0, 0, {}
+    ndof, ne_tot, degrees, nbytes = 0, 0, {}, 0
     for k in range(n_inst):
         prob = W.config4_instance(k)
         eng = W.prepare_instance(prob, stride=10)
         m.add(eng, prob.nsteps)
         ndof += prob.ndof
         ne_tot += prob.ne
+        nbytes += algorithmic_bytes(prob, 4, 24)
         degrees[prob.N] = degrees.get(prob.N, 0) + 1
     setup = time.time() - t0
     m.autotune(3)
@@ -120,8 +129,10 @@ def run_multi(steps, warmup, n_inst=256):
     ms = m.run_timed(steps)
     variants = {p: m.variant(p) for p in sorted(degrees)}
     m.close()
+    gbs = nbytes / (ms / steps / 1e3) / 1e9
     return dict(value=ndof * steps / (ms / 1e3), ms_per_step=ms / steps, instances=n_inst, elements=ne_tot,
                 dof=ndof, degrees=degrees, autotuned_variant=variants, setup_s=setup,
+                algorithmic_bytes=nbytes, achieved_GBs=gbs, hbm_frac=gbs / HBM_PEAK_GBS,
                 workload="config4: uneven batch, one persistent kernel per step, graph chunks of 32")
 
 
@@ -211,7 +222,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=600)
+    ap.add_argument("--cpu-steps", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32c", action="store_true")
     ap.add_argument("--no-batched", action="store_true")

@@ -37,6 +37,12 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kBWaves = 4;
 
+// Plain (cached) loads and stores throughout: a multi-column working set
+// (operator + state, a few hundred MB) stays largely in the 256 MB Infinity
+// Cache from one step to the next; non-temporal operator loads measured slower.
+__device__ inline f32x4 ld4(const float *p) { return *reinterpret_cast<const f32x4 *>(p); }
+__device__ inline void st4(float *p, f32x4 v) { *reinterpret_cast<f32x4 *>(p) = v; }
+
 __device__ inline int xcd_remap_b(int bid, int nwg)
 {
     const int per = nwg / 8, rem = nwg % 8;
@@ -49,7 +55,7 @@ struct BGeom {
     static constexpr int NLN = Geom<P>::NLN, D = Geom<P>::D;
     static constexpr int MT = (D + 31) / 32;  // 32-row tiles
     static constexpr int KP = 32 * MT;        // padded rows / contraction length
-    static constexpr int KB = KP / 8;         // k-blocks per block
+    static constexpr int KB = KP / 8;         // row octets per element
 };
 
 // Compensated update of the rows and columns a lane holds (four rows of one
@@ -100,9 +106,9 @@ __device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int h
                     acc[mt][nt][4 * g + q] = t;
                     bad |= !isfinite(t);
                 }
-                *reinterpret_cast<f32x4 *>(a.ucomp + idx) = cn;
-                *reinterpret_cast<f32x4 *>(a.uhi_out + idx) = tn;
-                *reinterpret_cast<f32x4 *>(a.winc + idx) = wn;
+                st4(a.ucomp + idx, cn);
+                st4(a.uhi_out + idx, tn);
+                st4(a.winc + idx, wn);
             }
     if (bad) atomicOr(a.flags, 1u);
 
@@ -165,9 +171,9 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
     constexpr int NB = PD + 1;
     f32x4 av[NB][KB][MT];
     f32x4 bv[NB][KB][NT];
-    // the element's own columns (slot 0) are also u^n of the update: kept
+    // the element's own columns (slot 0) are also u^n of the update: kept;
+    // the increment and compensation of the update are loaded with the last slot
     f32x4 own[KB][NT];
-    // increment and compensation of the update, loaded with the last slot
     f32x4 wv[KB][NT], cp[KB][NT];
     const size_t ebase = (size_t)e * KB * R * 8 + lofs;
     auto load = [&](int s, int buf) {
@@ -176,7 +182,7 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-            for (int mt = 0; mt < MT; ++mt) av[buf][kb][mt] = __builtin_nontemporal_load(Af + (mt * KB + kb) * 64 + lane);
+            for (int mt = 0; mt < MT; ++mt) av[buf][kb][mt] = Af[(mt * KB + kb) * 64 + lane];
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
@@ -190,8 +196,8 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
                     const size_t idx = ebase + (size_t)kb * R * 8 + nt * 256;
-                    wv[kb][nt] = *reinterpret_cast<const f32x4 *>(a.winc + idx);
-                    cp[kb][nt] = *reinterpret_cast<const f32x4 *>(a.ucomp + idx);
+                    wv[kb][nt] = ld4(a.winc + idx);
+                    cp[kb][nt] = ld4(a.ucomp + idx);
                 }
         }
     };
@@ -264,8 +270,8 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3(BatchArgs a)
         for (int c = 0; c < KC; ++c)
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
-                ah[buf][c][mt] = __builtin_nontemporal_load(Af + (mt * KC + c) * 128 + lane);
-                al[buf][c][mt] = __builtin_nontemporal_load(Af + (mt * KC + c) * 128 + 64 + lane);
+                ah[buf][c][mt] = Af[(mt * KC + c) * 128 + lane];
+                al[buf][c][mt] = Af[(mt * KC + c) * 128 + 64 + lane];
             }
 #pragma unroll
         for (int c = 0; c < KC; ++c) {
@@ -278,8 +284,8 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3(BatchArgs a)
             for (int kb = 0; kb < KB; ++kb) {
                 const size_t idx = ebase + (size_t)kb * R * 8;
                 uhv[kb][0] = *reinterpret_cast<const f32x4 *>(a.uhi + idx);
-                wv[kb][0] = *reinterpret_cast<const f32x4 *>(a.winc + idx);
-                cp[kb][0] = *reinterpret_cast<const f32x4 *>(a.ucomp + idx);
+                wv[kb][0] = ld4(a.winc + idx);
+                cp[kb][0] = ld4(a.ucomp + idx);
             }
         }
     };
@@ -422,15 +428,12 @@ __global__ void k_col_out(int ne, int nln, int KP, int R, int col, const float *
 // Launch shape per degree: one 32-column tile per wave (R/32 waves per
 // element); the operands of the next slot are prefetched while a slot's
 // products run when the registers allow it (up to two row tiles).
-#ifndef TSP_B32_PD
-#define TSP_B32_PD -1
-#endif
 
 template <int P, int R>
 int step_b32_p(const BatchArgs &a, hipStream_t s)
 {
     constexpr int WPE = R / 32, NT = 1;
-    constexpr int PD = TSP_B32_PD >= 0 ? TSP_B32_PD : (BGeom<P>::MT <= 2 ? 1 : 0);
+    constexpr int PD = BGeom<P>::MT <= 2 ? 1 : 0;
     const size_t waves = (size_t)a.ne * WPE;
     const unsigned nwg = (unsigned)((waves + kBWaves - 1) / kBWaves);
     hipLaunchKernelGGL((k_step_b32<P, NT, WPE, PD>), dim3(nwg), dim3(64 * kBWaves), 0, s, a);
@@ -441,7 +444,7 @@ template <int P, int R>
 int step_b16x3_p(const BatchArgs &a, hipStream_t s)
 {
     constexpr int WPE = R / 32;
-    constexpr int PD = TSP_B32_PD >= 0 ? TSP_B32_PD : (BGeom<P>::MT <= 2 ? 1 : 0);
+    constexpr int PD = BGeom<P>::MT <= 2 ? 1 : 0;
     const size_t waves = (size_t)a.ne * WPE;
     const unsigned nwg = (unsigned)((waves + kBWaves - 1) / kBWaves);
     hipLaunchKernelGGL((k_step_b16x3<P, WPE, PD>), dim3(nwg), dim3(64 * kBWaves), 0, s, a);
