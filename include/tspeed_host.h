@@ -58,6 +58,22 @@ int tsh_locate(int np, const double *points, int ne, const int *tri, int nq, con
 int tsh_ricker_table(int nsrc, const double *f0, const double *t0, const double *dirx, const double *diry,
                      double dt, int nrows, double *out);
 
+/* Plot output of FESpace::points_out / field_out (FESpace_imp.hpp:71-106): the
+ * images of the reference points (0.2,0.2), (0.8,0.2), (0.2,0.8), (0.5,0.5) in
+ * every element, out[(e*4+k)*2+c], and the field uh (component-major
+ * coefficients, degree N) at those points. */
+int tsh_field_points(int ne, const double *verts, double *out);
+int tsh_field_values(int N, int ne, const double *uh, double *out);
+
+/* Quadrature nodes of the degree-N space mapped into every element,
+ * out[(e*nq2+k)*2+c] (nq2 = (N+1)^2); the L2 projection of a field given by its
+ * values f at those nodes (FESpace::transform, FESpace_imp.hpp:137-179); and
+ * the L2 norm of uex - uh with uex given at those nodes (FESpace::L2error,
+ * FESpace_imp.hpp:108-135). */
+int tsh_quad_points(int N, int ne, const double *verts, double *out);
+int tsh_project(int N, int ne, const double *verts, const double *f, double *uh);
+int tsh_l2_error(int N, int ne, const double *verts, const double *uh, const double *uex, double *err);
+
 #ifdef __cplusplus
 }
 #endif

@@ -152,3 +152,43 @@ def test_config_receivers_file_format_roundtrip(tmp_path):
     for r in range(3):
         got = np.loadtxt(tmp_path / f"run_rcv_{r}.out")
         np.testing.assert_array_equal(got, s[:, r, :])
+
+
+# ------------------------------------------------------------------ plot output and error norm
+def _unit_square(n=3, N=3, seed=2):
+    pts, tri = capi.split_square(n, n, 0.0, 0.0, 1.0, 1.0, 0.2, seed)
+    return pts[tri].reshape(-1, 6), N
+
+
+def test_field_points_are_the_affine_images():
+    """FESpace::points_out (FESpace_imp.hpp:71-83): the four reference points
+    mapped by x = v0 + (v1 - v0) xi + (v2 - v0) eta."""
+    verts, _ = _unit_square()
+    p = capi.field_points(verts)
+    v = verts.reshape(-1, 3, 2)
+    ref = np.array([[0.2, 0.2], [0.8, 0.2], [0.2, 0.8], [0.5, 0.5]])
+    expect = v[:, None, 0] + ref[None, :, :1] * (v[:, None, 1] - v[:, None, 0]) + ref[None, :, 1:] * (v[:, None, 2] - v[:, None, 0])
+    np.testing.assert_allclose(p, expect, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("N", [1, 2, 4, 6])
+def test_projection_reproduces_polynomials_and_l2_error(N):
+    """FESpace::transform of a degree-N polynomial field is exact: its values at
+    the plot points equal the polynomial there (field_out), and its L2 error
+    (FESpace::L2error) vanishes; the L2 error of uh = 0 against (1, 2) is
+    sqrt(5 * area)."""
+    verts, _ = _unit_square(N=N)
+    def fun(x, y):
+        return (1 + x - 2 * y + x ** N, 0.5 * y ** N - x * y ** (N - 1) + 3)
+    uh = capi.project(N, verts, fun)
+    pts = capi.field_points(verts)
+    vals = capi.field_values(N, uh)
+    fx, fy = fun(pts[..., 0], pts[..., 1])
+    np.testing.assert_allclose(vals[..., 0], fx, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(vals[..., 1], fy, rtol=0, atol=1e-11)
+    assert capi.l2_error(N, verts, uh, fun) <= 1e-11
+    zero = np.zeros_like(uh)
+    assert capi.l2_error(N, verts, zero, lambda x, y: (1.0, 2.0)) == pytest.approx(np.sqrt(5.0), rel=1e-13)
+    # a degree N+1 field is not in the space: a positive projection error
+    assert capi.l2_error(N, verts, capi.project(N, verts, lambda x, y: (x ** (N + 1), 0 * x)),
+                         lambda x, y: (x ** (N + 1), 0 * x)) > 1e-10

@@ -81,6 +81,11 @@ SIGNATURES = {
                                ctypes.POINTER(_d)]),
     "tsp_server_stats": (_i, [_vp, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_longlong),
                               ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
+    "tsh_field_points": (_i, [_i, _dp, _dp]),
+    "tsh_field_values": (_i, [_i, _i, _dp, _dp]),
+    "tsh_quad_points": (_i, [_i, _i, _dp, _dp]),
+    "tsh_project": (_i, [_i, _i, _dp, _dp, _dp]),
+    "tsh_l2_error": (_i, [_i, _i, _dp, _dp, _dp, ctypes.POINTER(_d)]),
     "tsh_gauleg": (_i, [_i, _d, _d, _dp, _dp]),
     "tsh_dubiner": (_i, [_i, _d, _d, _dp, _vp]),
     "tsh_tables": (_i, [_i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -197,6 +202,56 @@ def ricker_table(f0, t0, dirx, diry, dt, nrows):
     out = np.zeros(max(1, nrows * ns * 2))
     assert lib().tsh_ricker_table(ns, f0, t0, dirx, diry, float(dt), nrows, out) == 0
     return out[: nrows * ns * 2].reshape(nrows, ns, 2)
+
+
+def field_points(verts):
+    """Images of the four plot points in every element, (ne, 4, 2)."""
+    verts = _c64(verts).reshape(-1, 6)
+    out = np.zeros((max(1, verts.shape[0]), 4, 2))
+    assert lib().tsh_field_points(verts.shape[0], verts, out) == 0
+    return out[: verts.shape[0]]
+
+
+def field_values(N, uh):
+    """uh (component-major, degree N) at the four plot points, (ne, 4, 2)."""
+    uh = _c64(uh)
+    ne = uh.size // (2 * nln(N))
+    out = np.zeros((max(1, ne), 4, 2))
+    assert lib().tsh_field_values(N, ne, uh, out) == 0
+    return out[:ne]
+
+
+def quad_points(N, verts):
+    """Quadrature nodes of the degree-N space in every element, (ne, (N+1)^2, 2)."""
+    verts = _c64(verts).reshape(-1, 6)
+    ne = verts.shape[0]
+    out = np.zeros((max(1, ne), (N + 1) ** 2, 2))
+    assert lib().tsh_quad_points(N, ne, verts, out) == 0
+    return out[:ne]
+
+
+def project(N, verts, fun):
+    """L2 projection of fun(x, y) -> (fx, fy) (vectorised over arrays) onto the
+    degree-N space: component-major coefficients (FESpace::transform)."""
+    verts = _c64(verts).reshape(-1, 6)
+    ne = verts.shape[0]
+    xy = quad_points(N, verts)
+    fx, fy = fun(xy[..., 0], xy[..., 1])
+    f = _c64(np.stack([np.broadcast_to(fx, xy.shape[:2]), np.broadcast_to(fy, xy.shape[:2])], -1))
+    uh = np.zeros(max(1, 2 * ne * nln(N)))
+    assert lib().tsh_project(N, ne, verts, f, uh) == 0
+    return uh[: 2 * ne * nln(N)]
+
+
+def l2_error(N, verts, uh, fun):
+    """L2 norm of fun - uh over the mesh (FESpace::L2error)."""
+    verts = _c64(verts).reshape(-1, 6)
+    xy = quad_points(N, verts)
+    fx, fy = fun(xy[..., 0], xy[..., 1])
+    f = _c64(np.stack([np.broadcast_to(fx, xy.shape[:2]), np.broadcast_to(fy, xy.shape[:2])], -1))
+    err = _d()
+    assert lib().tsh_l2_error(N, verts.shape[0], verts, _c64(uh), f, ctypes.byref(err)) == 0
+    return err.value
 
 
 # ---------------------------------------------------------------- device engine

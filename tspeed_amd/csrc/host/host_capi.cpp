@@ -142,4 +142,40 @@ int tsh_ricker_table(int nsrc, const double *f0, const double *t0, const double 
     return 0;
 }
 
+
+int tsh_field_points(int ne, const double *verts, double *out)
+{
+    if (ne < 0 || (ne && (!verts || !out))) return -1;
+    tspeed::field_points(ne, verts, out);
+    return 0;
+}
+
+int tsh_field_values(int N, int ne, const double *uh, double *out)
+{
+    if (N < 1 || N > 8 || ne < 0 || (ne && (!uh || !out))) return -1;
+    tspeed::field_values(N, ne, uh, out);
+    return 0;
+}
+
+int tsh_quad_points(int N, int ne, const double *verts, double *out)
+{
+    if (N < 1 || N > 8 || ne < 0 || (ne && (!verts || !out))) return -1;
+    tspeed::quad_points(N, ne, verts, out);
+    return 0;
+}
+
+int tsh_project(int N, int ne, const double *verts, const double *f, double *uh)
+{
+    if (N < 1 || N > 8 || ne < 0 || (ne && (!verts || !f || !uh))) return -1;
+    tspeed::project_nodes(N, ne, verts, f, uh);
+    return 0;
+}
+
+int tsh_l2_error(int N, int ne, const double *verts, const double *uh, const double *uex, double *err)
+{
+    if (N < 1 || N > 8 || ne < 0 || !err || (ne && (!verts || !uh || !uex))) return -1;
+    *err = tspeed::l2_error(N, ne, verts, uh, uex);
+    return 0;
+}
+
 }  // extern "C"

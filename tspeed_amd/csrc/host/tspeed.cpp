@@ -426,6 +426,157 @@ std::vector<double> FESpaceRT::transform(const std::function<std::array<double, 
     return uh;
 }
 
+// ---------------------------------------------------------------- plot output and error norm
+const double kPlotPoints[4][2] = {{0.2, 0.2}, {0.8, 0.2}, {0.2, 0.8}, {0.5, 0.5}};
+
+namespace {
+// affine map of the reference triangle onto element e (Geometry.cpp:199-207)
+inline void map_point(const double *P, double xi, double eta, double *x, double *y)
+{
+    *x = P[0] + (P[2] - P[0]) * xi + (P[4] - P[0]) * eta;
+    *y = P[1] + (P[3] - P[1]) * xi + (P[5] - P[1]) * eta;
+}
+inline double det_of(const double *P)
+{
+    return (P[2] - P[0]) * (P[5] - P[1]) - (P[3] - P[1]) * (P[4] - P[0]);
+}
+}  // namespace
+
+void field_points(int ne, const double *verts, double *out)
+{
+    for (int e = 0; e < ne; ++e)
+        for (int k = 0; k < 4; ++k)
+            map_point(verts + 6 * (size_t)e, kPlotPoints[k][0], kPlotPoints[k][1], &out[((size_t)e * 4 + k) * 2],
+                      &out[((size_t)e * 4 + k) * 2 + 1]);
+}
+
+void field_values(int N, int ne, const double *uh, double *out)
+{
+    const Dubiner sh(N);
+    const int nln = sh.gdl();
+    std::vector<double> vb(4 * (size_t)nln);  // basis at the plot points
+    for (int k = 0; k < 4; ++k)
+        for (int i = 0; i < nln; ++i) vb[(size_t)k * nln + i] = sh.phi(i, kPlotPoints[k][0], kPlotPoints[k][1]);
+    for (int e = 0; e < ne; ++e)
+        for (int k = 0; k < 4; ++k) {
+            double vx = 0, vy = 0;
+            for (int i = 0; i < nln; ++i) {
+                vx += vb[(size_t)k * nln + i] * uh[(size_t)e * nln + i];
+                vy += vb[(size_t)k * nln + i] * uh[(size_t)ne * nln + (size_t)e * nln + i];
+            }
+            out[((size_t)e * 4 + k) * 2] = vx;
+            out[((size_t)e * 4 + k) * 2 + 1] = vy;
+        }
+}
+
+void quad_points(int N, int ne, const double *verts, double *out)
+{
+    const Gauss q(N + 1);
+    const int nq2 = q.nqn2d();
+    for (int e = 0; e < ne; ++e)
+        for (int k = 0; k < nq2; ++k) {
+            const auto xy = q.inode(k);
+            map_point(verts + 6 * (size_t)e, xy[0], xy[1], &out[((size_t)e * nq2 + k) * 2],
+                      &out[((size_t)e * nq2 + k) * 2 + 1]);
+        }
+}
+
+void project_nodes(int N, int ne, const double *verts, const double *f, double *uh)
+{
+    const Gauss q(N + 1);
+    const Dubiner sh(N);
+    const int nq2 = q.nqn2d(), nln = sh.gdl();
+    std::vector<double> phi(nln), grad(2 * (size_t)nln);
+    std::vector<double> Bq((size_t)nq2 * nln);
+    for (int k = 0; k < nq2; ++k) {
+        const auto xy = q.inode(k);
+        sh.eval(xy[0], xy[1], phi.data(), grad.data());
+        for (int i = 0; i < nln; ++i) Bq[(size_t)k * nln + i] = phi[i];
+    }
+    for (int e = 0; e < ne; ++e) {
+        const double det = det_of(verts + 6 * (size_t)e);
+        std::vector<double> rhs(2 * (size_t)nln, 0.0);
+        for (int k = 0; k < nq2; ++k) {
+            const double dx = std::fabs(det) * q.iweight(k);
+            const double fx = f[((size_t)e * nq2 + k) * 2], fy = f[((size_t)e * nq2 + k) * 2 + 1];
+            for (int i = 0; i < nln; ++i) {
+                rhs[i] += fx * dx * Bq[(size_t)k * nln + i];
+                rhs[nln + i] += fy * dx * Bq[(size_t)k * nln + i];
+            }
+        }
+        for (int i = 0; i < nln; ++i) {
+            uh[(size_t)e * nln + i] = 1. / det * rhs[i];
+            uh[(size_t)ne * nln + (size_t)e * nln + i] = 1. / det * rhs[nln + i];
+        }
+    }
+}
+
+double l2_error(int N, int ne, const double *verts, const double *uh, const double *uex)
+{
+    const Gauss q(N + 1);
+    const Dubiner sh(N);
+    const int nq2 = q.nqn2d(), nln = sh.gdl();
+    std::vector<double> phi(nln), grad(2 * (size_t)nln);
+    std::vector<double> Bq((size_t)nq2 * nln);
+    for (int k = 0; k < nq2; ++k) {
+        const auto xy = q.inode(k);
+        sh.eval(xy[0], xy[1], phi.data(), grad.data());
+        for (int i = 0; i < nln; ++i) Bq[(size_t)k * nln + i] = phi[i];
+    }
+    double error = 0;
+    for (int e = 0; e < ne; ++e) {
+        double error_loc = 0;
+        for (int k = 0; k < nq2; ++k) {
+            double ax = 0, ay = 0;
+            for (int i = 0; i < nln; ++i) {
+                ax += Bq[(size_t)k * nln + i] * uh[(size_t)e * nln + i];
+                ay += Bq[(size_t)k * nln + i] * uh[(size_t)ne * nln + (size_t)e * nln + i];
+            }
+            const double dx = ax - uex[((size_t)e * nq2 + k) * 2], dy = ay - uex[((size_t)e * nq2 + k) * 2 + 1];
+            error_loc += (dx * dx + dy * dy) * q.iweight(k);
+        }
+        error += error_loc * det_of(verts + 6 * (size_t)e);
+    }
+    return std::sqrt(error);
+}
+
+void FESpaceRT::points_out(const std::string &fname) const
+{
+    const auto v = mesh_->vertex_array();
+    std::vector<double> pts(8 * (size_t)ne());
+    field_points(ne(), v.data(), pts.data());
+    std::ofstream out(fname);
+    if (!out.is_open()) throw std::runtime_error("File " + fname + " may not be opened");
+    for (size_t k = 0; k < 4 * (size_t)ne(); ++k) out << pts[2 * k] << " " << pts[2 * k + 1] << std::endl;
+}
+
+void FESpaceRT::field_out(const std::string &fname, const std::vector<double> &uh, unsigned step) const
+{
+    if (uh.size() != 2 * (size_t)ne() * nln_) throw std::invalid_argument("field_out: uh has the wrong size");
+    std::vector<double> val(8 * (size_t)ne());
+    field_values(N_, ne(), uh.data(), val.data());
+    const std::string name = fname + "u_" + std::to_string(step) + ".field";
+    std::ofstream out(name);
+    if (!out.is_open()) throw std::runtime_error("File " + name + " may not be opened");
+    for (size_t k = 0; k < 4 * (size_t)ne(); ++k) out << val[2 * k] << " " << val[2 * k + 1] << std::endl;
+}
+
+double FESpaceRT::L2error(const std::function<std::array<double, 2>(double, double)> &uex,
+                          const std::vector<double> &uh) const
+{
+    if (uh.size() != 2 * (size_t)ne() * nln_) throw std::invalid_argument("L2error: uh has the wrong size");
+    const auto v = mesh_->vertex_array();
+    const int nq2 = quad_.nqn2d();
+    std::vector<double> xy(2 * (size_t)ne() * nq2), f(2 * (size_t)ne() * nq2);
+    quad_points(N_, ne(), v.data(), xy.data());
+    for (size_t k = 0; k < (size_t)ne() * nq2; ++k) {
+        const auto F = uex(xy[2 * k], xy[2 * k + 1]);
+        f[2 * k] = F[0];
+        f[2 * k + 1] = F[1];
+    }
+    return l2_error(N_, ne(), v.data(), uh.data(), f.data());
+}
+
 // ---------------------------------------------------------------- Parameters
 Parameters::Parameters(Mesh_ptr m) : mesh_(std::move(m))
 {

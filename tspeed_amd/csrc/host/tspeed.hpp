@@ -145,6 +145,16 @@ public:
     // L2 projection of a vector field onto the space (FESpace_imp.hpp:137-179),
     // component-major coefficients.
     std::vector<double> transform(const std::function<std::array<double, 2>(double, double)> &fun) const;
+    // The reference's plot output (FESpace_imp.hpp:71-106): four points per
+    // element (the images of (0.2,0.2), (0.8,0.2), (0.2,0.8), (0.5,0.5)) as "x y"
+    // lines, and the field uh at those points as "ux uy" lines in
+    // fname + "u_" + step + ".field".
+    void points_out(const std::string &fname) const;
+    void field_out(const std::string &fname, const std::vector<double> &uh, unsigned step) const;
+    // L2 norm of uex - uh over the mesh with the space's quadrature
+    // (FESpace_imp.hpp:108-135).
+    double L2error(const std::function<std::array<double, 2>(double, double)> &uex,
+                   const std::vector<double> &uh) const;
 
 private:
     Mesh_ptr mesh_;
@@ -159,6 +169,15 @@ public:
     explicit FESpace(Mesh_ptr m) : FESpaceRT(std::move(m), N) {}
 };
 using FESpace_ptr = std::shared_ptr<FESpaceRT>;
+
+// Array forms of the output and error routines (also exported as tsh_*):
+// verts[e*6 + 2v + c]; uh component-major (c*ne*nln + e*nln + i).
+extern const double kPlotPoints[4][2];
+void field_points(int ne, const double *verts, double *out);                       // out[(e*4+k)*2+c]
+void field_values(int N, int ne, const double *uh, double *out);                    // out[(e*4+k)*2+c]
+void quad_points(int N, int ne, const double *verts, double *out);                  // out[(e*nq2+k)*2+c]
+void project_nodes(int N, int ne, const double *verts, const double *f, double *uh); // f at quad_points
+double l2_error(int N, int ne, const double *verts, const double *uh, const double *uex); // uex at quad_points
 
 // Per-element Lame parameters and density (FESpace.hpp:194-247).
 class Parameters {
