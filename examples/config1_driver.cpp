@@ -5,7 +5,8 @@
 //
 //   build:  g++ -O2 -std=c++17 -I../tspeed_amd/csrc/host config1_driver.cpp \
 //               -L../tspeed_amd -ltspeed_amd -Wl,-rpath,$ORIGIN/../tspeed_amd -o config1_driver
-//   run:    ./config1_driver [steps] [out_prefix] [precision 0|1]
+//   run:    ./config1_driver [steps] [out_prefix] [precision 0|1] [basis 0|1]
+//           basis 1 = FESpace<N, Dunavant<2N>, BoundaryAdapted<N>> (as Lamb.cpp:27)
 #include "tspeed.hpp"
 
 #include <cmath>
@@ -20,10 +21,12 @@ int main(int argc, char **argv)
     const int nsteps = argc > 1 ? std::atoi(argv[1]) : 2000;
     const std::string out = argc > 2 ? argv[2] : "config1";
     const int precision = argc > 3 ? std::atoi(argv[3]) : 0;
+    const int basis = argc > 4 ? std::atoi(argv[4]) : 0;
     const int N = 4;
 
     Mesh_ptr Th = Mesh::split_square(40, 40, 0.0, 0.0, 1000.0, 1000.0, 0.2, 1);
-    FESpace_ptr Xh(new FESpace<N>(Th));
+    FESpace_ptr Xh = basis ? FESpace_ptr(new FESpace<N, Dunavant<2 * N>, BoundaryAdapted<N>>(Th))
+                           : FESpace_ptr(new FESpace<N>(Th));
 
     Parameters p(Th);
     const double rho = 2000, vp = 3000, vs = 1732;

@@ -75,6 +75,16 @@ int tsp_set_stream(tsp_context *ctx, void *hip_stream);
 int tsp_set_tables(tsp_context *ctx, int N, const double *B, const double *G, const double *Bedge,
                    const double *Gedge, const double *w2, const double *w1);
 
+/* Non-orthonormal bases (FESpace<N,Q,BoundaryAdapted<N>>, Matrices_imp.hpp:183-190:
+ * invM = 1/rho * 1/detJ * base_invmass).  base_invmass is nln x nln row-major;
+ * NULL restores the orthonormal case (identity, the default).  Call after
+ * tsp_set_tables and before tsp_set_mesh / tsp_upload_blocks: the engine then
+ * stores the operator rows premultiplied, (I2 (x) base_invmass) B, keeps the
+ * per-element scalar 1/(rho detJ), and applies base_invmass to every source's
+ * basis values in tsp_set_sources; tsp_get_blocks returns the premultiplied
+ * blocks. */
+int tsp_set_mass(tsp_context *ctx, const double *base_invmass);
+
 /* Mesh and materials; assembles B = penalty*N^2*S + I + A and the inverse mass
  * on the device in FP64 (Matrices::Matrices, Matrices_imp.hpp:192-215;
  * TimeAdvance_imp.hpp:32).  Unlike the reference, whose set_penalty is a no-op

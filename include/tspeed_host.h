@@ -63,7 +63,7 @@ int tsh_ricker_table(int nsrc, const double *f0, const double *t0, const double 
  * every element, out[(e*4+k)*2+c], and the field uh (component-major
  * coefficients, degree N) at those points. */
 int tsh_field_points(int ne, const double *verts, double *out);
-int tsh_field_values(int N, int ne, const double *uh, double *out);
+int tsh_field_values(int N, int basis, int ne, const double *uh, double *out);
 
 /* Quadrature nodes of the degree-N space mapped into every element,
  * out[(e*nq2+k)*2+c] (nq2 = (N+1)^2); the L2 projection of a field given by its
@@ -71,8 +71,20 @@ int tsh_field_values(int N, int ne, const double *uh, double *out);
  * the L2 norm of uex - uh with uex given at those nodes (FESpace::L2error,
  * FESpace_imp.hpp:108-135). */
 int tsh_quad_points(int N, int ne, const double *verts, double *out);
-int tsh_project(int N, int ne, const double *verts, const double *f, double *uh);
-int tsh_l2_error(int N, int ne, const double *verts, const double *uh, const double *uex, double *err);
+int tsh_project(int N, int basis, int ne, const double *verts, const double *f, double *uh);
+int tsh_l2_error(int N, int basis, int ne, const double *verts, const double *uh, const double *uex, double *err);
+
+/* Basis selection (`basis` above and below): 0 = the orthonormal Dubiner basis
+ * (ShapeFunctions_imp.hpp:25-54), 1 = the BoundaryAdapted basis
+ * (ShapeFunctions_imp.hpp:56-118).  tsh_tables_basis is tsh_tables for either;
+ * tsh_base_invmass gives base_invmass (nln x nln, identity for Dubiner,
+ * FESpace_imp.hpp:49-67), which tsp_set_mass takes; tsh_basis_eval evaluates
+ * the basis (and its gradient, grad may be NULL) at n reference points
+ * xi[q*2+c]: phi[q*nln+i], grad[(q*nln+i)*2+c]. */
+int tsh_tables_basis(int N, int basis, double *B, double *G, double *Bedge, double *Gedge, double *w2, double *xy2,
+                     double *w1, double *x1);
+int tsh_base_invmass(int N, int basis, double *out);
+int tsh_basis_eval(int N, int basis, int n, const double *xi, double *phi, double *grad);
 
 #ifdef __cplusplus
 }

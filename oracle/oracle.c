@@ -159,15 +159,104 @@ void orc_dubiner(int N, double csi, double eta, double *phi, double *grad)
 
 /* Basis tables exactly as FESpace tabulates them (FESpace_imp.hpp:26-47),
  * with Q = Gauss<N+1> (FESpace.hpp:37). */
+/* ---- BoundaryAdapted basis (ShapeFunctions_imp.hpp:56-118), evaluated with
+ * forward-mode dual numbers (value, d/dxi, d/deta) so the gradient is the exact
+ * derivative of the same expression. */
+typedef struct { double v, x, y; } dual;
+static dual dconst(double c) { dual r = {c, 0, 0}; return r; }
+static dual dadd(dual a, dual b) { dual r = {a.v + b.v, a.x + b.x, a.y + b.y}; return r; }
+static dual dsub(dual a, dual b) { dual r = {a.v - b.v, a.x - b.x, a.y - b.y}; return r; }
+static dual dmul(dual a, dual b) { dual r = {a.v * b.v, a.x * b.v + a.v * b.x, a.y * b.v + a.v * b.y}; return r; }
+static dual dscale(dual a, double c) { dual r = {a.v * c, a.x * c, a.y * c}; return r; }
+static dual ddiv(dual a, dual b)
+{
+    const double q = a.v / b.v;
+    dual r = {q, (a.x - q * b.x) / b.v, (a.y - q * b.y) / b.v};
+    return r;
+}
+static dual dpowi(dual a, int n)
+{
+    dual r = dconst(1.0);
+    for (int k = 0; k < n; ++k) r = dmul(r, a);
+    return r;
+}
+/* the three-term recurrence of orc_jacobi on a dual argument */
+static dual djacobi(int n, int alpha, int beta, dual z)
+{
+    if (n == 0) return dconst(1.0);
+    dual p1 = dscale(dadd(dconst((double)(alpha - beta)), dscale(z, alpha + beta + 2.0)), 0.5);
+    if (n == 1) return p1;
+    const int apb = alpha + beta;
+    dual p2 = dconst(1.0), p = p1;
+    for (int k = 2; k <= n; ++k) {
+        double a1 = 2.0 * k * (k + apb) * (2.0 * k + apb - 2.0);
+        double a2 = (2.0 * k + apb - 1.0) * (alpha * alpha - beta * beta);
+        double a3 = (2.0 * k + apb - 2.0) * (2.0 * k + apb - 1.0) * (2.0 * k + apb);
+        double a4 = 2.0 * (k + alpha - 1.0) * (k + beta - 1.0) * (2.0 * k + apb);
+        a2 = a2 / a1;
+        a3 = a3 / a1;
+        a4 = a4 / a1;
+        p = dsub(dmul(dadd(dconst(a2), dscale(z, a3)), p1), dscale(p2, a4));
+        p2 = p1;
+        p1 = p;
+    }
+    return p;
+}
+
+void orc_boundary_adapted(int N, double csi, double eta, double *phi, double *grad)
+{
+    const dual x = {csi, 1, 0}, y = {eta, 0, 1};
+    const dual one = dconst(1.0);
+    const dual l0 = dsub(dsub(one, x), y);            /* 1 - xi - eta */
+    const dual om = dsub(one, y);                     /* 1 - eta */
+    const dual zx = dsub(dscale(ddiv(x, om), 2.0), one); /* 2 xi / (1 - eta) - 1 */
+    const dual zy = dsub(dscale(y, 2.0), one);        /* 2 eta - 1 */
+    dual f[(8 + 1) * (8 + 2) / 2];
+    int s = 0;
+    f[s++] = l0;                                       /* vertex modes */
+    f[s++] = x;
+    f[s++] = y;
+    for (int i = 1; i < N; ++i)                        /* edge eta = 0 */
+        f[s++] = dmul(dmul(dmul(l0, x), dpowi(om, i - 1)), djacobi(i - 1, 1, 1, zx));
+    for (int j = 1; j < N; ++j)                        /* edge xi = 0 */
+        f[s++] = dmul(dmul(l0, y), djacobi(j - 1, 1, 1, zy));
+    for (int j = 1; j < N; ++j)                        /* edge xi + eta = 1 */
+        f[s++] = dmul(dmul(x, y), djacobi(j - 1, 1, 1, zy));
+    for (int j = 1; j < N; ++j)                        /* interior bubbles */
+        for (int i = 1; i < N; ++i)
+            if (i + j < N)
+                f[s++] = dmul(dmul(dmul(dmul(dmul(l0, x), y), dpowi(om, i - 1)), djacobi(i - 1, 1, 1, zx)),
+                              djacobi(j - 1, 2 * i + 1, 1, zy));
+    for (int k = 0; k < s; ++k) {
+        if (phi) phi[k] = f[k].v;
+        if (grad) {
+            grad[2 * k] = f[k].x;
+            grad[2 * k + 1] = f[k].y;
+        }
+    }
+}
+
 int orc_tables(int N, double *B, double *G, double *Bedge, double *Gedge,
                double *w2, double *xy2, double *w1, double *x1)
+{
+    return orc_tables_basis(N, ORC_DUBINER, B, G, Bedge, Gedge, w2, xy2, w1, x1);
+}
+
+static void eval_basis(int N, int basis, double xi, double eta, double *phi, double *grad)
+{
+    if (basis == ORC_BOUNDARY_ADAPTED) orc_boundary_adapted(N, xi, eta, phi, grad);
+    else orc_dubiner(N, xi, eta, phi, grad);
+}
+
+int orc_tables_basis(int N, int basis, double *B, double *G, double *Bedge, double *Gedge,
+                     double *w2, double *xy2, double *w1, double *x1)
 {
     const int nln = orc_nln(N), nq1 = N + 1, nq2 = nq1 * nq1;
     double *phi = malloc(sizeof(double) * nln), *gr = malloc(sizeof(double) * 2 * nln);
     double exy[2 * 32];
     if (orc_gauss_rule(nq1, w2, xy2, w1, x1)) { free(phi); free(gr); return -1; }
     for (int k = 0; k < nq2; ++k) {
-        orc_dubiner(N, xy2[2 * k], xy2[2 * k + 1], phi, gr);
+        eval_basis(N, basis, xy2[2 * k], xy2[2 * k + 1], phi, gr);
         for (int i = 0; i < nln; ++i) {
             B[k * nln + i] = phi[i];
             G[(i * 2 + 0) * nq2 + k] = gr[2 * i];
@@ -177,7 +266,7 @@ int orc_tables(int N, double *B, double *G, double *Bedge, double *Gedge,
     for (int f = 0; f < 3; ++f) {
         orc_edge_nodes(nq1, x1, f, exy);
         for (int k = 0; k < nq1; ++k) {
-            orc_dubiner(N, exy[2 * k], exy[2 * k + 1], phi, gr);
+            eval_basis(N, basis, exy[2 * k], exy[2 * k + 1], phi, gr);
             for (int i = 0; i < nln; ++i) {
                 Bedge[(f * nq1 + k) * nln + i] = phi[i];
                 Gedge[((i * 3 + f) * 2 + 0) * nq1 + k] = gr[2 * i];
@@ -326,8 +415,65 @@ static double avg_p(double a, double b) { return (a + b == 0) ? 0 : 2 * a * b / 
 
 /* Matrices::Matrices (Matrices_imp.hpp:192-215) followed by
  * B = penalty*N*N*S + I + A (TimeAdvance_imp.hpp:32). */
+/* base_mass(i,j) = sum_k B(k,i) B(k,j) w_k and its inverse (FESpace_imp.hpp:49-67;
+ * identity for the orthonormal Dubiner basis).  Gauss-Jordan with partial pivoting
+ * in place of Eigen's inverse(). */
+int orc_base_invmass(int N, int basis, double *out)
+{
+    const int nln = orc_nln(N), nq2 = (N + 1) * (N + 1);
+    if (basis != ORC_BOUNDARY_ADAPTED) {
+        for (int i = 0; i < nln; ++i)
+            for (int j = 0; j < nln; ++j) out[i * nln + j] = i == j ? 1.0 : 0.0;
+        return 0;
+    }
+    double *Bt = malloc(sizeof(double) * nq2 * nln), *Gt = malloc(sizeof(double) * nln * 2 * nq2);
+    double *Be = malloc(sizeof(double) * 3 * (N + 1) * nln), *Ge = malloc(sizeof(double) * nln * 3 * 2 * (N + 1));
+    double w2[32 * 32], xy2[2 * 32 * 32], w1[32], x1[32];
+    orc_tables_basis(N, basis, Bt, Gt, Be, Ge, w2, xy2, w1, x1);
+    double *a = calloc((size_t)nln * 2 * nln, sizeof(double));   /* [M | I] */
+    for (int i = 0; i < nln; ++i) {
+        for (int j = 0; j < nln; ++j) {
+            double m = 0.0;
+            for (int k = 0; k < nq2; ++k) m += Bt[k * nln + i] * Bt[k * nln + j] * w2[k];
+            a[i * 2 * nln + j] = m;
+        }
+        a[i * 2 * nln + nln + i] = 1.0;
+    }
+    int rc = 0;
+    for (int c = 0; c < nln && !rc; ++c) {
+        int p = c;
+        for (int r = c + 1; r < nln; ++r)
+            if (fabs(a[r * 2 * nln + c]) > fabs(a[p * 2 * nln + c])) p = r;
+        if (a[p * 2 * nln + c] == 0.0) { rc = -1; break; }
+        if (p != c)
+            for (int j = 0; j < 2 * nln; ++j) {
+                const double t = a[c * 2 * nln + j];
+                a[c * 2 * nln + j] = a[p * 2 * nln + j];
+                a[p * 2 * nln + j] = t;
+            }
+        const double piv = a[c * 2 * nln + c];
+        for (int j = 0; j < 2 * nln; ++j) a[c * 2 * nln + j] /= piv;
+        for (int r = 0; r < nln; ++r)
+            if (r != c) {
+                const double fct = a[r * 2 * nln + c];
+                if (fct != 0.0)
+                    for (int j = 0; j < 2 * nln; ++j) a[r * 2 * nln + j] -= fct * a[c * 2 * nln + j];
+            }
+    }
+    for (int i = 0; i < nln; ++i)
+        for (int j = 0; j < nln; ++j) out[i * nln + j] = a[i * 2 * nln + nln + j];
+    free(a); free(Bt); free(Gt); free(Be); free(Ge);
+    return rc;
+}
+
 orc_op *orc_assemble(int N, int ne, const double *verts, const int *neigh, const int *neighedge,
                      const double *lambda, const double *mu, const double *rho, double penalty)
+{
+    return orc_assemble_basis(N, ORC_DUBINER, ne, verts, neigh, neighedge, lambda, mu, rho, penalty);
+}
+
+orc_op *orc_assemble_basis(int N, int basis, int ne, const double *verts, const int *neigh, const int *neighedge,
+                           const double *lambda, const double *mu, const double *rho, double penalty)
 {
     const int nln = orc_nln(N), nq1 = N + 1, nq2 = nq1 * nq1, nn = nln * nln;
     orc_op *op = calloc(1, sizeof(orc_op));
@@ -360,7 +506,9 @@ orc_op *orc_assemble(int N, int ne, const double *verts, const int *neigh, const
     double *Bt = malloc(sizeof(double) * nq2 * nln), *Gt = malloc(sizeof(double) * nln * 2 * nq2);
     double *Be = malloc(sizeof(double) * 3 * nq1 * nln), *Ge = malloc(sizeof(double) * nln * 3 * 2 * nq1);
     double w2[32 * 32], xy2[2 * 32 * 32], w1[32], x1[32];
-    orc_tables(N, Bt, Gt, Be, Ge, w2, xy2, w1, x1);
+    orc_tables_basis(N, basis, Bt, Gt, Be, Ge, w2, xy2, w1, x1);
+    double *bim = malloc(sizeof(double) * nn);
+    orc_base_invmass(N, basis, bim);
 
     /* per-element scratch: sigma/eps at area nodes [k][i][n][4] and edge nodes [k][i][f][n][4] */
     double *sig = malloc(sizeof(double) * nq2 * nln * 2 * 4), *eps = malloc(sizeof(double) * nq2 * nln * 2 * 4);
@@ -452,13 +600,13 @@ orc_op *orc_assemble(int N, int ne, const double *verts, const int *neigh, const
                             }
             }
         }
-        /* mass (Matrices_imp.hpp:183-190): invM = 1./rho*1./detJ * base_invmass, base = I */
+        /* mass (Matrices_imp.hpp:183-190): invM = 1./rho*1./detJ * base_invmass */
         const double sc = 1. / rho[e] * 1. / det;
         op->invm[e] = sc;
         for (int c = 0; c < 2; ++c)
             for (int i = 0; i < nln; ++i)
                 for (int j = 0; j < nln; ++j)
-                    blk(op->invM[c], e, nln)[i * nln + j] = sc * (i == j ? 1.0 : 0.0);
+                    blk(op->invM[c], e, nln)[i * nln + j] = sc * bim[i * nln + j];
     }
     /* I.symmetrize() (MyMat.hpp:321-328, MyMat.cpp:156-187): I <- I + I^T */
     for (int c = 0; c < 4; ++c) op->B[c] = malloc(sizeof(double) * (size_t)cnt * nn);
@@ -483,7 +631,7 @@ orc_op *orc_assemble(int N, int ne, const double *verts, const int *neigh, const
         }
         free(S[c]); free(I[c]); free(A[c]);
     }
-    free(Bt); free(Gt); free(Be); free(Ge); free(sig); free(eps); free(sige);
+    free(Bt); free(Gt); free(Be); free(Ge); free(sig); free(eps); free(sige); free(bim);
     return op;
 }
 

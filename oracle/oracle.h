@@ -29,11 +29,20 @@ double orc_jacobi(int n, int alpha, int beta, double z);
 int    orc_nln(int N);
 /* phi[nln], grad[nln*2] (d/dxi, d/deta) at one point */
 void   orc_dubiner(int N, double xi, double eta, double *phi, double *grad);
+/* the non-orthonormal BoundaryAdapted basis (ShapeFunctions_imp.hpp:56-118): vertex,
+ * edge and bubble modes in the reference's order */
+void   orc_boundary_adapted(int N, double xi, double eta, double *phi, double *grad);
+enum { ORC_DUBINER = 0, ORC_BOUNDARY_ADAPTED = 1 };
+/* base_invmass (nln x nln): identity for Dubiner, the inverse of the quadrature mass
+ * matrix otherwise (FESpace_imp.hpp:49-67) */
+int    orc_base_invmass(int N, int basis, double *out);
 /* Tables as FESpace builds them (FESpace_imp.hpp:26-47):
  *   B[k*nln+i], G[(i*2+a)*nq2+k], Bedge[(f*nq1+k)*nln+i], Gedge[((i*3+f)*2+a)*nq1+k],
  *   w2[nq2], xy2[nq2*2], w1[nq1], x1[nq1]  with nq1 = N+1, nq2 = nq1^2.              */
 int    orc_tables(int N, double *B, double *G, double *Bedge, double *Gedge,
                   double *w2, double *xy2, double *w1, double *x1);
+int    orc_tables_basis(int N, int basis, double *B, double *G, double *Bedge, double *Gedge,
+                        double *w2, double *xy2, double *w1, double *x1);
 
 /* ---- mesh and geometry (Mesh.cpp:33-128, Geometry.cpp:100-207) ---- */
 /* O(ne^2) neighbour search exactly as the reference; tri[ne*3] are vertex ids. */
@@ -48,6 +57,10 @@ int    orc_locate(int ne, const double *verts, double px, double py, double *xi)
 typedef struct orc_op orc_op;
 orc_op *orc_assemble(int N, int ne, const double *verts, const int *neigh, const int *neighedge,
                      const double *lambda, const double *mu, const double *rho, double penalty);
+/* the same with the basis selectable (invM blocks = 1/(rho detJ) * base_invmass) */
+orc_op *orc_assemble_basis(int N, int basis, int ne, const double *verts, const int *neigh,
+                           const int *neighedge, const double *lambda, const double *mu,
+                           const double *rho, double penalty);
 void   orc_free(orc_op *op);
 /* B in canonical merged form: out[((e*4+s)*d + I)*d + J], s=0 self, s=1+f neighbour across
  * edge f (zero block when absent), I = m*nln+i, J = n*nln+j, d = 2*nln. */

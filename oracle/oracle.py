@@ -45,6 +45,12 @@ def lib():
     L.orc_jacobi.restype = ctypes.c_double
     L.orc_dubiner.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
     L.orc_tables.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.orc_tables_basis.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.orc_boundary_adapted.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
+    L.orc_base_invmass.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+    L.orc_assemble_basis.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _ip, _ip, _dp, _dp, _dp,
+                                     ctypes.c_double]
+    L.orc_assemble_basis.restype = ctypes.c_void_p
     L.orc_neighbors.argtypes = [ctypes.c_int, _ip, _ip, _ip]
     L.orc_geometry.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
     L.orc_locate.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp]
@@ -96,12 +102,27 @@ def dubiner(N, xi, eta):
     return phi, gr.reshape(-1, 2)
 
 
-def tables(N):
+DUBINER, BOUNDARY_ADAPTED = 0, 1
+
+
+def boundary_adapted(N, xi, eta):
+    phi = np.zeros(nln(N)); gr = np.zeros(2 * nln(N))
+    lib().orc_boundary_adapted(N, float(xi), float(eta), phi, gr)
+    return phi, gr.reshape(-1, 2)
+
+
+def base_invmass(N, basis=DUBINER):
+    out = np.zeros(nln(N) * nln(N))
+    assert lib().orc_base_invmass(N, basis, out) == 0
+    return out.reshape(nln(N), nln(N))
+
+
+def tables(N, basis=DUBINER):
     n, nq1 = nln(N), N + 1
     nq2 = nq1 * nq1
     B = np.zeros(nq2 * n); G = np.zeros(n * 2 * nq2); Be = np.zeros(3 * nq1 * n); Ge = np.zeros(n * 3 * 2 * nq1)
     w2 = np.zeros(nq2); xy2 = np.zeros(nq2 * 2); w1 = np.zeros(nq1); x1 = np.zeros(nq1)
-    assert lib().orc_tables(N, B, G, Be, Ge, w2, xy2, w1, x1) == 0
+    assert lib().orc_tables_basis(N, basis, B, G, Be, Ge, w2, xy2, w1, x1) == 0
     return dict(B=B.reshape(nq2, n), G=G.reshape(n, 2, nq2), Bedge=Be.reshape(3, nq1, n),
                 Gedge=Ge.reshape(n, 3, 2, nq1), w2=w2, xy2=xy2.reshape(-1, 2), w1=w1, x1=x1)
 
@@ -137,11 +158,11 @@ def ricker(f0, t0, t):
 class Operator:
     """B = penalty*N^2*S + I + A assembled as the reference does (Matrices_imp.hpp:192-215)."""
 
-    def __init__(self, N, verts, neigh, neighedge, lam, mu, rho, penalty):
-        self.N, self.nln = N, nln(N)
+    def __init__(self, N, verts, neigh, neighedge, lam, mu, rho, penalty, basis=DUBINER):
+        self.N, self.nln, self.basis = N, nln(N), basis
         self.verts = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1)
         self.ne = self.verts.size // 6
-        self._h = lib().orc_assemble(N, self.ne, self.verts,
+        self._h = lib().orc_assemble_basis(N, basis, self.ne, self.verts,
                                      np.ascontiguousarray(neigh, dtype=np.int32).reshape(-1),
                                      np.ascontiguousarray(neighedge, dtype=np.int32).reshape(-1),
                                      np.ascontiguousarray(lam, dtype=np.float64),

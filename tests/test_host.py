@@ -192,3 +192,39 @@ def test_projection_reproduces_polynomials_and_l2_error(N):
     # a degree N+1 field is not in the space: a positive projection error
     assert capi.l2_error(N, verts, capi.project(N, verts, lambda x, y: (x ** (N + 1), 0 * x)),
                          lambda x, y: (x ** (N + 1), 0 * x)) > 1e-10
+
+
+# ------------------------------------------------------------------ BoundaryAdapted basis (host side)
+@pytest.mark.parametrize("N", [1, 2, 4, 8])
+def test_boundary_adapted_tables_match_oracle(N):
+    """The engine's BoundaryAdapted tables and base_invmass equal the oracle's
+    (same expressions, forward-mode gradients), and tsh_basis_eval agrees."""
+    h = capi.tables(N, capi.BOUNDARY_ADAPTED)
+    o = O.tables(N, O.BOUNDARY_ADAPTED)
+    np.testing.assert_allclose(h["B"].reshape(o["B"].shape), o["B"], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(h["G"].reshape(o["G"].shape), o["G"], rtol=1e-14, atol=1e-13)
+    np.testing.assert_allclose(h["Bedge"].reshape(o["Bedge"].shape), o["Bedge"], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(h["Gedge"].reshape(o["Gedge"].shape), o["Gedge"], rtol=1e-14, atol=1e-13)
+    np.testing.assert_allclose(capi.base_invmass(N, capi.BOUNDARY_ADAPTED), O.base_invmass(N, O.BOUNDARY_ADAPTED),
+                               rtol=1e-11, atol=1e-11)
+    np.testing.assert_array_equal(capi.base_invmass(N), np.eye(capi.nln(N)))
+    xi = np.array([[0.1, 0.2], [0.7, 0.25], [0.05, 0.9]])
+    np.testing.assert_allclose(capi.basis_eval(N, capi.BOUNDARY_ADAPTED, xi),
+                               np.array([O.boundary_adapted(N, *p)[0] for p in xi]), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("N", [2, 5])
+def test_boundary_adapted_projection_exact(N):
+    """FESpace::transform with the BoundaryAdapted basis (base_invmass applied)
+    reproduces degree-N polynomials at the plot points and in L2."""
+    verts, _ = _unit_square(N=N)
+    def fun(x, y):
+        return (1 + x - 2 * y + x ** N, 0.5 * y ** N - x * y + 3)
+    ba = capi.BOUNDARY_ADAPTED
+    uh = capi.project(N, verts, fun, ba)
+    pts = capi.field_points(verts)
+    vals = capi.field_values(N, uh, ba)
+    fx, fy = fun(pts[..., 0], pts[..., 1])
+    np.testing.assert_allclose(vals[..., 0], fx, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(vals[..., 1], fy, rtol=0, atol=1e-11)
+    assert capi.l2_error(N, verts, uh, fun, ba) <= 1e-11

@@ -348,3 +348,108 @@ def test_leapfrog_converges_to_an_exact_standing_wave(N):
     assert errs[0] < 0.05 / N
     # measured 3.35, 7.97, 14.7: the spatial order N+1 dominates on these meshes
     assert errs[0] / errs[1] >= 0.8 * 2 ** (N + 1), errs
+
+
+# ------------------------------------------------------------------ BoundaryAdapted basis
+def _ba_modes(N):
+    """(kind, edge set the mode is nonzero on) for the reference's mode order."""
+    kinds = [("v", None)] * 3 + [("e", "eta0")] * (N - 1) + [("e", "xi0")] * (N - 1) + [("e", "hyp")] * (N - 1)
+    kinds += [("b", None)] * ((N - 1) * (N - 2) // 2)
+    return kinds
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 8])
+def test_boundary_adapted_modes_and_traces(N):
+    """Vertex modes are the barycentric coordinates; an edge mode vanishes on the
+    two other edges and equals xi(1-xi) P_{i-1}^{1,1}(2xi-1) on eta = 0 (resp.
+    eta(1-eta) P^{1,1}_{j-1}(2eta-1) on xi = 0, xi eta P^{1,1}_{j-1}(2eta-1) on the
+    hypotenuse); bubbles vanish on every edge; the count is (N+1)(N+2)/2."""
+    from scipy.special import eval_jacobi
+    kinds = _ba_modes(N)
+    assert len(kinds) == O.nln(N)
+    for xi, eta in [(0.13, 0.21), (0.6, 0.3), (0.05, 0.9)]:
+        phi, _ = O.boundary_adapted(N, xi, eta)
+        np.testing.assert_allclose(phi[:3], [1 - xi - eta, xi, eta], rtol=0, atol=1e-15)
+    t = np.linspace(0.03, 0.97, 7)
+    for s in t:
+        on = {"eta0": O.boundary_adapted(N, s, 0.0)[0], "xi0": O.boundary_adapted(N, 0.0, s)[0],
+              "hyp": O.boundary_adapted(N, s, 1 - s)[0]}
+        ie = {"eta0": 0, "xi0": 0, "hyp": 0}
+        for k, (kind, edge) in enumerate(kinds):
+            if kind == "b":
+                for v in on.values():
+                    assert abs(v[k]) < 1e-14
+            if kind == "e":
+                for name, v in on.items():
+                    if name != edge:
+                        assert abs(v[k]) < 1e-14
+                i = ie[edge] = ie[edge] + 1
+                if edge == "eta0":
+                    expect = s * (1 - s) * eval_jacobi(i - 1, 1, 1, 2 * s - 1)
+                elif edge == "xi0":
+                    expect = s * (1 - s) * eval_jacobi(i - 1, 1, 1, 2 * s - 1)
+                else:
+                    expect = s * (1 - s) * eval_jacobi(i - 1, 1, 1, 2 * (1 - s) - 1)
+                assert on[edge][k] == pytest.approx(expect, rel=1e-12, abs=1e-14)
+
+
+@pytest.mark.parametrize("N", [2, 4, 7])
+def test_boundary_adapted_gradient_and_span(N):
+    """The gradient is the derivative (central differences), and the modes span
+    the polynomials of degree N: each is an exact combination of the Dubiner
+    modes (least-squares residual at 60 points ~ roundoff)."""
+    rng = np.random.default_rng(N)
+    h = 1e-6
+    pts = []
+    while len(pts) < 60:
+        x, y = rng.uniform(0.02, 0.95, 2)
+        if x + y < 0.97:
+            pts.append((x, y))
+    for x, y in pts[:8]:
+        _, g = O.boundary_adapted(N, x, y)
+        fx = (O.boundary_adapted(N, x + h, y)[0] - O.boundary_adapted(N, x - h, y)[0]) / (2 * h)
+        fy = (O.boundary_adapted(N, x, y + h)[0] - O.boundary_adapted(N, x, y - h)[0]) / (2 * h)
+        np.testing.assert_allclose(g[:, 0], fx, rtol=1e-6, atol=1e-6)
+        np.testing.assert_allclose(g[:, 1], fy, rtol=1e-6, atol=1e-6)
+    D = np.array([O.dubiner(N, x, y)[0] for x, y in pts])
+    A = np.array([O.boundary_adapted(N, x, y)[0] for x, y in pts])
+    T, *_ = np.linalg.lstsq(D, A, rcond=None)
+    assert np.abs(D @ T - A).max() < 1e-12
+
+
+@pytest.mark.parametrize("N", [1, 3, 6])
+def test_boundary_adapted_mass_inverse(N):
+    """base_invmass inverts the quadrature mass matrix, which is the exact
+    integral of phi_i phi_j (the rule is exact to degree 2N+1): checked through
+    the Dubiner change of basis, M = T^T T for phi_BA = phi_Dub T."""
+    T_ = O.tables(N, O.BOUNDARY_ADAPTED)
+    M = (T_["B"] * T_["w2"][:, None]).T @ T_["B"]
+    Minv = O.base_invmass(N, O.BOUNDARY_ADAPTED)
+    np.testing.assert_allclose(Minv @ M, np.eye(O.nln(N)), rtol=0, atol=1e-9)
+    D = O.tables(N)["B"]
+    Tc, *_ = np.linalg.lstsq(D, T_["B"], rcond=None)
+    np.testing.assert_allclose(M, Tc.T @ Tc, rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(O.base_invmass(N), np.eye(O.nln(N)), rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("N", [2, 4])
+def test_boundary_adapted_dynamics_equal_dubiner(N):
+    """The discrete dynamics do not depend on the basis: lambda_max(M^-1 B) and
+    the seismograms of the same source and receivers agree between the
+    BoundaryAdapted and the Dubiner spaces to roundoff."""
+    op_d, verts, nb, (lam, mu, rho) = _operator(N=N, nx=4, ny=3)
+    ng = O.neighbors(_mesh(4, 3, seed=3)[1])[1]
+    op_b = O.Operator(N, verts, nb, ng, lam, mu, rho, 10.0, basis=O.BOUNDARY_ADAPTED)
+    assert op_b.lambda_max(400) == pytest.approx(op_d.lambda_max(400), rel=1e-9)
+    sx, sy = 230.0, 170.0
+    es, xs = O.locate(verts, sx, sy)
+    recs = [(60.0, 350.0), (420.0, 90.0), (300.0, 300.0)]
+    rloc = [O.locate(verts, x, y) for x, y in recs]
+    dt = 0.4 / np.sqrt(op_d.lambda_max(400))
+    out = []
+    for op, fn in ((op_d, O.dubiner), (op_b, O.boundary_adapted)):
+        src = dict(elem=es, phi=fn(N, *xs)[0], f0=1.0 / (8 * dt), t0=10 * dt, dirx=0.3, diry=-1.0)
+        rphi = np.array([fn(N, *xi)[0] for _, xi in rloc])
+        _, s = op.run(dt, 120, src=src, rec_elem=np.array([e for e, _ in rloc]), rec_phi=rphi, stride=1)
+        out.append(s)
+    assert np.linalg.norm(out[1] - out[0]) / np.linalg.norm(out[0]) < 1e-11

@@ -82,10 +82,14 @@ SIGNATURES = {
     "tsp_server_stats": (_i, [_vp, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_longlong),
                               ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     "tsh_field_points": (_i, [_i, _dp, _dp]),
-    "tsh_field_values": (_i, [_i, _i, _dp, _dp]),
+    "tsh_field_values": (_i, [_i, _i, _i, _dp, _dp]),
     "tsh_quad_points": (_i, [_i, _i, _dp, _dp]),
-    "tsh_project": (_i, [_i, _i, _dp, _dp, _dp]),
-    "tsh_l2_error": (_i, [_i, _i, _dp, _dp, _dp, ctypes.POINTER(_d)]),
+    "tsh_project": (_i, [_i, _i, _i, _dp, _dp, _dp]),
+    "tsh_l2_error": (_i, [_i, _i, _i, _dp, _dp, _dp, ctypes.POINTER(_d)]),
+    "tsh_tables_basis": (_i, [_i, _i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "tsh_base_invmass": (_i, [_i, _i, _dp]),
+    "tsh_basis_eval": (_i, [_i, _i, _i, _dp, _dp, _vp]),
+    "tsp_set_mass": (_i, [_vp, _vp]),
     "tsh_gauleg": (_i, [_i, _d, _d, _dp, _dp]),
     "tsh_dubiner": (_i, [_i, _d, _d, _dp, _vp]),
     "tsh_tables": (_i, [_i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -153,12 +157,15 @@ def dubiner(N, xi, eta, grad=True):
     return (phi, g.reshape(-1, 2)) if grad else phi
 
 
-def tables(N):
+DUBINER, BOUNDARY_ADAPTED = 0, 1
+
+
+def tables(N, basis=DUBINER):
     n, nq1 = nln(N), N + 1
     nq2 = nq1 * nq1
     B = np.zeros(nq2 * n); G = np.zeros(n * 2 * nq2); Be = np.zeros(3 * nq1 * n); Ge = np.zeros(n * 3 * 2 * nq1)
     w2 = np.zeros(nq2); xy2 = np.zeros(2 * nq2); w1 = np.zeros(nq1); x1 = np.zeros(nq1)
-    assert lib().tsh_tables(N, B, G, Be, Ge, w2, xy2, w1, x1) == 0
+    assert lib().tsh_tables_basis(N, basis, B, G, Be, Ge, w2, xy2, w1, x1) == 0
     return dict(B=B, G=G, Bedge=Be, Gedge=Ge, w2=w2, xy2=xy2.reshape(-1, 2), w1=w1, x1=x1)
 
 
@@ -212,12 +219,28 @@ def field_points(verts):
     return out[: verts.shape[0]]
 
 
-def field_values(N, uh):
+def base_invmass(N, basis=DUBINER):
+    """base_invmass of the degree-N space (identity for Dubiner)."""
+    out = np.zeros(nln(N) * nln(N))
+    assert lib().tsh_base_invmass(N, basis, out) == 0
+    return out.reshape(nln(N), nln(N))
+
+
+def basis_eval(N, basis, xi):
+    """Basis values at reference points xi (n, 2): (n, nln)."""
+    xi = _c64(np.atleast_2d(xi))
+    n = xi.shape[0]
+    phi = np.zeros(max(1, n * nln(N)))
+    assert lib().tsh_basis_eval(N, basis, n, xi.reshape(-1), phi, None) == 0
+    return phi[: n * nln(N)].reshape(n, nln(N))
+
+
+def field_values(N, uh, basis=DUBINER):
     """uh (component-major, degree N) at the four plot points, (ne, 4, 2)."""
     uh = _c64(uh)
     ne = uh.size // (2 * nln(N))
     out = np.zeros((max(1, ne), 4, 2))
-    assert lib().tsh_field_values(N, ne, uh, out) == 0
+    assert lib().tsh_field_values(N, basis, ne, uh, out) == 0
     return out[:ne]
 
 
@@ -230,7 +253,7 @@ def quad_points(N, verts):
     return out[:ne]
 
 
-def project(N, verts, fun):
+def project(N, verts, fun, basis=DUBINER):
     """L2 projection of fun(x, y) -> (fx, fy) (vectorised over arrays) onto the
     degree-N space: component-major coefficients (FESpace::transform)."""
     verts = _c64(verts).reshape(-1, 6)
@@ -239,18 +262,18 @@ def project(N, verts, fun):
     fx, fy = fun(xy[..., 0], xy[..., 1])
     f = _c64(np.stack([np.broadcast_to(fx, xy.shape[:2]), np.broadcast_to(fy, xy.shape[:2])], -1))
     uh = np.zeros(max(1, 2 * ne * nln(N)))
-    assert lib().tsh_project(N, ne, verts, f, uh) == 0
+    assert lib().tsh_project(N, basis, ne, verts, f, uh) == 0
     return uh[: 2 * ne * nln(N)]
 
 
-def l2_error(N, verts, uh, fun):
+def l2_error(N, verts, uh, fun, basis=DUBINER):
     """L2 norm of fun - uh over the mesh (FESpace::L2error)."""
     verts = _c64(verts).reshape(-1, 6)
     xy = quad_points(N, verts)
     fx, fy = fun(xy[..., 0], xy[..., 1])
     f = _c64(np.stack([np.broadcast_to(fx, xy.shape[:2]), np.broadcast_to(fy, xy.shape[:2])], -1))
     err = _d()
-    assert lib().tsh_l2_error(N, verts.shape[0], verts, _c64(uh), f, ctypes.byref(err)) == 0
+    assert lib().tsh_l2_error(N, basis, verts.shape[0], verts, _c64(uh), f, ctypes.byref(err)) == 0
     return err.value
 
 
@@ -313,6 +336,11 @@ class Engine:
         self._ck(self._L.tsp_set_tables(self._h, N, _c64(tab["B"]), _c64(tab["G"]), _c64(tab["Bedge"]),
                                         _c64(tab["Gedge"]), _c64(tab["w2"]), _c64(tab["w1"])))
         self.N = N
+
+    def set_mass(self, base_invmass):
+        """tsp_set_mass: None = orthonormal basis."""
+        m = None if base_invmass is None else _c64(base_invmass).reshape(-1)
+        self._ck(self._L.tsp_set_mass(self._h, None if m is None else m.ctypes.data))
 
     def set_mesh(self, verts, neigh, neighedge, lam, mu, rho, penalty):
         verts = _c64(verts).reshape(-1)

@@ -241,6 +241,37 @@ __global__ void k_demote_quads(int ne, const double *src, float *dst)
     dst[q] = (float)src[b * D * D + (size_t)(col / 2) * 2 * D + row * 2 + (col & 1)];
 }
 
+// Non-orthonormal basis: fold base_invmass into the block rows, B' = (I2 (x) Mb^-1) B,
+// so the step's per-element scalar 1/(rho detJ) completes the dense inverse mass
+// (Matrices_imp.hpp:183-190).  One thread per column of one block, in place.
+template <int P>
+__global__ void k_premul_invmass(int ne, const double *minv, double *blk)
+{
+    constexpr int D = Geom<P>::D, NLN = Geom<P>::NLN;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)ne * 4 * D) return;
+    const size_t b = q / D;
+    const int J = (int)(q % D);
+    double *col = blk + b * D * D + (size_t)(J / 2) * 2 * D + (J & 1);
+    double v[D];
+#pragma unroll
+    for (int r = 0; r < D; ++r) v[r] = col[r * 2];
+    for (int r = 0; r < D; ++r) {
+        const int m = r / NLN, i = r % NLN;
+        double acc = 0.0;
+        for (int k = 0; k < NLN; ++k) acc += minv[i * NLN + k] * v[m * NLN + k];
+        col[r * 2] = acc;
+    }
+}
+
+template <int P>
+int premul_p(int ne, const double *minv, double *blk, hipStream_t s)
+{
+    const size_t n = (size_t)ne * 4 * Geom<P>::D;
+    hipLaunchKernelGGL(k_premul_invmass<P>, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, ne, minv, blk);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 template <int P>
 int assemble_p(const AssembleArgs &a, hipStream_t s)
 {
@@ -275,6 +306,13 @@ int conv_p(bool to_canon, int ne, const double *src, double *dst, hipStream_t s)
 int launch_assemble(int P, const AssembleArgs &a, hipStream_t s)
 {
 #define CALL(p) assemble_p<p>(a, s)
+    TSP_DISPATCH_P(P, CALL)
+#undef CALL
+}
+
+int launch_premul_invmass(int P, int ne, const double *minv, double *blk, hipStream_t s)
+{
+#define CALL(p) premul_p<p>(ne, minv, blk, s)
     TSP_DISPATCH_P(P, CALL)
 #undef CALL
 }

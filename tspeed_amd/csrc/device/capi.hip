@@ -59,6 +59,10 @@ struct tsp_context {
     // many columns (ncols > 1): packed MFMA operator, state rows padded to KP
     float *blkA = nullptr;
     int KP = 0;
+    // non-orthonormal basis: base_invmass (nln x nln), folded into the operator
+    // rows after assembly and applied to each source's basis values
+    std::vector<double> h_minv;
+    double *d_minv = nullptr;
     bool batched() const { return ncols > 1; }
     size_t state_len() const { return batched() ? (size_t)ne * KP * ncols : (size_t)ne * D; }
 
@@ -422,6 +426,10 @@ int tsp_set_tables(tsp_context *c, int N, const double *B, const double *G, cons
         if (N < 1 || N > kMaxDegree) fail(c, TSP_ERR_UNSUPPORTED, "degree %d outside 1..%d", N, kMaxDegree);
         if (!B || !G || !Be || !Ge || !w2 || !w1) fail(c, TSP_ERR_ARG, "null table pointer");
         if (c->P && c->P != N && c->ne) fail(c, TSP_ERR_STATE, "degree change after the mesh was set");
+        if (c->P != N) {
+            c->h_minv.clear();
+            dfree(c, c->d_minv);
+        }
         c->P = N;
         c->nln = nln_of(N);
         c->D = 2 * c->nln;
@@ -441,6 +449,23 @@ int tsp_set_tables(tsp_context *c, int N, const double *B, const double *G, cons
         h2d(c, c->t_w2, w2, c->nq2, "tables");
         h2d(c, c->t_w1, w1, c->nq1, "tables");
         hipcheck(c, hipStreamSynchronize(c->stream), "tables");
+    });
+}
+
+int tsp_set_mass(tsp_context *c, const double *base_invmass)
+{
+    return guarded(c, [&] {
+        if (!c->P) fail(c, TSP_ERR_STATE, "call tsp_set_tables before tsp_set_mass");
+        if (c->ne) fail(c, TSP_ERR_STATE, "call tsp_set_mass before tsp_set_mesh / tsp_upload_blocks");
+        c->h_minv.clear();
+        dfree(c, c->d_minv);
+        if (!base_invmass) return;  // back to the orthonormal (identity) case
+        for (int k = 0; k < c->nln * c->nln; ++k)
+            if (!std::isfinite(base_invmass[k])) fail(c, TSP_ERR_ARG, "non-finite base_invmass entry %d", k);
+        c->h_minv.assign(base_invmass, base_invmass + (size_t)c->nln * c->nln);
+        c->d_minv = dalloc<double>(c, (size_t)c->nln * c->nln, "invmass");
+        h2d(c, c->d_minv, base_invmass, (size_t)c->nln * c->nln, "invmass");
+        hipcheck(c, hipStreamSynchronize(c->stream), "invmass");
     });
 }
 
@@ -514,6 +539,7 @@ int tsp_set_mesh(tsp_context *c, int ne, const double *verts, const int *neigh, 
         a.blk = c->blk64;
         a.invm = c->invm;
         kcheck(c, launch_assemble(c->P, a, c->stream), "assemble");
+        if (c->d_minv) kcheck(c, launch_premul_invmass(c->P, ne, c->d_minv, c->blk64, c->stream), "invmass");
         hipcheck(c, hipStreamSynchronize(c->stream), "assemble");
         dfree(c, d_verts); dfree(c, d_nb); dfree(c, d_ng); dfree(c, d_mat); dfree(c, d_geo); dfree(c, d_face);
         operator_ready(c);
@@ -532,6 +558,7 @@ int tsp_upload_blocks(tsp_context *c, int ne, const int *neigh, const double *bl
         double *canon = dalloc<double>(c, n, "canonical blocks");
         h2d(c, canon, blocks, n, "blocks");
         kcheck(c, launch_canonical_to_blocks(c->P, ne, canon, c->blk64, c->stream), "layout");
+        if (c->d_minv) kcheck(c, launch_premul_invmass(c->P, ne, c->d_minv, c->blk64, c->stream), "invmass");
         h2d(c, c->invm, invm, ne, "invm");
         hipcheck(c, hipStreamSynchronize(c->stream), "upload");
         dfree(c, canon);
@@ -602,6 +629,19 @@ int tsp_set_sources(tsp_context *c, int nsrc, const int *col, const int *elem, c
         c->h_src_col.assign(col, col + nsrc);
         c->src_phi = dalloc<double>(c, (size_t)nsrc * c->nln, "source phi");
         c->wav = dalloc<double>(c, (size_t)c->ntab * nsrc * 2, "wavelet");
+        std::vector<double> mphi;
+        if (nsrc && !c->h_minv.empty()) {
+            // M^-1 f for f = F(t) phi: the folded operator needs the source in the same rows
+            mphi.assign((size_t)nsrc * c->nln, 0.0);
+            for (int k = 0; k < nsrc; ++k)
+                for (int i = 0; i < c->nln; ++i) {
+                    double acc = 0.0;
+                    for (int j = 0; j < c->nln; ++j)
+                        acc += c->h_minv[(size_t)i * c->nln + j] * phi[(size_t)k * c->nln + j];
+                    mphi[(size_t)k * c->nln + i] = acc;
+                }
+            phi = mphi.data();
+        }
         if (nsrc) {
             h2d(c, c->src_phi, phi, (size_t)nsrc * c->nln, "source phi");
             h2d(c, c->wav, wavelet, (size_t)c->ntab * nsrc * 2, "wavelet");
@@ -927,6 +967,7 @@ int tsp_share_operator(tsp_context *owner, tsp_context **out)
         c->blkA = owner->blkA;
         c->invm = owner->invm;
         c->KP = owner->KP;
+        c->h_minv = owner->h_minv;   // sources of a borrower get the same M^-1
         c->meta = dalloc<int4>(c, c->ne, "meta");
         if (c->precision != TSP_FP64) {
             c->sc_first = dalloc<float>(c, c->ne, "sc");

@@ -8,6 +8,11 @@
 
 using namespace tspeed;
 
+namespace {
+bool basis_ok(int basis) { return basis == 0 || basis == 1; }
+tspeed::BasisKind kind_of(int basis) { return basis == 1 ? tspeed::BasisKind::BoundaryAdapted : tspeed::BasisKind::Dubiner; }
+}  // namespace
+
 extern "C" {
 
 int tsh_gauleg(int n, double a, double b, double *x, double *w)
@@ -26,26 +31,7 @@ int tsh_dubiner(int N, double xi, double eta, double *phi, double *grad)
 
 int tsh_tables(int N, double *B, double *G, double *Be, double *Ge, double *w2, double *xy2, double *w1, double *x1)
 {
-    if (N < 1) return -1;
-    FESpaceRT X(nullptr, N);
-    const int nq1 = N + 1, nq2 = nq1 * nq1;
-    std::memcpy(B, X.B().data(), sizeof(double) * X.B().size());
-    std::memcpy(G, X.G().data(), sizeof(double) * X.G().size());
-    std::memcpy(Be, X.Bedge().data(), sizeof(double) * X.Bedge().size());
-    std::memcpy(Ge, X.Gedge().data(), sizeof(double) * X.Gedge().size());
-    for (int k = 0; k < nq2; ++k) {
-        w2[k] = X.quad().iweight(k);
-        if (xy2) {
-            const auto p = X.quad().inode(k);
-            xy2[2 * k] = p[0];
-            xy2[2 * k + 1] = p[1];
-        }
-    }
-    for (int k = 0; k < nq1; ++k) {
-        w1[k] = X.quad().eweight(k);
-        if (x1) x1[k] = X.quad().x1()[k];
-    }
-    return 0;
+    return tsh_tables_basis(N, 0, B, G, Be, Ge, w2, xy2, w1, x1);
 }
 
 int tsh_neighbors(int ne, const int *tri, int *neigh, int *neighedge)
@@ -150,10 +136,61 @@ int tsh_field_points(int ne, const double *verts, double *out)
     return 0;
 }
 
-int tsh_field_values(int N, int ne, const double *uh, double *out)
+int tsh_field_values(int N, int basis, int ne, const double *uh, double *out)
 {
-    if (N < 1 || N > 8 || ne < 0 || (ne && (!uh || !out))) return -1;
-    tspeed::field_values(N, ne, uh, out);
+    if (N < 1 || N > 8 || !basis_ok(basis) || ne < 0 || (ne && (!uh || !out))) return -1;
+    tspeed::field_values(N, ne, uh, out, kind_of(basis));
+    return 0;
+}
+
+int tsh_tables_basis(int N, int basis, double *B, double *G, double *Be, double *Ge, double *w2, double *xy2,
+                     double *w1, double *x1)
+{
+    if (N < 1 || N > 8 || !basis_ok(basis)) return -1;
+    try {
+        FESpaceRT X(nullptr, N, kind_of(basis));
+        const int nq1 = N + 1, nq2 = nq1 * nq1;
+        std::memcpy(B, X.B().data(), sizeof(double) * X.B().size());
+        std::memcpy(G, X.G().data(), sizeof(double) * X.G().size());
+        std::memcpy(Be, X.Bedge().data(), sizeof(double) * X.Bedge().size());
+        std::memcpy(Ge, X.Gedge().data(), sizeof(double) * X.Gedge().size());
+        for (int k = 0; k < nq2; ++k) {
+            w2[k] = X.quad().iweight(k);
+            if (xy2) {
+                const auto p = X.quad().inode(k);
+                xy2[2 * k] = p[0];
+                xy2[2 * k + 1] = p[1];
+            }
+        }
+        for (int k = 0; k < nq1; ++k) {
+            w1[k] = X.quad().eweight(k);
+            if (x1) x1[k] = X.quad().x1()[k];
+        }
+        return 0;
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
+int tsh_base_invmass(int N, int basis, double *out)
+{
+    if (N < 1 || N > 8 || !basis_ok(basis) || !out) return -1;
+    try {
+        const auto m = tspeed::base_invmass(kind_of(basis), N);
+        std::copy(m.begin(), m.end(), out);
+        return 0;
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
+int tsh_basis_eval(int N, int basis, int n, const double *xi, double *phi, double *grad)
+{
+    if (N < 0 || N > 8 || !basis_ok(basis) || n < 0 || (n && (!xi || !phi))) return -1;
+    const int nln = (N + 1) * (N + 2) / 2;
+    for (int q = 0; q < n; ++q)
+        tspeed::eval_basis(kind_of(basis), N, xi[2 * q], xi[2 * q + 1], phi + (size_t)q * nln,
+                           grad ? grad + (size_t)q * 2 * nln : nullptr);
     return 0;
 }
 
@@ -164,17 +201,17 @@ int tsh_quad_points(int N, int ne, const double *verts, double *out)
     return 0;
 }
 
-int tsh_project(int N, int ne, const double *verts, const double *f, double *uh)
+int tsh_project(int N, int basis, int ne, const double *verts, const double *f, double *uh)
 {
-    if (N < 1 || N > 8 || ne < 0 || (ne && (!verts || !f || !uh))) return -1;
-    tspeed::project_nodes(N, ne, verts, f, uh);
+    if (N < 1 || N > 8 || !basis_ok(basis) || ne < 0 || (ne && (!verts || !f || !uh))) return -1;
+    tspeed::project_nodes(N, ne, verts, f, uh, kind_of(basis));
     return 0;
 }
 
-int tsh_l2_error(int N, int ne, const double *verts, const double *uh, const double *uex, double *err)
+int tsh_l2_error(int N, int basis, int ne, const double *verts, const double *uh, const double *uex, double *err)
 {
-    if (N < 1 || N > 8 || ne < 0 || !err || (ne && (!verts || !uh || !uex))) return -1;
-    *err = tspeed::l2_error(N, ne, verts, uh, uex);
+    if (N < 1 || N > 8 || !basis_ok(basis) || ne < 0 || !err || (ne && (!verts || !uh || !uex))) return -1;
+    *err = tspeed::l2_error(N, ne, verts, uh, uex, kind_of(basis));
     return 0;
 }
 

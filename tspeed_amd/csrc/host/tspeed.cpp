@@ -67,6 +67,133 @@ std::array<double, 2> Gauss::edge_node(int f, int k) const
     return {s, 0.0};
 }
 
+namespace {
+// A value with its two partial derivatives (d/dxi, d/deta): forward-mode
+// differentiation of the basis expressions.
+struct Dual {
+    double v = 0, dx = 0, dy = 0;
+    Dual() = default;
+    Dual(double c) : v(c) {}
+    Dual(double a, double b, double c) : v(a), dx(b), dy(c) {}
+};
+inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.dx + b.dx, a.dy + b.dy}; }
+inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.dx - b.dx, a.dy - b.dy}; }
+inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.dx * b.v + a.v * b.dx, a.dy * b.v + a.v * b.dy}; }
+inline Dual operator/(Dual a, Dual b)
+{
+    const double q = a.v / b.v;
+    return {q, (a.dx - q * b.dx) / b.v, (a.dy - q * b.dy) / b.v};
+}
+inline Dual powi(Dual a, int n)
+{
+    Dual r(1.0);
+    for (int k = 0; k < n; ++k) r = r * a;
+    return r;
+}
+
+// jacobi_polynomial's recurrence on any arithmetic type
+template <typename T>
+T jacobi_t(int n, int alpha, int beta, T z)
+{
+    if (n == 0) return T(1.0);
+    const T first = T(0.5) * (T(double(alpha - beta)) + T(alpha + beta + 2.0) * z);
+    if (n == 1) return first;
+    const int apb = alpha + beta;
+    T poly(0.0), pm2(1.0), pm1 = first;
+    for (int k = 2; k <= n; ++k) {
+        const double a1 = 2.0 * k * (k + apb) * (2.0 * k + apb - 2.0);
+        double a2 = (2.0 * k + apb - 1.0) * (alpha * alpha - beta * beta);
+        double a3 = (2.0 * k + apb - 2.0) * (2.0 * k + apb - 1.0) * (2.0 * k + apb);
+        double a4 = 2.0 * (k + alpha - 1.0) * (k + beta - 1.0) * (2.0 * k + apb);
+        a2 /= a1;
+        a3 /= a1;
+        a4 /= a1;
+        poly = (T(a2) + T(a3) * z) * pm1 - T(a4) * pm2;
+        pm2 = pm1;
+        pm1 = poly;
+    }
+    return poly;
+}
+}  // namespace
+
+void BoundaryAdaptedBasis::eval(double xi, double eta, double *phi, double *grad) const
+{
+    const Dual x(xi, 1, 0), y(eta, 0, 1);
+    const Dual l0 = Dual(1.0) - x - y, om = Dual(1.0) - y;
+    const Dual zx = Dual(2.0) * x / om - Dual(1.0), zy = Dual(2.0) * y - Dual(1.0);
+    std::vector<Dual> f;
+    f.reserve(gdl());
+    f.push_back(l0);  // vertices
+    f.push_back(x);
+    f.push_back(y);
+    for (int i = 1; i < N_; ++i) f.push_back(l0 * x * powi(om, i - 1) * jacobi_t(i - 1, 1, 1, zx));  // eta = 0
+    for (int j = 1; j < N_; ++j) f.push_back(l0 * y * jacobi_t(j - 1, 1, 1, zy));                   // xi = 0
+    for (int j = 1; j < N_; ++j) f.push_back(x * y * jacobi_t(j - 1, 1, 1, zy));                    // xi + eta = 1
+    for (int j = 1; j < N_; ++j)
+        for (int i = 1; i < N_; ++i)
+            if (i + j < N_)
+                f.push_back(l0 * x * y * powi(om, i - 1) * jacobi_t(i - 1, 1, 1, zx) *
+                            jacobi_t(j - 1, 2 * i + 1, 1, zy));
+    for (size_t k = 0; k < f.size(); ++k) {
+        if (phi) phi[k] = f[k].v;
+        if (grad) {
+            grad[2 * k] = f[k].dx;
+            grad[2 * k + 1] = f[k].dy;
+        }
+    }
+}
+
+void eval_basis(BasisKind kind, int N, double xi, double eta, double *phi, double *grad)
+{
+    if (kind == BasisKind::BoundaryAdapted) BoundaryAdaptedBasis(N).eval(xi, eta, phi, grad);
+    else Dubiner(N).eval(xi, eta, phi, grad);
+}
+
+std::vector<double> base_invmass(BasisKind kind, int N)
+{
+    const int n = (N + 1) * (N + 2) / 2;
+    std::vector<double> inv((size_t)n * n, 0.0);
+    if (kind == BasisKind::Dubiner) {
+        for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+        return inv;
+    }
+    const Gauss q(N + 1);
+    std::vector<double> phi(n), M((size_t)n * n, 0.0);
+    for (int k = 0; k < q.nqn2d(); ++k) {
+        const auto xy = q.inode(k);
+        eval_basis(kind, N, xy[0], xy[1], phi.data(), nullptr);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) M[(size_t)i * n + j] += phi[i] * phi[j] * q.iweight(k);
+    }
+    // Gauss-Jordan elimination with partial pivoting on [M | I]
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(M[(size_t)r * n + c]) > std::fabs(M[(size_t)p * n + c])) p = r;
+        if (M[(size_t)p * n + c] == 0.0) throw std::runtime_error("base_invmass: singular mass matrix");
+        for (int j = 0; j < n; ++j) {
+            std::swap(M[(size_t)c * n + j], M[(size_t)p * n + j]);
+            std::swap(inv[(size_t)c * n + j], inv[(size_t)p * n + j]);
+        }
+        const double piv = M[(size_t)c * n + c];
+        for (int j = 0; j < n; ++j) {
+            M[(size_t)c * n + j] /= piv;
+            inv[(size_t)c * n + j] /= piv;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = M[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < n; ++j) {
+                M[(size_t)r * n + j] -= f * M[(size_t)c * n + j];
+                inv[(size_t)r * n + j] -= f * inv[(size_t)c * n + j];
+            }
+        }
+    }
+    return inv;
+}
+
 double jacobi_polynomial(int n, int alpha, int beta, double z)
 {
     if (n == 0) return 1.0;
@@ -355,7 +482,9 @@ int Mesh::locate(const Geo::Point &p, double *xi) const
 }
 
 // ---------------------------------------------------------------- FESpace
-FESpaceRT::FESpaceRT(Mesh_ptr m, int N) : mesh_(std::move(m)), N_(N), nln_((N + 1) * (N + 2) / 2), quad_(N + 1), sh_(N)
+FESpaceRT::FESpaceRT(Mesh_ptr m, int N, BasisKind kind)
+    : mesh_(std::move(m)), N_(N), nln_((N + 1) * (N + 2) / 2), kind_(kind), quad_(N + 1),
+      minv_(tspeed::base_invmass(kind, N))
 {
     const int nq1 = N + 1, nq2 = nq1 * nq1;
     B_.resize((size_t)nq2 * nln_);
@@ -365,7 +494,7 @@ FESpaceRT::FESpaceRT(Mesh_ptr m, int N) : mesh_(std::move(m)), N_(N), nln_((N + 
     std::vector<double> phi(nln_), gr(2 * nln_);
     for (int k = 0; k < nq2; ++k) {
         const auto xy = quad_.inode(k);
-        sh_.eval(xy[0], xy[1], phi.data(), gr.data());
+        basis(xy[0], xy[1], phi.data(), gr.data());
         for (int i = 0; i < nln_; ++i) {
             B_[(size_t)k * nln_ + i] = phi[i];
             G_[((size_t)i * 2) * nq2 + k] = gr[2 * i];
@@ -375,7 +504,7 @@ FESpaceRT::FESpaceRT(Mesh_ptr m, int N) : mesh_(std::move(m)), N_(N), nln_((N + 
     for (int f = 0; f < 3; ++f)
         for (int k = 0; k < nq1; ++k) {
             const auto xy = quad_.edge_node(f, k);
-            sh_.eval(xy[0], xy[1], phi.data(), gr.data());
+            basis(xy[0], xy[1], phi.data(), gr.data());
             for (int i = 0; i < nln_; ++i) {
                 Be_[((size_t)f * nq1 + k) * nln_ + i] = phi[i];
                 Ge_[(((size_t)i * 3 + f) * 2) * nq1 + k] = gr[2 * i];
@@ -418,9 +547,15 @@ std::vector<double> FESpaceRT::transform(const std::function<std::array<double, 
                 rhs[nln_ + i] += F[1] * dx * B_[(size_t)k * nln_ + i];
             }
         }
+        // uh = 1/detJ * base_invmass * rhs (FESpace_imp.hpp:151-152)
         for (int i = 0; i < nln_; ++i) {
-            uh[(size_t)e * nln_ + i] = 1. / det * rhs[i];
-            uh[(size_t)n * nln_ + (size_t)e * nln_ + i] = 1. / det * rhs[nln_ + i];
+            double sx = 0, sy = 0;
+            for (int j = 0; j < nln_; ++j) {
+                sx += minv_[(size_t)i * nln_ + j] * rhs[j];
+                sy += minv_[(size_t)i * nln_ + j] * rhs[nln_ + j];
+            }
+            uh[(size_t)e * nln_ + i] = 1. / det * sx;
+            uh[(size_t)n * nln_ + (size_t)e * nln_ + i] = 1. / det * sy;
         }
     }
     return uh;
@@ -450,13 +585,11 @@ void field_points(int ne, const double *verts, double *out)
                       &out[((size_t)e * 4 + k) * 2 + 1]);
 }
 
-void field_values(int N, int ne, const double *uh, double *out)
+void field_values(int N, int ne, const double *uh, double *out, BasisKind kind)
 {
-    const Dubiner sh(N);
-    const int nln = sh.gdl();
+    const int nln = (N + 1) * (N + 2) / 2;
     std::vector<double> vb(4 * (size_t)nln);  // basis at the plot points
-    for (int k = 0; k < 4; ++k)
-        for (int i = 0; i < nln; ++i) vb[(size_t)k * nln + i] = sh.phi(i, kPlotPoints[k][0], kPlotPoints[k][1]);
+    for (int k = 0; k < 4; ++k) eval_basis(kind, N, kPlotPoints[k][0], kPlotPoints[k][1], &vb[(size_t)k * nln], nullptr);
     for (int e = 0; e < ne; ++e)
         for (int k = 0; k < 4; ++k) {
             double vx = 0, vy = 0;
@@ -481,17 +614,15 @@ void quad_points(int N, int ne, const double *verts, double *out)
         }
 }
 
-void project_nodes(int N, int ne, const double *verts, const double *f, double *uh)
+void project_nodes(int N, int ne, const double *verts, const double *f, double *uh, BasisKind kind)
 {
     const Gauss q(N + 1);
-    const Dubiner sh(N);
-    const int nq2 = q.nqn2d(), nln = sh.gdl();
-    std::vector<double> phi(nln), grad(2 * (size_t)nln);
+    const int nq2 = q.nqn2d(), nln = (N + 1) * (N + 2) / 2;
+    const auto minv = tspeed::base_invmass(kind, N);
     std::vector<double> Bq((size_t)nq2 * nln);
     for (int k = 0; k < nq2; ++k) {
         const auto xy = q.inode(k);
-        sh.eval(xy[0], xy[1], phi.data(), grad.data());
-        for (int i = 0; i < nln; ++i) Bq[(size_t)k * nln + i] = phi[i];
+        eval_basis(kind, N, xy[0], xy[1], &Bq[(size_t)k * nln], nullptr);
     }
     for (int e = 0; e < ne; ++e) {
         const double det = det_of(verts + 6 * (size_t)e);
@@ -505,23 +636,25 @@ void project_nodes(int N, int ne, const double *verts, const double *f, double *
             }
         }
         for (int i = 0; i < nln; ++i) {
-            uh[(size_t)e * nln + i] = 1. / det * rhs[i];
-            uh[(size_t)ne * nln + (size_t)e * nln + i] = 1. / det * rhs[nln + i];
+            double sx = 0, sy = 0;
+            for (int j = 0; j < nln; ++j) {
+                sx += minv[(size_t)i * nln + j] * rhs[j];
+                sy += minv[(size_t)i * nln + j] * rhs[nln + j];
+            }
+            uh[(size_t)e * nln + i] = 1. / det * sx;
+            uh[(size_t)ne * nln + (size_t)e * nln + i] = 1. / det * sy;
         }
     }
 }
 
-double l2_error(int N, int ne, const double *verts, const double *uh, const double *uex)
+double l2_error(int N, int ne, const double *verts, const double *uh, const double *uex, BasisKind kind)
 {
     const Gauss q(N + 1);
-    const Dubiner sh(N);
-    const int nq2 = q.nqn2d(), nln = sh.gdl();
-    std::vector<double> phi(nln), grad(2 * (size_t)nln);
+    const int nq2 = q.nqn2d(), nln = (N + 1) * (N + 2) / 2;
     std::vector<double> Bq((size_t)nq2 * nln);
     for (int k = 0; k < nq2; ++k) {
         const auto xy = q.inode(k);
-        sh.eval(xy[0], xy[1], phi.data(), grad.data());
-        for (int i = 0; i < nln; ++i) Bq[(size_t)k * nln + i] = phi[i];
+        eval_basis(kind, N, xy[0], xy[1], &Bq[(size_t)k * nln], nullptr);
     }
     double error = 0;
     for (int e = 0; e < ne; ++e) {
@@ -554,7 +687,7 @@ void FESpaceRT::field_out(const std::string &fname, const std::vector<double> &u
 {
     if (uh.size() != 2 * (size_t)ne() * nln_) throw std::invalid_argument("field_out: uh has the wrong size");
     std::vector<double> val(8 * (size_t)ne());
-    field_values(N_, ne(), uh.data(), val.data());
+    field_values(N_, ne(), uh.data(), val.data(), kind_);
     const std::string name = fname + "u_" + std::to_string(step) + ".field";
     std::ofstream out(name);
     if (!out.is_open()) throw std::runtime_error("File " + name + " may not be opened");
@@ -574,7 +707,7 @@ double FESpaceRT::L2error(const std::function<std::array<double, 2>(double, doub
         f[2 * k] = F[0];
         f[2 * k + 1] = F[1];
     }
-    return l2_error(N_, ne(), v.data(), uh.data(), f.data());
+    return l2_error(N_, ne(), v.data(), uh.data(), f.data(), kind_);
 }
 
 // ---------------------------------------------------------------- Parameters
@@ -627,7 +760,7 @@ void PointWiseEntity::add_point(const FESpaceRT &Xh, const Geo::Point &p)
     ie_.push_back(e);
     relp_.emplace_back(xi[0], xi[1]);
     std::vector<double> s(Xh.nln());
-    Xh.shape().eval(xi[0], xi[1], s.data(), nullptr);
+    Xh.basis(xi[0], xi[1], s.data(), nullptr);
     shape_.push_back(std::move(s));
 }
 
@@ -682,6 +815,8 @@ LeapFrog::LeapFrog(FESpace_ptr Xh, const Parameters &p, const Receivers &r, doub
     const auto w2 = Xh_->w2(), w1 = Xh_->w1();
     check(tsp_set_tables(ctx_, Xh_->degree(), Xh_->B().data(), Xh_->G().data(), Xh_->Bedge().data(),
                          Xh_->Gedge().data(), w2.data(), w1.data()));
+    // invM = 1/rho 1/detJ base_invmass (Matrices_imp.hpp:183-190)
+    if (!Xh_->orthonormal()) check(tsp_set_mass(ctx_, Xh_->base_invmass().data()));
     const Mesh &m = *Xh_->mesh();
     const auto v = m.vertex_array();
     check(tsp_set_mesh(ctx_, m.ne(), v.data(), m.neigh().data(), m.neighedge().data(), p.lambda().data(),

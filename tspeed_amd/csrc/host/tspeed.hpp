@@ -69,6 +69,38 @@ private:
     int N_;
 };
 
+// The reference's non-orthonormal BoundaryAdapted basis (ShapeFunctions_imp.hpp:
+// 56-118): three vertex modes, N-1 modes per edge and the interior bubbles, in
+// the reference's order; gradients by forward-mode differentiation of the same
+// expressions.
+class BoundaryAdaptedBasis {
+public:
+    explicit BoundaryAdaptedBasis(int N) : N_(N) {}
+    int degree() const { return N_; }
+    int gdl() const { return (N_ + 1) * (N_ + 2) / 2; }
+    void eval(double xi, double eta, double *phi, double *grad) const;
+
+private:
+    int N_;
+};
+
+enum class BasisKind { Dubiner = 0, BoundaryAdapted = 1 };
+void eval_basis(BasisKind kind, int N, double xi, double eta, double *phi, double *grad);
+// base_invmass (nln x nln, row-major): identity for Dubiner, the inverse of the
+// quadrature mass matrix sum_k phi_i phi_j w_k otherwise (FESpace_imp.hpp:49-67)
+std::vector<double> base_invmass(BasisKind kind, int N);
+
+// Type tags for source compatibility with FESpace<N, Q, S> (FESpace.hpp:37):
+// FESpace<N, Dunavant<2*N>, BoundaryAdapted<N>> selects the BoundaryAdapted
+// basis. The quadrature is always the collapsed Gauss rule with N+1 nodes per
+// direction (exact to degree 2N on the triangle, as Dunavant<2N>, and N+1 edge
+// nodes exact to degree 2N+1): every integral of the operator is polynomial of
+// degree <= 2N, so either rule assembles the same operator up to roundoff.
+template <int M> struct BoundaryAdapted {};
+template <int M> struct Dunavant {};
+template <int M> struct GaussRule {};
+
+
 // ------------------------------------------------------------- geometry / mesh
 namespace Geo {
 struct Point {
@@ -128,14 +160,17 @@ using Mesh_ptr = std::shared_ptr<Mesh>;
 // quadrature nodes, in the layout include/tspeed_amd.h documents.
 class FESpaceRT {
 public:
-    FESpaceRT(Mesh_ptr m, int N);
+    FESpaceRT(Mesh_ptr m, int N, BasisKind kind = BasisKind::Dubiner);
     virtual ~FESpaceRT() = default;
     Mesh_ptr mesh() const { return mesh_; }
     int degree() const { return N_; }
     int nln() const { return nln_; }
     int ne() const { return mesh_ ? mesh_->ne() : 0; }
     const Gauss &quad() const { return quad_; }
-    const Dubiner &shape() const { return sh_; }
+    BasisKind basis_kind() const { return kind_; }
+    bool orthonormal() const { return kind_ == BasisKind::Dubiner; }
+    void basis(double xi, double eta, double *phi, double *grad) const { eval_basis(kind_, N_, xi, eta, phi, grad); }
+    const std::vector<double> &base_invmass() const { return minv_; }
     const std::vector<double> &B() const { return B_; }
     const std::vector<double> &G() const { return G_; }
     const std::vector<double> &Bedge() const { return Be_; }
@@ -159,14 +194,24 @@ public:
 private:
     Mesh_ptr mesh_;
     int N_, nln_;
+    BasisKind kind_;
     Gauss quad_;
-    Dubiner sh_;
-    std::vector<double> B_, G_, Be_, Ge_;
+    std::vector<double> B_, G_, Be_, Ge_, minv_;
 };
-template <int N>
+
+template <typename S>
+struct basis_of {
+    static constexpr BasisKind kind = BasisKind::Dubiner;
+};
+template <int M>
+struct basis_of<BoundaryAdapted<M>> {
+    static constexpr BasisKind kind = BasisKind::BoundaryAdapted;
+};
+
+template <int N, typename Q = void, typename S = void>
 class FESpace : public FESpaceRT {
 public:
-    explicit FESpace(Mesh_ptr m) : FESpaceRT(std::move(m), N) {}
+    explicit FESpace(Mesh_ptr m) : FESpaceRT(std::move(m), N, basis_of<S>::kind) {}
 };
 using FESpace_ptr = std::shared_ptr<FESpaceRT>;
 
@@ -174,10 +219,13 @@ using FESpace_ptr = std::shared_ptr<FESpaceRT>;
 // verts[e*6 + 2v + c]; uh component-major (c*ne*nln + e*nln + i).
 extern const double kPlotPoints[4][2];
 void field_points(int ne, const double *verts, double *out);                       // out[(e*4+k)*2+c]
-void field_values(int N, int ne, const double *uh, double *out);                    // out[(e*4+k)*2+c]
+void field_values(int N, int ne, const double *uh, double *out,
+                  BasisKind kind = BasisKind::Dubiner);                              // out[(e*4+k)*2+c]
 void quad_points(int N, int ne, const double *verts, double *out);                  // out[(e*nq2+k)*2+c]
-void project_nodes(int N, int ne, const double *verts, const double *f, double *uh); // f at quad_points
-double l2_error(int N, int ne, const double *verts, const double *uh, const double *uex); // uex at quad_points
+void project_nodes(int N, int ne, const double *verts, const double *f, double *uh,
+                   BasisKind kind = BasisKind::Dubiner);                             // f at quad_points
+double l2_error(int N, int ne, const double *verts, const double *uh, const double *uex,
+                BasisKind kind = BasisKind::Dubiner);                                // uex at quad_points
 
 // Per-element Lame parameters and density (FESpace.hpp:194-247).
 class Parameters {

@@ -45,12 +45,18 @@ class Problem:
     rec_xy: np.ndarray          # (nrec, 2)
     stride: int = 1
     meta: dict = field(default_factory=dict)
+    basis: int = capi.DUBINER   # or capi.BOUNDARY_ADAPTED
 
     def __post_init__(self):
         self.verts = self.pts[self.tri].reshape(-1, 6)
         self.neigh, self.neighedge = capi.neighbors(self.tri)
-        self.src_elem, _, self.src_phi = capi.locate(self.pts, self.tri, self.src_xy[:, 0], self.src_xy[:, 1], self.N)
-        self.rec_elem, _, self.rec_phi = capi.locate(self.pts, self.tri, self.rec_xy[:, 0], self.rec_xy[:, 1], self.N)
+        self.src_elem, sxi, self.src_phi = capi.locate(self.pts, self.tri, self.src_xy[:, 0], self.src_xy[:, 1],
+                                                       self.N)
+        self.rec_elem, rxi, self.rec_phi = capi.locate(self.pts, self.tri, self.rec_xy[:, 0], self.rec_xy[:, 1],
+                                                       self.N)
+        if self.basis != capi.DUBINER:
+            self.src_phi = capi.basis_eval(self.N, self.basis, sxi)
+            self.rec_phi = capi.basis_eval(self.N, self.basis, rxi)
         if (self.src_elem < 0).any() or (self.rec_elem < 0).any():
             raise ValueError(f"{self.name}: a source or receiver lies outside the mesh")
 
@@ -298,7 +304,9 @@ def prepare_instance(prob: Problem, share=None, stride=None, nsteps=None):
 def run_engine(prob: Problem, precision="fp64", engine=None, u0=None, v0=None, stride=None):
     """Assemble and run `prob` on the device; returns (engine, u_final, seismograms)."""
     eng = engine or capi.Engine(precision)
-    eng.set_tables(prob.N)
+    eng.set_tables(prob.N, capi.tables(prob.N, prob.basis))
+    if prob.basis != capi.DUBINER:
+        eng.set_mass(capi.base_invmass(prob.N, prob.basis))
     eng.set_mesh(prob.verts, prob.neigh, prob.neighedge, prob.lam, prob.mu, prob.rho, prob.penalty)
     stride = prob.stride if stride is None else stride
     eng.set_sources(prob.src_elem, prob.src_phi, prob.wavelet())
