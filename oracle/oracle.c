@@ -12,7 +12,7 @@
  *
  * Pinning of each part (what fails if this restatement is wrong):
  *   orc_gauleg / orc_gauss_rule .. numpy leggauss golden vectors; monomial exactness
- *                                  a!b!/(a+b+2)! on the triangle (tests/test_oracle_tables.py)
+ *                                  a!b!/(a+b+2)! on the triangle (tests/test_oracle.py)
  *   orc_jacobi .................... scipy.special.eval_jacobi golden vectors
  *   orc_dubiner (phi) ............. orthonormality on the triangle, phi_00 = sqrt(2)
  *   orc_dubiner (grad) ............ central finite differences of phi
@@ -25,7 +25,17 @@
  *                                   constant-strain energy; S kills continuous fields;
  *                                   invM = 1/(rho*detJ)
  *   orc_run ....................... discrete energy conservation with f = 0; one-element
- *                                   hand step; Taylor-start identity
+ *                                   hand step; Taylor-start identity; convergence to an
+ *                                   exact standing wave at order N+1 (tests/test_oracle.py)
+ *   orc_boundary_adapted .......... vertex modes = barycentric coordinates; edge/bubble
+ *                                   traces vanish where they must and equal the closed
+ *                                   Jacobi forms on their edge; gradient by central
+ *                                   differences; span = P_N (exact Dubiner combination)
+ *   orc_base_invmass .............. inverse of the quadrature mass; mass = T^T T through
+ *                                   the Dubiner change of basis
+ *   orc_assemble_basis ............ lambda_max(M^-1 B) and seismograms identical for the
+ *                                   BoundaryAdapted and Dubiner spaces (basis-independent
+ *                                   dynamics)
  *   Only running the compiled reference would pin its exact FP64 trajectory (penalty
  *   tensor with 2*mu_bar, the 1/2-weighted consistency term, Eigen's summation order).
  */
