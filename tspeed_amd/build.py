@@ -26,14 +26,32 @@ HOST_SRC = ["host/tspeed.cpp", "host/host_capi.cpp", "host/server.cpp"]
 # Packed FP32 VALU forms are switched off for every device compile.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-Wall", "-Wno-unused-function"]
+             "-Wall", "-Wno-unused-function", "-Rpass-analysis=kernel-resource-usage"]
+
+# Step kernels run every time step: a register spill there costs bandwidth (and
+# one spilling variant was measured wrong), so the build refuses any.
+HOT_KERNELS = ("k_step64", "k_step32c", "k_multi32c", "k_step_b32", "k_step_b16x3")
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
+
+
+def _check_scratch(out):
+    """Raise if a hot kernel in the resource-usage remarks uses scratch."""
+    name = None
+    for line in out.splitlines():
+        if "Function Name:" in line:
+            name = line.split("Function Name:")[1].split()[0]
+        elif "ScratchSize" in line and name and any(k in name for k in HOT_KERNELS):
+            size = int(line.split("ScratchSize [bytes/lane]:")[1].split()[0])
+            if size:
+                raise RuntimeError(f"hot kernel {name} spills {size} bytes/lane to scratch")
 
 
 def _run(cmd):
     p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if p.returncode == 0 and "-Rpass-analysis=kernel-resource-usage" in cmd:
+        _check_scratch(p.stdout)
     out = "\n".join(l for l in p.stdout.splitlines() if "packed-fp32-ops" not in l and
-                    "not a recognized feature for this target" not in l)
+                    "not a recognized feature for this target" not in l and "remark:" not in l)
     if p.returncode != 0:
         raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
     return out

@@ -264,6 +264,26 @@ __global__ void k_premul_invmass(int ne, const double *minv, double *blk)
     }
 }
 
+// Uploaded operators: the blocks of absent neighbours are set to zero, as the
+// device assembly leaves them (the step kernels read every slot).
+template <int P>
+__global__ void k_zero_missing(int ne, const int *neigh, double *blk)
+{
+    constexpr int D = Geom<P>::D;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)ne * 3 || neigh[q] >= 0) return;
+    double *b = blk + ((q / 3) * 4 + 1 + q % 3) * D * D;
+    for (int k = 0; k < D * D; ++k) b[k] = 0.0;
+}
+
+template <int P>
+int zero_missing_p(int ne, const int *neigh, double *blk, hipStream_t s)
+{
+    const size_t n = (size_t)ne * 3;
+    hipLaunchKernelGGL(k_zero_missing<P>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ne, neigh, blk);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 template <int P>
 int premul_p(int ne, const double *minv, double *blk, hipStream_t s)
 {
@@ -306,6 +326,13 @@ int conv_p(bool to_canon, int ne, const double *src, double *dst, hipStream_t s)
 int launch_assemble(int P, const AssembleArgs &a, hipStream_t s)
 {
 #define CALL(p) assemble_p<p>(a, s)
+    TSP_DISPATCH_P(P, CALL)
+#undef CALL
+}
+
+int launch_zero_missing(int P, int ne, const int *neigh, double *blk, hipStream_t s)
+{
+#define CALL(p) zero_missing_p<p>(ne, neigh, blk, s)
     TSP_DISPATCH_P(P, CALL)
 #undef CALL
 }

@@ -558,10 +558,14 @@ int tsp_upload_blocks(tsp_context *c, int ne, const int *neigh, const double *bl
         double *canon = dalloc<double>(c, n, "canonical blocks");
         h2d(c, canon, blocks, n, "blocks");
         kcheck(c, launch_canonical_to_blocks(c->P, ne, canon, c->blk64, c->stream), "layout");
+        int *d_nb = dalloc<int>(c, (size_t)ne * 3, "neigh");
+        h2d(c, d_nb, neigh, (size_t)ne * 3, "neigh");
+        kcheck(c, launch_zero_missing(c->P, ne, d_nb, c->blk64, c->stream), "zero blocks");
         if (c->d_minv) kcheck(c, launch_premul_invmass(c->P, ne, c->d_minv, c->blk64, c->stream), "invmass");
         h2d(c, c->invm, invm, ne, "invm");
         hipcheck(c, hipStreamSynchronize(c->stream), "upload");
         dfree(c, canon);
+        dfree(c, d_nb);
         operator_ready(c);
     });
 }

@@ -78,6 +78,7 @@ int launch_assemble(int P, const AssembleArgs &a, hipStream_t s);
 int launch_blocks_to_canonical(int P, int ne, const double *blk, double *canon, hipStream_t s);
 int launch_canonical_to_blocks(int P, int ne, const double *canon, double *blk, hipStream_t s);
 int launch_premul_invmass(int P, int ne, const double *minv, double *blk, hipStream_t s);
+int launch_zero_missing(int P, int ne, const int *neigh, double *blk, hipStream_t s);
 int launch_demote_blocks(int P, int ne, const double *src, float *dst, hipStream_t s);
 
 struct StepArgs {

@@ -79,7 +79,7 @@ __device__ inline void multi_elem(const MInst &I, int e, int lane, float (*xs)[k
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     float acc[G::RPL];
-    spmv<G, float, 4>(I.blk, e, m, xv, lane, acc);
+    spmv<G, float, 4>(I.blk, e, xv, lane, acc);
 
     StepArgs a{};
     a.nrec = I.nrec;

@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kThreads) void k_step64(StepArgs a)
     if (!act) return;
 
     double acc[G::RPL];
-    spmv<G, double, 2>(a.blk64, e, m, xs, lane, acc);
+    spmv<G, double, 2>(a.blk64, e, xs, lane, acc);
 
     const int4 ex = m.w >= 0 ? a.extra[m.w] : make_int4(0, 0, 0, 0);
     double unew[G::RPL];
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kThreads) void k_step32c(StepArgs a)
     if (!act) return;
 
     float acc[G::RPL];
-    spmv<G, float, 4>(a.blk32, e, m, xs, lane, acc);
+    spmv<G, float, 4>(a.blk32, e, xs, lane, acc);
 
     const int4 ex = m.w >= 0 ? a.extra[m.w] : make_int4(0, 0, 0, 0);
     float unew[G::RPL];
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kThreads) void k_apply_minvB(int ne, const int4 *me
     __syncthreads();
     if (!act) return;
     double acc[G::RPL];
-    spmv<G, double, 2>(blk, e, m, xs, lane, acc);
+    spmv<G, double, 2>(blk, e, xs, lane, acc);
     const int g = lane / G::RW, r = lane % G::RW;
     const double iv = invm[e];
 #pragma unroll
