@@ -149,7 +149,13 @@ def run_serving(n_req=32, loads=(0.5, 0.8), sample_frac=0.05, seed=11):
     work = sum(p.ndof * len(p.src_elem) * p.nsteps for p in reqs)
     out = dict(requests=n_req, dof_steps=work,
                workload="config5: Poisson arrivals, HBM-footprint admission at 80%, continuous batching "
-                        "(chunks of 32 steps)")
+                        "(chunks of 32 steps); each request one 32-column MFMA context on its own stream")
+    # the other execution mode (one instance per source in the shared set), saturated, for comparison
+    srv, ids, lat, wall = W.serve(reqs, rate=0, mode=capi.Server.INSTANCES)
+    st = srv.stats()
+    out["instances_mode_saturated"] = dict(value=st["dof_steps"] / st["busy_s"], wall_s=wall,
+                                           p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)))
+    srv.close()
     srv, ids, lat, wall = W.serve(reqs, rate=0)
     st = srv.stats()
     cap = n_req / wall

@@ -269,12 +269,19 @@ typedef struct {
 
 int    tsp_server_create(int device, double mem_fraction, int chunk, tsp_server **out);
 void   tsp_server_destroy(tsp_server *s);
+/* How requests run (before the first submit):
+ *   TSP_SERVE_COLUMNS (default): one 32-column context per request (sources as
+ *     columns of a block SpMM on MFMA), each request's chunk on its own stream;
+ *   TSP_SERVE_INSTANCES: one single-column context per source (sharing the
+ *     request's operator) in one tsp_multi set. */
+enum { TSP_SERVE_INSTANCES = 0, TSP_SERVE_COLUMNS = 1 };
+int    tsp_server_set_mode(tsp_server *s, int mode);
 const char *tsp_server_last_error(const tsp_server *s);
 double tsp_server_now(void);
 /* Device bytes the loop books for a request while it runs (resident) and the
  * extra it needs while its operator is assembled (peak).  A request is
  * admitted when used + resident + peak <= budget. */
-int    tsp_server_footprint(const tsp_request *req, long long *resident, long long *peak);
+int    tsp_server_footprint(const tsp_request *req, int mode, long long *resident, long long *peak);
 /* Queue a request (its arrays are copied) that arrives at time `arrival`.
  * TSP_ERR_NOMEM if resident + peak exceeds the whole budget. */
 int    tsp_server_submit(tsp_server *s, const tsp_request *req, double arrival, int *id);

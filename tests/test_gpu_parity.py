@@ -603,45 +603,55 @@ def _req(k, nsteps=120, n=(8, 14), nsrc=(1, 4)):
     return W.config5_request(k, n_range=n, nsrc_range=nsrc, steps_range=(nsteps, nsteps + 40), nrec_range=(6, 12))
 
 
-def test_server_result_is_the_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("mode", ["columns", "instances"])
+def test_server_result_is_the_same_alone_and_in_the_mix(mode):
     """A request's seismograms are identical whether it runs alone or admitted
-    among five others with staggered arrivals; they equal the single-context FP32
-    path source by source, and stay within 1e-4 of the FP64 path."""
+    among five others with staggered arrivals, and stay within 1e-4 of the FP64
+    path source by source.  Instances mode: equal to the single-context FP32
+    path bit for bit; columns mode: equal to the same request run as a
+    32-column context."""
     import copy
+    md = capi.Server.COLUMNS if mode == "columns" else capi.Server.INSTANCES
     target = _req(0, nsrc=(3, 3))
-    srv, ids, lat, _ = W.serve([target], rate=0)
+    srv, ids, lat, _ = W.serve([target], rate=0, mode=md)
     alone, _ = srv.result(ids[0])
     srv.close()
     mix = [_req(k) for k in range(1, 6)]
-    srv, ids, lat, wall = W.serve(mix[:2] + [target] + mix[2:], rate=40.0, seed=3, chunk=8)
+    srv, ids, lat, wall = W.serve(mix[:2] + [target] + mix[2:], rate=40.0, seed=3, chunk=8, mode=md)
     mixed, t = srv.result(ids[2])
-    say(f"  mix of 6: latencies {np.round(lat, 3).tolist()} s, wall {wall:.2f} s")
+    say(f"  {mode}: mix of 6: latencies {np.round(lat, 3).tolist()} s, wall {wall:.2f} s")
     srv.close()
     np.testing.assert_array_equal(mixed, alone)
+    if md == capi.Server.COLUMNS:
+        _, sb = W.run_batched(target, 32)
+        np.testing.assert_array_equal(alone, sb[..., :3, :].transpose(2, 0, 1, 3))
     for k in range(len(target.src_elem)):
         q = copy.copy(target)
         q.src_xy, q.f0, q.t0, q.dirs = target.src_xy[k:k + 1], target.f0[k:k + 1], target.t0[k:k + 1], \
             target.dirs[k:k + 1]
         q.__post_init__()
-        _, _, s32 = W.run_engine(q, "fp32c")
-        np.testing.assert_array_equal(alone[k], s32)
+        if md == capi.Server.INSTANCES:
+            _, _, s32 = W.run_engine(q, "fp32c")
+            np.testing.assert_array_equal(alone[k], s32)
         _, _, s64 = W.run_engine(q, "fp64")
         err = rel_l2(alone[k], s64)
         say(f"  source {k}: vs FP64 {err:.3e}")
         assert err <= 1e-4
 
 
-def test_server_admission_respects_the_memory_budget():
+@pytest.mark.parametrize("mode", ["columns", "instances"])
+def test_server_admission_respects_the_memory_budget(mode):
     """With a budget that holds one request at a time, three requests run one
     after the other (each admitted no earlier than the previous completion), and
     all three results equal the request run with an ample budget; a request
     larger than the budget is refused at submission."""
     from tspeed_amd import capi as C
+    md = C.Server.COLUMNS if mode == "columns" else C.Server.INSTANCES
     _, total = ctypes_mem()
     req = _req(7, nsteps=60, nsrc=(2, 2))
-    res, peak = C.Server.footprint(req)
+    res, peak = C.Server.footprint(req, mode=md)
     frac = (res + peak + res // 2) / total
-    srv, ids, lat, _ = W.serve([req, req, req], rate=0, mem_fraction=frac, chunk=16)
+    srv, ids, lat, _ = W.serve([req, req, req], rate=0, mem_fraction=frac, chunk=16, mode=md)
     times = sorted((srv.result(i)[1] for i in ids), key=lambda t: t["admitted"])
     for a, b in zip(times, times[1:]):
         assert b["admitted"] >= a["completed"]
@@ -649,7 +659,7 @@ def test_server_admission_respects_the_memory_budget():
     with pytest.raises(C.EngineError):
         srv.submit(_req(8, nsteps=60, n=(30, 30), nsrc=(4, 4)), srv.now())
     srv.close()
-    srv, ids, _, _ = W.serve([req], rate=0)
+    srv, ids, _, _ = W.serve([req], rate=0, mode=md)
     ref = srv.result(ids[0])[0]
     srv.close()
     for o in outs:

@@ -74,7 +74,8 @@ SIGNATURES = {
     "tsp_server_destroy": (None, [_vp]),
     "tsp_server_last_error": (ctypes.c_char_p, [_vp]),
     "tsp_server_now": (_d, []),
-    "tsp_server_footprint": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
+    "tsp_server_footprint": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
+    "tsp_server_set_mode": (_i, [_vp, _i]),
     "tsp_server_submit": (_i, [_vp, _vp, _d, ctypes.POINTER(_i)]),
     "tsp_server_tick": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "tsp_server_result": (_i, [_vp, _i, _vp, ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d),
@@ -518,14 +519,18 @@ class Request(ctypes.Structure):
 class Server:
     """The request loop (tsp_server_*)."""
 
-    def __init__(self, device=0, mem_fraction=0.8, chunk=32):
+    INSTANCES, COLUMNS = 0, 1
+
+    def __init__(self, device=0, mem_fraction=0.8, chunk=32, mode=COLUMNS):
         self._L = lib()
         h = _vp()
         rc = self._L.tsp_server_create(device, float(mem_fraction), int(chunk), ctypes.byref(h))
         if rc != TSP_OK:
             raise EngineError(rc, "tsp_server_create failed")
         self._h = h
+        self.mode = mode
         self.meta = {}
+        self._ck(self._L.tsp_server_set_mode(self._h, int(mode)))
 
     def _ck(self, rc):
         if rc != TSP_OK:
@@ -559,12 +564,12 @@ class Server:
         return i.value
 
     @staticmethod
-    def footprint(prob, stride=1):
+    def footprint(prob, stride=1, mode=COLUMNS):
         """(resident, peak) device bytes the loop books for `prob`."""
         q = Request(prob.N, prob.ne, None, None, None, None, None, None, prob.penalty, prob.dt, prob.nsteps,
                     len(prob.src_elem), None, None, None, len(prob.rec_elem), None, None, int(stride))
         a, b = ctypes.c_longlong(), ctypes.c_longlong()
-        rc = lib().tsp_server_footprint(ctypes.byref(q), ctypes.byref(a), ctypes.byref(b))
+        rc = lib().tsp_server_footprint(ctypes.byref(q), int(mode), ctypes.byref(a), ctypes.byref(b))
         if rc != TSP_OK:
             raise EngineError(rc, "tsp_server_footprint: bad request")
         return a.value, b.value

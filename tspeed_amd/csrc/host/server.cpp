@@ -4,11 +4,15 @@
 // batching: admissions and retirements happen between chunks of steps, while
 // the others keep their fields resident), and leave with their seismograms.
 //
-// A request is one mesh with 1..16 point sources; it becomes one owner context
-// that assembles the operator on the device and (nsrc-1) contexts sharing that
-// operator, all added to one tsp_multi set.  Each instance's arithmetic is the
-// same whatever else is in the set, so a request's result does not depend on
-// the mix it ran in.
+// A request is one mesh with 1..16 point sources.  Two ways to run it:
+//   columns (default): one 32-column context, one source per column, stepped as
+//     a block SpMM on the MFMA cores; every running request's chunk of steps is
+//     queued on its own stream, so requests overlap on the device;
+//   instances: one owner context that assembles the operator and (nsrc-1)
+//     contexts sharing it, all added to one tsp_multi set (one launch per degree
+//     per step for everything running).
+// Either way a request's arithmetic does not depend on what else runs, so its
+// result is the same alone and in a mix.
 //
 // Uses only the public C-ABI (include/tspeed_amd.h, include/tspeed_host.h).
 #include "../../../include/tspeed_amd.h"
@@ -42,15 +46,30 @@ struct Request {
     std::vector<double> verts, lambda, mu, rho, src_phi, wavelet, rec_phi;
     std::vector<int> neigh, neighedge, src_elem, rec_elem;
     long long footprint = 0;
-    std::vector<int> inst;        // multi-set ids, owner first
+    std::vector<int> inst;        // multi-set ids, owner first (instances mode)
+    tsp_context *ctx = nullptr;   // the multi-column context (columns mode)
+    int ncols = 0, done = 0;
     std::vector<double> seis;     // [src][sample][rec][2]
     int nsamples = 0;
     int state = 0;                // 0 queued, 1 running, 2 done, 3 failed
     std::string error;
 };
 
-long long footprint_of(const Request &r)
+long long footprint_columns(const Request &r)
 {
+    const long long nln = (long long)(r.N + 1) * (r.N + 2) / 2, d = 2 * nln;
+    const long long kp = 32 * ((d + 31) / 32), R = r.nsrc <= 32 ? 32 : 64;
+    const long long ns = 1 + r.nsteps / std::max(r.stride, 1);
+    return (long long)r.ne * 4 * kp * kp * 4          // MFMA operator
+           + (long long)r.ne * kp * R * (4 * 4 + 2 * 8)  // FP32 state x4, FP64 staging x2
+           + ns * r.nrec * R * 16                     // seismograms of every column
+           + (long long)(r.nsteps + 1) * r.nsrc * 16  // force table
+           + (long long)r.ne * (d * 8 + 64);          // staging vector, per-element data
+}
+
+long long footprint_of(const Request &r, int mode)
+{
+    if (mode == TSP_SERVE_COLUMNS) return footprint_columns(r);
     const long long nln = (long long)(r.N + 1) * (r.N + 2) / 2, d = 2 * nln;
     const long long op32 = (long long)r.ne * 4 * d * d * 4;       // FP32 column-quad operator
     const long long per_col = (long long)r.ne * d * 4 * 4          // uhi x2, comp, increment
@@ -70,6 +89,7 @@ long long assembly_peak(const Request &r)
 
 struct tsp_server {
     int device = 0;
+    int mode = TSP_SERVE_COLUMNS;
     tsp_multi *multi = nullptr;
     long long budget = 0, used = 0;
     int chunk = 32;
@@ -104,10 +124,64 @@ const Tables &tables_for(tsp_server *s, int N)
     return s->tables.emplace(N, std::move(t)).first->second;
 }
 
+// columns mode: one multi-column context, first step taken, steps queued by tick
+bool admit_columns(tsp_server *s, Request &r)
+{
+    const Tables &t = tables_for(s, r.N);
+    r.ncols = r.nsrc <= 32 ? 32 : 64;
+    tsp_context *c = nullptr;
+    auto bail = [&](const char *what) {
+        r.error = std::string(what) + ": " + (c ? tsp_last_error(c) : "");
+        if (c) tsp_destroy(c);
+        r.ctx = nullptr;
+        r.state = 3;
+        return false;
+    };
+    if (tsp_create(s->device, TSP_FP32C, r.ncols, &c) != TSP_OK) return bail("create");
+    if (tsp_set_tables(c, r.N, t.B.data(), t.G.data(), t.Be.data(), t.Ge.data(), t.w2.data(), t.w1.data()))
+        return bail("tables");
+    if (tsp_set_mesh(c, r.ne, r.verts.data(), r.neigh.data(), r.neighedge.data(), r.lambda.data(), r.mu.data(),
+                     r.rho.data(), r.penalty))
+        return bail("mesh");
+    if (tsp_release_fp64(c)) return bail("release");
+    std::vector<int> cols(r.nsrc);
+    for (int k = 0; k < r.nsrc; ++k) cols[k] = k;
+    if (tsp_set_sources(c, r.nsrc, cols.data(), r.src_elem.data(), r.src_phi.data(), r.nsteps + 1, r.wavelet.data()))
+        return bail("sources");
+    if (tsp_set_receivers(c, r.nrec, r.rec_elem.data(), r.rec_phi.data(), r.stride,
+                          1 + r.nsteps / std::max(r.stride, 1)))
+        return bail("receivers");
+    if (tsp_set_dt(c, r.dt) || tsp_first_step(c)) return bail("first step");
+    r.ctx = c;
+    r.done = 0;
+    r.state = 1;
+    return true;
+}
+
+void retire_columns(Request &r)
+{
+    r.nsamples = 1 + r.nsteps / std::max(r.stride, 1);
+    std::vector<double> all((size_t)r.nsamples * r.nrec * r.ncols * 2);
+    int ns = 0;
+    tsp_get_seismograms(r.ctx, all.data(), r.nsamples, &ns);
+    const size_t per = (size_t)r.nsamples * r.nrec * 2;
+    r.seis.assign(per * r.nsrc, 0.0);
+    for (int k = 0; k < r.nsrc; ++k)
+        for (int smp = 0; smp < r.nsamples; ++smp)
+            for (int rec = 0; rec < r.nrec; ++rec)
+                for (int c = 0; c < 2; ++c)
+                    r.seis[per * k + ((size_t)smp * r.nrec + rec) * 2 + c] =
+                        all[(((size_t)smp * r.nrec + rec) * r.ncols + k) * 2 + c];
+    tsp_destroy(r.ctx);
+    r.ctx = nullptr;
+    r.state = 2;
+}
+
 // build the contexts of one request and add them to the set; on failure every
 // context made so far is destroyed and the request is marked failed
 bool admit(tsp_server *s, Request &r)
 {
+    if (s->mode == TSP_SERVE_COLUMNS) return admit_columns(s, r);
     const Tables &t = tables_for(s, r.N);
     const int nln = (r.N + 1) * (r.N + 2) / 2;
     std::vector<tsp_context *> made;
@@ -208,8 +282,21 @@ int tsp_server_create(int device, double mem_fraction, int chunk, tsp_server **o
 void tsp_server_destroy(tsp_server *s)
 {
     if (!s) return;
+    for (auto &r : s->req)
+        if (r.ctx) tsp_destroy(r.ctx);
     tsp_multi_destroy(s->multi);
     delete s;
+}
+
+int tsp_server_set_mode(tsp_server *s, int mode)
+{
+    if (!s || (mode != TSP_SERVE_INSTANCES && mode != TSP_SERVE_COLUMNS)) return TSP_ERR_ARG;
+    if (!s->req.empty()) {
+        s->err = "the mode is fixed once requests were submitted";
+        return TSP_ERR_STATE;
+    }
+    s->mode = mode;
+    return TSP_OK;
 }
 
 const char *tsp_server_last_error(const tsp_server *s) { return s ? s->err.c_str() : "null server"; }
@@ -235,11 +322,11 @@ Request sizes_of(const tsp_request *q)
 }
 }  // namespace
 
-int tsp_server_footprint(const tsp_request *q, long long *resident, long long *peak)
+int tsp_server_footprint(const tsp_request *q, int mode, long long *resident, long long *peak)
 {
-    if (!request_ok(q)) return TSP_ERR_ARG;
+    if (!request_ok(q) || (mode != TSP_SERVE_INSTANCES && mode != TSP_SERVE_COLUMNS)) return TSP_ERR_ARG;
     const Request r = sizes_of(q);
-    if (resident) *resident = footprint_of(r);
+    if (resident) *resident = footprint_of(r, mode);
     if (peak) *peak = assembly_peak(r);
     return TSP_OK;
 }
@@ -268,7 +355,7 @@ int tsp_server_submit(tsp_server *s, const tsp_request *q, double arrival, int *
     r.wavelet.assign(q->wavelet, q->wavelet + (size_t)(q->nsteps + 1) * q->nsrc * 2);
     r.rec_elem.assign(q->rec_elem, q->rec_elem + q->nrec);
     r.rec_phi.assign(q->rec_phi, q->rec_phi + (size_t)q->nrec * nln);
-    r.footprint = footprint_of(r);
+    r.footprint = footprint_of(r, s->mode);
     if (r.footprint + assembly_peak(r) > s->budget) {
         s->err = "request does not fit the device-memory budget";
         return TSP_ERR_NOMEM;
@@ -309,7 +396,38 @@ int tsp_server_tick(tsp_server *s, int *completed, int *running, int *queued)
     }
     int done = s->failed_pending;
     s->failed_pending = 0;
-    if (!s->running.empty()) {
+    if (!s->running.empty() && s->mode == TSP_SERVE_COLUMNS) {
+        // queue a chunk on every running request's stream, then collect
+        for (int id : s->running) {
+            Request &r = s->req[id];
+            const int n = std::min(s->chunk, r.nsteps - r.done);
+            if (n > 0 && tsp_run(r.ctx, n) != TSP_OK) {
+                s->err = tsp_last_error(r.ctx);
+                return TSP_ERR_HIP;
+            }
+            r.done += std::max(n, 0);
+        }
+        for (int id : s->running)
+            if (tsp_synchronize(s->req[id].ctx) != TSP_OK) {
+                s->err = tsp_last_error(s->req[id].ctx);
+                return TSP_ERR_HIP;
+            }
+        const double t = clock_s();
+        for (auto it = s->running.begin(); it != s->running.end();) {
+            Request &r = s->req[*it];
+            if (r.done >= r.nsteps) {
+                retire_columns(r);
+                r.completed = t;
+                s->used -= r.footprint;
+                s->t_last = t;
+                s->dof_steps += (long long)r.ne * (r.N + 1) * (r.N + 2) * r.nsrc * r.nsteps;
+                ++done;
+                it = s->running.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    } else if (!s->running.empty()) {
         int active = 0;
         if (tsp_multi_run(s->multi, s->chunk, &active) != TSP_OK) {
             s->err = tsp_multi_last_error(s->multi);

@@ -257,11 +257,12 @@ def config5_request(k, seed=5, n_range=(20, 40), degrees=(3, 4, 5), nsrc_range=(
                    np.stack([rx, np.full(nrec, 990.0)], 1), 1, dict(n=n, seed=seed))
 
 
-def serve(requests, rate, seed=0, mem_fraction=0.8, chunk=32, stride=1, server=None):
+def serve(requests, rate, seed=0, mem_fraction=0.8, chunk=32, stride=1, server=None, mode=None):
     """Submit `requests` (Problems) with Poisson arrivals at `rate` per second and
     run the request loop until all complete.  Returns (server, ids, latencies [s],
     wall seconds)."""
-    srv = server or capi.Server(mem_fraction=mem_fraction, chunk=chunk)
+    srv = server or capi.Server(mem_fraction=mem_fraction, chunk=chunk,
+                                mode=capi.Server.COLUMNS if mode is None else mode)
     gaps = np.random.default_rng(seed).exponential(1.0 / rate, len(requests)) if rate > 0 else np.zeros(len(requests))
     t0 = srv.now() + 0.05
     ids = [srv.submit(p, t0 + float(a), stride=stride) for p, a in zip(requests, np.cumsum(gaps))]
