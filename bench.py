@@ -3,7 +3,7 @@
 
 A "step" is one LeapFrog::step over the whole mesh: block-ELL SpMV + inverse
 mass + leap-frog update + point source + receiver sampling, fused in one
-kernel launch.  The workload is config 2 (4000 m x 2000 m, 160k triangles,
+kernel launch.  The workload is layered_ricker (4000 m x 2000 m, 160k triangles,
 p = 6, 8.96M DOF, three layers with lognormal moduli, one Ricker source,
 128 receivers), the largest single-GPU configuration; synthetic inputs.
 
@@ -64,10 +64,10 @@ def run_variant(prob, precision, steps, warmup):
 
 
 def run_batched(steps, warmup, ncols=64, precision="fp32c"):
-    """Config 3: config 1's mesh at p=4 with 64 sources stepped as 64 columns of one
+    """The 3200-triangle square at p=4 with 64 sources stepped as 64 columns of one
     block SpMM on the MFMA cores (FP32 or BF16x3 products, compensated update)."""
     from tspeed_amd import capi, workloads as W
-    prob = W.config3(nsrc=ncols, nsteps=steps + warmup)
+    prob = W.multisource(nsrc=ncols, nsteps=steps + warmup)
     eng = capi.Engine(precision, ncols=ncols)
     eng.set_tables(prob.N)
     eng.set_mesh(prob.verts, prob.neigh, prob.neighedge, prob.lam, prob.mu, prob.rho, prob.penalty)
@@ -98,7 +98,7 @@ def run_batched(steps, warmup, ncols=64, precision="fp32c"):
     gbs = nbytes / (ms_step / 1e3) / 1e9
     return dict(value=prob.ndof * ncols * steps / (ms / 1e3), ms_per_step=ms_step, columns=ncols,
                 precision=precision,
-                workload="config3: config-1 mesh, p=4, 64 Ricker sources as 64 columns, 128 receivers",
+                workload="multisource: 1 km square, 40x40 split squares (3200 triangles), p=4, 64 Ricker sources as 64 columns, 128 receivers",
                 tflops_algorithmic=flops_alg / (ms_step / 1e3) / 1e12,
                 tflops_mfma_issued=flops_mfma / (ms_step / 1e3) / 1e12, mfma_peak_tflops=peak,
                 mfma_frac=flops_mfma / (ms_step / 1e3) / 1e12 / peak,
@@ -106,7 +106,7 @@ def run_batched(steps, warmup, ncols=64, precision="fp32c"):
 
 
 def run_multi(steps, warmup, n_inst=256):
-    """Config 4: n_inst independent meshes (500-20000 triangles, p in 2..8,
+    """Uneven batch: n_inst independent meshes (500-20000 triangles, p in 2..8,
     random homogeneous materials, one Ricker source each, 500-5000 steps) in one
     persistent kernel per step, graph-captured, lane mapping autotuned per p."""
     from tspeed_amd import capi, workloads as W
@@ -114,7 +114,7 @@ def run_multi(steps, warmup, n_inst=256):
     m = capi.Multi(chunk=32)
     ndof, ne_tot, degrees, nbytes = 0, 0, {}, 0
     for k in range(n_inst):
-        prob = W.config4_instance(k)
+        prob = W.uneven_instance(k)
         eng = W.prepare_instance(prob, stride=10)
         m.add(eng, prob.nsteps)
         ndof += prob.ndof
@@ -133,11 +133,11 @@ def run_multi(steps, warmup, n_inst=256):
     return dict(value=ndof * steps / (ms / 1e3), ms_per_step=ms / steps, instances=n_inst, elements=ne_tot,
                 dof=ndof, degrees=degrees, autotuned_variant=variants, setup_s=setup,
                 algorithmic_bytes=nbytes, achieved_GBs=gbs, hbm_frac=gbs / HBM_PEAK_GBS,
-                workload="config4: uneven batch, one persistent kernel per step, graph chunks of 32")
+                workload="uneven batch: 256 meshes of 500-20000 triangles, p 2..8, one launch per degree per step, graph chunks of 32")
 
 
 def run_serving(n_req=32, loads=(0.5, 0.8), sample_frac=0.05, seed=11):
-    """Config 5: requests from the multi-source distribution (20-40 squares per
+    """Serving: requests from the multi-source distribution (20-40 squares per
     side, p in {3,4,5}, random homogeneous materials, 1-16 sources, 500-5000
     steps, 16-128 receivers) arrive as a Poisson process; the loop admits them
     against 80% of device memory and steps everything admitted together.  First
@@ -145,10 +145,10 @@ def run_serving(n_req=32, loads=(0.5, 0.8), sample_frac=0.05, seed=11):
     fractions of that capacity.  A seeded 5% sample is re-run source by source
     on the FP64 path and its seismogram error reported."""
     from tspeed_amd import capi, workloads as W
-    reqs = [W.config5_request(k) for k in range(n_req)]
+    reqs = [W.serving_request(k) for k in range(n_req)]
     work = sum(p.ndof * len(p.src_elem) * p.nsteps for p in reqs)
     out = dict(requests=n_req, dof_steps=work,
-               workload="config5: Poisson arrivals, HBM-footprint admission at 80%, continuous batching "
+               workload="serving: Poisson arrivals, HBM-footprint admission at 80%, continuous batching "
                         "(chunks of 32 steps); each request one 32-column MFMA context on its own stream")
     # the other execution mode (one instance per source in the shared set), saturated, for comparison
     srv, ids, lat, wall = W.serve(reqs, rate=0, mode=capi.Server.INSTANCES)
@@ -190,16 +190,16 @@ def run_serving(n_req=32, loads=(0.5, 0.8), sample_frac=0.05, seed=11):
 
 def cpu_baseline(sample_steps):
     """The FP64 oracle (single thread, reference Release flags) on a 40x20 sub-grid
-    of config 2 (same p, materials model and per-element work)."""
+    of the layered workload (same p, materials model and per-element work)."""
     from oracle import oracle as O
     from tspeed_amd import workloads as W
-    p = W.config2(nx=40, ny=20, nsteps=sample_steps)
+    p = W.layered_ricker(nx=40, ny=20, nsteps=sample_steps)
     op = O.Operator(p.N, p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
     t = time.perf_counter()
     op.run(p.dt, p.nsteps, src=p.source_dict(0), rec_elem=p.rec_elem, rec_phi=p.rec_phi, stride=1)
     dt = time.perf_counter() - t
     return dict(value=p.ndof * sample_steps / dt, unit="DOF*steps/s", cores=1, kind="port",
-                sample=f"oracle (FP64 C restatement of LeapFrog::step, -O3 -msse4, 1 thread) on config 2's "
+                sample=f"oracle (FP64 C restatement of LeapFrog::step, -O3 -msse4, 1 thread) on the layered workload's "
                        f"model at 40x20 squares ({p.ne} triangles, p=6, {p.ndof} DOF), {sample_steps} steps, "
                        f"{dt:.1f} s")
 
@@ -245,7 +245,7 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
 
-    prob = W.config2(nsteps=args.steps + args.warmup)
+    prob = W.layered_ricker(nsteps=args.steps + args.warmup)
     r64 = run_variant(prob, "fp64", args.steps, args.warmup)
     ms_step = r64["ms_total"] / args.steps
     value = prob.ndof * args.steps / (r64["ms_total"] / 1e3)
@@ -270,7 +270,7 @@ def main():
         variants["serving"] = run_serving(args.serve)
 
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_step64_config2.json")
+    tf = os.path.join(ROOT, "profiles", "traffic_step64_layered_400x200_p6.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("bytes_per_launch")
 
@@ -287,7 +287,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "config2: 4000x2000 m, 400x200 split squares (160000 triangles), p=6, "
+        "config": {"workload": "layered_ricker: 4000x2000 m, 400x200 split squares (160000 triangles), p=6, "
                                "8.96M DOF, 3 layers + lognormal moduli, 1 Ricker source, 128 receivers; "
                                "plain FP64 leap-frog",
                    "ne": prob.ne, "p": prob.N, "dof": prob.ndof, "dt": prob.dt, "precision": "fp64"},

@@ -1,11 +1,11 @@
-// config1_driver.cpp -- a tspeed-style driver (same shape as the reference's
-// Examples/src/Lamb.cpp) running config 1 through the tspeed::LeapFrog facade:
+// square_driver.cpp -- a tspeed-style driver (same shape as the reference's
+// Examples/src/Lamb.cpp) running the 1 km square workload through the tspeed::LeapFrog facade:
 // 1000 m x 1000 m, 40x40 split squares, p = 4, vertical Ricker at (500, 950),
 // 32 receivers at y = 990.  Writes <out>_rcv_<i>.out.
 //
-//   build:  g++ -O2 -std=c++17 -I../tspeed_amd/csrc/host config1_driver.cpp \
-//               -L../tspeed_amd -ltspeed_amd -Wl,-rpath,$ORIGIN/../tspeed_amd -o config1_driver
-//   run:    ./config1_driver [steps] [out_prefix] [precision 0|1] [basis 0|1]
+//   build:  g++ -O2 -std=c++17 -I../tspeed_amd/csrc/host square_driver.cpp \
+//               -L../tspeed_amd -ltspeed_amd -Wl,-rpath,$ORIGIN/../tspeed_amd -o square_driver
+//   run:    ./square_driver [steps] [out_prefix] [precision 0|1] [basis 0|1]
 //           basis 1 = FESpace<N, Dunavant<2N>, BoundaryAdapted<N>> (as Lamb.cpp:27)
 #include "tspeed.hpp"
 
@@ -19,7 +19,7 @@ using namespace tspeed;
 int main(int argc, char **argv)
 {
     const int nsteps = argc > 1 ? std::atoi(argv[1]) : 2000;
-    const std::string out = argc > 2 ? argv[2] : "config1";
+    const std::string out = argc > 2 ? argv[2] : "square";
     const int precision = argc > 3 ? std::atoi(argv[3]) : 0;
     const int basis = argc > 4 ? std::atoi(argv[4]) : 0;
     const int N = 4;
@@ -63,6 +63,6 @@ int main(int argc, char **argv)
     const auto &u = TA.get_uh();
     double nrm = 0;
     for (double v : u) nrm += v * v;
-    std::printf("config1_driver: %d steps, dt=%.6e, |u|=%.17e\n", step, dt, std::sqrt(nrm));
+    std::printf("square_driver: %d steps, dt=%.6e, |u|=%.17e\n", step, dt, std::sqrt(nrm));
     return 0;
 }

@@ -29,7 +29,7 @@ def say(*a):
 
 
 def small_problem(N, nx=6, ny=5, nsteps=200, hetero=True, seed=4, nrec=7, stride=1):
-    p = W.config1(nx=nx, ny=ny, N=N, nsteps=nsteps, nrec=nrec, seed=seed, stride=stride)
+    p = W.square_ricker(nx=nx, ny=ny, N=N, nsteps=nsteps, nrec=nrec, seed=seed, stride=stride)
     if hetero:
         rng = np.random.default_rng(seed)
         f = rng.uniform(0.6, 1.4, p.ne)
@@ -148,11 +148,11 @@ def test_fp64_trajectory_matches_oracle(N):
     assert eng.status() == 0
 
 
-def test_config1_full_parity():
-    """Config 1 at full size (3200 triangles, p=4, 2000 steps, 32 receivers):
+def test_square_ricker_full_parity():
+    """The 1 km square at full size (3200 triangles, p=4, 2000 steps, 32 receivers):
     final displacement and every seismogram within 1e-12 of the oracle."""
-    p = W.config1()
-    say(f"config1 ne={p.ne} dof={p.ndof} dt={p.dt:.4e} steps={p.nsteps}")
+    p = W.square_ricker()
+    say(f"square_ricker ne={p.ne} dof={p.ndof} dt={p.dt:.4e} steps={p.nsteps}")
     t = time.time()
     eng, u, s = W.run_engine(p, "fp64")
     say(f"  gpu done in {time.time() - t:.2f}s")
@@ -164,14 +164,14 @@ def test_config1_full_parity():
     say(f"  u rel L2 {e_u:.3e}  seis rel L2 {e_s:.3e}  worst receiver {e_r:.3e}  |s|max {np.abs(sref).max():.3e}")
     assert np.abs(sref).max() > 0
     assert e_u <= 1e-12 and e_s <= 1e-12 and e_r <= 1e-12
-    np.save(os.path.join(os.environ.get("TSP_OUT", "/tmp"), "config1_seis_fp64.npy"), s) if os.environ.get("TSP_OUT") else None
+    np.save(os.path.join(os.environ.get("TSP_OUT", "/tmp"), "square_ricker_seis_fp64.npy"), s) if os.environ.get("TSP_OUT") else None
 
 
-def test_fp32c_error_vs_fp64_oracle_config1():
+def test_fp32c_error_vs_fp64_oracle_square_ricker():
     """FP32 blocks + compensated update stay within 1e-4 of the FP64 oracle on
-    config 1's seismograms over 2000 steps."""
-    p = W.config1()
-    say(f"fp32c config1 steps={p.nsteps}")
+    the 1 km square's seismograms over 2000 steps."""
+    p = W.square_ricker()
+    say(f"fp32c square_ricker steps={p.nsteps}")
     eng, u, s = W.run_engine(p, "fp32c")
     uref, sref = oracle_run(p)
     e_u = rel_l2(u, uref)
@@ -285,7 +285,7 @@ def test_single_element_mesh():
     assert rel_l2(u, uref) <= 1e-12 and rel_l2(s, sref) <= 1e-12
 
 
-# ------------------------------------------------------------------ full-size config 2
+# ------------------------------------------------------------------ full-size layered workload
 def _submesh(p, centre):
     """Elements `centre` plus their neighbours, renumbered; returns the sub-problem
     arrays and the local index of each centre element."""
@@ -302,14 +302,14 @@ def _submesh(p, centre):
     return pts, tri, nb, ng, keep, loc
 
 
-def test_config2_full_size_blocks_and_linearity():
-    """Config 2 at full size (160k triangles, p=6, 16 GB of FP64 blocks):
+def test_layered_full_size_blocks_and_linearity():
+    """The layered workload at full size (160k triangles, p=6, 16 GB of FP64 blocks):
     (a) blocks of 64 sampled elements equal the oracle's blocks assembled on the
         sub-mesh of those elements and their neighbours;
     (b) doubling the force doubles every displacement and sample bit for bit
         (every operation of the step is linear and scaling by 2 is exact)."""
-    p = W.config2(nsteps=20)
-    say(f"config2 ne={p.ne} dof={p.ndof}")
+    p = W.layered_ricker(nsteps=20)
+    say(f"layered_ricker ne={p.ne} dof={p.ndof}")
     eng = capi.Engine("fp64")
     t = time.time()
     eng.set_tables(p.N)
@@ -348,11 +348,11 @@ def test_config2_full_size_blocks_and_linearity():
 
 @pytest.mark.parametrize("basis", [0, 1])
 def test_cpp_facade_driver_matches_oracle(tmp_path, basis):
-    """examples/config1_driver (the C++ LeapFrog facade, reference-style loop with
+    """examples/square_driver (the C++ LeapFrog facade, reference-style loop with
     one step() and one eval_receivers() per iteration) against the oracle, with
     the Dubiner space and with FESpace<N, Dunavant<2N>, BoundaryAdapted<N>>."""
     import subprocess
-    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "examples", "config1_driver")
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "examples", "square_driver")
     assert os.path.exists(exe), "build the example with python -m tspeed_amd.build"
     steps = 150
     out = str(tmp_path / "c1")
@@ -361,7 +361,7 @@ def test_cpp_facade_driver_matches_oracle(tmp_path, basis):
     assert r.returncode == 0
     from tspeed_amd import io
     s = io.read_receivers(out, 32)
-    p = W.config1(nsteps=steps)
+    p = W.square_ricker(nsteps=steps)
     if basis:
         p.basis = capi.BOUNDARY_ADAPTED
         p.__post_init__()
@@ -381,16 +381,16 @@ def test_batched_columns_match_oracle_and_single_column_path():
     """64 sources as 64 columns of one block SpMM (FP32 on MFMA, compensated
     update): every column's seismograms within 1e-4 of the FP64 oracle run of
     that source alone, and close to the single-column FP32 path."""
-    p = W.config3(nsteps=400)
+    p = W.multisource(nsteps=400)
     p.t0 = np.full(64, 20 * p.dt)               # early wavelets: signal within 400 steps
     p.f0 = 1.0 / (10 * p.dt) * np.linspace(0.6, 1.0, 64)
-    say(f"batched config3 ne={p.ne} cols=64 steps={p.nsteps}")
+    say(f"batched multisource ne={p.ne} cols=64 steps={p.nsteps}")
     eng, s = W.run_batched(p, 64)
     assert s.shape == (p.nsamples, 128, 64, 2)
     assert eng.status() == 0
     worst = 0.0
     for k in (0, 17, 40, 63):
-        q = W.config3_column(k, nsteps=p.nsteps)
+        q = W.multisource_column(k, nsteps=p.nsteps)
         q.t0, q.f0 = p.t0[k:k + 1], p.f0[k:k + 1]
         _, sref = oracle_run(q)
         e_s, e_r = seis_err(s[:, :, k, :], sref)
@@ -403,11 +403,11 @@ def test_batched_columns_match_oracle_and_single_column_path():
 
 
 def test_batched_bf16x3_error_against_the_oracle():
-    """The BF16x3 split-product variant on the config-3 workload (64 columns,
+    """The BF16x3 split-product variant on the 64-source square (64 columns,
     400 steps): each sampled column's seismograms within 2e-3 of the FP64
     oracle (the measured error is printed; the FP32 products stay within
     1e-4)."""
-    p = W.config3(nsteps=400)
+    p = W.multisource(nsteps=400)
     p.t0 = np.full(64, 20 * p.dt)
     p.f0 = 1.0 / (10 * p.dt) * np.linspace(0.6, 1.0, 64)
     eng, s = W.run_batched(p, 64, precision="bf16x3")
@@ -415,7 +415,7 @@ def test_batched_bf16x3_error_against_the_oracle():
     _, s32 = W.run_batched(p, 64)
     worst = 0.0
     for k in (0, 17, 40, 63):
-        q = W.config3_column(k, nsteps=p.nsteps)
+        q = W.multisource_column(k, nsteps=p.nsteps)
         q.t0, q.f0 = p.t0[k:k + 1], p.f0[k:k + 1]
         _, sref = oracle_run(q)
         e_s, e_r = seis_err(s[:, :, k, :], sref)
@@ -480,7 +480,7 @@ def test_batched_every_degree(N, ncols, precision):
 def test_batched_state_roundtrip_and_eval_receivers():
     """set_state / get_u per column, eval_receivers, and linearity across columns:
     column 1 carries twice column 0's source and initial state."""
-    p = W.config3(nsrc=2, nx=6, ny=5, nsteps=40)
+    p = W.multisource(nsrc=2, nx=6, ny=5, nsteps=40)
     p.src_xy[1] = p.src_xy[0]
     p.f0[1], p.t0[1] = p.f0[0], p.t0[0]
     p.dirs[1] = 2 * p.dirs[0]
@@ -505,7 +505,7 @@ def test_batched_state_roundtrip_and_eval_receivers():
     np.testing.assert_allclose(u_1, 2 * u_0, rtol=0, atol=1e-6 * np.abs(u_0).max())
     np.testing.assert_allclose(s[:, :, 1], 2 * s[:, :, 0], rtol=0, atol=1e-6 * np.abs(s[:, :, 0]).max())
     # against the FP64 oracle for column 0
-    q = W.config3_column(0, nx=6, ny=5, nsteps=p.nsteps, nrec=p.rec_xy.shape[0])
+    q = W.multisource_column(0, nx=6, ny=5, nsteps=p.nsteps, nrec=p.rec_xy.shape[0])
     q.src_xy, q.f0, q.t0, q.dirs = p.src_xy[:1], p.f0[:1], p.t0[:1], p.dirs[:1]
     q.__post_init__()
     uref, _ = oracle_run(q, u0=u0)
@@ -600,7 +600,7 @@ def test_multi_shared_operator_and_remove():
 
 # ------------------------------------------------------------------ request loop
 def _req(k, nsteps=120, n=(8, 14), nsrc=(1, 4)):
-    return W.config5_request(k, n_range=n, nsrc_range=nsrc, steps_range=(nsteps, nsteps + 40), nrec_range=(6, 12))
+    return W.serving_request(k, n_range=n, nsrc_range=nsrc, steps_range=(nsteps, nsteps + 40), nrec_range=(6, 12))
 
 
 @pytest.mark.parametrize("mode", ["columns", "instances"])

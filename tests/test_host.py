@@ -42,7 +42,7 @@ def test_edge_hash_neighbors_equal_bruteforce(nx, ny, jit, seed):
     np.testing.assert_array_equal(ng, ngo)
 
 
-def test_split_square_is_valid_at_config2_size():
+def test_split_square_is_valid_at_layered_size():
     """160k triangles, jitter 0.2h: every triangle counter-clockwise, boundary
     vertices on the boundary, interior jitter within +-0.2h."""
     pts, tri = capi.split_square(400, 200, 0.0, 0.0, 4000.0, 2000.0, 0.2, 2)
@@ -89,14 +89,14 @@ def test_ricker_table_times():
 
 
 def test_config_builders():
-    p = W.config1()
+    p = W.square_ricker()
     assert p.ne == 3200 and p.ndof == 96000 and p.N == 4
     assert p.rec_xy.shape == (32, 2) and (p.rec_elem >= 0).all()
     lam, mu = p.lam[0], p.mu[0]
     np.testing.assert_allclose(np.sqrt((lam + 2 * mu) / 2000.0), 3000.0)
     np.testing.assert_allclose(np.sqrt(mu / 2000.0), 1732.0)
     assert 0 < p.dt < 1e-4
-    small = W.config2(nx=40, ny=20)
+    small = W.layered_ricker(nx=40, ny=20)
     vs = np.sqrt(small.mu / small.rho)
     assert vs.min() > 800 * np.exp(-0.6) and vs.max() < 2500 * np.exp(0.6)
     np.testing.assert_allclose((small.lam / small.mu), 1.0, rtol=1e-12)   # vp = sqrt(3) vs

@@ -76,8 +76,8 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     # example driver using the C++ facade (links the library just built)
     ex = os.path.join(os.path.dirname(HERE), "examples")
-    _run([CXX, *CXX_FLAGS, "-I", os.path.join(CSRC, "host"), os.path.join(ex, "config1_driver.cpp"),
-          "-L", HERE, "-ltspeed_amd", "-Wl,-rpath,$ORIGIN/../tspeed_amd", "-o", os.path.join(ex, "config1_driver")])
+    _run([CXX, *CXX_FLAGS, "-I", os.path.join(CSRC, "host"), os.path.join(ex, "square_driver.cpp"),
+          "-L", HERE, "-ltspeed_amd", "-Wl,-rpath,$ORIGIN/../tspeed_amd", "-o", os.path.join(ex, "square_driver")])
     return LIB
 
 

@@ -104,7 +104,7 @@ def lame(rho, vp, vs):
     return rho * vp * vp - 2 * mu, mu
 
 
-def config1(nx=40, ny=40, N=4, nsteps=2000, nrec=32, seed=1, stride=1):
+def square_ricker(nx=40, ny=40, N=4, nsteps=2000, nrec=32, seed=1, stride=1):
     """Plain method: 1000 m x 1000 m, 40x40 split squares (3200 triangles), p=4,
     homogeneous rho=2000, vp=3000, vs=1732, vertical Ricker f0=10 Hz at
     (500, 950), 32 receivers along y=990, 2000 steps.  nx/ny/nsteps shrink it
@@ -114,13 +114,13 @@ def config1(nx=40, ny=40, N=4, nsteps=2000, nrec=32, seed=1, stride=1):
     ne = tri.shape[0]
     lam, mu = lame(2000.0, 3000.0, 1732.0)
     rx = (np.arange(nrec) + 0.5) * L / nrec
-    return Problem("config1", N, pts, tri, np.full(ne, lam), np.full(ne, mu), np.full(ne, 2000.0), PENALTY,
+    return Problem("square_ricker", N, pts, tri, np.full(ne, lam), np.full(ne, mu), np.full(ne, 2000.0), PENALTY,
                    cfl_dt(pts, tri, N, 3000.0), nsteps, np.array([[500.0, 950.0]]), np.array([10.0]),
                    np.array([1.2 / 10.0]), np.array([[0.0, -1.0]]), np.stack([rx, np.full(nrec, 990.0)], 1),
                    stride, dict(nx=nx, ny=ny, seed=seed))
 
 
-def config2(nx=400, ny=200, N=6, nsteps=5000, nrec=128, seed=2, stride=1):
+def layered_ricker(nx=400, ny=200, N=6, nsteps=5000, nrec=128, seed=2, stride=1):
     """Larger single run: 4000 m x 2000 m, 400x200 split squares (160k triangles),
     p=6; three horizontal layers (equal thirds, top to bottom vs = 800, 1500,
     2500 m/s, vp = sqrt(3) vs, rho = 1800, 2100, 2400) with every element's
@@ -139,12 +139,12 @@ def config2(nx=400, ny=200, N=6, nsteps=5000, nrec=128, seed=2, stride=1):
     lam, mu = lam * fac, mu * fac
     vp_max = float(np.sqrt((lam + 2 * mu) / rho).max())
     rx = (np.arange(nrec) + 0.5) * Lx / nrec
-    return Problem("config2", N, pts, tri, lam, mu, rho, PENALTY, cfl_dt(pts, tri, N, vp_max), nsteps,
+    return Problem("layered_ricker", N, pts, tri, lam, mu, rho, PENALTY, cfl_dt(pts, tri, N, vp_max), nsteps,
                    np.array([[Lx / 2, 0.9 * Ly]]), np.array([8.0]), np.array([1.2 / 8.0]), np.array([[0.0, -1.0]]),
                    np.stack([rx, np.full(nrec, Ly - 0.5 * Ly / ny)], 1), stride, dict(nx=nx, ny=ny, seed=seed))
 
 
-def config3_sources(nsrc=64, seed=3, L=1000.0):
+def multisource_positions(nsrc=64, seed=3, L=1000.0):
     """Sources of the multi-source workload: positions uniform in the top 10% of
     the domain, f0 ~ U(5, 15) Hz, direction uniform on the unit circle."""
     rng = np.random.default_rng(seed)
@@ -154,24 +154,24 @@ def config3_sources(nsrc=64, seed=3, L=1000.0):
     return xy, f0, 1.2 / f0, np.stack([np.cos(ang), np.sin(ang)], 1)
 
 
-def config3_column(k, nx=40, ny=40, N=4, nsteps=3000, nrec=128, seed=3, stride=1):
+def multisource_column(k, nx=40, ny=40, N=4, nsteps=3000, nrec=128, seed=3, stride=1):
     """Column k of the 64-source workload as a single-source problem."""
-    base = config1(nx, ny, N, nsteps, nrec=nrec, stride=stride)
-    xy, f0, t0, dirs = config3_sources(64, seed)
-    base.name = f"config3[{k}]"
+    base = square_ricker(nx, ny, N, nsteps, nrec=nrec, stride=stride)
+    xy, f0, t0, dirs = multisource_positions(64, seed)
+    base.name = f"multisource[{k}]"
     base.src_xy, base.f0, base.t0, base.dirs = xy[k:k + 1], f0[k:k + 1], t0[k:k + 1], dirs[k:k + 1]
     base.__post_init__()
     return base
 
 
-def config3(nsrc=64, nx=40, ny=40, N=4, nsteps=3000, nrec=128, seed=3, stride=1):
-    """Multi-source batching: config 1's mesh and materials at p=4 with `nsrc`
+def multisource(nsrc=64, nx=40, ny=40, N=4, nsteps=3000, nrec=128, seed=3, stride=1):
+    """Multi-source batching: the square_ricker mesh and materials at p=4 with `nsrc`
     Ricker sources (positions uniform in the top 10% of the domain, f0 ~ U(5,15),
     random directions), all advanced together as nsrc right-hand-side columns;
     the same 128 receivers along y = 990 record every column; 3000 steps."""
-    base = config1(nx, ny, N, nsteps, nrec=nrec, stride=stride)
-    xy, f0, t0, dirs = config3_sources(nsrc, seed)
-    base.name = f"config3x{nsrc}"
+    base = square_ricker(nx, ny, N, nsteps, nrec=nrec, stride=stride)
+    xy, f0, t0, dirs = multisource_positions(nsrc, seed)
+    base.name = f"multisource x{nsrc}"
     base.src_xy, base.f0, base.t0, base.dirs = xy, f0, t0, dirs
     base.__post_init__()
     return base
@@ -196,7 +196,7 @@ def run_batched(prob: Problem, ncols=64, stride=None, precision="fp32c"):
     return eng, eng.seismograms()
 
 
-def config4_instance(k, seed=4, ne_range=(500, 20000), p_range=(2, 8), steps_range=(500, 5000), nrec=8,
+def uneven_instance(k, seed=4, ne_range=(500, 20000), p_range=(2, 8), steps_range=(500, 5000), nrec=8,
                      h=10.0):
     """Instance k of the uneven batch: ne log-uniform in ne_range (split squares of
     side h with aspect ratio U(1,4)), p uniform in p_range, homogeneous
@@ -221,13 +221,13 @@ def config4_instance(k, seed=4, ne_range=(500, 20000), p_range=(2, 8), steps_ran
     lam, mu = lame(rho, vp, vs)
     src = np.array([[rng.uniform(0.2, 0.8) * Lx, rng.uniform(0.2, 0.8) * Ly]])
     rx = (np.arange(nrec) + 0.5) * Lx / nrec
-    return Problem(f"config4[{k}]", N, pts, tri, np.full(n, lam), np.full(n, mu), np.full(n, rho), PENALTY,
+    return Problem(f"uneven[{k}]", N, pts, tri, np.full(n, lam), np.full(n, mu), np.full(n, rho), PENALTY,
                    cfl_dt(pts, tri, N, vp), nsteps, src, np.array([f0]), np.array([1.2 / f0]),
                    np.array([[np.cos(ang), np.sin(ang)]]), np.stack([rx, np.full(nrec, Ly - 0.5 * h)], 1), 1,
                    dict(nx=nx, ny=ny, seed=seed))
 
 
-def config5_request(k, seed=5, n_range=(20, 40), degrees=(3, 4, 5), nsrc_range=(1, 16), steps_range=(500, 5000),
+def serving_request(k, seed=5, n_range=(20, 40), degrees=(3, 4, 5), nsrc_range=(1, 16), steps_range=(500, 5000),
                     nrec_range=(16, 128)):
     """Request k of the serving workload, drawn like the multi-source workload
     (1000 m x 1000 m split squares, sources in the top 10%, receivers along
@@ -252,7 +252,7 @@ def config5_request(k, seed=5, n_range=(20, 40), degrees=(3, 4, 5), nsrc_range=(
     f0 = rng.uniform(5.0, 15.0, nsrc)
     ang = rng.uniform(0.0, 2 * np.pi, nsrc)
     rx = (np.arange(nrec) + 0.5) * L / nrec
-    return Problem(f"config5[{k}]", N, pts, tri, np.full(ne, lam), np.full(ne, mu), np.full(ne, rho), PENALTY,
+    return Problem(f"request[{k}]", N, pts, tri, np.full(ne, lam), np.full(ne, mu), np.full(ne, rho), PENALTY,
                    cfl_dt(pts, tri, N, vp), nsteps, xy, f0, 1.2 / f0, np.stack([np.cos(ang), np.sin(ang)], 1),
                    np.stack([rx, np.full(nrec, 990.0)], 1), 1, dict(n=n, seed=seed))
 
