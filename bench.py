@@ -160,7 +160,8 @@ def run_serving(n_req=32, loads=(0.5, 0.8), sample_frac=0.05, seed=11):
     st = srv.stats()
     cap = n_req / wall
     out["saturated"] = dict(value=st["dof_steps"] / st["busy_s"], wall_s=wall, requests_per_s=cap,
-                            p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)))
+                            p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                            host_admit_s=st["admit_s"], host_run_s=st["run_s"], host_retire_s=st["retire_s"])
     seis = {i: srv.result(i)[0] for i in ids}
     srv.close()
     for f in loads:

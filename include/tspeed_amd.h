@@ -17,6 +17,8 @@
 #ifndef TSPEED_AMD_H
 #define TSPEED_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -171,6 +173,19 @@ int tsp_get_u(tsp_context *ctx, int col, double *out);
  * the count. */
 int tsp_get_seismograms(tsp_context *ctx, double *out, int max_samples, int *nsamples);
 
+/* Columns col0 .. col0+ncols_out-1 of the seismograms, compacted:
+ * out[((sample*nrec + r)*ncols_out + k)*2 + c] for column col0 + k.  With
+ * async != 0 the copy is only queued on the context's stream: `out` must stay
+ * valid (page-locked memory from tsp_host_alloc for a truly asynchronous copy)
+ * until tsp_stream_idle reports the stream idle. */
+int tsp_get_seismograms_cols(tsp_context *ctx, int col0, int ncols_out, double *out, int max_samples,
+                             int *nsamples, int async);
+/* *idle = 1 when every operation queued on the context's stream has finished. */
+int tsp_stream_idle(tsp_context *ctx, int *idle);
+/* Page-locked host memory (hipHostMalloc) for asynchronous downloads. */
+void *tsp_host_alloc(size_t bytes);
+void tsp_host_free(void *p);
+
 /* Step-size estimate and stability check (no counterpart in the reference,
  * which hard-codes dt, Examples/src/Lamb.cpp:18): lambda_max of invM*B by FP64
  * power iteration on the device (`iters` iterations), and
@@ -298,6 +313,10 @@ int    tsp_server_result(tsp_server *s, int id, double *seis, int *nsamples, dou
  * device bytes held by running requests, and the budget. */
 int    tsp_server_stats(const tsp_server *s, double *busy_seconds, long long *dof_steps, long long *mem_used,
                         long long *mem_budget);
+/* Host seconds tsp_server_tick has spent admitting (context set-up, assembly,
+ * first step), running chunks (queueing and waiting for the device) and
+ * retiring (seismogram download, release). */
+int    tsp_server_phases(const tsp_server *s, double *admit_seconds, double *run_seconds, double *retire_seconds);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,11 @@ SIGNATURES = {
     "tsp_server_now": (_d, []),
     "tsp_server_footprint": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     "tsp_server_set_mode": (_i, [_vp, _i]),
+    "tsp_get_seismograms_cols": (_i, [_vp, _i, _i, _vp, _i, ctypes.POINTER(_i), _i]),
+    "tsp_stream_idle": (_i, [_vp, ctypes.POINTER(_i)]),
+    "tsp_host_alloc": (_vp, [ctypes.c_size_t]),
+    "tsp_host_free": (None, [_vp]),
+    "tsp_server_phases": (_i, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(_d)]),
     "tsp_server_submit": (_i, [_vp, _vp, _d, ctypes.POINTER(_i)]),
     "tsp_server_tick": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "tsp_server_result": (_i, [_vp, _i, _vp, ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d),
@@ -593,4 +598,7 @@ class Server:
         b, d, u, g = _d(), ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
         self._ck(self._L.tsp_server_stats(self._h, ctypes.byref(b), ctypes.byref(d), ctypes.byref(u),
                                           ctypes.byref(g)))
-        return dict(busy_s=b.value, dof_steps=d.value, mem_used=u.value, mem_budget=g.value)
+        a, r, t = _d(), _d(), _d()
+        self._ck(self._L.tsp_server_phases(self._h, ctypes.byref(a), ctypes.byref(r), ctypes.byref(t)))
+        return dict(busy_s=b.value, dof_steps=d.value, mem_used=u.value, mem_budget=g.value,
+                    admit_s=a.value, run_s=r.value, retire_s=t.value)

@@ -59,6 +59,11 @@ struct tsp_context {
     // many columns (ncols > 1): packed MFMA operator, state rows padded to KP
     float *blkA = nullptr;
     int KP = 0;
+    // multi-column stepping replayed from a captured graph of graph_len steps;
+    // the device step counter feeds the per-step values (BatchArgs::ctr)
+    int *d_ctr = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_len = 0;
     // non-orthonormal basis: base_invmass (nln x nln), folded into the operator
     // rows after assembly and applied to each source's basis values
     std::vector<double> h_minv;
@@ -161,8 +166,18 @@ int guarded(tsp_context *c, F &&f)
 }
 
 // Rebuild the per-element source/receiver lists and the aux column of meta.
+// a captured graph holds buffer pointers, strides and sample limits: any call
+// that changes them drops it
+void drop_graph(tsp_context *c)
+{
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+    c->graph_len = 0;
+}
+
 void rebuild_extra(tsp_context *c)
 {
+    drop_graph(c);
     if (!c->ne) return;
     std::vector<std::vector<int>> s(c->ne), r(c->ne);
     for (int k = 0; k < c->nsrc; ++k) s[c->h_src_elem[k]].push_back(k);
@@ -239,10 +254,61 @@ void alloc_state(tsp_context *c)
     }
 }
 
+BatchArgs batch_args(tsp_context *c);
+
 int launch_batched(tsp_context *c, const BatchArgs &b)
 {
     return c->precision == TSP_BF16X3 ? launch_step_b16x3(c->P, c->ncols, b, c->stream)
                                        : launch_step_b32(c->P, c->ncols, b, c->stream);
+}
+
+// Runs of at least this many multi-column steps replay a captured graph (one
+// launch instead of one per step and one per counter advance).
+constexpr int kGraphMinSteps = 8;
+
+// samples so far are exactly the stride-driven ones (the graph derives the
+// sample slot of step n as n / stride)
+bool regular_samples(const tsp_context *c)
+{
+    if (c->nrec == 0 || c->stride == 0) return true;
+    return c->nsamples == std::min(c->max_samples, 1 + c->steps / c->stride);
+}
+
+void run_batched_graph(tsp_context *c, int nsteps)
+{
+    if (!c->d_ctr) c->d_ctr = dalloc<int>(c, 1, "step counter");
+    if (!c->graph || c->graph_len != nsteps) {
+        drop_graph(c);
+        BatchArgs b = batch_args(c);
+        b.sc32 = c->sc_step;
+        b.ctr = c->d_ctr;
+        b.wav = c->wav;
+        b.stride = c->nrec ? c->stride : 0;
+        b.max_samples = c->max_samples;
+        b.uhi_pair[0] = c->uhi[0];
+        b.uhi_pair[1] = c->uhi[1];
+        hipcheck(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "capture");
+        int rc = 0;
+        for (int k = 0; k < nsteps && !rc; ++k) {
+            rc = launch_batched(c, b);
+            if (!rc) rc = launch_ctr_advance(c->d_ctr, c->stream);
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            fail(c, TSP_ERR_HIP, "graph capture of %d steps failed", nsteps);
+        }
+        const hipError_t ei = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hipcheck(c, ei, "graph instantiate");
+        c->graph_len = nsteps;
+    }
+    kcheck(c, launch_ctr_set(c->d_ctr, c->steps, c->stream), "step counter");
+    hipcheck(c, hipGraphLaunch(c->graph, c->stream), "graph launch");
+    c->steps += nsteps;
+    c->cur = (c->steps + 1) & 1;
+    if (c->nrec > 0 && c->stride > 0) c->nsamples = std::min(c->max_samples, 1 + c->steps / c->stride);
 }
 
 void operator_ready(tsp_context *c)
@@ -407,6 +473,7 @@ void tsp_destroy(tsp_context *c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
@@ -487,6 +554,7 @@ static void check_mesh_args(tsp_context *c, int ne, const int *neigh, const int 
 
 static void reset_mesh(tsp_context *c, int ne)
 {
+    drop_graph(c);
     dfree(c, c->meta); dfree(c, c->blk64); dfree(c, c->invm); dfree(c, c->blk32); dfree(c, c->blkA);
     dfree(c, c->sc_first); dfree(c, c->sc_step);
     for (int k = 0; k < 2; ++k) { dfree(c, c->u64[k]); dfree(c, c->uhi[k]); }
@@ -698,6 +766,7 @@ int tsp_set_stride(tsp_context *c, int stride)
 {
     return guarded(c, [&] {
         if (stride < 0) fail(c, TSP_ERR_ARG, "negative stride");
+        if (stride != c->stride) drop_graph(c);
         c->stride = stride;
     });
 }
@@ -775,6 +844,10 @@ int tsp_run(tsp_context *c, int nsteps)
             const int need = (c->steps + nsteps) / c->stride - c->steps / c->stride;
             if (c->nsamples + need > c->max_samples)
                 fail(c, TSP_ERR_ARG, "seismogram buffer of %d samples cannot hold %d more", c->max_samples, need);
+        }
+        if (c->batched() && nsteps >= kGraphMinSteps && regular_samples(c)) {
+            run_batched_graph(c, nsteps);
+            return;
         }
         StepArgs a = base_args(c);
         const double dt = c->dt;
@@ -890,6 +963,48 @@ int tsp_get_seismograms(tsp_context *c, double *out, int max_samples, int *nsamp
         if (nsamples) *nsamples = c->nsamples;
         if (out && ns > 0 && c->nrec > 0) d2h(c, out, c->seis, (size_t)ns * c->nrec * c->ncols * 2, "seismograms");
     });
+}
+
+int tsp_get_seismograms_cols(tsp_context *c, int col0, int ncols_out, double *out, int max_samples, int *nsamples,
+                             int async)
+{
+    return guarded(c, [&] {
+        if (col0 < 0 || ncols_out < 1 || col0 + ncols_out > c->ncols)
+            fail(c, TSP_ERR_ARG, "columns [%d, %d) outside 0..%d", col0, col0 + ncols_out, c->ncols);
+        const int ns = std::min(c->nsamples, std::max(0, max_samples));
+        if (nsamples) *nsamples = c->nsamples;
+        if (!out || ns <= 0 || c->nrec <= 0) return;
+        const size_t width = sizeof(double) * 2 * ncols_out, pitch = sizeof(double) * 2 * c->ncols;
+        hipcheck(c, hipMemcpy2DAsync(out, width, c->seis + (size_t)col0 * 2, pitch, width, (size_t)ns * c->nrec,
+                                     hipMemcpyDeviceToHost, c->stream), "seismograms");
+        if (!async) hipcheck(c, hipStreamSynchronize(c->stream), "seismograms");
+    });
+}
+
+int tsp_stream_idle(tsp_context *c, int *idle)
+{
+    return guarded(c, [&] {
+        if (!idle) fail(c, TSP_ERR_ARG, "null output");
+        const hipError_t e = hipStreamQuery(c->stream);
+        if (e == hipErrorNotReady) {
+            *idle = 0;
+            return;
+        }
+        hipcheck(c, e, "stream query");
+        *idle = 1;
+    });
+}
+
+void *tsp_host_alloc(size_t bytes)
+{
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+    return p;
+}
+
+void tsp_host_free(void *p)
+{
+    if (p) (void)hipHostFree(p);
 }
 
 int tsp_cfl(tsp_context *c, double factor, int iters, double *dt_out, double *lmax_out)

@@ -125,7 +125,8 @@ int launch_apply_minvB(int P, int ne, const int4 *meta, const double *blk, const
                        const double *x, double *y, hipStream_t s);
 int launch_dot(size_t n, const double *x, const double *y, double *partial, int nparts, hipStream_t s);
 
-// Many right-hand sides (spmm.hip): R = ncols columns, state u[(e*KP + k)*R + j].
+// Many right-hand sides (spmm.hip): R = ncols columns, state in row octets
+// u[((e*KB + k/8)*R + j)*8 + k%8], KB = KP/8.
 struct BatchArgs {
     int ne, nsrc, nrec;
     const int4 *meta;
@@ -140,7 +141,17 @@ struct BatchArgs {
     const float *sc32;
     const float *uhi;
     float *uhi_out, *ucomp, *winc;
+    // Graph replay: when ctr is set, the step reads its 1-based index n = *ctr + 1
+    // and derives the force row, the sample slot (n / stride) and the ping-pong
+    // buffers (u^n in uhi_pair[n & 1]) on the device; wav_row, sample, uhi and
+    // uhi_out above are then ignored.
+    const int *ctr;
+    const double *wav;
+    int stride, max_samples;
+    float *uhi_pair[2];
 };
+int launch_ctr_set(int *ctr, int value, hipStream_t s);
+int launch_ctr_advance(int *ctr, hipStream_t s);
 int batch_kp(int P);                 // padded rows per element (multiple of 32)
 size_t batch_block_floats(int P);    // floats of one element's four packed blocks
 int launch_step_b32(int P, int ncols, const BatchArgs &a, hipStream_t s);

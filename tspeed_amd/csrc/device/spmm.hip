@@ -23,7 +23,7 @@
 //   operator  A-fragment order: for block (e,s), row tile mt and k-octet kb,
 //             64 lanes x 4 floats, lane l / slot t holding
 //             B_{e,s}(32*mt + l%32, 8*kb + 4*(l/32) + t); one 16-byte
-//             non-temporal load per lane feeds four MFMAs.
+//             load per lane feeds four MFMAs.
 #include "engine.hpp"
 
 #pragma clang fp contract(off)
@@ -140,6 +140,21 @@ __device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int h
     }
 }
 
+// Per-step values of a graph-replayed step (see BatchArgs::ctr).
+__device__ inline void resolve_step(BatchArgs &a)
+{
+    if (!a.ctr) return;
+    const int n = *a.ctr + 1;
+    a.wav_row = a.nsrc ? a.wav + (size_t)n * a.nsrc * 2 : nullptr;
+    a.sample = (a.stride > 0 && n % a.stride == 0 && n / a.stride < a.max_samples) ? n / a.stride : -1;
+    const bool odd = n & 1;  // selects, not an index: keeps the arguments out of scratch
+    a.uhi = odd ? a.uhi_pair[1] : a.uhi_pair[0];
+    a.uhi_out = odd ? a.uhi_pair[0] : a.uhi_pair[1];
+}
+
+__global__ void k_ctr_set(int *ctr, int value) { *ctr = value; }
+__global__ void k_ctr_advance(int *ctr) { *ctr += 1; }
+
 // One wave computes one element's rows for NT column tiles (WPE waves share an
 // element, each with its own NT*32 columns).  A missing neighbour's block is
 // zero in the packed operator, so its slot multiplies the element's own
@@ -148,6 +163,7 @@ __device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int h
 template <int P, int NT, int WPE, int PD>
 __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 {
+    resolve_step(a);
     using G = BGeom<P>;
     constexpr int MT = G::MT, KB = G::KB, D = G::D, NLN = G::NLN;
     constexpr int R = 32 * NT * WPE;
@@ -241,6 +257,7 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 template <int P, int WPE, int PD>
 __global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3(BatchArgs a)
 {
+    resolve_step(a);
     using G = BGeom<P>;
     constexpr int MT = G::MT, KB = G::KB, KC = KB / 2, NT = 1;
     constexpr int R = 32 * WPE;
@@ -487,6 +504,18 @@ int sample_b32_p(const BatchArgs &a, int R, hipStream_t s)
     case 7: return CALL(7); case 8: return CALL(8);               \
     default: return -2;                                           \
     }
+
+int launch_ctr_set(int *ctr, int value, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ctr_set, dim3(1), dim3(1), 0, s, ctr, value);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_ctr_advance(int *ctr, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ctr_advance, dim3(1), dim3(1), 0, s, ctr);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 int batch_kp(int P) { return 32 * ((2 * nln_of(P) + 31) / 32); }
 size_t batch_block_floats(int P) { return (size_t)4 * batch_kp(P) / 32 * (batch_kp(P) / 8) * 256; }

@@ -49,9 +49,10 @@ struct Request {
     std::vector<int> inst;        // multi-set ids, owner first (instances mode)
     tsp_context *ctx = nullptr;   // the multi-column context (columns mode)
     int ncols = 0, done = 0;
-    std::vector<double> seis;     // [src][sample][rec][2]
+    std::vector<double> seis;     // [src][sample][rec][2] (instances mode)
+    double *pinned = nullptr;     // [sample][rec][src][2] (columns mode, page-locked)
     int nsamples = 0;
-    int state = 0;                // 0 queued, 1 running, 2 done, 3 failed
+    int state = 0;                // 0 queued, 1 running, 2 done, 3 failed, 4 results in flight
     std::string error;
 };
 
@@ -97,10 +98,12 @@ struct tsp_server {
     std::vector<Request> req;
     std::deque<int> queue;   // ids waiting, arrival order
     std::vector<int> running;
+    std::vector<int> draining;  // columns mode: finished, results in flight
     std::string err;
     double t_first = -1, t_last = -1;
     long long dof_steps = 0;
     int failed_pending = 0;
+    double t_admit = 0, t_run = 0, t_retire = 0;  // host seconds spent in each phase of tick()
 };
 
 namespace {
@@ -158,20 +161,30 @@ bool admit_columns(tsp_server *s, Request &r)
     return true;
 }
 
-void retire_columns(Request &r)
+// columns mode: queue the download of the request's own columns into
+// page-locked memory behind its last steps; the context is released once the
+// copy has landed (finish_columns)
+bool start_retire_columns(Request &r)
 {
     r.nsamples = 1 + r.nsteps / std::max(r.stride, 1);
-    std::vector<double> all((size_t)r.nsamples * r.nrec * r.ncols * 2);
+    r.pinned = static_cast<double *>(tsp_host_alloc(sizeof(double) * (size_t)r.nsamples * r.nrec * r.nsrc * 2));
     int ns = 0;
-    tsp_get_seismograms(r.ctx, all.data(), r.nsamples, &ns);
-    const size_t per = (size_t)r.nsamples * r.nrec * 2;
-    r.seis.assign(per * r.nsrc, 0.0);
-    for (int k = 0; k < r.nsrc; ++k)
-        for (int smp = 0; smp < r.nsamples; ++smp)
-            for (int rec = 0; rec < r.nrec; ++rec)
-                for (int c = 0; c < 2; ++c)
-                    r.seis[per * k + ((size_t)smp * r.nrec + rec) * 2 + c] =
-                        all[(((size_t)smp * r.nrec + rec) * r.ncols + k) * 2 + c];
+    if (!r.pinned || tsp_get_seismograms_cols(r.ctx, 0, r.nsrc, r.pinned, r.nsamples, &ns, 1) != TSP_OK) {
+        r.error = std::string("seismograms: ") + tsp_last_error(r.ctx);
+        return false;
+    }
+    r.state = 4;
+    return true;
+}
+
+bool copy_landed(Request &r)
+{
+    int idle = 0;
+    return tsp_stream_idle(r.ctx, &idle) != TSP_OK || idle;
+}
+
+void finish_columns(Request &r)
+{
     tsp_destroy(r.ctx);
     r.ctx = nullptr;
     r.state = 2;
@@ -282,8 +295,10 @@ int tsp_server_create(int device, double mem_fraction, int chunk, tsp_server **o
 void tsp_server_destroy(tsp_server *s)
 {
     if (!s) return;
-    for (auto &r : s->req)
+    for (auto &r : s->req) {
         if (r.ctx) tsp_destroy(r.ctx);
+        tsp_host_free(r.pinned);
+    }
     tsp_multi_destroy(s->multi);
     delete s;
 }
@@ -373,6 +388,7 @@ int tsp_server_tick(tsp_server *s, int *completed, int *running, int *queued)
 {
     if (!s) return TSP_ERR_ARG;
     const double now = clock_s();
+    const double t0 = now;
     // admit every arrived request that fits (first fit in arrival order)
     for (auto it = s->queue.begin(); it != s->queue.end();) {
         Request &r = s->req[*it];
@@ -396,7 +412,9 @@ int tsp_server_tick(tsp_server *s, int *completed, int *running, int *queued)
     }
     int done = s->failed_pending;
     s->failed_pending = 0;
-    if (!s->running.empty() && s->mode == TSP_SERVE_COLUMNS) {
+    const double t1 = clock_s();
+    s->t_admit += t1 - t0;
+    if ((!s->running.empty() || !s->draining.empty()) && s->mode == TSP_SERVE_COLUMNS) {
         // queue a chunk on every running request's stream, then collect
         for (int id : s->running) {
             Request &r = s->req[id];
@@ -413,15 +431,35 @@ int tsp_server_tick(tsp_server *s, int *completed, int *running, int *queued)
                 return TSP_ERR_HIP;
             }
         const double t = clock_s();
-        for (auto it = s->running.begin(); it != s->running.end();) {
+        s->t_run += t - t1;
+        // results of earlier ticks that have landed: release their contexts
+        for (auto it = s->draining.begin(); it != s->draining.end();) {
             Request &r = s->req[*it];
-            if (r.done >= r.nsteps) {
-                retire_columns(r);
+            if (copy_landed(r)) {
+                finish_columns(r);
                 r.completed = t;
                 s->used -= r.footprint;
                 s->t_last = t;
                 s->dof_steps += (long long)r.ne * (r.N + 1) * (r.N + 2) * r.nsrc * r.nsteps;
                 ++done;
+                it = s->draining.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        for (auto it = s->running.begin(); it != s->running.end();) {
+            Request &r = s->req[*it];
+            if (r.done >= r.nsteps) {
+                if (!start_retire_columns(r)) {
+                    tsp_destroy(r.ctx);
+                    r.ctx = nullptr;
+                    r.state = 3;
+                    r.completed = t;
+                    s->used -= r.footprint;
+                    ++done;
+                } else {
+                    s->draining.push_back(*it);
+                }
                 it = s->running.erase(it);
             } else {
                 ++it;
@@ -434,6 +472,7 @@ int tsp_server_tick(tsp_server *s, int *completed, int *running, int *queued)
             return TSP_ERR_HIP;
         }
         const double t = clock_s();
+        s->t_run += t - t1;
         for (auto it = s->running.begin(); it != s->running.end();) {
             Request &r = s->req[*it];
             int steps = 0, n = 0;
@@ -451,8 +490,9 @@ int tsp_server_tick(tsp_server *s, int *completed, int *running, int *queued)
             }
         }
     }
+    s->t_retire += clock_s() - t1 - 0.0;
     if (completed) *completed = done;
-    if (running) *running = (int)s->running.size();
+    if (running) *running = (int)(s->running.size() + s->draining.size());
     if (queued) *queued = (int)s->queue.size();
     return TSP_OK;
 }
@@ -468,10 +508,33 @@ int tsp_server_result(tsp_server *s, int id, double *seis, int *nsamples, double
     }
     if (r.state != 2) return TSP_ERR_STATE;
     if (nsamples) *nsamples = r.nsamples;
-    if (seis) std::memcpy(seis, r.seis.data(), sizeof(double) * r.seis.size());
+    if (seis && r.pinned) {
+        // [sample][rec][src][2] as downloaded -> [src][sample][rec][2]
+        const size_t per = (size_t)r.nsamples * r.nrec * 2;
+        for (int smp = 0; smp < r.nsamples; ++smp)
+            for (int rec = 0; rec < r.nrec; ++rec) {
+                const double *in = r.pinned + ((size_t)smp * r.nrec + rec) * r.nsrc * 2;
+                for (int k = 0; k < r.nsrc; ++k) {
+                    double *o = seis + per * k + ((size_t)smp * r.nrec + rec) * 2;
+                    o[0] = in[2 * k];
+                    o[1] = in[2 * k + 1];
+                }
+            }
+    } else if (seis) {
+        std::memcpy(seis, r.seis.data(), sizeof(double) * r.seis.size());
+    }
     if (arrival) *arrival = r.arrival;
     if (admitted) *admitted = r.admitted;
     if (completed) *completed = r.completed;
+    return TSP_OK;
+}
+
+int tsp_server_phases(const tsp_server *s, double *admit_seconds, double *run_seconds, double *retire_seconds)
+{
+    if (!s) return TSP_ERR_ARG;
+    if (admit_seconds) *admit_seconds = s->t_admit;
+    if (run_seconds) *run_seconds = s->t_run;
+    if (retire_seconds) *retire_seconds = s->t_retire - s->t_run;
     return TSP_OK;
 }
 
