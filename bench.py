@@ -107,8 +107,9 @@ def run_batched(steps, warmup, ncols=64, precision="fp32c"):
 
 def run_multi(steps, warmup, n_inst=256):
     """Uneven batch: n_inst independent meshes (500-20000 triangles, p in 2..8,
-    random homogeneous materials, one Ricker source each, 500-5000 steps) in one
-    persistent kernel per step, graph-captured, lane mapping autotuned per p."""
+    random homogeneous materials, one Ricker source each, 500-5000 steps) stepped
+    together, graph-captured; lane mapping per p and launch mode (one kernel per
+    degree, or one kernel for all) autotuned."""
     from tspeed_amd import capi, workloads as W
     t0 = time.time()
     m = capi.Multi(chunk=32)
@@ -123,6 +124,15 @@ def run_multi(steps, warmup, n_inst=256):
         degrees[prob.N] = degrees.get(prob.N, 0) + 1
     setup = time.time() - t0
     m.autotune(3)
+    chosen = m.launch()
+    # each launch mode on one graph chunk (every instance runs >= 500 steps: the
+    # autotuner's trials, these 96 steps, the warm-up and at most 200 timed steps
+    # leave every instance running to the end of the timed run)
+    modes = {}
+    for mode, name in ((m.PER_DEGREE, "per_degree"), (m.FUSED, "fused"), (m.FUSED_PERSISTENT, "fused_persistent")):
+        m.set_launch(mode)
+        modes[name] = m.run_timed(32) / 32
+    m.set_launch(chosen)
     if warmup:
         m.run(warmup)
     m.synchronize()
@@ -130,10 +140,13 @@ def run_multi(steps, warmup, n_inst=256):
     variants = {p: m.variant(p) for p in sorted(degrees)}
     m.close()
     gbs = nbytes / (ms / steps / 1e3) / 1e9
+    mode_name = {0: "per_degree", 1: "fused", 2: "fused_persistent"}[chosen]
     return dict(value=ndof * steps / (ms / 1e3), ms_per_step=ms / steps, instances=n_inst, elements=ne_tot,
-                dof=ndof, degrees=degrees, autotuned_variant=variants, setup_s=setup,
+                dof=ndof, degrees=degrees, autotuned_variant=variants, launch_mode=mode_name,
+                ms_per_step_by_mode=modes, setup_s=setup,
                 algorithmic_bytes=nbytes, achieved_GBs=gbs, hbm_frac=gbs / HBM_PEAK_GBS,
-                workload="uneven batch: 256 meshes of 500-20000 triangles, p 2..8, one launch per degree per step, graph chunks of 32")
+                workload="uneven batch: 256 meshes of 500-20000 triangles, p 2..8, graph chunks of 32, "
+                         f"launch mode {mode_name} (autotuned)")
 
 
 def run_serving(n_req=32, loads=(0.5, 0.8), sample_frac=0.05, seed=11):
@@ -189,20 +202,48 @@ def run_serving(n_req=32, loads=(0.5, 0.8), sample_frac=0.05, seed=11):
     return out
 
 
-def cpu_baseline(sample_steps):
-    """The FP64 oracle (single thread, reference Release flags) on a 40x20 sub-grid
-    of the layered workload (same p, materials model and per-element work)."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sample_steps, runs=3):
+    """The FP64 oracle (reference Release flags) on a 40x20 sub-grid of the layered
+    workload (same p, materials model and per-element work): the median of `runs`
+    single-threaded runs, and beside it the same source built with OpenMP over
+    elements on every core this process may use."""
     from oracle import oracle as O
     from tspeed_amd import workloads as W
-    p = W.layered_ricker(nx=40, ny=20, nsteps=sample_steps)
-    op = O.Operator(p.N, p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
-    t = time.perf_counter()
-    op.run(p.dt, p.nsteps, src=p.source_dict(0), rec_elem=p.rec_elem, rec_phi=p.rec_phi, stride=1)
-    dt = time.perf_counter() - t
-    return dict(value=p.ndof * sample_steps / dt, unit="DOF*steps/s", cores=1, kind="port",
+
+    def timed(omp, steps):
+        p = W.layered_ricker(nx=40, ny=20, nsteps=steps)
+        op = O.Operator(p.N, p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty, omp=omp)
+        rates = []
+        for _ in range(runs):
+            t = time.perf_counter()
+            op.run(p.dt, p.nsteps, src=p.source_dict(0), rec_elem=p.rec_elem, rec_phi=p.rec_phi, stride=1)
+            rates.append(p.ndof * steps / (time.perf_counter() - t))
+        return p, float(np.median(rates)), rates
+
+    p, v1, r1 = timed(False, sample_steps)
+    ncpu = len(os.sched_getaffinity(0))
+    nthr = O.threads(min(ncpu, int(os.environ.get("OMP_NUM_THREADS", ncpu))), omp=True)
+    _, vm, rm = timed(True, 3 * sample_steps)
+    model = _cpu_model()
+    return dict(value=v1, unit="DOF*steps/s", cores=1, kind="port",
                 sample=f"oracle (FP64 C restatement of LeapFrog::step, -O3 -msse4, 1 thread) on the layered workload's "
-                       f"model at 40x20 squares ({p.ne} triangles, p=6, {p.ndof} DOF), {sample_steps} steps, "
-                       f"{dt:.1f} s")
+                       f"model at 40x20 squares ({p.ne} triangles, p=6, {p.ndof} DOF), {sample_steps} steps; "
+                       f"median of {runs} runs ({', '.join(f'{r:.3g}' for r in r1)})",
+                cpu_model=model, nproc=ncpu,
+                openmp=dict(value=vm, threads=nthr, runs=[float(r) for r in rm],
+                            sample=f"same source built with -fopenmp (rows split over {nthr} threads), "
+                                   f"{3 * sample_steps} steps, median of {runs} runs"))
 
 
 def copy_calibration(gib=2, reps=5):
@@ -229,7 +270,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=3000)
+    ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32c", action="store_true")
     ap.add_argument("--no-batched", action="store_true")

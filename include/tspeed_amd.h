@@ -218,9 +218,11 @@ int tsp_release_fp64(tsp_context *ctx);
  * single-column TSP_FP32C contexts, built with the calls above (mesh or
  * tsp_share_operator, sources, receivers with a stride, dt, state) and
  * stopped after tsp_first_step; the set takes ownership of each.  A step of
- * the set is one persistent-kernel launch over every element of every running
- * instance, whatever its size and degree; chunks of steps replay a captured HIP
- * graph.  Results are read through tsp_multi_context + tsp_get_u /
+ * the set covers every element of every running instance, whatever its size
+ * and degree, either as one launch per degree present (side by side in the
+ * graph) or as one launch of a single kernel that dispatches each element to
+ * its degree's step (tsp_multi_set_launch); chunks of steps replay a captured
+ * HIP graph.  Results are read through tsp_multi_context + tsp_get_u /
  * tsp_get_seismograms.
  * ------------------------------------------------------------------------ */
 typedef struct tsp_multi tsp_multi;
@@ -238,8 +240,20 @@ int  tsp_multi_set_chunk(tsp_multi *m, int chunk);
 int  tsp_multi_run(tsp_multi *m, int nsteps, int *active);
 /* tsp_multi_run timed with HIP events on the set's stream (milliseconds). */
 int  tsp_multi_run_timed(tsp_multi *m, int nsteps, float *ms);
+/* How a step is launched:
+ *   TSP_MULTI_PER_DEGREE (default): one kernel per degree present, each with the
+ *     register budget of its own degree;
+ *   TSP_MULTI_FUSED: one kernel for all degrees, one element per wave, largest
+ *     degree first;
+ *   TSP_MULTI_FUSED_PERSISTENT: the same kernel with resident waves looping over
+ *     the elements.
+ * Results are bitwise the same in every mode. */
+enum { TSP_MULTI_PER_DEGREE = 0, TSP_MULTI_FUSED = 1, TSP_MULTI_FUSED_PERSISTENT = 2 };
+int  tsp_multi_set_launch(tsp_multi *m, int mode);
+int  tsp_multi_launch(const tsp_multi *m);
 /* Choose, per degree, the fastest launch variant of the element step by timing
- * each on `steps_per_trial` real steps of that degree's instances. */
+ * each on `steps_per_trial` real steps of that degree's instances; then, with
+ * a graph chunk set, the fastest launch mode on one chunk of the whole set. */
 int  tsp_multi_autotune(tsp_multi *m, int steps_per_trial);
 /* The launch variant in use for a degree: bit 0 = one row per lane across the
  * wave (else column groups of 16/32 lanes, d <= 32 only), bit 1 = one item per

@@ -49,6 +49,28 @@
 
 #define ORC_PI 3.14159265358979323846
 
+/* Built twice: liboracle.so with the reference's single-threaded Release flags,
+ * and liboracle_omp.so with -fopenmp as a labelled multi-core CPU baseline.  The
+ * parallel loops split rows or vector entries only, so each output entry keeps
+ * the same summation order and both builds give bitwise equal results. */
+#ifdef _OPENMP
+#include <omp.h>
+#define ORC_PAR _Pragma("omp parallel for schedule(static)")
+#else
+#define ORC_PAR
+#endif
+
+int orc_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
 /* Gauss-Legendre nodes/weights on [a,b] by Newton iteration on P_n
  * (QuadratureRule_imp.hpp:23-61; the textbook "gauleg" recurrence).  Nodes ascend. */
 int orc_gauleg(int n, double a, double b, double *x, double *w)
@@ -679,6 +701,7 @@ void orc_export_invm(const orc_op *op, double *out) { memcpy(out, op->invm, size
 static void vecmult(const orc_op *op, const double *M, const double *x, double *out)
 {
     const int nln = op->nln;
+    ORC_PAR
     for (int r = 0; r < op->ne; ++r) {
         double *o = out + (size_t)r * nln;
         for (int b = op->rowptr[r]; b < op->rowptr[r + 1]; ++b) {
@@ -699,9 +722,11 @@ void orc_apply_B(const orc_op *op, const double *x, double *out)
     double *t0 = malloc(sizeof(double) * hs), *t1 = malloc(sizeof(double) * hs);
     vecmult(op, op->B[0], x, t0);
     vecmult(op, op->B[1], x + hs, t1);
+    ORC_PAR
     for (size_t q = 0; q < hs; ++q) out[q] = t0[q] + t1[q];
     vecmult(op, op->B[2], x, t0);
     vecmult(op, op->B[3], x + hs, t1);
+    ORC_PAR
     for (size_t q = 0; q < hs; ++q) out[hs + q] = t0[q] + t1[q];
     free(t0);
     free(t1);
@@ -712,7 +737,8 @@ static void apply_invM(const orc_op *op, const double *x, double *out)
 {
     const int nln = op->nln;
     const size_t hs = (size_t)op->ne * nln;
-    for (int c = 0; c < 2; ++c)
+    for (int c = 0; c < 2; ++c) {
+        ORC_PAR
         for (int e = 0; e < op->ne; ++e) {
             const double *M = op->invM[c] + (size_t)e * nln * nln, *xs = x + c * hs + (size_t)e * nln;
             double *o = out + c * hs + (size_t)e * nln;
@@ -722,6 +748,7 @@ static void apply_invM(const orc_op *op, const double *x, double *out)
                 o[i] = acc;
             }
         }
+    }
 }
 
 double orc_ricker(double f0, double t0, double t)
@@ -786,8 +813,10 @@ int orc_run(const orc_op *op, double dt, int nsteps, const orc_source *src,
         const double t = last_time + dt;
         force_eval(op, src, t, f);
         orc_apply_B(op, uhold, Bu);
+        ORC_PAR
         for (size_t q = 0; q < n; ++q) w[q] = -dt * dt * (Bu[q] - f[q]);
         apply_invM(op, w, iw);
+        ORC_PAR
         for (size_t q = 0; q < n; ++q) uh[q] = 2 * uhold[q] - uholdold[q] + iw[q];
         double *tmp = uholdold;
         uholdold = uhold;

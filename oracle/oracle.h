@@ -87,6 +87,10 @@ int    orc_run(const orc_op *op, double dt, int nsteps, const orc_source *src,
 /* Ricker wavelet value as the drivers write it (Examples/src/Lamb.cpp:37) */
 double orc_ricker(double f0, double t0, double t);
 /* largest eigenvalue of invM*B by power iteration (FP64); iters fixed */
+/* Threads the parallel loops use (0 = keep): the single-threaded build
+ * (liboracle.so) always reports 1; liboracle_omp.so sets and reports its
+ * OpenMP team size. */
+int    orc_threads(int n);
 double orc_lambda_max(const orc_op *op, int iters, unsigned long long seed);
 
 #ifdef __cplusplus

@@ -15,7 +15,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
-_lib = None
+_OMP_PATH = os.path.join(_HERE, "liboracle_omp.so")
+_libs = {}
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -27,17 +28,20 @@ class Source(ctypes.Structure):
 
 
 def build():
-    """Compile liboracle.so with the reference's Release flags."""
+    """Compile liboracle.so with the reference's Release flags (and
+    liboracle_omp.so, the same source with -fopenmp)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(_LIB_PATH):
+def lib(omp=False):
+    """The single-threaded oracle, or with omp=True its OpenMP build (bitwise
+    equal results; only for timing a multi-core CPU baseline)."""
+    if omp in _libs:
+        return _libs[omp]
+    path = _OMP_PATH if omp else _LIB_PATH
+    if not os.path.exists(path):
         build()
-    L = ctypes.CDLL(_LIB_PATH)
+    L = ctypes.CDLL(path)
     L.orc_gauleg.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
     L.orc_gauss_rule.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
     L.orc_edge_nodes.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp]
@@ -66,8 +70,15 @@ def lib():
     L.orc_ricker.restype = ctypes.c_double
     L.orc_lambda_max.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_ulonglong]
     L.orc_lambda_max.restype = ctypes.c_double
-    _lib = L
+    L.orc_threads.argtypes = [ctypes.c_int]
+    L.orc_threads.restype = ctypes.c_int
+    _libs[omp] = L
     return L
+
+
+def threads(n=0, omp=False):
+    """Set (n > 0) and report the thread count of the chosen build."""
+    return lib(omp).orc_threads(int(n))
 
 
 def nln(N):
@@ -158,11 +169,12 @@ def ricker(f0, t0, t):
 class Operator:
     """B = penalty*N^2*S + I + A assembled as the reference does (Matrices_imp.hpp:192-215)."""
 
-    def __init__(self, N, verts, neigh, neighedge, lam, mu, rho, penalty, basis=DUBINER):
+    def __init__(self, N, verts, neigh, neighedge, lam, mu, rho, penalty, basis=DUBINER, omp=False):
         self.N, self.nln, self.basis = N, nln(N), basis
+        self._L = lib(omp)
         self.verts = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1)
         self.ne = self.verts.size // 6
-        self._h = lib().orc_assemble_basis(N, basis, self.ne, self.verts,
+        self._h = self._L.orc_assemble_basis(N, basis, self.ne, self.verts,
                                      np.ascontiguousarray(neigh, dtype=np.int32).reshape(-1),
                                      np.ascontiguousarray(neighedge, dtype=np.int32).reshape(-1),
                                      np.ascontiguousarray(lam, dtype=np.float64),
@@ -171,7 +183,7 @@ class Operator:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().orc_free(self._h)
+            self._L.orc_free(self._h)
             self._h = None
 
     @property
@@ -181,27 +193,27 @@ class Operator:
     def blocks(self):
         d = 2 * self.nln
         out = np.zeros(self.ne * 4 * d * d)
-        lib().orc_export_blocks(self._h, out)
+        self._L.orc_export_blocks(self._h, out)
         return out.reshape(self.ne, 4, d, d)
 
     def invm(self):
         out = np.zeros(self.ne)
-        lib().orc_export_invm(self._h, out)
+        self._L.orc_export_invm(self._h, out)
         return out
 
     def apply(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
         out = np.zeros_like(x)
-        lib().orc_apply_B(self._h, x, out)
+        self._L.orc_apply_B(self._h, x, out)
         return out
 
     def lambda_max(self, iters=200, seed=7):
-        return lib().orc_lambda_max(self._h, iters, seed)
+        return self._L.orc_lambda_max(self._h, iters, seed)
 
     def run(self, dt, nsteps, src=None, rec_elem=None, rec_phi=None, stride=1, u0=None, v0=None):
         """first_step + nsteps steps; src = dict(elem, phi, f0, t0, dirx, diry) or None.
         Returns (u_final component-major, seismograms [nsamples, nrec, 2])."""
-        L = lib()
+        L = self._L
         s = Source(-1, None, 0.0, 0.0, 0.0, 0.0)
         keep = []
         if src is not None:

@@ -453,3 +453,27 @@ def test_boundary_adapted_dynamics_equal_dubiner(N):
         _, s = op.run(dt, 120, src=src, rec_elem=np.array([e for e, _ in rloc]), rec_phi=rphi, stride=1)
         out.append(s)
     assert np.linalg.norm(out[1] - out[0]) / np.linalg.norm(out[0]) < 1e-11
+
+
+def test_openmp_build_is_bitwise_equal():
+    """The multi-core oracle build (timed as a labelled CPU baseline) splits rows
+    and vector entries over threads only, so its trajectory equals the
+    single-threaded build's bit for bit."""
+    pts, tri, verts = _mesh(8, 6, seed=5)
+    nb, ng = O.neighbors(tri)
+    ne = tri.shape[0]
+    rng = np.random.default_rng(5)
+    lam, mu, rho = rng.uniform(3e9, 9e9, ne), rng.uniform(3e9, 9e9, ne), rng.uniform(1600, 2400, ne)
+    e, xi = O.locate(verts, 260.0, 190.0)
+    src = dict(elem=e, phi=O.dubiner(3, *xi)[0], f0=12.0, t0=0.05, dirx=0.6, diry=-0.8)
+    assert O.threads(omp=False) == 1
+    O.threads(4, omp=True)
+    assert O.threads(omp=True) == 4
+    runs = []
+    for omp in (False, True):
+        op = O.Operator(3, verts, nb, ng, lam, mu, rho, 10.0, omp=omp)
+        dt = 0.5 * 2 / np.sqrt(op.lambda_max(200))
+        runs.append(op.run(dt, 40, src=src, rec_elem=np.array([e, 0], np.int32),
+                           rec_phi=np.concatenate([src["phi"], O.dubiner(3, 0.3, 0.3)[0]]), stride=1))
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])

@@ -65,6 +65,8 @@ SIGNATURES = {
     "tsp_multi_run_timed": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_float)]),
     "tsp_multi_autotune": (_i, [_vp, _i]),
     "tsp_multi_variant": (_i, [_vp, _i]),
+    "tsp_multi_set_launch": (_i, [_vp, _i]),
+    "tsp_multi_launch": (_i, [_vp]),
     "tsp_multi_info": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "tsp_multi_context": (_vp, [_vp, _i]),
     "tsp_multi_remove": (_i, [_vp, _i]),
@@ -445,6 +447,8 @@ class Engine:
 class Multi:
     """A set of independent simulations stepped together (tsp_multi_*)."""
 
+    PER_DEGREE, FUSED, FUSED_PERSISTENT = 0, 1, 2
+
     def __init__(self, device=0, chunk=32):
         self._L = lib()
         h = _vp()
@@ -495,6 +499,14 @@ class Multi:
 
     def variant(self, degree):
         return self._L.tsp_multi_variant(self._h, int(degree))
+
+    def set_launch(self, mode):
+        """PER_DEGREE (one kernel per degree), FUSED or FUSED_PERSISTENT (one
+        kernel for every degree)."""
+        self._ck(self._L.tsp_multi_set_launch(self._h, int(mode)))
+
+    def launch(self):
+        return self._L.tsp_multi_launch(self._h)
 
     def info(self, i):
         a, b = _i(), _i()

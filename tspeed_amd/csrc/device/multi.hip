@@ -9,16 +9,23 @@
 // budget (and occupancy) of the single-mesh kernel -- whose resident waves walk
 // that degree's items, each running the same element step as k_step32c
 // (block-ELL SpMV over column quads + compensated update + source + receivers).
-// The per-degree kernels of a step are independent graph nodes, so they run
-// side by side; k_multi_advance, which bumps the step counter of every running
-// instance, depends on all of them.  Step counters live on the device, so a
-// chunk of steps is one graph, built once and replayed.  Instances stop at their
+// In a captured chunk of steps each degree is its own chain of graph nodes
+// (step kernel, then k_multi_advance bumping the step counters of that degree's
+// instances), so the degrees run side by side and none waits for another at a
+// step boundary.  Step counters live on the device, so a chunk of steps is one
+// graph, built once and replayed.  Instances stop at their
 // own step count; finished ones are dropped from the items at the next chunk.
+//
+// The other launch mode, k_multi_fused, is one kernel for every degree: one
+// item list, largest degree first, each item dispatched on its instance's
+// degree to the same element step.
 //
 // Autotuning: per degree, tsp_multi_autotune times the launch variants on real
 // steps of that degree's instances and keeps the fastest: persistent loop or
 // one item per wave, and for d <= 32 also the lane mapping (column groups of
-// 16/32 lanes reduced by shuffles, or one row per lane across the wave).
+// 16/32 lanes reduced by shuffles, or one row per lane across the wave).  Then
+// it times one graph chunk of the whole set in each launch mode (per-degree
+// kernels, fused one-shot, fused persistent) and keeps the fastest.
 #include "../../../include/tspeed_amd.h"
 #include "internal.hpp"
 #include "step_common.hpp"
@@ -156,6 +163,63 @@ __global__ __launch_bounds__(64 * kMWaves, PERSIST ? (G::D > 64 ? 2 : 4) : 1) vo
 constexpr int kVarRows = 1, kVarOneShot = 2;
 
 template <int P>
+__device__ inline void fused_item(const MInst &I, int e, int lane, float (*xs)[kMaxDP])
+{
+    if constexpr (2 * ((P + 1) * (P + 2) / 2) <= 32) {
+        if (I.variant & kVarRows) {
+            multi_elem<Geom<P, 64>>(I, e, lane, xs);
+            return;
+        }
+    }
+    multi_elem<Geom<P>>(I, e, lane, xs);
+}
+
+// Every degree in one launch: the items of all running instances in one list,
+// largest degree first, each dispatched (on the wave-uniform degree of its
+// instance) to that degree's element step with the degree's tuned lane mapping.
+// The kernel carries the register budget of its largest body, so the small
+// degrees run at a lower occupancy than in their own kernels; in exchange the
+// whole step is one launch and the dispatcher balances all items together.
+__device__ inline void fused_dispatch(const MInst &I, int e, int lane, float (*xs)[kMaxDP])
+{
+    switch (I.P) {
+    case 1: fused_item<1>(I, e, lane, xs); break;
+    case 2: fused_item<2>(I, e, lane, xs); break;
+    case 3: fused_item<3>(I, e, lane, xs); break;
+    case 4: fused_item<4>(I, e, lane, xs); break;
+    case 5: fused_item<5>(I, e, lane, xs); break;
+    case 6: fused_item<6>(I, e, lane, xs); break;
+    case 7: fused_item<7>(I, e, lane, xs); break;
+    case 8: fused_item<8>(I, e, lane, xs); break;
+    default: break;
+    }
+}
+
+template <bool PERSIST>
+__global__ __launch_bounds__(64 * kMWaves) void k_multi_fused(const MInst *insts, const int2 *items, int nitems)
+{
+    __shared__ __attribute__((aligned(16))) float xs_all[kMWaves][4][kMaxDP];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    float(*xs)[kMaxDP] = &xs_all[wave][0];
+    if (PERSIST) {
+        for (int it = blockIdx.x * kMWaves + wave; it < nitems; it += gridDim.x * kMWaves) {
+            const int2 t = items[it];
+            const MInst I = insts[t.x];
+            if (!I.live || I.steps >= I.nsteps) continue;
+            fused_dispatch(I, t.y, lane, xs);
+        }
+    } else {
+        const int it = blockIdx.x * kMWaves + wave;
+        if (it >= nitems) return;
+        const int2 t = items[it];
+        const MInst I = insts[t.x];
+        if (!I.live || I.steps >= I.nsteps) return;
+        fused_dispatch(I, t.y, lane, xs);
+    }
+}
+
+template <int P>
 const void *kern_p(int v)
 {
     constexpr bool narrow = 2 * ((P + 1) * (P + 2) / 2) <= 32;
@@ -181,10 +245,12 @@ const void *multi_kernel(int P, int v)
     }
 }
 
-__global__ void k_multi_advance(MInst *insts, int n)
+// Advance the step counter of every running instance of degree P (0 = all).
+__global__ void k_multi_advance(MInst *insts, int n, int P)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && insts[i].sel && insts[i].live && insts[i].steps < insts[i].nsteps) insts[i].steps += 1;
+    if (i < n && (P == 0 || insts[i].P == P) && insts[i].sel && insts[i].live && insts[i].steps < insts[i].nsteps)
+        insts[i].steps += 1;
 }
 
 }  // namespace
@@ -215,6 +281,7 @@ struct tsp_multi {
     std::vector<Range> ranges;          // items of one degree: [off, off+count)
     std::map<int, int> variant_of_p;    // autotuned lane mapping per degree
     std::vector<int> only_p;            // restrict the item list (autotuning)
+    int launch = TSP_MULTI_PER_DEGREE;  // per-degree kernels or one fused kernel
 };
 
 namespace {
@@ -306,7 +373,12 @@ void rebuild(tsp_multi *m)
     std::vector<int> order;
     for (int i = 0; i < n; ++i)
         if (m->host[i].sel) order.push_back(i);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return m->host[a].P < m->host[b].P; });
+    // per-degree launches need each degree's items contiguous; the fused kernel
+    // takes the largest degree (the longest items) first
+    const bool fused = m->launch != TSP_MULTI_PER_DEGREE;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return fused ? m->host[a].P > m->host[b].P : m->host[a].P < m->host[b].P;
+    });
     for (int i : order)
         for (int e = 0; e < m->host[i].ne; ++e) items.push_back(make_int2(i, e));
     m->nitems = (int)items.size();
@@ -323,7 +395,25 @@ void rebuild(tsp_multi *m)
     mhip(m, hipGetDeviceProperties(&prop, m->device), "properties");
     const int cus = prop.multiProcessorCount;
     m->ranges.clear();
-    for (int q = 0; q < m->nitems;) {
+    if (fused && m->nitems) {
+        tsp_multi::Range g;
+        g.P = 0;
+        g.variant = 0;
+        g.off = 0;
+        g.count = m->nitems;
+        const bool persist = m->launch == TSP_MULTI_FUSED_PERSISTENT;
+        g.kern = persist ? reinterpret_cast<const void *>(&k_multi_fused<true>)
+                         : reinterpret_cast<const void *>(&k_multi_fused<false>);
+        const int want = (g.count + kMWaves - 1) / kMWaves;
+        g.grid = want;
+        if (persist) {
+            int per_cu = 0;
+            mhip(m, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, g.kern, 64 * kMWaves, 0), "occupancy");
+            g.grid = std::max(1, std::min(want, std::max(1, per_cu) * cus));
+        }
+        m->ranges.push_back(g);
+    }
+    for (int q = 0; q < m->nitems && !fused;) {
         const int P = m->host[items[q].x].P;
         int r = q;
         while (r < m->nitems && m->host[items[r].x].P == P) ++r;
@@ -357,25 +447,29 @@ void launch_steps(tsp_multi *m, int nsteps)
             void *args[] = {&m->d_inst, &items, &g.count};
             mhip(m, hipLaunchKernel(g.kern, dim3(g.grid), dim3(64 * kMWaves), args, 0, m->stream), "multi step");
         }
-        hipLaunchKernelGGL(k_multi_advance, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->d_inst, n);
+        hipLaunchKernelGGL(k_multi_advance, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->d_inst, n, 0);
     }
     mhip(m, hipGetLastError(), "multi step launch");
 }
 
-// one graph for `chunk` steps: per step, the degree kernels side by side, then
-// the counter advance that depends on all of them
+// one graph for `chunk` steps: one chain per launch range (degree), each step
+// of a chain followed by the counter advance of that range's instances only.
+// Instances of different degrees never read each other's data, so the chains
+// are independent: a degree that finishes its steps early leaves the device to
+// the others instead of waiting at every step for the slowest degree.
 void build_graph(tsp_multi *m)
 {
     drop_graph(m);
     hipGraph_t g = nullptr;
     mhip(m, hipGraphCreate(&g, 0), "graph");
     int n = (int)m->host.size();
-    hipGraphNode_t prev = nullptr;
-    for (int k = 0; k < m->chunk; ++k) {
-        std::vector<hipGraphNode_t> step_nodes;
-        for (auto &r : m->ranges) {
-            const int2 *items = m->d_items + r.off;
-            void *args[] = {&m->d_inst, &items, &r.count};
+    for (auto &r : m->ranges) {
+        const int2 *items = m->d_items + r.off;
+        void *args[] = {&m->d_inst, &items, &r.count};
+        int adv_p = r.P;  // 0 for the fused range: every instance
+        void *aargs[] = {&m->d_inst, &n, &adv_p};
+        hipGraphNode_t prev = nullptr;
+        for (int k = 0; k < m->chunk; ++k) {
             hipKernelNodeParams kp{};
             kp.func = const_cast<void *>(r.kern);
             kp.gridDim = dim3(r.grid);
@@ -385,19 +479,15 @@ void build_graph(tsp_multi *m)
             kp.extra = nullptr;
             hipGraphNode_t node;
             mhip(m, hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &kp), "graph node");
-            step_nodes.push_back(node);
+            hipKernelNodeParams ap{};
+            ap.func = reinterpret_cast<void *>(&k_multi_advance);
+            ap.gridDim = dim3((n + 255) / 256);
+            ap.blockDim = dim3(256);
+            ap.sharedMemBytes = 0;
+            ap.kernelParams = aargs;
+            ap.extra = nullptr;
+            mhip(m, hipGraphAddKernelNode(&prev, g, &node, 1, &ap), "graph node");
         }
-        void *aargs[] = {&m->d_inst, &n};
-        hipKernelNodeParams ap{};
-        ap.func = reinterpret_cast<void *>(&k_multi_advance);
-        ap.gridDim = dim3((n + 255) / 256);
-        ap.blockDim = dim3(256);
-        ap.sharedMemBytes = 0;
-        ap.kernelParams = aargs;
-        ap.extra = nullptr;
-        hipGraphNode_t adv;
-        mhip(m, hipGraphAddKernelNode(&adv, g, step_nodes.data(), step_nodes.size(), &ap), "graph node");
-        prev = adv;
     }
     hipError_t e = hipGraphInstantiate(&m->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -549,6 +639,9 @@ int tsp_multi_run_timed(tsp_multi *m, int nsteps, float *ms)
         if (!ms) mfail(m, TSP_ERR_ARG, "null output");
         sync_progress(m);
         rebuild(m);
+        // graph instantiation is host work: keep it out of the timed span
+        if (m->nitems && m->chunk > 0 && nsteps >= m->chunk && (!m->graph || m->graph_chunk != m->chunk))
+            build_graph(m);
         hipEvent_t a, b;
         mhip(m, hipEventCreate(&a), "event");
         mhip(m, hipEventCreate(&b), "event");
@@ -606,8 +699,58 @@ int tsp_multi_autotune(tsp_multi *m, int steps_per_trial)
         m->only_p.clear();
         m->chunk = saved_chunk;
         m->dirty = true;
+        // launch mode, on one graph chunk of the whole set (the per-degree kernels
+        // only run side by side inside a graph)
+        if (m->chunk > 0) {
+            bool enough = false;
+            for (const auto &h : m->host)
+                if (h.live && h.steps + 3 * m->chunk <= h.nsteps) enough = true;
+            if (enough) {
+                float best = 0;
+                int bestm = TSP_MULTI_PER_DEGREE;
+                for (int mode : {TSP_MULTI_PER_DEGREE, TSP_MULTI_FUSED, TSP_MULTI_FUSED_PERSISTENT}) {
+                    m->launch = mode;
+                    m->dirty = true;
+                    rebuild(m);
+                    build_graph(m);
+                    hipEvent_t a, b;
+                    mhip(m, hipEventCreate(&a), "event");
+                    mhip(m, hipEventCreate(&b), "event");
+                    mhip(m, hipEventRecord(a, m->stream), "event");
+                    advance(m, m->chunk);
+                    mhip(m, hipEventRecord(b, m->stream), "event");
+                    mhip(m, hipEventSynchronize(b), "event");
+                    float t = 0;
+                    mhip(m, hipEventElapsedTime(&t, a, b), "event");
+                    (void)hipEventDestroy(a);
+                    (void)hipEventDestroy(b);
+                    sync_progress(m);
+                    if (mode == TSP_MULTI_PER_DEGREE || t < best) {
+                        best = t;
+                        bestm = mode;
+                    }
+                }
+                m->launch = bestm;
+                m->dirty = true;
+            }
+        }
     });
 }
+
+int tsp_multi_set_launch(tsp_multi *m, int mode)
+{
+    return mguard(m, [&] {
+        if (mode != TSP_MULTI_PER_DEGREE && mode != TSP_MULTI_FUSED && mode != TSP_MULTI_FUSED_PERSISTENT)
+            mfail(m, TSP_ERR_ARG, "unknown launch mode %d", mode);
+        if (mode != m->launch) {
+            sync_progress(m);
+            m->launch = mode;
+            m->dirty = true;
+        }
+    });
+}
+
+int tsp_multi_launch(const tsp_multi *m) { return m ? m->launch : -1; }
 
 int tsp_multi_variant(const tsp_multi *m, int degree)
 {
