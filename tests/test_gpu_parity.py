@@ -273,6 +273,60 @@ def test_nonfinite_detected_and_errors_reported():
         e.set_mesh(p.verts, bad, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
 
 
+def test_state_errors_and_zero_step_runs():
+    """Calls out of order or past a buffer are refused with the documented code
+    and leave the context usable; a zero-step run changes nothing."""
+    p = small_problem(3, nsteps=30, nrec=4)
+    e = capi.Engine("fp64")
+    e.set_tables(3)
+    e.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+    with pytest.raises(capi.EngineError) as ex:       # the operator is already assembled
+        e.set_mass(np.eye(p.nln))
+    assert ex.value.code == capi.TSP_ERR_STATE
+    e.set_sources(p.src_elem, p.src_phi, p.wavelet(31))
+    e.set_receivers(p.rec_elem, p.rec_phi, 1, 11)     # room for u1 + 10 samples
+    e.set_dt(p.dt)
+    e.set_state()
+    e.first_step()
+    e.run(0)
+    assert e.steps_done() == 0
+    e.run(10)
+    with pytest.raises(capi.EngineError) as ex:       # the 12th sample has no slot
+        e.run(1)
+    assert ex.value.code == capi.TSP_ERR_ARG
+    assert e.steps_done() == 10
+    u = e.get_u()
+    e.run(0)
+    np.testing.assert_array_equal(e.get_u(), u)
+    # the same 10 steps as a fresh run with a large buffer
+    _, u_ref, s_ref = W.run_engine(small_problem(3, nsteps=10, nrec=4), "fp64")
+    np.testing.assert_array_equal(u, u_ref)
+    np.testing.assert_array_equal(e.seismograms(), s_ref)
+    e.close()
+    # a set takes only first-stepped single-column FP32C contexts
+    m = capi.Multi(chunk=4)
+    for prec, ncols in (("fp64", 1), ("fp32c", 32)):
+        q = capi.Engine(prec, ncols=ncols)
+        q.set_tables(3)
+        q.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+        q.set_dt(p.dt)
+        for c in range(ncols):
+            q.set_state(col=c)
+        q.first_step()
+        with pytest.raises(capi.EngineError) as ex:
+            m.add(q, 5)
+        assert ex.value.code == capi.TSP_ERR_UNSUPPORTED
+        q.close()
+    q = capi.Engine("fp32c")
+    q.set_tables(3)
+    q.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+    with pytest.raises(capi.EngineError) as ex:       # no first step yet
+        m.add(q, 5)
+    assert ex.value.code == capi.TSP_ERR_STATE
+    q.close()
+    m.close()
+
+
 def test_single_element_mesh():
     """A lone triangle: every face is a boundary face, B = A."""
     pts = np.array([[0.0, 0.0], [100.0, 10.0], [20.0, 90.0]])
