@@ -162,7 +162,8 @@ def run_serving(n_req=32, loads=(0.5, 0.8), sample_frac=0.05, seed=11):
     work = sum(p.ndof * len(p.src_elem) * p.nsteps for p in reqs)
     out = dict(requests=n_req, dof_steps=work,
                workload="serving: Poisson arrivals, HBM-footprint admission at 80%, continuous batching "
-                        "(chunks of 32 steps); each request one 32-column MFMA context on its own stream")
+                        "(chunks of 32 steps); each request one MFMA context of the narrowest width (8/16/32/64 "
+                        "columns) that holds its sources, on its own stream")
     # the other execution mode (one instance per source in the shared set), saturated, for comparison
     srv, ids, lat, wall = W.serve(reqs, rate=0, mode=capi.Server.INSTANCES)
     st = srv.stats()

@@ -54,9 +54,11 @@ int tsp_device_memory(int device, long long *free_bytes, long long *total_bytes)
 /* Context lifetime.  Replaces the construction of TimeAdvance/LeapFrog
  * (TimeAdvance_imp.hpp:26-40).  `device` is the HIP ordinal; `precision` one
  * of tsp_precision; `ncols` the number of right-hand-side columns: 1 = the
- * reference's single simulation; 32 or 64 = that many simulations on one mesh
- * (one point source each), stepped together as a block SpMM on the MFMA cores
- * (TSP_FP32C or TSP_BF16X3). */
+ * reference's single simulation; 8, 16, 32 or 64 = that many simulations on
+ * one mesh (one point source each), stepped together as a block SpMM on the
+ * MFMA cores (TSP_FP32C or TSP_BF16X3).  The MFMA tile is 32 columns wide, so
+ * 8 and 16 cost the operator stream and products of 32 but carry (and move)
+ * only their own columns' state. */
 int  tsp_create(int device, int precision, int ncols, tsp_context **out);
 void tsp_destroy(tsp_context *ctx);
 const char *tsp_last_error(const tsp_context *ctx);
@@ -299,7 +301,8 @@ typedef struct {
 int    tsp_server_create(int device, double mem_fraction, int chunk, tsp_server **out);
 void   tsp_server_destroy(tsp_server *s);
 /* How requests run (before the first submit):
- *   TSP_SERVE_COLUMNS (default): one 32-column context per request (sources as
+ *   TSP_SERVE_COLUMNS (default): one 8/16/32/64-column context per request, the
+ *     narrowest that holds its sources (sources as
  *     columns of a block SpMM on MFMA), each request's chunk on its own stream;
  *   TSP_SERVE_INSTANCES: one single-column context per source (sharing the
  *     request's operator) in one tsp_multi set. */

@@ -481,8 +481,33 @@ def test_batched_bf16x3_error_against_the_oracle():
 
 
 @pytest.mark.parametrize("precision", ["fp32c", "bf16x3"])
+@pytest.mark.parametrize("N", [2, 4, 5, 7])
+def test_narrow_contexts_equal_the_wide_ones_bitwise(N, precision):
+    """8- and 16-column contexts fill part of the 32-column MFMA tile: every
+    output column is summed on its own, so a source's seismograms and final
+    state are bit for bit those it gets in a 32- or 64-column context."""
+    p = small_problem(N, nx=5, ny=4, nsteps=30, seed=50 + N, nrec=6)
+    p.src_xy = np.array([[220.0, 640.0], [700.0, 500.0], [480.0, 180.0]])
+    p.f0 = np.repeat(p.f0, 3)
+    p.t0 = np.repeat(p.t0, 3)
+    p.dirs = np.array([[0.0, -1.0], [0.6, 0.8], [1.0, 0.0]])
+    p.__post_init__()
+    runs = {}
+    for ncols in (8, 16, 32, 64):
+        eng, s = W.run_batched(p, ncols, precision=precision)
+        assert eng.status() == 0
+        runs[ncols] = (s[:, :, :3], [eng.get_u(c) for c in range(3)])
+        assert not s[:, :, 3:].any()
+        eng.close()
+    for ncols in (8, 16, 64):
+        np.testing.assert_array_equal(runs[ncols][0], runs[32][0])
+        for a, b in zip(runs[ncols][1], runs[32][1]):
+            np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", ["fp32c", "bf16x3"])
 @pytest.mark.parametrize("N", [1, 2, 3, 5, 6, 7, 8])
-@pytest.mark.parametrize("ncols", [32, 64])
+@pytest.mark.parametrize("ncols", [8, 16, 32, 64])
 def test_batched_every_degree(N, ncols, precision):
     """Every degree on the MFMA path (row tiles 1..3, padded row octets): three
     sources in columns 0, 5 and ncols-1, each within 1e-4 (FP32 products) or
@@ -739,8 +764,9 @@ def test_server_result_is_the_same_alone_and_in_the_mix(mode):
     """A request's seismograms are identical whether it runs alone or admitted
     among five others with staggered arrivals, and stay within 1e-4 of the FP64
     path source by source.  Instances mode: equal to the single-context FP32
-    path bit for bit; columns mode: equal to the same request run as a
-    32-column context."""
+    path bit for bit; columns mode: equal to the same request run as an
+    8-column context (the width the loop picks for 3 sources) and as a
+    32-column one."""
     import copy
     md = capi.Server.COLUMNS if mode == "columns" else capi.Server.INSTANCES
     target = _req(0, nsrc=(3, 3))
@@ -754,8 +780,9 @@ def test_server_result_is_the_same_alone_and_in_the_mix(mode):
     srv.close()
     np.testing.assert_array_equal(mixed, alone)
     if md == capi.Server.COLUMNS:
-        _, sb = W.run_batched(target, 32)
-        np.testing.assert_array_equal(alone, sb[..., :3, :].transpose(2, 0, 1, 3))
+        for width in (8, 32):
+            _, sb = W.run_batched(target, width)
+            np.testing.assert_array_equal(alone, sb[..., :3, :].transpose(2, 0, 1, 3))
     for k in range(len(target.src_elem)):
         q = copy.copy(target)
         q.src_xy, q.f0, q.t0, q.dirs = target.src_xy[k:k + 1], target.f0[k:k + 1], target.t0[k:k + 1], \

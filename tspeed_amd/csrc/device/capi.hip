@@ -442,7 +442,7 @@ int tsp_create(int device, int precision, int ncols, tsp_context **out)
     if (!out) return TSP_ERR_ARG;
     *out = nullptr;
     if (precision != TSP_FP64 && precision != TSP_FP32C && precision != TSP_BF16X3) return TSP_ERR_UNSUPPORTED;
-    if (ncols != 1 && ncols != 32 && ncols != 64) return TSP_ERR_UNSUPPORTED;
+    if (ncols != 1 && ncols != 8 && ncols != 16 && ncols != 32 && ncols != 64) return TSP_ERR_UNSUPPORTED;
     if (ncols > 1 && precision == TSP_FP64) return TSP_ERR_UNSUPPORTED;
     if (ncols == 1 && precision == TSP_BF16X3) return TSP_ERR_UNSUPPORTED;
     tsp_context *c = new (std::nothrow) tsp_context;

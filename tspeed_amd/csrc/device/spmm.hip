@@ -1,7 +1,7 @@
 // spmm.hip -- many right-hand sides on one mesh: the leap-frog step as a block
 // SpMM on the MFMA cores (FP32, compensated update).
 //
-// R columns (R = 32 or 64) share the operator; each column is an independent
+// R columns (R = 8, 16, 32 or 64) share the operator; each column is an independent
 // simulation with its own point source, i.e. the reference's LeapFrog::step
 // (TimeAdvance.cpp:41-59) applied to R vectors at once.  Per element the step is
 //   Y (d x R) = sum_s B_{e,s} (d x d) * U^n_{nb(e,s)} (d x R)
@@ -58,11 +58,20 @@ struct BGeom {
     static constexpr int KB = KP / 8;         // row octets per element
 };
 
+// Waves per element and columns per wave: a wave owns one 32-column MFMA tile;
+// a context of fewer than 32 columns fills only the first R columns of it (the
+// other lanes read column 0 of the element, use zeros and store nothing).
+template <int R>
+struct BCols {
+    static constexpr int WPE = R >= 32 ? R / 32 : 1;
+    static constexpr int CW = R >= 32 ? 32 : R;
+};
+
 // Compensated update of the rows and columns a lane holds (four rows of one
 // column per 16-byte access), the NaN/Inf flag, and the receiver dot products
 // of every column: shared by the FP32 and the BF16x3 products.
 template <int P, int NT, int R>
-__device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int half, int jj, size_t ebase,
+__device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int half, int jj, bool colok, size_t ebase,
                                 f32x16 (&acc)[BGeom<P>::MT][NT], const f32x4 (&uhv)[BGeom<P>::KB][NT],
                                 const f32x4 (&wv)[BGeom<P>::KB][NT], const f32x4 (&cp)[BGeom<P>::KB][NT])
 {
@@ -106,9 +115,11 @@ __device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int h
                     acc[mt][nt][4 * g + q] = t;
                     bad |= !isfinite(t);
                 }
-                st4(a.ucomp + idx, cn);
-                st4(a.uhi_out + idx, tn);
-                st4(a.winc + idx, wn);
+                if (colok) {
+                    st4(a.ucomp + idx, cn);
+                    st4(a.uhi_out + idx, tn);
+                    st4(a.winc + idx, wn);
+                }
             }
     if (bad) atomicOr(a.flags, 1u);
 
@@ -130,7 +141,7 @@ __device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int h
                     }
                 sx += __shfl_xor(sx, 32);
                 sy += __shfl_xor(sy, 32);
-                if (half == 0) {
+                if (half == 0 && colok) {
                     const size_t o = (((size_t)a.sample * a.nrec + rec) * R + c0 + nt * 32 + jj) * 2;
                     a.seis[o] = sx;
                     a.seis[o + 1] = sy;
@@ -155,27 +166,28 @@ __device__ inline void resolve_step(BatchArgs &a)
 __global__ void k_ctr_set(int *ctr, int value) { *ctr = value; }
 __global__ void k_ctr_advance(int *ctr) { *ctr += 1; }
 
-// One wave computes one element's rows for NT column tiles (WPE waves share an
-// element, each with its own NT*32 columns).  A missing neighbour's block is
+// One wave computes one element's rows for one 32-column tile (R/32 waves share
+// an element when R >= 32; below that the tile is partly empty).  A missing neighbour's block is
 // zero in the packed operator, so its slot multiplies the element's own
 // columns by zero and the slot loop needs no branch.  Padded rows stay zero:
 // their operator rows and columns are zero and they carry no source.
-template <int P, int NT, int WPE, int PD>
+template <int P, int R, int PD>
 __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 {
     resolve_step(a);
     using G = BGeom<P>;
-    constexpr int MT = G::MT, KB = G::KB, D = G::D, NLN = G::NLN;
-    constexpr int R = 32 * NT * WPE;
+    constexpr int MT = G::MT, KB = G::KB, D = G::D, NLN = G::NLN, NT = 1;
+    constexpr int WPE = BCols<R>::WPE, CW = BCols<R>::CW;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int gw = xcd_remap_b(blockIdx.x, gridDim.x) * kBWaves + wave;
-    const int e = gw / WPE, c0 = (gw % WPE) * NT * 32;
+    const int e = gw / WPE, c0 = (gw % WPE) * 32;
     if (e >= a.ne) return;  // no block-level barrier in this kernel
     const int half = lane >> 5, jj = lane & 31;
+    const bool colok = CW == 32 || jj < CW;
     const int4 m = a.meta[e];
     const int nbs[4] = {e, m.x >= 0 ? m.x : e, m.y >= 0 ? m.y : e, m.z >= 0 ? m.z : e};
-    // this lane's 16-byte piece of column c0 + nt*32 + jj in any row octet
-    const int lofs = (c0 + jj) * 8 + 4 * half;
+    // this lane's 16-byte piece of column c0 + jj in any row octet
+    const int lofs = (c0 + (colok ? jj : 0)) * 8 + 4 * half;
 
     f32x16 acc[MT][NT];
 #pragma unroll
@@ -203,7 +215,8 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
         for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
-                const f32x4 x = *reinterpret_cast<const f32x4 *>(X + (size_t)kb * R * 8 + nt * 256);
+                f32x4 x = *reinterpret_cast<const f32x4 *>(X + (size_t)kb * R * 8 + nt * 256);
+                if (!colok) x = f32x4{};
                 if (s == 0) own[kb][nt] = x; else bv[buf][kb][nt] = x;
             }
         if (s == 3) {
@@ -212,8 +225,8 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
                     const size_t idx = ebase + (size_t)kb * R * 8 + nt * 256;
-                    wv[kb][nt] = ld4(a.winc + idx);
-                    cp[kb][nt] = ld4(a.ucomp + idx);
+                    wv[kb][nt] = colok ? ld4(a.winc + idx) : f32x4{};
+                    cp[kb][nt] = colok ? ld4(a.ucomp + idx) : f32x4{};
                 }
         }
     };
@@ -241,7 +254,7 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
         if constexpr (PD > 0) __builtin_amdgcn_sched_barrier(0);
     }
 
-    b_finish<P, NT, R>(a, e, m, c0, half, jj, ebase, acc, own, wv, cp);
+    b_finish<P, NT, R>(a, e, m, c0, half, jj, colok, ebase, acc, own, wv, cp);
 }
 
 // The same step with the block products split into three BF16 products on
@@ -254,22 +267,24 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 // lane l's B operand is octet 2c + l/32 of its column: 32 contiguous bytes.
 // Operator: per block, row tile mt and k-chunk c, 64 lanes x 8 BF16 of a_hi
 // then 64 lanes x 8 of a_lo (the same bytes as the FP32 operator).
-template <int P, int WPE, int PD>
+template <int P, int R, int PD>
 __global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3(BatchArgs a)
 {
     resolve_step(a);
     using G = BGeom<P>;
     constexpr int MT = G::MT, KB = G::KB, KC = KB / 2, NT = 1;
-    constexpr int R = 32 * WPE;
+    constexpr int WPE = BCols<R>::WPE, CW = BCols<R>::CW;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int gw = xcd_remap_b(blockIdx.x, gridDim.x) * kBWaves + wave;
     const int e = gw / WPE, c0 = (gw % WPE) * 32;
     if (e >= a.ne) return;  // no block-level barrier in this kernel
     const int half = lane >> 5, jj = lane & 31;
+    const bool colok = CW == 32 || jj < CW;
+    const int jc = colok ? jj : 0;
     const int4 m = a.meta[e];
     const int nbs[4] = {e, m.x >= 0 ? m.x : e, m.y >= 0 ? m.y : e, m.z >= 0 ? m.z : e};
-    const int lofs = (c0 + jj) * 8 + 4 * half;   // update: rows 4h..4h+3 of an octet
-    const int bofs = (c0 + jj) * 8;              // products: a whole octet
+    const int lofs = (c0 + jc) * 8 + 4 * half;   // update: rows 4h..4h+3 of an octet
+    const int bofs = (c0 + jc) * 8;              // products: a whole octet
 
     f32x16 acc[MT][NT];
 #pragma unroll
@@ -293,16 +308,16 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3(BatchArgs a)
 #pragma unroll
         for (int c = 0; c < KC; ++c) {
             const float *o = X + (size_t)(2 * c + half) * R * 8;
-            xv[buf][c][0] = *reinterpret_cast<const f32x4 *>(o);
-            xv[buf][c][1] = *reinterpret_cast<const f32x4 *>(o + 4);
+            xv[buf][c][0] = colok ? *reinterpret_cast<const f32x4 *>(o) : f32x4{};
+            xv[buf][c][1] = colok ? *reinterpret_cast<const f32x4 *>(o + 4) : f32x4{};
         }
         if (s == 3) {
 #pragma unroll
             for (int kb = 0; kb < KB; ++kb) {
                 const size_t idx = ebase + (size_t)kb * R * 8;
-                uhv[kb][0] = *reinterpret_cast<const f32x4 *>(a.uhi + idx);
-                wv[kb][0] = ld4(a.winc + idx);
-                cp[kb][0] = ld4(a.ucomp + idx);
+                uhv[kb][0] = colok ? *reinterpret_cast<const f32x4 *>(a.uhi + idx) : f32x4{};
+                wv[kb][0] = colok ? ld4(a.winc + idx) : f32x4{};
+                cp[kb][0] = colok ? ld4(a.ucomp + idx) : f32x4{};
             }
         }
     };
@@ -336,7 +351,7 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3(BatchArgs a)
         product(s % NB);
         if constexpr (PD > 0) __builtin_amdgcn_sched_barrier(0);
     }
-    b_finish<P, NT, R>(a, e, m, c0, half, jj, ebase, acc, uhv, wv, cp);
+    b_finish<P, NT, R>(a, e, m, c0, half, jj, colok, ebase, acc, uhv, wv, cp);
 }
 
 // FP64 column-pair operator -> BF16 hi/lo fragments (layout above); blocks of
@@ -449,22 +464,20 @@ __global__ void k_col_out(int ne, int nln, int KP, int R, int col, const float *
 template <int P, int R>
 int step_b32_p(const BatchArgs &a, hipStream_t s)
 {
-    constexpr int WPE = R / 32, NT = 1;
     constexpr int PD = BGeom<P>::MT <= 2 ? 1 : 0;
-    const size_t waves = (size_t)a.ne * WPE;
+    const size_t waves = (size_t)a.ne * BCols<R>::WPE;
     const unsigned nwg = (unsigned)((waves + kBWaves - 1) / kBWaves);
-    hipLaunchKernelGGL((k_step_b32<P, NT, WPE, PD>), dim3(nwg), dim3(64 * kBWaves), 0, s, a);
+    hipLaunchKernelGGL((k_step_b32<P, R, PD>), dim3(nwg), dim3(64 * kBWaves), 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int P, int R>
 int step_b16x3_p(const BatchArgs &a, hipStream_t s)
 {
-    constexpr int WPE = R / 32;
     constexpr int PD = BGeom<P>::MT <= 2 ? 1 : 0;
-    const size_t waves = (size_t)a.ne * WPE;
+    const size_t waves = (size_t)a.ne * BCols<R>::WPE;
     const unsigned nwg = (unsigned)((waves + kBWaves - 1) / kBWaves);
-    hipLaunchKernelGGL((k_step_b16x3<P, WPE, PD>), dim3(nwg), dim3(64 * kBWaves), 0, s, a);
+    hipLaunchKernelGGL((k_step_b16x3<P, R, PD>), dim3(nwg), dim3(64 * kBWaves), 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -522,6 +535,16 @@ size_t batch_block_floats(int P) { return (size_t)4 * batch_kp(P) / 32 * (batch_
 
 int launch_step_b16x3(int P, int ncols, const BatchArgs &a, hipStream_t s)
 {
+    if (ncols == 8) {
+#define CALL(p) step_b16x3_p<p, 8>(a, s)
+        TSP_DISPATCH_P(P, CALL)
+#undef CALL
+    }
+    if (ncols == 16) {
+#define CALL(p) step_b16x3_p<p, 16>(a, s)
+        TSP_DISPATCH_P(P, CALL)
+#undef CALL
+    }
     if (ncols == 32) {
 #define CALL(p) step_b16x3_p<p, 32>(a, s)
         TSP_DISPATCH_P(P, CALL)
@@ -544,6 +567,16 @@ int launch_pack_bf16x3(int P, int ne, const int4 *meta, const double *src, float
 
 int launch_step_b32(int P, int ncols, const BatchArgs &a, hipStream_t s)
 {
+    if (ncols == 8) {
+#define CALL(p) step_b32_p<p, 8>(a, s)
+        TSP_DISPATCH_P(P, CALL)
+#undef CALL
+    }
+    if (ncols == 16) {
+#define CALL(p) step_b32_p<p, 16>(a, s)
+        TSP_DISPATCH_P(P, CALL)
+#undef CALL
+    }
     if (ncols == 32) {
 #define CALL(p) step_b32_p<p, 32>(a, s)
         TSP_DISPATCH_P(P, CALL)

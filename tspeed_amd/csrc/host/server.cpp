@@ -5,8 +5,9 @@
 // the others keep their fields resident), and leave with their seismograms.
 //
 // A request is one mesh with 1..16 point sources.  Two ways to run it:
-//   columns (default): one 32-column context, one source per column, stepped as
-//     a block SpMM on the MFMA cores; every running request's chunk of steps is
+//   columns (default): one multi-column context of the narrowest width (8, 16,
+//     32 or 64) that holds the sources, one source per column, stepped as a
+//     block SpMM on the MFMA cores; every running request's chunk of steps is
 //     queued on its own stream, so requests overlap on the device;
 //   instances: one owner context that assembles the operator and (nsrc-1)
 //     contexts sharing it, all added to one tsp_multi set (one launch per degree
@@ -56,10 +57,15 @@ struct Request {
     std::string error;
 };
 
+// Columns of the context a request runs in: the narrowest width that holds
+// its sources (a column's state is streamed every step whether it carries a
+// source or not).
+int columns_for(int nsrc) { return nsrc <= 8 ? 8 : nsrc <= 16 ? 16 : nsrc <= 32 ? 32 : 64; }
+
 long long footprint_columns(const Request &r)
 {
     const long long nln = (long long)(r.N + 1) * (r.N + 2) / 2, d = 2 * nln;
-    const long long kp = 32 * ((d + 31) / 32), R = r.nsrc <= 32 ? 32 : 64;
+    const long long kp = 32 * ((d + 31) / 32), R = columns_for(r.nsrc);
     const long long ns = 1 + r.nsteps / std::max(r.stride, 1);
     return (long long)r.ne * 4 * kp * kp * 4          // MFMA operator
            + (long long)r.ne * kp * R * (4 * 4 + 2 * 8)  // FP32 state x4, FP64 staging x2
@@ -131,7 +137,7 @@ const Tables &tables_for(tsp_server *s, int N)
 bool admit_columns(tsp_server *s, Request &r)
 {
     const Tables &t = tables_for(s, r.N);
-    r.ncols = r.nsrc <= 32 ? 32 : 64;
+    r.ncols = columns_for(r.nsrc);
     tsp_context *c = nullptr;
     auto bail = [&](const char *what) {
         r.error = std::string(what) + ": " + (c ? tsp_last_error(c) : "");
