@@ -262,8 +262,9 @@ int launch_batched(tsp_context *c, const BatchArgs &b)
                                        : launch_step_b32(c->P, c->ncols, b, c->stream);
 }
 
-// Runs of at least this many multi-column steps replay a captured graph (one
-// launch instead of one per step and one per counter advance).
+// Runs of at least this many multi-column steps replay a captured graph: one
+// launch for the run, each captured step reading its index from a device
+// counter (set before the replay) plus its place in the run.
 constexpr int kGraphMinSteps = 8;
 
 // samples so far are exactly the stride-driven ones (the graph derives the
@@ -290,8 +291,8 @@ void run_batched_graph(tsp_context *c, int nsteps)
         hipcheck(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "capture");
         int rc = 0;
         for (int k = 0; k < nsteps && !rc; ++k) {
+            b.koff = k;  // the step's place in the run: no counter kernel between steps
             rc = launch_batched(c, b);
-            if (!rc) rc = launch_ctr_advance(c->d_ctr, c->stream);
         }
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(c->stream, &g);

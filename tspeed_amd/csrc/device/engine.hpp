@@ -141,17 +141,18 @@ struct BatchArgs {
     const float *sc32;
     const float *uhi;
     float *uhi_out, *ucomp, *winc;
-    // Graph replay: when ctr is set, the step reads its 1-based index n = *ctr + 1
-    // and derives the force row, the sample slot (n / stride) and the ping-pong
-    // buffers (u^n in uhi_pair[n & 1]) on the device; wav_row, sample, uhi and
-    // uhi_out above are then ignored.
+    // Graph replay: when ctr is set, the step reads its 1-based index
+    // n = *ctr + 1 + koff (*ctr = steps done before the replay, koff = the step's
+    // place in the captured run) and derives the force row, the sample slot
+    // (n / stride) and the ping-pong buffers (u^n in uhi_pair[n & 1]) on the
+    // device; wav_row, sample, uhi and uhi_out above are then ignored.
     const int *ctr;
+    int koff;
     const double *wav;
     int stride, max_samples;
     float *uhi_pair[2];
 };
 int launch_ctr_set(int *ctr, int value, hipStream_t s);
-int launch_ctr_advance(int *ctr, hipStream_t s);
 int batch_kp(int P);                 // padded rows per element (multiple of 32)
 size_t batch_block_floats(int P);    // floats of one element's four packed blocks
 int launch_step_b32(int P, int ncols, const BatchArgs &a, hipStream_t s);

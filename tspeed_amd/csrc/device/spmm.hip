@@ -155,7 +155,7 @@ __device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int h
 __device__ inline void resolve_step(BatchArgs &a)
 {
     if (!a.ctr) return;
-    const int n = *a.ctr + 1;
+    const int n = *a.ctr + 1 + a.koff;
     a.wav_row = a.nsrc ? a.wav + (size_t)n * a.nsrc * 2 : nullptr;
     a.sample = (a.stride > 0 && n % a.stride == 0 && n / a.stride < a.max_samples) ? n / a.stride : -1;
     const bool odd = n & 1;  // selects, not an index: keeps the arguments out of scratch
@@ -164,7 +164,6 @@ __device__ inline void resolve_step(BatchArgs &a)
 }
 
 __global__ void k_ctr_set(int *ctr, int value) { *ctr = value; }
-__global__ void k_ctr_advance(int *ctr) { *ctr += 1; }
 
 // One wave computes one element's rows for one 32-column tile (R/32 waves share
 // an element when R >= 32; below that the tile is partly empty).  A missing neighbour's block is
@@ -521,12 +520,6 @@ int sample_b32_p(const BatchArgs &a, int R, hipStream_t s)
 int launch_ctr_set(int *ctr, int value, hipStream_t s)
 {
     hipLaunchKernelGGL(k_ctr_set, dim3(1), dim3(1), 0, s, ctr, value);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int launch_ctr_advance(int *ctr, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_ctr_advance, dim3(1), dim3(1), 0, s, ctr);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
