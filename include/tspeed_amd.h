@@ -317,7 +317,10 @@ int    tsp_server_footprint(const tsp_request *req, int mode, long long *residen
 /* Queue a request (its arrays are copied) that arrives at time `arrival`.
  * TSP_ERR_NOMEM if resident + peak exceeds the whole budget. */
 int    tsp_server_submit(tsp_server *s, const tsp_request *req, double arrival, int *id);
-/* One pass of the loop: admit, run one chunk, retire.  Reports how many
+/* One pass of the loop: admit, run, retire.  Instances mode runs one chunk
+ * of the set and waits for it; columns mode does not wait: it queues the next
+ * two chunks on every request whose stream has run dry (and the result
+ * download behind a request's last chunk), so call it in a loop.  Reports how many
  * requests finished in this pass (completed, or failed at admission: then
  * tsp_server_result returns TSP_ERR_STATE with the reason in
  * tsp_server_last_error) and how many are running / queued. */
@@ -331,8 +334,9 @@ int    tsp_server_result(tsp_server *s, int id, double *seis, int *nsamples, dou
 int    tsp_server_stats(const tsp_server *s, double *busy_seconds, long long *dof_steps, long long *mem_used,
                         long long *mem_budget);
 /* Host seconds tsp_server_tick has spent admitting (context set-up, assembly,
- * first step), running chunks (queueing and waiting for the device) and
- * retiring (seismogram download, release). */
+ * first step), running chunks (instances mode: queueing and waiting for the
+ * device; columns mode: polling the requests' streams and queueing the next
+ * chunks) and retiring (seismogram download, release). */
 int    tsp_server_phases(const tsp_server *s, double *admit_seconds, double *run_seconds, double *retire_seconds);
 
 #ifdef __cplusplus

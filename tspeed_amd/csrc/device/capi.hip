@@ -170,7 +170,11 @@ int guarded(tsp_context *c, F &&f)
 // that changes them drops it
 void drop_graph(tsp_context *c)
 {
-    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    if (c->graph) {
+        // tsp_run only queues work: a replay of this graph may still be running
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipGraphExecDestroy(c->graph);
+    }
     c->graph = nullptr;
     c->graph_len = 0;
 }

@@ -25,6 +25,7 @@
 #include <deque>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -420,25 +421,51 @@ int tsp_server_tick(tsp_server *s, int *completed, int *running, int *queued)
     s->failed_pending = 0;
     const double t1 = clock_s();
     s->t_admit += t1 - t0;
+    bool nap = false;
     if ((!s->running.empty() || !s->draining.empty()) && s->mode == TSP_SERVE_COLUMNS) {
-        // queue a chunk on every running request's stream, then collect
-        for (int id : s->running) {
-            Request &r = s->req[id];
-            const int n = std::min(s->chunk, r.nsteps - r.done);
-            if (n > 0 && tsp_run(r.ctx, n) != TSP_OK) {
+        // No wait on the whole set: a request whose stream has run dry gets its
+        // next two chunks queued (so it never waits for a slower one), and the
+        // download of its results is queued right behind its last chunk.
+        bool moved = false;
+        for (auto it = s->running.begin(); it != s->running.end();) {
+            Request &r = s->req[*it];
+            int idle = 0;
+            if (tsp_stream_idle(r.ctx, &idle) != TSP_OK) {
                 s->err = tsp_last_error(r.ctx);
                 return TSP_ERR_HIP;
             }
-            r.done += std::max(n, 0);
-        }
-        for (int id : s->running)
-            if (tsp_synchronize(s->req[id].ctx) != TSP_OK) {
-                s->err = tsp_last_error(s->req[id].ctx);
-                return TSP_ERR_HIP;
+            if (!idle) {
+                ++it;
+                continue;
             }
+            moved = true;
+            for (int k = 0; k < 2 && r.done < r.nsteps; ++k) {
+                const int n = std::min(s->chunk, r.nsteps - r.done);
+                if (tsp_run(r.ctx, n) != TSP_OK) {
+                    s->err = tsp_last_error(r.ctx);
+                    return TSP_ERR_HIP;
+                }
+                r.done += n;
+            }
+            if (r.done < r.nsteps) {
+                ++it;
+                continue;
+            }
+            if (!start_retire_columns(r)) {
+                tsp_destroy(r.ctx);
+                r.ctx = nullptr;
+                r.state = 3;
+                r.completed = clock_s();
+                s->used -= r.footprint;
+                ++done;
+            } else {
+                s->draining.push_back(*it);
+            }
+            it = s->running.erase(it);
+        }
         const double t = clock_s();
         s->t_run += t - t1;
-        // results of earlier ticks that have landed: release their contexts
+        // results that have landed: release their contexts
         for (auto it = s->draining.begin(); it != s->draining.end();) {
             Request &r = s->req[*it];
             if (copy_landed(r)) {
@@ -448,29 +475,13 @@ int tsp_server_tick(tsp_server *s, int *completed, int *running, int *queued)
                 s->t_last = t;
                 s->dof_steps += (long long)r.ne * (r.N + 1) * (r.N + 2) * r.nsrc * r.nsteps;
                 ++done;
+                moved = true;
                 it = s->draining.erase(it);
             } else {
                 ++it;
             }
         }
-        for (auto it = s->running.begin(); it != s->running.end();) {
-            Request &r = s->req[*it];
-            if (r.done >= r.nsteps) {
-                if (!start_retire_columns(r)) {
-                    tsp_destroy(r.ctx);
-                    r.ctx = nullptr;
-                    r.state = 3;
-                    r.completed = t;
-                    s->used -= r.footprint;
-                    ++done;
-                } else {
-                    s->draining.push_back(*it);
-                }
-                it = s->running.erase(it);
-            } else {
-                ++it;
-            }
-        }
+        nap = !moved && done == 0;
     } else if (!s->running.empty()) {
         int active = 0;
         if (tsp_multi_run(s->multi, s->chunk, &active) != TSP_OK) {
@@ -496,7 +507,9 @@ int tsp_server_tick(tsp_server *s, int *completed, int *running, int *queued)
             }
         }
     }
-    s->t_retire += clock_s() - t1 - 0.0;
+    s->t_retire += clock_s() - t1;
+    // columns mode with nothing to do until some stream runs dry: yield the core briefly
+    if (nap) std::this_thread::sleep_for(std::chrono::microseconds(20));
     if (completed) *completed = done;
     if (running) *running = (int)(s->running.size() + s->draining.size());
     if (queued) *queued = (int)s->queue.size();
