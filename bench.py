@@ -275,6 +275,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32c", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-square", action="store_true", help="skip the 3200-triangle FP64 leg")
     ap.add_argument("--multi", type=int, default=256, help="instances in the uneven-batch variant (0 = skip)")
     ap.add_argument("--serve", type=int, default=32, help="requests in the serving variant (0 = skip)")
     args = ap.parse_args()
@@ -303,6 +304,18 @@ def main():
         variants["fp32c"] = dict(value=prob.ndof * args.steps / (r32["ms_total"] / 1e3), ms_per_step=ms32,
                                  achieved_GBs=b32 / (ms32 / 1e3) / 1e9, algorithmic_bytes=b32,
                                  status=r32["status"])
+
+    if not args.no_square:
+        # the reference's CPU-sized case at its own length: 2000 steps of one 2000-step run
+        sq = W.square_ricker(nsteps=2000 + 20)
+        rsq = run_variant(sq, "fp64", 2000, 20)
+        ms_sq = rsq["ms_total"] / 2000
+        bsq = algorithmic_bytes(sq, 8, 24)
+        variants["square_fp64"] = dict(
+            value=sq.ndof * 2000 / (rsq["ms_total"] / 1e3), ms_per_step=ms_sq, steps=2000,
+            achieved_GBs=bsq / (ms_sq / 1e3) / 1e9, algorithmic_bytes=bsq, status=rsq["status"],
+            workload="square_ricker: 1000x1000 m, 40x40 split squares (3200 triangles), p=4, 96k DOF, "
+                     "32 receivers, 2000 steps; plain FP64")
 
     if not args.no_batched:
         variants["batched_fp32_64col"] = run_batched(args.steps, args.warmup)
