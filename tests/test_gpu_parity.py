@@ -88,6 +88,42 @@ def _ba_problem(N, nsteps=60, seed=50):
     return p
 
 
+@pytest.mark.parametrize("N", [2, 4, 5])
+def test_boundary_adapted_on_the_batched_and_multi_paths(N):
+    """The BoundaryAdapted inverse mass is folded into the operator rows and the
+    source values on upload, so the multi-column MFMA path and the set of
+    instances take it unchanged: three sources as columns of an 8-column
+    context match the single-context FP32 run of each source and the FP64
+    oracle, and a BoundaryAdapted instance in a set is bitwise its single-context
+    FP32 run."""
+    import copy
+    p = _ba_problem(N, nsteps=50)
+    p.src_xy = np.array([[300.0, 700.0], [650.0, 420.0], [820.0, 160.0]])
+    p.f0 = np.repeat(p.f0, 3)
+    p.t0 = np.repeat(p.t0, 3)
+    p.dirs = np.array([[0.0, -1.0], [0.6, 0.8], [1.0, 0.0]])
+    p.__post_init__()
+    eng, s = W.run_batched(p, 8)
+    assert eng.status() == 0
+    for k in range(3):
+        q = copy.copy(p)
+        q.src_xy, q.f0, q.t0, q.dirs = p.src_xy[k:k + 1], p.f0[k:k + 1], p.t0[k:k + 1], p.dirs[k:k + 1]
+        q.__post_init__()
+        _, _, s32 = W.run_engine(q, "fp32c")
+        _, sref = oracle_run(q)
+        e_o, e_32 = rel_l2(s[:, :, k], sref), rel_l2(s[:, :, k], s32)
+        say(f"  BA p={N} column {k}: vs oracle {e_o:.2e}, vs single-column FP32 {e_32:.2e}")
+        assert e_o <= 1e-3 and e_32 <= 1e-3
+        if k == 0:
+            m = capi.Multi(chunk=8)
+            i = m.add(W.prepare_instance(q), q.nsteps)
+            while m.run(13):
+                pass
+            np.testing.assert_array_equal(m.engine(i).seismograms(), s32)
+            m.close()
+    eng.close()
+
+
 @pytest.mark.parametrize("N", range(1, 9))
 def test_boundary_adapted_operator_and_fp64_trajectory(N):
     """Non-orthonormal basis (dense inverse mass, Matrices_imp.hpp:183-190): the

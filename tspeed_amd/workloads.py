@@ -183,7 +183,7 @@ def run_batched(prob: Problem, ncols=64, stride=None, precision="fp32c"):
     [sample, rec, col, 2])."""
     assert len(prob.src_elem) <= ncols
     eng = capi.Engine(precision, ncols=ncols)
-    eng.set_tables(prob.N)
+    set_space(eng, prob)
     eng.set_mesh(prob.verts, prob.neigh, prob.neighedge, prob.lam, prob.mu, prob.rho, prob.penalty)
     stride = prob.stride if stride is None else stride
     eng.set_sources(prob.src_elem, prob.src_phi, prob.wavelet(), col=np.arange(len(prob.src_elem)))
@@ -289,7 +289,7 @@ def prepare_instance(prob: Problem, share=None, stride=None, nsteps=None):
     stride = prob.stride if stride is None else stride
     if share is None:
         eng = capi.Engine("fp32c")
-        eng.set_tables(prob.N)
+        set_space(eng, prob)
         eng.set_mesh(prob.verts, prob.neigh, prob.neighedge, prob.lam, prob.mu, prob.rho, prob.penalty)
         eng.release_fp64()
     else:
@@ -302,12 +302,17 @@ def prepare_instance(prob: Problem, share=None, stride=None, nsteps=None):
     return eng
 
 
-def run_engine(prob: Problem, precision="fp64", engine=None, u0=None, v0=None, stride=None):
-    """Assemble and run `prob` on the device; returns (engine, u_final, seismograms)."""
-    eng = engine or capi.Engine(precision)
+def set_space(eng, prob: Problem):
+    """Tables of prob's basis and, for a non-orthonormal one, its inverse mass."""
     eng.set_tables(prob.N, capi.tables(prob.N, prob.basis))
     if prob.basis != capi.DUBINER:
         eng.set_mass(capi.base_invmass(prob.N, prob.basis))
+
+
+def run_engine(prob: Problem, precision="fp64", engine=None, u0=None, v0=None, stride=None):
+    """Assemble and run `prob` on the device; returns (engine, u_final, seismograms)."""
+    eng = engine or capi.Engine(precision)
+    set_space(eng, prob)
     eng.set_mesh(prob.verts, prob.neigh, prob.neighedge, prob.lam, prob.mu, prob.rho, prob.penalty)
     stride = prob.stride if stride is None else stride
     eng.set_sources(prob.src_elem, prob.src_phi, prob.wavelet())
