@@ -66,6 +66,23 @@ __device__ inline void record(const StepArgs &a, int4 ex, const V *unew, const i
     }
 }
 
+// Gather u^n of one element (nb < 0: zeros) into an LDS slice.
+template <typename G, typename T>
+__device__ inline void gather_one(T *x, const T *u, int nb, int lane)
+{
+    T v[G::RPL];
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const int idx = lane + 64 * q;
+        v[q] = (nb >= 0 && idx < G::D) ? u[(size_t)nb * G::D + idx] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const int idx = lane + 64 * q;
+        if (idx < G::D) x[idx] = v[q];
+    }
+}
+
 // Gather u^n of the element and its three neighbours into the wave's LDS slice;
 // every load is issued before the first store so the four fetches overlap.
 template <typename G, typename T>
@@ -101,8 +118,11 @@ using vec_t = T __attribute__((ext_vector_type(W)));
 // divergent branch, all loads of a slot are in flight together (a per-row
 // branch made the compiler wait on each load of the first slot in turn:
 // 24 -> 17 us per step on a 3200-triangle mesh at p = 4).
+// Slots s0 .. s0+ns-1 only (xs[k] holds slot s0+k's coefficients): the
+// four-waves-per-element FP64 step gives each wave one slot.
 template <typename G, typename T, int W>
-__device__ inline void spmv(const T *blk, int e, const T (*xs)[G::DP], int lane, T (&acc)[G::RPL])
+__device__ inline void spmv(const T *blk, int e, const T (*xs)[G::DP], int lane, T (&acc)[G::RPL], int s0 = 0,
+                            int ns = 4)
 {
     constexpr int NFULL = G::D / W, TAIL = G::D % W;
     constexpr int NIT = (NFULL + G::NG - 1) / G::NG;
@@ -113,8 +133,8 @@ __device__ inline void spmv(const T *blk, int e, const T (*xs)[G::DP], int lane,
     // one slot per iteration: unrolling the four would keep every load of the
     // element live at once and spill
 #pragma unroll 1
-    for (int s = 0; s < 4; ++s) {
-        const T *Bs = blk + ((size_t)e * 4 + s) * G::D * G::D;
+    for (int s = 0; s < ns; ++s) {
+        const T *Bs = blk + ((size_t)e * 4 + s0 + s) * G::D * G::D;
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
             const int jraw = it * G::NG + g;
