@@ -518,10 +518,15 @@ def test_batched_bf16x3_error_against_the_oracle():
 
 @pytest.mark.parametrize("precision", ["fp32c", "bf16x3"])
 @pytest.mark.parametrize("N", [2, 4, 5, 7])
-def test_narrow_contexts_equal_the_wide_ones_bitwise(N, precision):
-    """8- and 16-column contexts fill part of the 32-column MFMA tile: every
+def test_narrow_and_wide_contexts_agree(N, precision):
+    """8- and 16-column contexts fill part of the 32-column MFMA tile and every
     output column is summed on its own, so a source's seismograms and final
-    state are bit for bit those it gets in a 32- or 64-column context."""
+    state are bit for bit the same at every width that runs the same kernel:
+    8 = 16 and 32 = 64 always, and all four where one kernel serves every width
+    (BF16x3 products; blocks of more than one 32-row tile).  With FP32 products
+    and one-tile blocks, full tiles run one slot per wave and add the four slot
+    sums in another order than the partly empty ones: there the two families
+    agree to FP32 rounding."""
     p = small_problem(N, nx=5, ny=4, nsteps=30, seed=50 + N, nrec=6)
     p.src_xy = np.array([[220.0, 640.0], [700.0, 500.0], [480.0, 180.0]])
     p.f0 = np.repeat(p.f0, 3)
@@ -535,10 +540,21 @@ def test_narrow_contexts_equal_the_wide_ones_bitwise(N, precision):
         runs[ncols] = (s[:, :, :3], [eng.get_u(c) for c in range(3)])
         assert not s[:, :, 3:].any()
         eng.close()
-    for ncols in (8, 16, 64):
-        np.testing.assert_array_equal(runs[ncols][0], runs[32][0])
-        for a, b in zip(runs[ncols][1], runs[32][1]):
-            np.testing.assert_array_equal(a, b)
+    one_kernel = precision == "bf16x3" or 2 * (N + 1) * (N + 2) // 2 > 32
+
+    def same(a, b, exact):
+        if exact:
+            np.testing.assert_array_equal(a[0], b[0])
+            for x, y in zip(a[1], b[1]):
+                np.testing.assert_array_equal(x, y)
+        else:
+            assert rel_l2(a[0], b[0]) <= 1e-5
+            for x, y in zip(a[1], b[1]):
+                assert rel_l2(x, y) <= 1e-5
+
+    same(runs[16], runs[8], True)
+    same(runs[64], runs[32], True)
+    same(runs[8], runs[32], one_kernel)
 
 
 @pytest.mark.parametrize("precision", ["fp32c", "bf16x3"])
@@ -801,8 +817,8 @@ def test_server_result_is_the_same_alone_and_in_the_mix(mode):
     among five others with staggered arrivals, and stay within 1e-4 of the FP64
     path source by source.  Instances mode: equal to the single-context FP32
     path bit for bit; columns mode: equal to the same request run as an
-    8-column context (the width the loop picks for 3 sources) and as a
-    32-column one."""
+    8-column context (the width the loop picks for 3 sources), and to FP32
+    rounding to it run as a 32-column one."""
     import copy
     md = capi.Server.COLUMNS if mode == "columns" else capi.Server.INSTANCES
     target = _req(0, nsrc=(3, 3))
@@ -816,9 +832,10 @@ def test_server_result_is_the_same_alone_and_in_the_mix(mode):
     srv.close()
     np.testing.assert_array_equal(mixed, alone)
     if md == capi.Server.COLUMNS:
-        for width in (8, 32):
-            _, sb = W.run_batched(target, width)
-            np.testing.assert_array_equal(alone, sb[..., :3, :].transpose(2, 0, 1, 3))
+        _, sb = W.run_batched(target, 8)
+        np.testing.assert_array_equal(alone, sb[..., :3, :].transpose(2, 0, 1, 3))
+        _, sb = W.run_batched(target, 32)   # one slot per wave at full tiles: FP32 rounding apart
+        assert rel_l2(alone, sb[..., :3, :].transpose(2, 0, 1, 3)) <= 1e-5
     for k in range(len(target.src_elem)):
         q = copy.copy(target)
         q.src_xy, q.f0, q.t0, q.dirs = target.src_xy[k:k + 1], target.f0[k:k + 1], target.t0[k:k + 1], \
