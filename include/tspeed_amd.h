@@ -38,7 +38,7 @@ enum tsp_precision {
     TSP_FP64 = 0,   /* plain FP64 blocks and update: the correctness path            */
     TSP_FP32C = 1,  /* FP32 blocks, FP32 increments, Kahan-compensated displacement */
     TSP_BF16X3 = 2  /* as TSP_FP32C, with each block product split into three BF16
-                       products accumulated in FP32 (32/64 columns only)            */
+                       products accumulated in FP32 (multi-column contexts only)   */
 };
 
 /* Status bits reported by tsp_status (no counterpart in the reference, which has

@@ -6,8 +6,9 @@
 // (TimeAdvance.cpp:41-59) applied to R vectors at once.  Per element the step is
 //   Y (d x R) = sum_s B_{e,s} (d x d) * U^n_{nb(e,s)} (d x R)
 // computed with v_mfma_f32_32x32x2f32 in 32x32 output tiles (MT row tiles of
-// the block, NT column tiles per wave), followed by the same compensated update
-// as the single-column FP32 path and the receiver dot products of every column.
+// the block, one 32-column tile per wave), followed by the same compensated
+// update as the single-column FP32 path and the receiver dot products of every
+// column.
 //
 // MFMA operand/result layout of v_mfma_f32_32x32x2f32 (wave64): A lane l holds
 // A[l%32][l/32], B lane l holds B[l/32][l%32], result register r of lane l
