@@ -749,6 +749,39 @@ def test_multi_graph_and_plain_launches_agree_and_autotune():
     m.close()
 
 
+def test_multi_every_launch_variant():
+    """Every launch variant of every degree (persistent or one item per wave,
+    whole-wave rows for d <= 32, one element per work-group with a slot per
+    wave): the default (one item per wave) is bitwise the single-context FP32
+    run, every other variant agrees with it to FP32 rounding."""
+    probs = _mixed_set()
+    degrees = sorted({p.N for p in probs})
+    ref = [W.run_engine(p, "fp32c")[2] for p in probs]
+    for var in range(8):
+        m = capi.Multi(chunk=8)
+        ids = [m.add(W.prepare_instance(p), p.nsteps) for p in probs]
+        for N in degrees:
+            narrow = (N + 1) * (N + 2) <= 32
+            v = var if (narrow or not (var & m.ROWS) or (var & m.SLOTS)) else var & ~m.ROWS
+            m.set_variant(N, v)
+            assert m.variant(N) == v
+        while m.run(9):
+            pass
+        for i, p, s1 in zip(ids, probs, ref):
+            s = m.engine(i).seismograms()
+            if var == m.ONE_SHOT:
+                np.testing.assert_array_equal(s, s1)
+            else:
+                assert rel_l2(s, s1) <= 1e-5, (var, p.N)
+        m.close()
+    m = capi.Multi(chunk=8)
+    with pytest.raises(capi.EngineError):
+        m.set_variant(6, m.ROWS)          # d = 56: no whole-wave rows
+    with pytest.raises(capi.EngineError):
+        m.set_variant(3, 9)
+    m.close()
+
+
 def test_multi_launch_modes_are_bitwise_equal():
     """The fused kernel (every degree in one launch, one-shot or persistent)
     runs each element through the same step as the per-degree kernels: the six

@@ -65,6 +65,7 @@ SIGNATURES = {
     "tsp_multi_run_timed": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_float)]),
     "tsp_multi_autotune": (_i, [_vp, _i]),
     "tsp_multi_variant": (_i, [_vp, _i]),
+    "tsp_multi_set_variant": (_i, [_vp, _i, _i]),
     "tsp_multi_set_launch": (_i, [_vp, _i]),
     "tsp_multi_launch": (_i, [_vp]),
     "tsp_multi_info": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
@@ -499,6 +500,12 @@ class Multi:
 
     def variant(self, degree):
         return self._L.tsp_multi_variant(self._h, int(degree))
+
+    ROWS, ONE_SHOT, SLOTS = 1, 2, 4
+
+    def set_variant(self, degree, variant):
+        """Pin a degree's launch variant (bits ROWS | ONE_SHOT | SLOTS)."""
+        self._ck(self._L.tsp_multi_set_variant(self._h, int(degree), int(variant)))
 
     def set_launch(self, mode):
         """PER_DEGREE (one kernel per degree), FUSED or FUSED_PERSISTENT (one

@@ -158,9 +158,111 @@ __global__ __launch_bounds__(64 * kMWaves, PERSIST ? (G::D > 64 ? 2 : 4) : 1) vo
     }
 }
 
+// One element per work-group, one slot per wave: wave s gathers slot s's
+// coefficients and streams block (e, s); waves 1..3 leave their partial sums in
+// LDS and wave 0 adds them in slot order and finishes the step as multi_elem
+// does.  Shorter streams per wave, four times the waves; the sums are added in
+// another order than in multi_elem, so this is an autotuner candidate, never
+// the default (the default keeps every instance bitwise equal to its
+// single-context FP32c run).
+template <typename G>
+__device__ inline void multi_elem_slots(const MInst &I, int e, int lane, int s, float *x,
+                                        float (*part)[G::RPL][64])
+{
+    const int n = I.steps + 1;
+    const int cur = (I.steps + 1) & 1;
+    const float *uin = cur ? I.uhi1 : I.uhi0;
+    float *uout = cur ? I.uhi0 : I.uhi1;
+    const int4 m = I.meta[e];
+    const int nb = s == 0 ? e : (s == 1 ? m.x : (s == 2 ? m.y : m.z));
+    const int g = lane / G::RW, r = lane % G::RW;
+    float wi[G::RPL], cp[G::RPL];
+    int rows[G::RPL];
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const int row = r + 64 * q;
+        rows[q] = (s == 0 && g == 0 && row < G::D) ? row : -1;
+        const size_t idx = (size_t)e * G::D + row;
+        wi[q] = rows[q] >= 0 ? I.winc[idx] : 0.f;
+        cp[q] = rows[q] >= 0 ? I.ucomp[idx] : 0.f;
+    }
+    const float sc = s == 0 ? I.sc[e] : 0.f;
+    gather_one<G, float>(x, uin, nb, lane);
+    // the wave reads only its own LDS slice: order its stores before its loads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const float(*xv)[G::DP] = reinterpret_cast<const float(*)[G::DP]>(x);
+    float acc[G::RPL];
+    spmv<G, float, 4>(I.blk, e, xv, lane, acc, s, 1);
+    if (s > 0) {
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) part[s - 1][q][lane] = acc[q];
+    }
+    __syncthreads();
+    if (s > 0) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) acc[q] += part[k][q][lane];
+
+    StepArgs a{};
+    a.nrec = I.nrec;
+    a.src_sorted = I.src_sorted;
+    a.rec_sorted = I.rec_sorted;
+    a.src_phi = I.src_phi;
+    a.wav_row = I.nsrc ? I.wav + (size_t)n * I.nsrc * 2 : nullptr;
+    a.rec_phi = I.rec_phi;
+    a.seis = I.seis;
+    const int slot = (I.stride > 0 && n % I.stride == 0) ? n / I.stride : -1;
+    a.sample = (slot >= 0 && slot < I.max_samples) ? slot : -1;
+    const int4 ex = m.w >= 0 ? I.extra[m.w] : make_int4(0, 0, 0, 0);
+    float unew[G::RPL];
+    bool bad = false;
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        unew[q] = 0.f;
+        const int row = rows[q];
+        if (row < 0) continue;
+        const size_t idx = (size_t)e * G::D + row;
+        const float f = (ex.y > ex.x) ? (float)source_term(a, ex, row, G::NLN) : 0.f;
+        const float w = wi[q] + sc * (acc[q] - f);
+        const float uh = xv[0][row];
+        const float y = w - cp[q];
+        const float t = uh + y;
+        I.ucomp[idx] = (t - uh) - y;
+        uout[idx] = t;
+        I.winc[idx] = w;
+        unew[q] = t;
+        bad |= !isfinite(t);
+    }
+    if (bad) atomicOr(I.flags, 1u);
+    if (a.sample >= 0 && ex.w > ex.z) record<G, float>(a, ex, unew, rows, G::RPL);
+}
+
+template <typename G, bool PERSIST>
+__global__ __launch_bounds__(64 * kMWaves) void k_multi32s(const MInst *insts, const int2 *items, int nitems)
+{
+    static_assert(kMWaves == 4, "one wave per slot");
+    __shared__ __attribute__((aligned(16))) float xs_all[kMWaves][G::DP];
+    __shared__ float part[kMWaves - 1][G::RPL][64];
+    const int lane = threadIdx.x & 63;
+    const int s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int step = PERSIST ? gridDim.x : nitems;
+    for (int it = blockIdx.x; it < nitems; it += step) {
+        // the whole work-group takes the same item: every branch below is uniform over it
+        const int2 t = items[it];
+        const MInst I = insts[t.x];
+        if (!I.live || I.steps >= I.nsteps) continue;
+        multi_elem_slots<G>(I, t.y, lane, s, xs_all[s], part);
+        __syncthreads();  // the next item's partial sums overwrite these
+    }
+}
+
 // Launch variants: bit 0 = whole-wave rows (degrees with d <= 32 only),
-// bit 1 = one item per wave (else persistent).
-constexpr int kVarRows = 1, kVarOneShot = 2;
+// bit 1 = one item per wave / per work-group (else persistent), bit 2 = one
+// element per work-group and one slot per wave (k_multi32s; bit 0 ignored).
+constexpr int kVarRows = 1, kVarOneShot = 2, kVarSlots = 4;
 
 template <int P>
 __device__ inline void fused_item(const MInst &I, int e, int lane, float (*xs)[kMaxDP])
@@ -222,6 +324,9 @@ __global__ __launch_bounds__(64 * kMWaves) void k_multi_fused(const MInst *insts
 template <int P>
 const void *kern_p(int v)
 {
+    if (v & kVarSlots)
+        return (v & kVarOneShot) ? reinterpret_cast<const void *>(&k_multi32s<Geom<P>, false>)
+                                 : reinterpret_cast<const void *>(&k_multi32s<Geom<P>, true>);
     constexpr bool narrow = 2 * ((P + 1) * (P + 2) / 2) <= 32;
     if (narrow && (v & kVarRows))
         return (v & kVarOneShot) ? reinterpret_cast<const void *>(&k_multi32c<Geom<P, 64>, false>)
@@ -424,7 +529,8 @@ void rebuild(tsp_multi *m)
         g.count = r - q;
         g.kern = multi_kernel(P, g.variant);
         if (!g.kern) mfail(m, TSP_ERR_UNSUPPORTED, "degree %d not built", P);
-        const int want = (g.count + kMWaves - 1) / kMWaves;
+        // items per work-group: one per wave, or one in all with a slot per wave
+        const int want = (g.variant & kVarSlots) ? g.count : (g.count + kMWaves - 1) / kMWaves;
         if (g.variant & kVarOneShot) {
             g.grid = want;
         } else {
@@ -669,7 +775,9 @@ int tsp_multi_autotune(tsp_multi *m, int steps_per_trial)
         const int saved_chunk = m->chunk;
         m->chunk = 0;
         for (int p : degrees) {
-            const std::vector<int> cand = 2 * nln_of(p) <= 32 ? std::vector<int>{0, 1, 2, 3} : std::vector<int>{0, 2};
+            std::vector<int> cand = 2 * nln_of(p) <= 32 ? std::vector<int>{0, 1, 2, 3} : std::vector<int>{0, 2};
+            cand.push_back(kVarSlots);
+            cand.push_back(kVarSlots | kVarOneShot);
             float best = 0;
             int bestv = kVarOneShot;
             for (int var : cand) {
@@ -751,6 +859,20 @@ int tsp_multi_set_launch(tsp_multi *m, int mode)
 }
 
 int tsp_multi_launch(const tsp_multi *m) { return m ? m->launch : -1; }
+
+int tsp_multi_set_variant(tsp_multi *m, int degree, int variant)
+{
+    return mguard(m, [&] {
+        if (degree < 1 || degree > 8) mfail(m, TSP_ERR_ARG, "degree %d outside 1..8", degree);
+        if (variant < 0 || variant > 7) mfail(m, TSP_ERR_ARG, "unknown variant %d", variant);
+        if ((variant & kVarRows) && !(variant & kVarSlots) && 2 * nln_of(degree) > 32)
+            mfail(m, TSP_ERR_ARG, "whole-wave rows need d <= 32 (degree %d has d = %d)", degree,
+                  2 * nln_of(degree));
+        sync_progress(m);
+        m->variant_of_p[degree] = variant;
+        m->dirty = true;
+    });
+}
 
 int tsp_multi_variant(const tsp_multi *m, int degree)
 {
