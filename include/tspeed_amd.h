@@ -301,6 +301,8 @@ typedef struct {
     const int *rec_elem;               /* nrec                                      */
     const double *rec_phi;             /* nrec*nln                                  */
     int stride;                        /* receiver sampling stride (>= 1)           */
+    int basis;                         /* 0 Dubiner, 1 BoundaryAdapted (src_phi and
+                                          rec_phi in that basis); 0 when zeroed     */
 } tsp_request;
 
 int    tsp_server_create(int device, double mem_fraction, int chunk, tsp_server **out);

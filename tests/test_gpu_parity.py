@@ -884,6 +884,37 @@ def test_server_result_is_the_same_alone_and_in_the_mix(mode):
 
 
 @pytest.mark.parametrize("mode", ["columns", "instances"])
+def test_server_boundary_adapted_request(mode):
+    """A request in the BoundaryAdapted basis (tsp_request.basis = 1): the loop
+    sets the basis's tables and inverse mass on its contexts, so the result is
+    the request run directly, bit for bit (an 8-column context in columns mode,
+    each source in its own FP32 context in instances mode), and within the
+    basis's FP32 tolerance of the FP64 oracle."""
+    import copy
+    md = capi.Server.COLUMNS if mode == "columns" else capi.Server.INSTANCES
+    target = _req(3, nsteps=80, nsrc=(2, 2))
+    target.basis = capi.BOUNDARY_ADAPTED
+    target.__post_init__()
+    srv, ids, lat, _ = W.serve([target], rate=0, mode=md)
+    res, _ = srv.result(ids[0])
+    srv.close()
+    if md == capi.Server.COLUMNS:
+        _, sb = W.run_batched(target, 8)
+        np.testing.assert_array_equal(res, sb[..., :2, :].transpose(2, 0, 1, 3))
+    for k in range(2):
+        q = copy.copy(target)
+        q.src_xy, q.f0, q.t0, q.dirs = target.src_xy[k:k + 1], target.f0[k:k + 1], target.t0[k:k + 1], \
+            target.dirs[k:k + 1]
+        q.__post_init__()
+        if md == capi.Server.INSTANCES:
+            np.testing.assert_array_equal(res[k], W.run_engine(q, "fp32c")[2])
+        _, sref = oracle_run(q)
+        err = rel_l2(res[k], sref)
+        say(f"  BA request p={target.N} source {k}: vs oracle {err:.2e}")
+        assert err <= 1e-3
+
+
+@pytest.mark.parametrize("mode", ["columns", "instances"])
 def test_server_admission_respects_the_memory_budget(mode):
     """With a budget that holds one request at a time, three requests run one
     after the other (each admitted no earlier than the previous completion), and

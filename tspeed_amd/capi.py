@@ -537,7 +537,7 @@ class Request(ctypes.Structure):
     _fields_ = [("N", _i), ("ne", _i), ("verts", _vp), ("neigh", _vp), ("neighedge", _vp), ("lam", _vp),
                 ("mu", _vp), ("rho", _vp), ("penalty", _d), ("dt", _d), ("nsteps", _i), ("nsrc", _i),
                 ("src_elem", _vp), ("src_phi", _vp), ("wavelet", _vp), ("nrec", _i), ("rec_elem", _vp),
-                ("rec_phi", _vp), ("stride", _i)]
+                ("rec_phi", _vp), ("stride", _i), ("basis", _i)]
 
 
 class Server:
@@ -581,7 +581,8 @@ class Server:
                     arrs["neighedge"].ctypes.data, arrs["lam"].ctypes.data, arrs["mu"].ctypes.data,
                     arrs["rho"].ctypes.data, prob.penalty, prob.dt, prob.nsteps, len(prob.src_elem),
                     arrs["src_elem"].ctypes.data, arrs["src_phi"].ctypes.data, arrs["wavelet"].ctypes.data,
-                    len(prob.rec_elem), arrs["rec_elem"].ctypes.data, arrs["rec_phi"].ctypes.data, int(stride))
+                    len(prob.rec_elem), arrs["rec_elem"].ctypes.data, arrs["rec_phi"].ctypes.data, int(stride),
+                    int(getattr(prob, "basis", DUBINER)))
         i = _i()
         self._ck(self._L.tsp_server_submit(self._h, ctypes.byref(q), float(arrival), ctypes.byref(i)))
         self.meta[i.value] = (len(prob.src_elem), len(prob.rec_elem))

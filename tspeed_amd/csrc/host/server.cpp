@@ -38,12 +38,13 @@ double clock_s()
 
 struct Tables {
     std::vector<double> B, G, Be, Ge, w2, xy2, w1, x1;
+    std::vector<double> minv;  // base inverse mass of a non-orthonormal basis (else empty)
 };
 
 struct Request {
     int id = 0;
     double arrival = 0, admitted = -1, completed = -1;
-    int N = 0, ne = 0, nsteps = 0, nsrc = 0, nrec = 0, stride = 1;
+    int N = 0, ne = 0, nsteps = 0, nsrc = 0, nrec = 0, stride = 1, basis = 0;
     double penalty = 0, dt = 0;
     std::vector<double> verts, lambda, mu, rho, src_phi, wavelet, rec_phi;
     std::vector<int> neigh, neighedge, src_elem, rec_elem;
@@ -115,9 +116,10 @@ struct tsp_server {
 
 namespace {
 
-const Tables &tables_for(tsp_server *s, int N)
+const Tables &tables_for(tsp_server *s, int N, int basis)
 {
-    auto it = s->tables.find(N);
+    const int key = 2 * N + basis;
+    auto it = s->tables.find(key);
     if (it != s->tables.end()) return it->second;
     Tables t;
     const int nln = (N + 1) * (N + 2) / 2, nq1 = N + 1, nq2 = nq1 * nq1;
@@ -129,15 +131,27 @@ const Tables &tables_for(tsp_server *s, int N)
     t.xy2.resize(2 * nq2);
     t.w1.resize(nq1);
     t.x1.resize(nq1);
-    tsh_tables(N, t.B.data(), t.G.data(), t.Be.data(), t.Ge.data(), t.w2.data(), t.xy2.data(), t.w1.data(),
-               t.x1.data());
-    return s->tables.emplace(N, std::move(t)).first->second;
+    tsh_tables_basis(N, basis, t.B.data(), t.G.data(), t.Be.data(), t.Ge.data(), t.w2.data(), t.xy2.data(),
+                     t.w1.data(), t.x1.data());
+    if (basis != 0) {
+        t.minv.resize((size_t)nln * nln);
+        tsh_base_invmass(N, basis, t.minv.data());
+    }
+    return s->tables.emplace(key, std::move(t)).first->second;
+}
+
+// tables, and for a non-orthonormal basis its inverse mass, on a new context
+int set_space(tsp_context *c, const Request &r, const Tables &t)
+{
+    int rc = tsp_set_tables(c, r.N, t.B.data(), t.G.data(), t.Be.data(), t.Ge.data(), t.w2.data(), t.w1.data());
+    if (rc == TSP_OK && !t.minv.empty()) rc = tsp_set_mass(c, t.minv.data());
+    return rc;
 }
 
 // columns mode: one multi-column context, first step taken, steps queued by tick
 bool admit_columns(tsp_server *s, Request &r)
 {
-    const Tables &t = tables_for(s, r.N);
+    const Tables &t = tables_for(s, r.N, r.basis);
     r.ncols = columns_for(r.nsrc);
     tsp_context *c = nullptr;
     auto bail = [&](const char *what) {
@@ -148,8 +162,7 @@ bool admit_columns(tsp_server *s, Request &r)
         return false;
     };
     if (tsp_create(s->device, TSP_FP32C, r.ncols, &c) != TSP_OK) return bail("create");
-    if (tsp_set_tables(c, r.N, t.B.data(), t.G.data(), t.Be.data(), t.Ge.data(), t.w2.data(), t.w1.data()))
-        return bail("tables");
+    if (set_space(c, r, t)) return bail("tables");
     if (tsp_set_mesh(c, r.ne, r.verts.data(), r.neigh.data(), r.neighedge.data(), r.lambda.data(), r.mu.data(),
                      r.rho.data(), r.penalty))
         return bail("mesh");
@@ -202,7 +215,7 @@ void finish_columns(Request &r)
 bool admit(tsp_server *s, Request &r)
 {
     if (s->mode == TSP_SERVE_COLUMNS) return admit_columns(s, r);
-    const Tables &t = tables_for(s, r.N);
+    const Tables &t = tables_for(s, r.N, r.basis);
     const int nln = (r.N + 1) * (r.N + 2) / 2;
     std::vector<tsp_context *> made;
     auto bail = [&](tsp_context *c, const char *what) {
@@ -214,8 +227,7 @@ bool admit(tsp_server *s, Request &r)
     tsp_context *owner = nullptr;
     if (tsp_create(s->device, TSP_FP32C, 1, &owner) != TSP_OK) return bail(nullptr, "create");
     made.push_back(owner);
-    if (tsp_set_tables(owner, r.N, t.B.data(), t.G.data(), t.Be.data(), t.Ge.data(), t.w2.data(), t.w1.data()))
-        return bail(owner, "tables");
+    if (set_space(owner, r, t)) return bail(owner, "tables");
     if (tsp_set_mesh(owner, r.ne, r.verts.data(), r.neigh.data(), r.neighedge.data(), r.lambda.data(), r.mu.data(),
                      r.rho.data(), r.penalty))
         return bail(owner, "mesh");
@@ -329,7 +341,7 @@ namespace {
 bool request_ok(const tsp_request *q)
 {
     return q && q->N >= 1 && q->N <= 8 && q->ne >= 1 && q->nsteps >= 0 && q->nsrc >= 1 && q->nsrc <= 16 &&
-           q->nrec >= 0 && q->stride >= 1 && q->dt > 0;
+           q->nrec >= 0 && q->stride >= 1 && q->dt > 0 && (q->basis == 0 || q->basis == 1);
 }
 Request sizes_of(const tsp_request *q)
 {
@@ -340,6 +352,7 @@ Request sizes_of(const tsp_request *q)
     r.nsrc = q->nsrc;
     r.nrec = q->nrec;
     r.stride = q->stride;
+    r.basis = q->basis;
     return r;
 }
 }  // namespace
