@@ -12,8 +12,8 @@
 // reference's four copies per step (update_variables, TimeAdvance.hpp:230)
 // become a pointer swap.
 //
-// Work mapping: FP64, one element per work-group and one slot (block) per wave;
-// FP32c, one element per wave walking its four slots.
+// Work mapping: FP64, one element per work-group and one or two waves per slot
+// (block); FP32c, one element per wave walking its four slots.
 // SpMV lane mapping: lane = block row; the wave walks the block's column
 // groups (pairs of FP64 columns, quads of FP32 columns -- see engine.hpp), each
 // lane fetching its row's entries of one group with a single 16-byte
@@ -37,22 +37,32 @@
 namespace tspeed_dev {
 
 // ---------------------------------------------------------------- FP64 plain
-// One element per work-group, one slot per wave: wave s streams block (e, s)
-// against the coefficients of slot s's element (gathered into its own LDS
-// slice), waves 1..3 leave their partial sums in LDS, and wave 0 (slot 0 = the
-// element itself) adds them in slot order and finishes the update.  Four short
-// streams per element instead of one long one keep more loads in flight: on
-// the 160 000-triangle p = 6 mesh 2.41 ms per step against 2.48 with one wave
-// walking all four slots, and 15.8 us against 17.2 on 3200 triangles at p = 4.
+// One element per work-group; each of the four slots (blocks) goes to its own
+// wave, or to two waves (one per half of the block's columns) when the block
+// is one column group wide (d > 32, p >= 5).  A wave gathers the coefficients
+// of its slot's element into its LDS slice and streams its part of the block;
+// the other waves leave their partial sums in LDS and wave 0 (slot 0, first
+// half: the element itself) adds them in wave order and finishes the update.
+// Short streams per wave keep more loads in flight than one wave walking all
+// four blocks: on the 160 000-triangle p = 6 mesh 2.38 ms per step (85% of HBM)
+// against 2.42 with one wave per slot and 2.48 with one wave per element; on
+// 3200 triangles at p = 4, 14.4 us against 17.2.
+template <int P>
+struct StepShape {
+    static constexpr int H = Geom<P>::NG == 1 ? 2 : 1;  // waves per slot
+    static constexpr int WAVES = 4 * H;
+};
+
 template <int P, bool FIRST>
-__global__ __launch_bounds__(kThreads) void k_step64(StepArgs a)
+__global__ __launch_bounds__(64 * StepShape<P>::WAVES) void k_step64(StepArgs a)
 {
     using G = Geom<P>;
-    static_assert(kWaves == 4, "one wave per slot");
-    __shared__ __attribute__((aligned(16))) double xs_all[kWaves][G::DP];
-    __shared__ double part[kWaves - 1][G::RPL][64];
+    constexpr int H = StepShape<P>::H, NW = StepShape<P>::WAVES;
+    __shared__ __attribute__((aligned(16))) double xs_all[NW][G::DP];
+    __shared__ double part[NW - 1][G::RPL][64];
     const int lane = threadIdx.x & 63;
-    const int s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // this wave's slot
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int s = w / H, h = w % H;  // slot, half of the block's columns
     const int e = xcd_remap(blockIdx.x, gridDim.x);
     const bool act = e < a.ne;
     const int4 m = act ? a.meta[e] : make_int4(-1, -1, -1, -1);
@@ -64,31 +74,32 @@ __global__ __launch_bounds__(kThreads) void k_step64(StepArgs a)
 #pragma unroll
     for (int q = 0; q < G::RPL; ++q) {
         const int row = r + 64 * q;
-        rows[q] = (act && s == 0 && g == 0 && row < G::D) ? row : -1;
+        rows[q] = (act && w == 0 && g == 0 && row < G::D) ? row : -1;
         up[q] = rows[q] >= 0 ? a.uprev[(size_t)e * G::D + row] : 0.0;
     }
-    const double invm = (act && s == 0) ? a.invm[e] : 0.0;
-    double(*xs)[G::DP] = &xs_all[s];
+    const double invm = (act && w == 0) ? a.invm[e] : 0.0;
+    double(*xs)[G::DP] = &xs_all[w];
     gather_one<G, double>(xs[0], a.ucur, nb, lane);
     // the wave reads only its own LDS slice: order its stores before its loads
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     double acc[G::RPL];
-    if (act) {
+    if constexpr (H == 1) {
         spmv<G, double, 2>(a.blk64, e, xs, lane, acc, s, 1);
     } else {
-#pragma unroll
-        for (int q = 0; q < G::RPL; ++q) acc[q] = 0.0;
+        constexpr int NIT = G::D / 2, HALF = NIT / 2;
+        if (h == 0) spmv_cols<G, double, 2, 0, HALF>(a.blk64, e, s, xs[0], lane, acc);
+        else        spmv_cols<G, double, 2, HALF, NIT>(a.blk64, e, s, xs[0], lane, acc);
     }
-    if (s > 0) {
+    if (w > 0) {
 #pragma unroll
-        for (int q = 0; q < G::RPL; ++q) part[s - 1][q][lane] = acc[q];
+        for (int q = 0; q < G::RPL; ++q) part[w - 1][q][lane] = acc[q];
     }
     __syncthreads();
-    if (!act || s > 0) return;
+    if (!act || w > 0) return;
 #pragma unroll
-    for (int k = 0; k < kWaves - 1; ++k)
+    for (int k = 0; k < NW - 1; ++k)
 #pragma unroll
         for (int q = 0; q < G::RPL; ++q) acc[q] += part[k][q][lane];
 
@@ -101,13 +112,13 @@ __global__ __launch_bounds__(kThreads) void k_step64(StepArgs a)
         const int row = rows[q];
         if (row < 0) continue;
         const double f = (ex.y > ex.x) ? source_term(a, ex, row, G::NLN) : 0.0;
-        const double w = a.coef * (acc[q] - f);
+        const double wv = a.coef * (acc[q] - f);
         double un;
         if (FIRST) {
             // u1 = u0 + dt*v0 - invM*(dt*dt/2*(B*u0 - f(0)))  (TimeAdvance.cpp:30)
-            un = xs[0][row] + a.dt * up[q] - invm * w;
+            un = xs[0][row] + a.dt * up[q] - invm * wv;
         } else {
-            un = 2 * xs[0][row] - up[q] + invm * w;
+            un = 2 * xs[0][row] - up[q] + invm * wv;
         }
         a.uprev[(size_t)e * G::D + row] = un;
         unew[q] = un;
@@ -296,8 +307,9 @@ template <int P>
 int step64_p(bool first, const StepArgs &a, hipStream_t s)
 {
     // one element per work-group
-    if (first) hipLaunchKernelGGL((k_step64<P, true>), dim3((unsigned)a.ne), dim3(kThreads), 0, s, a);
-    else       hipLaunchKernelGGL((k_step64<P, false>), dim3((unsigned)a.ne), dim3(kThreads), 0, s, a);
+    const dim3 block(64 * StepShape<P>::WAVES);
+    if (first) hipLaunchKernelGGL((k_step64<P, true>), dim3((unsigned)a.ne), block, 0, s, a);
+    else       hipLaunchKernelGGL((k_step64<P, false>), dim3((unsigned)a.ne), block, 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 template <int P>

@@ -170,5 +170,29 @@ __device__ inline void spmv(const T *blk, int e, const T (*xs)[G::DP], int lane,
     }
 }
 
+
+// Columns [it0, it1) of one slot's block (one column group, no tail pair): the
+// FP64 step gives each slot to two waves, one per half of the columns.
+template <typename G, typename T, int W, int it0, int it1>
+__device__ inline void spmv_cols(const T *blk, int e, int slot, const T *x, int lane, T (&acc)[G::RPL])
+{
+    static_assert(G::NG == 1 && G::D % W == 0, "one column group, no tail");
+    const int r = lane % G::RW;
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) acc[q] = T(0);
+    const T *Bs = blk + ((size_t)e * 4 + slot) * G::D * G::D;
+#pragma unroll
+    for (int jg = it0; jg < it1; ++jg) {
+        const vec_t<T, W> xv = *reinterpret_cast<const vec_t<T, W> *>(&x[W * jg]);
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            const int row = min(r + 64 * q, G::D - 1);
+            const vec_t<T, W> b = __builtin_nontemporal_load(
+                reinterpret_cast<const vec_t<T, W> *>(Bs + ((size_t)jg * G::D + row) * W));
+#pragma unroll
+            for (int t = 0; t < W; ++t) acc[q] = fma(b[t], xv[t], acc[q]);
+        }
+    }
+}
 }  // namespace
 }  // namespace tspeed_dev
