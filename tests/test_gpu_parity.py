@@ -946,3 +946,89 @@ def ctypes_mem():
     f, t = ct.c_longlong(), ct.c_longlong()
     assert capi.lib().tsp_device_memory(0, ct.byref(f), ct.byref(t)) == 0
     return f.value, t.value
+
+
+# ------------------------------------------------------------------ the reference's Wedge example
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _wedge_initial_fields(dt, x0=1480.0):
+    """The Rayleigh-wave initial displacement and velocity of Examples/src/Wedge.cpp:
+    wedge_init_param there receives (rho, cf) = (10, 2200) in those argument
+    slots, which makes the pulse ~0.1 m wide on elements tens of metres across
+    (its projection is numerically zero); the evident rho = 2200, cf = 10 Hz
+    gives a ~20 m surface pulse, used here.  The example starts it at
+    x0 = 1000 m and needs ~5000 steps to reach its first receiver at 1500 m; a
+    short test starts it at 1480 m so that the receivers record it."""
+    lam, mu, rho, cf, csurf = 1.76e10, 8.8e9, 2200.0, 10.0, 1865.05
+    ymax = 2000.0
+    beta = np.pi ** 2 * (cf * 3) ** 2
+    c2 = np.sqrt(mu / rho)
+    m = mu / (lam + 2 * mu)
+    k = cf * 3 / csurf
+    q = k * np.sqrt(1 - m * (csurf / c2) ** 2)
+    s = k * np.sqrt(1 - (csurf / c2) ** 2)
+
+    def depth_x(y):
+        return k * (np.exp(-q * (ymax - y)) - 2 * q * s / (s * s + k * k) * np.exp(-s * (ymax - y)))
+
+    def depth_y(y):
+        return q * (np.exp(-q * (ymax - y)) - 2 * k * k / (s * s + k * k) * np.exp(-s * (ymax - y)))
+
+    def u0(x, y):
+        a = (0 - (x - x0) / csurf)
+        g = (1 - beta * a ** 2) * np.exp(-beta * a ** 2)
+        return depth_x(y) * g, depth_y(y) * g
+
+    def v0(x, y):
+        a = dt + (x - x0) / csurf
+        g = 2 * (beta * a ** 2 - 2) * beta * a * np.exp(-beta * a ** 2)
+        return depth_x(y) * g, depth_y(y) * g
+
+    return u0, v0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_reference_wedge_example(precision):
+    """The reference's own Wedge example input (Examples/Meshes/wedge.msh and
+    Examples/Receivers/wedge.rcv, kept as fixtures in tests/golden): an
+    unstructured 2360-triangle mesh with two materials meeting at an internal
+    interface, p = 5, dt = 5e-5, no force, a Rayleigh pulse as initial u and v
+    (L2-projected as FESpace::transform does) started next to the first
+    receiver, 100 receivers on the surface sampled every second step
+    (Wedge.cpp:93-146).  FP64 matches the oracle to 1e-12 in the final
+    displacement and every seismogram; FP32c to 1e-4."""
+    N, dt, nsteps, stride = 5, 5e-5, 300, 2
+    pts, tri, reg, _ = capi.read_msh(os.path.join(GOLDEN, "wedge.msh"))
+    nb, ng = capi.neighbors(tri)
+    assert sorted(set(reg.tolist())) == [1, 2]
+    lam = np.where(reg == 1, 6.25e9, 1.76e10)
+    mu = np.where(reg == 1, 3.125e9, 8.8e9)
+    rho = np.where(reg == 1, 2000.0, 2200.0)
+    verts = pts[tri].reshape(-1, 6)
+    rxy = np.loadtxt(os.path.join(GOLDEN, "wedge.rcv")).reshape(-1, 2)
+    rel, _, rphi = capi.locate(pts, tri, rxy[:, 0], rxy[:, 1], N=N)
+    assert (rel >= 0).all() and rxy.shape[0] == 100
+    fu, fv = _wedge_initial_fields(dt)
+    u0, v0 = capi.project(N, verts, fu), capi.project(N, verts, fv)
+    assert np.abs(u0).max() > 0 and np.abs(v0).max() > 0
+    op = O.Operator(N, verts, nb, ng, lam, mu, rho, 2.0)
+    assert dt < 2 / np.sqrt(op.lambda_max(300))          # the example's step is stable
+    uref, sref = op.run(dt, nsteps, rec_elem=rel, rec_phi=rphi, stride=stride, u0=u0, v0=v0)
+    e = capi.Engine(precision)
+    e.set_tables(N)
+    e.set_mesh(verts, nb, ng, lam, mu, rho, 2.0)
+    e.set_receivers(rel, rphi, stride, 1 + nsteps // stride)
+    e.set_dt(dt)
+    e.set_state(u0, v0)
+    e.first_step()
+    e.run(nsteps)
+    u, s = e.get_u(), e.seismograms()
+    assert e.status() == 0
+    e.close()
+    assert np.abs(sref).max() > 1e-3 * np.abs(uref).max()   # the receivers record the pulse
+    tol = 1e-12 if precision == "fp64" else 1e-4
+    eu, es = rel_l2(u, uref), seis_err(s, sref)
+    say(f"  wedge {precision}: u {eu:.2e}, seismograms {es[0]:.2e} (worst receiver {es[1]:.2e})")
+    assert s.shape == sref.shape
+    assert eu <= tol and es[0] <= tol and es[1] <= tol
