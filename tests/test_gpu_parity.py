@@ -1032,3 +1032,59 @@ def test_reference_wedge_example(precision):
     say(f"  wedge {precision}: u {eu:.2e}, seismograms {es[0]:.2e} (worst receiver {es[1]:.2e})")
     assert s.shape == sref.shape
     assert eu <= tol and es[0] <= tol and es[1] <= tol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_reference_lamb_example(precision):
+    """The reference's own Lamb example input (Examples/Meshes/
+    Lamb_fullyunstruct3.msh and Examples/Receivers/lamb.rcv, fixtures in
+    tests/golden) with its set-up (Lamb.cpp): the BoundaryAdapted basis at p = 5
+    (FESpace<N, Dunavant<2N>, BoundaryAdapted<N>>), one material, dt = 5e-5 and
+    a vertical Ricker force (f0 = 10 Hz, t0 = 0.1 s, sign as Lamb.cpp:37) at
+    (1500, 1950).  Its two receivers are 700-1200 m away, beyond a short run,
+    so a third receiver right above the source is added; 1500 steps, samples
+    every second step.  FP64 matches the oracle's BoundaryAdapted run to
+    5e-12: the device applies the operator premultiplied by the inverse mass,
+    (M^-1 B) u, where the reference computes M^-1 (B u), and with this basis's
+    ill-conditioned mass matrix that reassociation drifts by ~1e-13 per step
+    (measured 2.3e-12 after 1500 steps; the orthonormal Dubiner path stays at
+    1e-13 over 2000).  FP32c to 1e-3 (the same conditioning in FP32)."""
+    N, dt, nsteps, stride = 5, 5e-5, 1500, 2
+    BA = capi.BOUNDARY_ADAPTED
+    pts, tri, reg, _ = capi.read_msh(os.path.join(GOLDEN, "lamb_fullyunstruct3.msh"))
+    assert set(reg.tolist()) == {6}
+    nb, ng = capi.neighbors(tri)
+    ne = tri.shape[0]
+    lam, mu, rho = np.full(ne, 6.8270e9), np.full(ne, 6.8265125e9), np.full(ne, 2000.0)
+    verts = pts[tri].reshape(-1, 6)
+    rxy = np.vstack([np.loadtxt(os.path.join(GOLDEN, "lamb.rcv")).reshape(-1, 2), [[1500.0, 2000.0]]])
+    rel, rxi, _ = capi.locate(pts, tri, rxy[:, 0], rxy[:, 1])
+    assert (rel >= 0).all()
+    rphi = capi.basis_eval(N, BA, rxi)
+    sel, sxi, _ = capi.locate(pts, tri, 1500.0, 1950.0)
+    sphi = capi.basis_eval(N, BA, sxi)[0]
+    f0, t0 = 10.0, 0.1
+    op = O.Operator(N, verts, nb, ng, lam, mu, rho, 2.0, basis=O.BOUNDARY_ADAPTED)
+    assert dt < 2 / np.sqrt(op.lambda_max(300))
+    uref, sref = op.run(dt, nsteps, src=dict(elem=int(sel[0]), phi=sphi, f0=f0, t0=t0, dirx=0.0, diry=-1.0),
+                        rec_elem=rel, rec_phi=rphi, stride=stride)
+    assert np.abs(sref[:, 2]).max() > 1e-3 * np.abs(uref).max()   # the added receiver records the source
+    e = capi.Engine(precision)
+    e.set_tables(N, capi.tables(N, BA))
+    e.set_mass(capi.base_invmass(N, BA))
+    e.set_mesh(verts, nb, ng, lam, mu, rho, 2.0)
+    wav = capi.ricker_table(f0, t0, 0.0, -1.0, dt, nsteps + 1)
+    e.set_sources(sel, sphi[None, :], wav)
+    e.set_receivers(rel, rphi, stride, 1 + nsteps // stride)
+    e.set_dt(dt)
+    e.set_state()
+    e.first_step()
+    e.run(nsteps)
+    u, s = e.get_u(), e.seismograms()
+    assert e.status() == 0
+    e.close()
+    tol = 5e-12 if precision == "fp64" else 1e-3
+    eu, es = rel_l2(u, uref), seis_err(s, sref)
+    say(f"  lamb {precision}: u {eu:.2e}, seismograms {es[0]:.2e} (worst receiver {es[1]:.2e})")
+    assert s.shape == sref.shape
+    assert eu <= tol and es[0] <= tol and es[1] <= tol
