@@ -82,11 +82,12 @@ int tsp_set_tables(tsp_context *ctx, int N, const double *B, const double *G, co
 /* Non-orthonormal bases (FESpace<N,Q,BoundaryAdapted<N>>, Matrices_imp.hpp:183-190:
  * invM = 1/rho * 1/detJ * base_invmass).  base_invmass is nln x nln row-major;
  * NULL restores the orthonormal case (identity, the default).  Call after
- * tsp_set_tables and before tsp_set_mesh / tsp_upload_blocks: the engine then
- * stores the operator rows premultiplied, (I2 (x) base_invmass) B, keeps the
- * per-element scalar 1/(rho detJ), and applies base_invmass to every source's
- * basis values in tsp_set_sources; tsp_get_blocks returns the premultiplied
- * blocks. */
+ * tsp_set_tables and before tsp_set_mesh / tsp_upload_blocks.  A TSP_FP64
+ * context keeps B and applies invM after the block product in the reference's
+ * order, M^-1 (B u - f); TSP_FP32C and multi-column contexts store the operator
+ * rows premultiplied, (I2 (x) base_invmass) B, keep the per-element scalar
+ * 1/(rho detJ) and apply base_invmass to every source's basis values in
+ * tsp_set_sources.  tsp_get_blocks returns the blocks as stored. */
 int tsp_set_mass(tsp_context *ctx, const double *base_invmass);
 
 /* Mesh and materials; assembles B = penalty*N^2*S + I + A and the inverse mass

@@ -126,24 +126,37 @@ def test_boundary_adapted_on_the_batched_and_multi_paths(N):
 
 @pytest.mark.parametrize("N", range(1, 9))
 def test_boundary_adapted_operator_and_fp64_trajectory(N):
-    """Non-orthonormal basis (dense inverse mass, Matrices_imp.hpp:183-190): the
-    device stores (I2 (x) Mb^-1) B with B assembled from the BoundaryAdapted
-    tables (within 1e-13 of the oracle's B premultiplied), and the FP64 run
-    matches the oracle's BoundaryAdapted run to <= 1e-12 in u and seismograms."""
+    """Non-orthonormal basis (dense inverse mass, Matrices_imp.hpp:183-190): an
+    FP64 context keeps B assembled from the BoundaryAdapted tables (equal to the
+    oracle's B to 1e-13) and applies M^-1 = invm_e * Mb^-1 after the product in
+    the reference's order, so its run matches the oracle's to <= 1e-12 in u and
+    seismograms and its CFL estimate is lambda_max(M^-1 B); an FP32C context
+    folds the inverse mass into the operator, (I2 (x) Mb^-1) B."""
     p = _ba_problem(N)
     eng, u, s = W.run_engine(p, "fp64")
     blk, invm = eng.get_blocks()
     op = oracle_op(p)
-    K = np.kron(np.eye(2), O.base_invmass(N, O.BOUNDARY_ADAPTED))
-    ref = np.einsum("ij,esjk->esik", K, op.blocks())
+    ref = op.blocks()
     err = np.abs(blk - ref).max() / np.abs(ref).max()
     np.testing.assert_array_equal(invm, op.invm())
     uref, sref = oracle_run(p)
     e_u, (e_s, e_r) = rel_l2(u, uref), seis_err(s, sref)
-    say(f"  BA p={N}: blocks {err:.2e}; u {e_u:.2e}; seis {e_s:.2e} (worst receiver {e_r:.2e})")
+    _, lm = eng.cfl(1.0, 400)
+    e_l = abs(lm / op.lambda_max(400) - 1)
+    say(f"  BA p={N}: blocks {err:.2e}; u {e_u:.2e}; seis {e_s:.2e} (worst receiver {e_r:.2e}); "
+        f"lambda_max {e_l:.1e}")
     assert err <= 1e-13
     assert e_u <= 1e-12 and e_s <= 1e-12 and e_r <= 1e-12
+    assert e_l <= 1e-6
     eng.close()
+    f = capi.Engine("fp32c")
+    W.set_space(f, p)
+    f.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+    blk32, _ = f.get_blocks()
+    K = np.kron(np.eye(2), O.base_invmass(N, O.BOUNDARY_ADAPTED))
+    folded = np.einsum("ij,esjk->esik", K, ref)
+    assert np.abs(blk32 - folded).max() / np.abs(folded).max() <= 1e-13
+    f.close()
 
 
 @pytest.mark.parametrize("N", [3, 5])
@@ -459,11 +472,9 @@ def test_cpp_facade_driver_matches_oracle(tmp_path, basis):
     assert s.shape == sref.shape
     e_s, e_r = seis_err(s, sref)
     say(f"  facade driver: seis rel L2 {e_s:.3e} worst receiver {e_r:.3e}")
-    # BoundaryAdapted: the coefficients of the same field are ~25x larger with
-    # cancelling signs (|u| 4.2e-14 against 1.7e-15 here), so summation-order
-    # roundoff shows ~100x larger in the seismograms (measured 4.6e-12)
-    tol = 1e-12 if basis == 0 else 2e-11
-    assert e_s <= tol and e_r <= tol
+    # both bases apply the inverse mass in the reference's order (BoundaryAdapted:
+    # after the block product, dense per element), so one bar holds for both
+    assert e_s <= 1e-12 and e_r <= 1e-12
 
 
 # ------------------------------------------------------------------ many columns (MFMA)
@@ -1043,12 +1054,10 @@ def test_reference_lamb_example(precision):
     a vertical Ricker force (f0 = 10 Hz, t0 = 0.1 s, sign as Lamb.cpp:37) at
     (1500, 1950).  Its two receivers are 700-1200 m away, beyond a short run,
     so a third receiver right above the source is added; 1500 steps, samples
-    every second step.  FP64 matches the oracle's BoundaryAdapted run to
-    5e-12: the device applies the operator premultiplied by the inverse mass,
-    (M^-1 B) u, where the reference computes M^-1 (B u), and with this basis's
-    ill-conditioned mass matrix that reassociation drifts by ~1e-13 per step
-    (measured 2.3e-12 after 1500 steps; the orthonormal Dubiner path stays at
-    1e-13 over 2000).  FP32c to 1e-3 (the same conditioning in FP32)."""
+    every second step.  FP64 (inverse mass applied after the block product, as
+    the reference does) matches the oracle's BoundaryAdapted run to 1e-12;
+    FP32c, which folds the inverse mass into the operator, to 1e-3 (the basis's
+    mass matrix is ill-conditioned in FP32)."""
     N, dt, nsteps, stride = 5, 5e-5, 1500, 2
     BA = capi.BOUNDARY_ADAPTED
     pts, tri, reg, _ = capi.read_msh(os.path.join(GOLDEN, "lamb_fullyunstruct3.msh"))
@@ -1083,7 +1092,7 @@ def test_reference_lamb_example(precision):
     u, s = e.get_u(), e.seismograms()
     assert e.status() == 0
     e.close()
-    tol = 5e-12 if precision == "fp64" else 1e-3
+    tol = 1e-12 if precision == "fp64" else 1e-3
     eu, es = rel_l2(u, uref), seis_err(s, sref)
     say(f"  lamb {precision}: u {eu:.2e}, seismograms {es[0]:.2e} (worst receiver {es[1]:.2e})")
     assert s.shape == sref.shape

@@ -69,6 +69,10 @@ struct tsp_context {
     std::vector<double> h_minv;
     double *d_minv = nullptr;
     bool batched() const { return ncols > 1; }
+    // non-orthonormal basis: FP64 contexts apply the dense inverse mass after
+    // the block product (the reference's order); FP32C and multi-column
+    // contexts fold it into the operator rows and the sources
+    bool minv_after() const { return precision == TSP_FP64 && !h_minv.empty(); }
     size_t state_len() const { return batched() ? (size_t)ne * KP * ncols : (size_t)ne * D; }
 
     unsigned *flags = nullptr;
@@ -372,6 +376,7 @@ StepArgs base_args(tsp_context *c)
     a.flags = c->flags;
     a.blk64 = c->blk64;
     a.invm = c->invm;
+    a.minv = c->minv_after() ? c->d_minv : nullptr;
     a.blk32 = c->blk32;
     return a;
 }
@@ -619,7 +624,8 @@ int tsp_set_mesh(tsp_context *c, int ne, const double *verts, const int *neigh, 
         a.blk = c->blk64;
         a.invm = c->invm;
         kcheck(c, launch_assemble(c->P, a, c->stream), "assemble");
-        if (c->d_minv) kcheck(c, launch_premul_invmass(c->P, ne, c->d_minv, c->blk64, c->stream), "invmass");
+        if (c->d_minv && !c->minv_after())
+            kcheck(c, launch_premul_invmass(c->P, ne, c->d_minv, c->blk64, c->stream), "invmass");
         hipcheck(c, hipStreamSynchronize(c->stream), "assemble");
         dfree(c, d_verts); dfree(c, d_nb); dfree(c, d_ng); dfree(c, d_mat); dfree(c, d_geo); dfree(c, d_face);
         operator_ready(c);
@@ -641,7 +647,8 @@ int tsp_upload_blocks(tsp_context *c, int ne, const int *neigh, const double *bl
         int *d_nb = dalloc<int>(c, (size_t)ne * 3, "neigh");
         h2d(c, d_nb, neigh, (size_t)ne * 3, "neigh");
         kcheck(c, launch_zero_missing(c->P, ne, d_nb, c->blk64, c->stream), "zero blocks");
-        if (c->d_minv) kcheck(c, launch_premul_invmass(c->P, ne, c->d_minv, c->blk64, c->stream), "invmass");
+        if (c->d_minv && !c->minv_after())
+            kcheck(c, launch_premul_invmass(c->P, ne, c->d_minv, c->blk64, c->stream), "invmass");
         h2d(c, c->invm, invm, ne, "invm");
         hipcheck(c, hipStreamSynchronize(c->stream), "upload");
         dfree(c, canon);
@@ -714,7 +721,7 @@ int tsp_set_sources(tsp_context *c, int nsrc, const int *col, const int *elem, c
         c->src_phi = dalloc<double>(c, (size_t)nsrc * c->nln, "source phi");
         c->wav = dalloc<double>(c, (size_t)c->ntab * nsrc * 2, "wavelet");
         std::vector<double> mphi;
-        if (nsrc && !c->h_minv.empty()) {
+        if (nsrc && !c->h_minv.empty() && !c->minv_after()) {
             // M^-1 f for f = F(t) phi: the folded operator needs the source in the same rows
             mphi.assign((size_t)nsrc * c->nln, 0.0);
             for (int k = 0; k < nsrc; ++k)
@@ -1046,7 +1053,8 @@ int tsp_cfl(tsp_context *c, double factor, int iters, double *dt_out, double *lm
         };
         double lam = 0, nx = norm(x);
         for (int it = 0; it < iters; ++it) {
-            kcheck(c, launch_apply_minvB(c->P, c->ne, c->meta, c->blk64, c->invm, x, y, c->stream), "apply");
+            kcheck(c, launch_apply_minvB(c->P, c->ne, c->meta, c->blk64, c->invm,
+                                         c->minv_after() ? c->d_minv : nullptr, x, y, c->stream), "apply");
             const double ny = norm(y);
             lam = ny / nx;
             std::swap(x, y);
@@ -1101,6 +1109,7 @@ int tsp_share_operator(tsp_context *owner, tsp_context **out)
         c->invm = owner->invm;
         c->KP = owner->KP;
         c->h_minv = owner->h_minv;   // sources of a borrower get the same M^-1
+        c->d_minv = owner->d_minv;   // borrowed, as the operator
         c->meta = dalloc<int4>(c, c->ne, "meta");
         if (c->precision != TSP_FP64) {
             c->sc_first = dalloc<float>(c, c->ne, "sc");

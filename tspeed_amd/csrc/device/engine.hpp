@@ -101,6 +101,9 @@ struct StepArgs {
     double dt;                // first step only
     const double *ucur;       // u^n (first step: u0)
     double *uprev;            // u^{n-1} in, u^{n+1} out (first step: v0 in, u1 out)
+    const double *minv;       // non-orthonormal basis: base inverse mass (nln x nln),
+                              // applied after the block product as the reference does
+                              // (M^-1 (B u - f), M^-1 = invm_e * minv); null otherwise
     // FP32 compensated
     const float *blk32;
     const float *sc32;        // per element (float)(invm * coef)
@@ -122,7 +125,7 @@ int launch_sample32(int P, const StepArgs &a, const float *hi, const float *comp
 
 // Power iteration pieces for the step-size check: y = invM * B * x (FP64).
 int launch_apply_minvB(int P, int ne, const int4 *meta, const double *blk, const double *invm,
-                       const double *x, double *y, hipStream_t s);
+                       const double *minv, const double *x, double *y, hipStream_t s);
 int launch_dot(size_t n, const double *x, const double *y, double *partial, int nparts, hipStream_t s);
 
 // Many right-hand sides (spmm.hip): R = ncols columns, state in row octets

@@ -60,6 +60,7 @@ __global__ __launch_bounds__(64 * StepShape<P>::WAVES) void k_step64(StepArgs a)
     constexpr int H = StepShape<P>::H, NW = StepShape<P>::WAVES;
     __shared__ __attribute__((aligned(16))) double xs_all[NW][G::DP];
     __shared__ double part[NW - 1][G::RPL][64];
+    __shared__ double wrow[G::DP];   // dense inverse mass: the element's w rows
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int s = w / H, h = w % H;  // slot, half of the block's columns
@@ -104,6 +105,29 @@ __global__ __launch_bounds__(64 * StepShape<P>::WAVES) void k_step64(StepArgs a)
         for (int q = 0; q < G::RPL; ++q) acc[q] += part[k][q][lane];
 
     const int4 ex = m.w >= 0 ? a.extra[m.w] : make_int4(0, 0, 0, 0);
+    double wv[G::RPL];
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const int row = rows[q];
+        const double f = (row >= 0 && ex.y > ex.x) ? source_term(a, ex, row, G::NLN) : 0.0;
+        wv[q] = a.coef * (acc[q] - f);
+    }
+    double iw[G::RPL];
+    if (a.minv) {
+        // non-orthonormal basis: iw = M^-1 w with the element's dense inverse mass
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q)
+            if (rows[q] >= 0) wrow[rows[q]] = wv[q];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q)
+            iw[q] = rows[q] >= 0 ? apply_minv_row<G>(a.minv, invm, wrow, rows[q]) : 0.0;
+    } else {
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) iw[q] = invm * wv[q];
+    }
     double unew[G::RPL];
     bool bad = false;
 #pragma unroll
@@ -111,14 +135,12 @@ __global__ __launch_bounds__(64 * StepShape<P>::WAVES) void k_step64(StepArgs a)
         unew[q] = 0.0;
         const int row = rows[q];
         if (row < 0) continue;
-        const double f = (ex.y > ex.x) ? source_term(a, ex, row, G::NLN) : 0.0;
-        const double wv = a.coef * (acc[q] - f);
         double un;
         if (FIRST) {
             // u1 = u0 + dt*v0 - invM*(dt*dt/2*(B*u0 - f(0)))  (TimeAdvance.cpp:30)
-            un = xs[0][row] + a.dt * up[q] - invm * wv;
+            un = xs[0][row] + a.dt * up[q] - iw[q];
         } else {
-            un = 2 * xs[0][row] - up[q] + invm * wv;
+            un = 2 * xs[0][row] - up[q] + iw[q];
         }
         a.uprev[(size_t)e * G::D + row] = un;
         unew[q] = un;
@@ -238,10 +260,12 @@ __global__ void k_scale_demote(size_t n, const double *src, double coef, float *
 // y = invM * B * x for the power iteration (FP64).
 template <int P>
 __global__ __launch_bounds__(kThreads) void k_apply_minvB(int ne, const int4 *meta, const double *blk,
-                                                          const double *invm, const double *x, double *y)
+                                                          const double *invm, const double *minv,
+                                                          const double *x, double *y)
 {
     using G = Geom<P>;
     __shared__ __attribute__((aligned(16))) double xs_all[kWaves][4][G::DP];
+    __shared__ double trow[kWaves][G::DP];   // dense inverse mass: the element's (B x) rows
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int e = xcd_remap(blockIdx.x, gridDim.x) * kWaves + wave;
     const bool act = e < ne;
@@ -254,10 +278,21 @@ __global__ __launch_bounds__(kThreads) void k_apply_minvB(int ne, const int4 *me
     spmv<G, double, 2>(blk, e, xs, lane, acc);
     const int g = lane / G::RW, r = lane % G::RW;
     const double iv = invm[e];
+    if (minv) {
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            const int row = r + 64 * q;
+            if (g == 0 && row < G::D) trow[wave][row] = acc[q];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
 #pragma unroll
     for (int q = 0; q < G::RPL; ++q) {
         const int row = r + 64 * q;
-        if (g == 0 && row < G::D) y[(size_t)e * G::D + row] = iv * acc[q];
+        if (g == 0 && row < G::D)
+            y[(size_t)e * G::D + row] = minv ? apply_minv_row<G>(minv, iv, trow[wave], row) : iv * acc[q];
     }
 }
 
@@ -332,10 +367,10 @@ int sample32_p(const StepArgs &a, const float *hi, const float *comp, hipStream_
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 template <int P>
-int minvb_p(int ne, const int4 *meta, const double *blk, const double *invm, const double *x, double *y,
-            hipStream_t s)
+int minvb_p(int ne, const int4 *meta, const double *blk, const double *invm, const double *minv, const double *x,
+            double *y, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_apply_minvB<P>, dim3(nwg_for(ne)), dim3(kThreads), 0, s, ne, meta, blk, invm, x, y);
+    hipLaunchKernelGGL(k_apply_minvB<P>, dim3(nwg_for(ne)), dim3(kThreads), 0, s, ne, meta, blk, invm, minv, x, y);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 }  // namespace
@@ -373,10 +408,10 @@ int launch_sample32(int P, const StepArgs &a, const float *hi, const float *comp
     TSP_DISPATCH_P(P, CALL)
 #undef CALL
 }
-int launch_apply_minvB(int P, int ne, const int4 *meta, const double *blk, const double *invm, const double *x,
-                       double *y, hipStream_t s)
+int launch_apply_minvB(int P, int ne, const int4 *meta, const double *blk, const double *invm,
+                       const double *minv, const double *x, double *y, hipStream_t s)
 {
-#define CALL(p) minvb_p<p>(ne, meta, blk, invm, x, y, s)
+#define CALL(p) minvb_p<p>(ne, meta, blk, invm, minv, x, y, s)
     TSP_DISPATCH_P(P, CALL)
 #undef CALL
 }

@@ -194,5 +194,18 @@ __device__ inline void spmv_cols(const T *blk, int e, int slot, const T *x, int 
         }
     }
 }
+
+// y = (invm_e * minv) w on one element's rows, in the reference's order: the
+// entries of the dense inverse mass are formed as invm_e * minv_ij, then each
+// row's sum runs over j from 0 (MyMatMultiDimBlockDiag::vecMult, MyMat.hpp:
+// 304-310).  w holds the element's d rows (component-major) in LDS.
+template <typename G>
+__device__ inline double apply_minv_row(const double *minv, double invm, const double *w, int row)
+{
+    const int c = row / G::NLN, i = row % G::NLN;
+    double acc = 0.0;
+    for (int j = 0; j < G::NLN; ++j) acc += (invm * minv[i * G::NLN + j]) * w[c * G::NLN + j];
+    return acc;
+}
 }  // namespace
 }  // namespace tspeed_dev
