@@ -68,17 +68,36 @@ struct BCols {
     static constexpr int CW = R >= 32 ? 32 : R;
 };
 
+// Source term of rows i0..i0+3 of column j: the point forces of the element's
+// sources that sit in column j, summed in FP64 and rounded once.
+template <int P>
+__device__ inline f32x4 b_source(const BatchArgs &a, int4 ex, int i0, int j)
+{
+    constexpr int D = BGeom<P>::D, NLN = BGeom<P>::NLN;
+    double fd[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = ex.x; k < ex.y; ++k) {
+        const int src = a.src_sorted[k];
+        if (a.src_col[src] != j) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = i0 + q;
+            if (i < D) fd[q] += a.wav_row[src * 2 + i / NLN] * a.src_phi[src * NLN + i % NLN];
+        }
+    }
+    return f32x4{(float)fd[0], (float)fd[1], (float)fd[2], (float)fd[3]};
+}
+
 // Compensated update of the rows and columns a lane holds (four rows of one
-// column per 16-byte access), the NaN/Inf flag, and the receiver dot products
-// of every column: shared by the FP32 and the BF16x3 products.
-template <int P, int NT, int R>
-__device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int half, int jj, bool colok, size_t ebase,
+// column per 16-byte access) and the NaN/Inf flag; acc holds B u^n on entry
+// and u^{n+1} on return.  fsrc(mt, nt, g, i0, j) gives the source term of the
+// four rows i0.. of column j.
+template <int P, int NT, int R, class FSrc>
+__device__ inline void b_update(const BatchArgs &a, int e, int c0, int half, int jj, bool colok, size_t ebase,
                                 f32x16 (&acc)[BGeom<P>::MT][NT], const f32x4 (&uhv)[BGeom<P>::KB][NT],
-                                const f32x4 (&wv)[BGeom<P>::KB][NT], const f32x4 (&cp)[BGeom<P>::KB][NT])
+                                const f32x4 (&wv)[BGeom<P>::KB][NT], const f32x4 (&cp)[BGeom<P>::KB][NT], FSrc fsrc)
 {
     using G = BGeom<P>;
-    constexpr int MT = G::MT, D = G::D, NLN = G::NLN;
-    const int4 ex = m.w >= 0 ? a.extra[m.w] : make_int4(0, 0, 0, 0);
+    constexpr int MT = G::MT, D = G::D;
     const float sc = a.sc32[e];
     bool bad = false;
 #pragma unroll
@@ -92,17 +111,7 @@ __device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int h
                 const int j = c0 + nt * 32 + jj;
                 const int kb = mt * 4 + g;
                 const size_t idx = ebase + (size_t)kb * R * 8 + nt * 256;
-                double fd[4] = {0.0, 0.0, 0.0, 0.0};
-                for (int k = ex.x; k < ex.y; ++k) {
-                    const int src = a.src_sorted[k];
-                    if (a.src_col[src] != j) continue;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int i = i0 + q;
-                        if (i < D) fd[q] += a.wav_row[src * 2 + i / NLN] * a.src_phi[src * NLN + i % NLN];
-                    }
-                }
-                const f32x4 f = {(float)fd[0], (float)fd[1], (float)fd[2], (float)fd[3]};
+                const f32x4 f = fsrc(mt, nt, g, i0, j);
                 const f32x4 uh = uhv[kb][nt];
                 f32x4 wn, tn, cn;
 #pragma unroll
@@ -123,33 +132,53 @@ __device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int h
                 }
             }
     if (bad) atomicOr(a.flags, 1u);
+}
 
-    // receivers: every receiver records all R columns (this wave's NT tiles)
-    if (a.sample >= 0) {
-        for (int k = ex.z; k < ex.w; ++k) {
-            const int rec = a.rec_sorted[k];
+// Receiver samples of this wave's NT column tiles for receivers ex.z + k0,
+// ex.z + k0 + kstep, ...; tval(mt, nt, r) is u^{n+1} in result-register layout.
+template <int P, int NT, int R, class TVal>
+__device__ inline void b_receivers(const BatchArgs &a, int4 ex, int k0, int kstep, int c0, int half, int jj, bool colok,
+                                   TVal tval)
+{
+    using G = BGeom<P>;
+    constexpr int MT = G::MT, D = G::D, NLN = G::NLN;
+    for (int k = ex.z + k0; k < ex.w; k += kstep) {
+        const int rec = a.rec_sorted[k];
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                float sx = 0.f, sy = 0.f;
+        for (int nt = 0; nt < NT; ++nt) {
+            float sx = 0.f, sy = 0.f;
 #pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
+            for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const int i = mt * 32 + 8 * (r / 4) + 4 * half + (r % 4);
-                        if (i >= D) continue;
-                        const float v = (float)a.rec_phi[rec * NLN + i % NLN] * acc[mt][nt][r];
-                        if (i < NLN) sx += v; else sy += v;
-                    }
-                sx += __shfl_xor(sx, 32);
-                sy += __shfl_xor(sy, 32);
-                if (half == 0 && colok) {
-                    const size_t o = (((size_t)a.sample * a.nrec + rec) * R + c0 + nt * 32 + jj) * 2;
-                    a.seis[o] = sx;
-                    a.seis[o + 1] = sy;
+                for (int r = 0; r < 16; ++r) {
+                    const int i = mt * 32 + 8 * (r / 4) + 4 * half + (r % 4);
+                    if (i >= D) continue;
+                    const float v = (float)a.rec_phi[rec * NLN + i % NLN] * tval(mt, nt, r);
+                    if (i < NLN) sx += v; else sy += v;
                 }
+            sx += __shfl_xor(sx, 32);
+            sy += __shfl_xor(sy, 32);
+            if (half == 0 && colok) {
+                const size_t o = (((size_t)a.sample * a.nrec + rec) * R + c0 + nt * 32 + jj) * 2;
+                a.seis[o] = sx;
+                a.seis[o + 1] = sy;
             }
         }
     }
+}
+
+// The update, the source term and every receiver of the element, by one wave.
+template <int P, int NT, int R>
+__device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int half, int jj, bool colok, size_t ebase,
+                                f32x16 (&acc)[BGeom<P>::MT][NT], const f32x4 (&uhv)[BGeom<P>::KB][NT],
+                                const f32x4 (&wv)[BGeom<P>::KB][NT], const f32x4 (&cp)[BGeom<P>::KB][NT])
+{
+    const int4 ex = m.w >= 0 ? a.extra[m.w] : make_int4(0, 0, 0, 0);
+    b_update<P, NT, R>(a, e, c0, half, jj, colok, ebase, acc, uhv, wv, cp,
+                       [&](int, int, int, int i0, int j) { return b_source<P>(a, ex, i0, j); });
+    if (a.sample >= 0)
+        b_receivers<P, NT, R>(a, ex, 0, 1, c0, half, jj, colok,
+                              [&](int mt, int nt, int r) { return acc[mt][nt][r]; });
 }
 
 // Per-step values of a graph-replayed step (see BatchArgs::ctr).
@@ -176,7 +205,7 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 {
     resolve_step(a);
     using G = BGeom<P>;
-    constexpr int MT = G::MT, KB = G::KB, D = G::D, NLN = G::NLN, NT = 1;
+    constexpr int MT = G::MT, KB = G::KB, NT = 1;
     constexpr int WPE = BCols<R>::WPE, CW = BCols<R>::CW;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int gw = xcd_remap_b(blockIdx.x, gridDim.x) * kBWaves + wave;
@@ -259,24 +288,29 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 
 // One slot per wave, for blocks of one 32-row tile (p <= 4) and full 32-column
 // tiles (R >= 32): a work-group of four waves takes one (element, 32-column
-// tile); wave s multiplies block (e, s) into its own accumulator, waves 1..3
-// leave theirs in LDS and wave 0 adds them in slot order and finishes.  Four
-// short chains instead of one wave walking the four slots: 39.3 against
-// 40.7 us per step on the 64-source square (p = 4).  Larger blocks keep
-// k_step_b32 (this form would run there at 1-2 waves per SIMD), and so do 8-
-// and 16-column contexts, whose partly empty tiles measured 3.5% slower this
-// way in the request loop.
+// tile).  Wave s multiplies block (e, s) into its own accumulator and leaves
+// it in LDS; then wave g updates row octet g of the tile, adding the four
+// partial sums in slot order, so no wave holds the whole tile's update
+// operands (4-7 waves per SIMD instead of 3).  Wave 1 evaluates the element's
+// source term first; elements with receivers take a second barrier and the
+// four waves share the receivers.  Larger blocks keep k_step_b32 (this form
+// would run there at 1-2 waves per SIMD), and so do 8- and 16-column
+// contexts, whose partly empty tiles measured 3.5% slower this way in the
+// request loop.
 template <int P, int R>
 __global__ __launch_bounds__(64 * kBWaves) void k_step_b32s(BatchArgs a)
 {
     resolve_step(a);
     using G = BGeom<P>;
-    constexpr int MT = G::MT, KB = G::KB, NT = 1;
+    constexpr int KB = G::KB;
     constexpr int WPE = BCols<R>::WPE, CW = BCols<R>::CW;
-    static_assert(kBWaves == 4, "one wave per slot");
-    __shared__ f32x16 part[3][MT][64];
+    static_assert(kBWaves == 4 && G::MT == 1 && KB == 4, "one wave per slot and per row octet");
+    // part[s][g][lane]: registers 4g..4g+3 of slot s's accumulator; the same
+    // layout holds the source term (fsrc) and, for the receivers, u^{n+1} (part[0])
+    __shared__ f32x4 part[4][4][64];
+    __shared__ f32x4 fsrc[4][64];
     const int lane = threadIdx.x & 63;
-    const int s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // this wave's slot
+    const int s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // this wave's slot, then its octet
     const int item = xcd_remap_b(blockIdx.x, gridDim.x);
     const int e = item / WPE, c0 = (item % WPE) * 32;
     const bool act = e < a.ne;
@@ -286,51 +320,78 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32s(BatchArgs a)
     // a missing neighbour's block is zero: its slot reads the element's own columns
     const int nb = s == 0 ? e : (s == 1 ? (m.x >= 0 ? m.x : e) : (s == 2 ? (m.y >= 0 ? m.y : e) : (m.z >= 0 ? m.z : e)));
     const int lofs = (c0 + (colok ? jj : 0)) * 8 + 4 * half;
-    f32x16 acc[MT][NT];
+    // the element's sources and receivers: the same for the whole work-group
+    const int4 ex = act && m.w >= 0 ? a.extra[m.w] : make_int4(0, 0, 0, 0);
+    const bool src_here = ex.x < ex.y, rec_here = a.sample >= 0 && ex.z < ex.w;
+    // wave 1 evaluates the source term (most elements have none) before it
+    // loads its own operands, so the FP64 sums add no register pressure
+    if (s == 1 && src_here) {
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) acc[mt][0] = f32x16{};
-    f32x4 av[KB][MT], bv[KB][NT], wv[KB][NT], cp[KB][NT];
-    const size_t ebase = (size_t)e * KB * R * 8 + lofs;
+        for (int g = 0; g < 4; ++g)
+            fsrc[g][lane] = 8 * g < G::D ? b_source<P>(a, ex, 8 * g + 4 * half, c0 + jj) : f32x4{};
+    }
+    // octet s of the tile: the rows this wave updates
+    const bool upd = act && 8 * s < G::D;
+    const size_t idx = (size_t)e * KB * R * 8 + lofs + (size_t)s * R * 8;
+    f32x4 uh{}, wv{}, cp{};
+    f32x16 acc = f32x16{};
     if (act) {
-        const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * MT * KB * 256);
+        const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * KB * 256);
         const float *X = a.uhi + (size_t)nb * KB * R * 8 + lofs;
+        f32x4 av[KB], bv[KB];
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb)
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) av[kb][mt] = Af[(mt * KB + kb) * 64 + lane];
+        for (int kb = 0; kb < KB; ++kb) av[kb] = Af[kb * 64 + lane];
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) {
             const f32x4 x = *reinterpret_cast<const f32x4 *>(X + (size_t)kb * R * 8);
-            bv[kb][0] = colok ? x : f32x4{};
+            bv[kb] = colok ? x : f32x4{};
         }
-        if (s == 0) {
-#pragma unroll
-            for (int kb = 0; kb < KB; ++kb) {
-                const size_t idx = ebase + (size_t)kb * R * 8;
-                wv[kb][0] = colok ? ld4(a.winc + idx) : f32x4{};
-                cp[kb][0] = colok ? ld4(a.ucomp + idx) : f32x4{};
-            }
+        if (upd && colok) {
+            uh = *reinterpret_cast<const f32x4 *>(a.uhi + idx);
+            wv = ld4(a.winc + idx);
+            cp = ld4(a.ucomp + idx);
         }
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kb][t], bv[kb][t], acc, 0, 0, 0);
 #pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-                    acc[mt][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kb][mt][t], bv[kb][0][t], acc[mt][0], 0, 0, 0);
-    }
-    if (s > 0) {
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) part[s - 1][mt][lane] = acc[mt][0];
+        for (int g = 0; g < 4; ++g)
+            part[s][g][lane] = f32x4{acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]};
     }
     __syncthreads();
-    if (!act || s > 0) return;
+    if (upd) {
+        // B u^n in slot order, then - f (x - 0 is x exactly), then the update
+        f32x4 y = part[0][s][lane];
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
+        for (int k = 1; k < 4; ++k) y += part[k][s][lane];
+        if (src_here) y -= fsrc[s][lane];
+        const float sc = a.sc32[e];
+        bool bad = false;
+        f32x4 wn, tn, cn;
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) acc[mt][0] += part[k][mt][lane];
-    // slot 0's columns are the element's own u^n
-    b_finish<P, NT, R>(a, e, m, c0, half, jj, colok, ebase, acc, bv, wv, cp);
+        for (int q = 0; q < 4; ++q) {
+            const float w = wv[q] + sc * y[q];
+            const float z = w - cp[q];
+            const float t = uh[q] + z;
+            cn[q] = (t - uh[q]) - z;
+            tn[q] = t;
+            wn[q] = w;
+            bad |= !isfinite(t);
+        }
+        if (colok) {
+            st4(a.ucomp + idx, cn);
+            st4(a.uhi_out + idx, tn);
+            st4(a.winc + idx, wn);
+        }
+        if (bad) atomicOr(a.flags, 1u);
+        if (rec_here) part[0][s][lane] = tn;
+    }
+    if (!rec_here) return;
+    // the four waves share the element's receivers, reading u^{n+1} from LDS
+    __syncthreads();
+    b_receivers<P, 1, R>(a, ex, s, kBWaves, c0, half, jj, colok,
+                         [&](int, int, int r) { return part[0][r / 4][lane][r % 4]; });
 }
 
 // The same step with the block products split into three BF16 products on
