@@ -288,21 +288,20 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 
 // One slot per wave, for blocks of one 32-row tile (p <= 4) and full 32-column
 // tiles (R >= 32): a work-group of four waves takes one (element, 32-column
-// tile).  Wave s multiplies block (e, s) into its own accumulator and leaves
-// it in LDS; then wave g updates row octet g of the tile, adding the four
-// partial sums in slot order, so no wave holds the whole tile's update
-// operands (4-7 waves per SIMD instead of 3).  Wave 1 evaluates the element's
-// source term first; elements with receivers take a second barrier and the
-// four waves share the receivers.  Larger blocks keep k_step_b32 (this form
-// would run there at 1-2 waves per SIMD), and so do 8- and 16-column
-// contexts, whose partly empty tiles measured 3.5% slower this way in the
-// request loop.
-template <int P, int R>
-__global__ __launch_bounds__(64 * kBWaves) void k_step_b32s(BatchArgs a)
+// tile).  Wave s multiplies block (e, s) into its own accumulator (FP32
+// products, or BF16x3 when BF) and leaves it in LDS; then wave g updates row
+// octet g of the tile, adding the four partial sums in slot order, so no wave
+// holds the whole tile's update operands (6-7 waves per SIMD at p <= 4 instead
+// of 3).  Wave 1 evaluates the element's source term first; elements with
+// receivers take a second barrier and the four waves share the receivers.
+// Larger blocks keep the one-wave-per-tile kernels (this form would run there
+// at 1-2 waves per SIMD), and so do 8- and 16-column contexts, whose partly
+// empty tiles measured 3.5% slower this way in the request loop.
+template <int P, int R, bool BF>
+__device__ inline void step_slots(BatchArgs &a)
 {
-    resolve_step(a);
     using G = BGeom<P>;
-    constexpr int KB = G::KB;
+    constexpr int KB = G::KB, KC = KB / 2;
     constexpr int WPE = BCols<R>::WPE, CW = BCols<R>::CW;
     static_assert(kBWaves == 4 && G::MT == 1 && KB == 4, "one wave per slot and per row octet");
     // part[s][g][lane]: registers 4g..4g+3 of slot s's accumulator; the same
@@ -337,24 +336,63 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32s(BatchArgs a)
     f32x16 acc = f32x16{};
     if (act) {
         const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * KB * 256);
-        const float *X = a.uhi + (size_t)nb * KB * R * 8 + lofs;
-        f32x4 av[KB], bv[KB];
+        if constexpr (!BF) {
+            const float *X = a.uhi + (size_t)nb * KB * R * 8 + lofs;
+            f32x4 av[KB], bv[KB];
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb) av[kb] = Af[kb * 64 + lane];
+            for (int kb = 0; kb < KB; ++kb) av[kb] = Af[kb * 64 + lane];
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb) {
-            const f32x4 x = *reinterpret_cast<const f32x4 *>(X + (size_t)kb * R * 8);
-            bv[kb] = colok ? x : f32x4{};
+            for (int kb = 0; kb < KB; ++kb) {
+                const f32x4 x = *reinterpret_cast<const f32x4 *>(X + (size_t)kb * R * 8);
+                bv[kb] = colok ? x : f32x4{};
+            }
+            if (upd && colok) {
+                uh = *reinterpret_cast<const f32x4 *>(a.uhi + idx);
+                wv = ld4(a.winc + idx);
+                cp = ld4(a.ucomp + idx);
+            }
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kb][t], bv[kb][t], acc, 0, 0, 0);
+        } else {
+            // a lane's B operand of K-chunk c is octet 2c + half of its column
+            const float *X = a.uhi + (size_t)nb * KB * R * 8 + (c0 + (colok ? jj : 0)) * 8;
+            f32x4 ah[KC], al[KC], xv[KC][2];
+#pragma unroll
+            for (int c = 0; c < KC; ++c) {
+                ah[c] = Af[c * 128 + lane];
+                al[c] = Af[c * 128 + 64 + lane];
+            }
+#pragma unroll
+            for (int c = 0; c < KC; ++c) {
+                const float *o = X + (size_t)(2 * c + half) * R * 8;
+                xv[c][0] = colok ? *reinterpret_cast<const f32x4 *>(o) : f32x4{};
+                xv[c][1] = colok ? *reinterpret_cast<const f32x4 *>(o + 4) : f32x4{};
+            }
+            if (upd && colok) {
+                uh = *reinterpret_cast<const f32x4 *>(a.uhi + idx);
+                wv = ld4(a.winc + idx);
+                cp = ld4(a.ucomp + idx);
+            }
+#pragma unroll
+            for (int c = 0; c < KC; ++c) {
+                bf16x8 xh, xl;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const float x = xv[c][q / 4][q % 4];
+                    const __bf16 h = (__bf16)x;
+                    xh[q] = h;
+                    xl[q] = (__bf16)(x - (float)h);
+                }
+                const bf16x8 Ah = __builtin_bit_cast(bf16x8, ah[c]);
+                const bf16x8 Al = __builtin_bit_cast(bf16x8, al[c]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, xh, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, xl, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, xh, acc, 0, 0, 0);
+            }
         }
-        if (upd && colok) {
-            uh = *reinterpret_cast<const f32x4 *>(a.uhi + idx);
-            wv = ld4(a.winc + idx);
-            cp = ld4(a.ucomp + idx);
-        }
-#pragma unroll
-        for (int kb = 0; kb < KB; ++kb)
-#pragma unroll
-            for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kb][t], bv[kb][t], acc, 0, 0, 0);
 #pragma unroll
         for (int g = 0; g < 4; ++g)
             part[s][g][lane] = f32x4{acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]};
@@ -392,6 +430,21 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32s(BatchArgs a)
     __syncthreads();
     b_receivers<P, 1, R>(a, ex, s, kBWaves, c0, half, jj, colok,
                          [&](int, int, int r) { return part[0][r / 4][lane][r % 4]; });
+}
+
+template <int P, int R>
+__global__ __launch_bounds__(64 * kBWaves) void k_step_b32s(BatchArgs a)
+{
+    resolve_step(a);
+    step_slots<P, R, false>(a);
+}
+
+// BF16x3 products (operator layout and split below, at k_step_b16x3).
+template <int P, int R>
+__global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3s(BatchArgs a)
+{
+    resolve_step(a);
+    step_slots<P, R, true>(a);
 }
 
 // The same step with the block products split into three BF16 products on
@@ -615,10 +668,14 @@ int step_b32_p(const BatchArgs &a, hipStream_t s)
 template <int P, int R>
 int step_b16x3_p(const BatchArgs &a, hipStream_t s)
 {
-    constexpr int PD = BGeom<P>::MT <= 2 ? 1 : 0;
-    const size_t waves = (size_t)a.ne * BCols<R>::WPE;
-    const unsigned nwg = (unsigned)((waves + kBWaves - 1) / kBWaves);
-    hipLaunchKernelGGL((k_step_b16x3<P, R, PD>), dim3(nwg), dim3(64 * kBWaves), 0, s, a);
+    const size_t items = (size_t)a.ne * BCols<R>::WPE;
+    if constexpr (BGeom<P>::MT == 1 && R >= 32) {
+        hipLaunchKernelGGL((k_step_b16x3s<P, R>), dim3((unsigned)items), dim3(64 * kBWaves), 0, s, a);
+    } else {
+        constexpr int PD = BGeom<P>::MT <= 2 ? 1 : 0;
+        const unsigned nwg = (unsigned)((items + kBWaves - 1) / kBWaves);
+        hipLaunchKernelGGL((k_step_b16x3<P, R, PD>), dim3(nwg), dim3(64 * kBWaves), 0, s, a);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
