@@ -195,11 +195,13 @@ __device__ inline void resolve_step(BatchArgs &a)
 
 __global__ void k_ctr_set(int *ctr, int value) { *ctr = value; }
 
-// One wave computes one element's rows for one 32-column tile (R/32 waves share
-// an element when R >= 32; below that the tile is partly empty).  A missing neighbour's block is
-// zero in the packed operator, so its slot multiplies the element's own
-// columns by zero and the slot loop needs no branch.  Padded rows stay zero:
-// their operator rows and columns are zero and they carry no source.
+// Blocks of more than one 32-row tile (p >= 5): one wave computes one
+// element's rows for one 32-column tile (R/32 waves share an element when
+// R >= 32; below that the tile is partly empty), walking the four slots.  A
+// missing neighbour's block is zero in the packed operator, so its slot
+// multiplies the element's own columns by zero and the slot loop needs no
+// branch.  Padded rows stay zero: their operator rows and columns are zero and
+// they carry no source.
 template <int P, int R, int PD>
 __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 {
@@ -286,17 +288,17 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
     b_finish<P, NT, R>(a, e, m, c0, half, jj, colok, ebase, acc, own, wv, cp);
 }
 
-// One slot per wave, for blocks of one 32-row tile (p <= 4) and full 32-column
-// tiles (R >= 32): a work-group of four waves takes one (element, 32-column
-// tile).  Wave s multiplies block (e, s) into its own accumulator (FP32
-// products, or BF16x3 when BF) and leaves it in LDS; then wave g updates row
-// octet g of the tile, adding the four partial sums in slot order, so no wave
-// holds the whole tile's update operands (6-7 waves per SIMD at p <= 4 instead
-// of 3).  Wave 1 evaluates the element's source term first; elements with
-// receivers take a second barrier and the four waves share the receivers.
-// Larger blocks keep the one-wave-per-tile kernels (this form would run there
-// at 1-2 waves per SIMD), and so do 8- and 16-column contexts, whose partly
-// empty tiles measured 3.5% slower this way in the request loop.
+// One slot per wave, for blocks of one 32-row tile (p <= 4), at every width: a
+// work-group of four waves takes one (element, 32-column tile).  Wave s
+// multiplies block (e, s) into its own accumulator (FP32 products, or BF16x3
+// when BF) and leaves it in LDS; then wave g updates row octet g of the tile,
+// adding the four partial sums in slot order, so no wave holds the whole
+// tile's update operands (6-7 waves per SIMD at p <= 4 instead of 3).  Wave 1
+// evaluates the element's source term first; elements with receivers take a
+// second barrier and the four waves share the receivers.  On the 3200-triangle
+// square (p = 4) a step takes 35.4 us at 64 columns against 39.5 in the
+// one-wave-per-tile form, 15.2 against 21.2 at 8.  Larger blocks keep the
+// one-wave-per-tile kernels (this form would run there at 1-2 waves per SIMD).
 template <int P, int R, bool BF>
 __device__ inline void step_slots(BatchArgs &a)
 {
@@ -655,7 +657,7 @@ template <int P, int R>
 int step_b32_p(const BatchArgs &a, hipStream_t s)
 {
     const size_t items = (size_t)a.ne * BCols<R>::WPE;  // (element, 32-column tile)
-    if constexpr (BGeom<P>::MT == 1 && R >= 32) {
+    if constexpr (BGeom<P>::MT == 1) {
         hipLaunchKernelGGL((k_step_b32s<P, R>), dim3((unsigned)items), dim3(64 * kBWaves), 0, s, a);
     } else {
         constexpr int PD = BGeom<P>::MT <= 2 ? 1 : 0;
@@ -669,7 +671,7 @@ template <int P, int R>
 int step_b16x3_p(const BatchArgs &a, hipStream_t s)
 {
     const size_t items = (size_t)a.ne * BCols<R>::WPE;
-    if constexpr (BGeom<P>::MT == 1 && R >= 32) {
+    if constexpr (BGeom<P>::MT == 1) {
         hipLaunchKernelGGL((k_step_b16x3s<P, R>), dim3((unsigned)items), dim3(64 * kBWaves), 0, s, a);
     } else {
         constexpr int PD = BGeom<P>::MT <= 2 ? 1 : 0;
