@@ -195,7 +195,7 @@ __device__ inline void resolve_step(BatchArgs &a)
 
 __global__ void k_ctr_set(int *ctr, int value) { *ctr = value; }
 
-// Blocks of more than one 32-row tile (p >= 5): one wave computes one
+// Blocks of three 32-row tiles (p = 7, 8): one wave computes one
 // element's rows for one 32-column tile (R/32 waves share an element when
 // R >= 32; below that the tile is partly empty), walking the four slots.  A
 // missing neighbour's block is zero in the packed operator, so its slot
@@ -288,30 +288,32 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
     b_finish<P, NT, R>(a, e, m, c0, half, jj, colok, ebase, acc, own, wv, cp);
 }
 
-// One slot per wave, for blocks of one 32-row tile (p <= 4), at every width: a
-// work-group of four waves takes one (element, 32-column tile).  Wave s
-// multiplies block (e, s) into its own accumulator (FP32 products, or BF16x3
-// when BF) and leaves it in LDS; then wave g updates row octet g of the tile,
-// adding the four partial sums in slot order, so no wave holds the whole
-// tile's update operands (6-7 waves per SIMD at p <= 4 instead of 3).  Wave 1
-// evaluates the element's source term first; elements with receivers take a
-// second barrier and the four waves share the receivers.  On the 3200-triangle
-// square (p = 4) a step takes 35.4 us at 64 columns against 39.5 in the
-// one-wave-per-tile form, 15.2 against 21.2 at 8.  Larger blocks keep the
-// one-wave-per-tile kernels (this form would run there at 1-2 waves per SIMD).
+// One slot per wave, for blocks of one or two 32-row tiles (p <= 6), at every
+// width: a work-group of four waves takes one (element, 32-column tile).  Wave
+// s multiplies block (e, s) into its own accumulators (FP32 products, or BF16x3
+// when BF) and leaves them in LDS; then wave g updates row octets g and g + 4
+// of the tile, adding the four partial sums in slot order, so no wave holds
+// the whole tile's update operands (p <= 4: 6-7 waves per SIMD instead of 3;
+// p = 5, 6: 2-3 instead of 1).  Wave 1 evaluates the element's source term
+// first; elements with receivers take a second barrier and the four waves
+// share the receivers.  On the 3200-triangle square (p = 4) a step takes 35.4
+// us at 64 columns against 39.5 in the one-wave-per-tile form, 15.2 against
+// 21.2 at 8; at p = 5, 8 columns, 45.6 against 59.6.  Blocks of three row
+// tiles (p = 7, 8) keep the one-wave-per-tile kernels.
 template <int P, int R, bool BF>
 __device__ inline void step_slots(BatchArgs &a)
 {
     using G = BGeom<P>;
-    constexpr int KB = G::KB, KC = KB / 2;
+    constexpr int MT = G::MT, KB = G::KB, KC = KB / 2, NO = 4 * MT;  // NO row octets per tile column
     constexpr int WPE = BCols<R>::WPE, CW = BCols<R>::CW;
-    static_assert(kBWaves == 4 && G::MT == 1 && KB == 4, "one wave per slot and per row octet");
-    // part[s][g][lane]: registers 4g..4g+3 of slot s's accumulator; the same
-    // layout holds the source term (fsrc) and, for the receivers, u^{n+1} (part[0])
-    __shared__ f32x4 part[4][4][64];
-    __shared__ f32x4 fsrc[4][64];
+    static_assert(kBWaves == 4 && MT <= 2, "one wave per slot, MT row octets per wave");
+    // part[s][o][lane]: registers 4g..4g+3 of slot s's accumulator of row tile
+    // o / 4 (g = o % 4), i.e. row octet o; the same layout holds the source term
+    // (fsrc) and, for the receivers, u^{n+1} (part[0])
+    __shared__ f32x4 part[4][NO][64];
+    __shared__ f32x4 fsrc[NO][64];
     const int lane = threadIdx.x & 63;
-    const int s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // this wave's slot, then its octet
+    const int s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // this wave's slot; it updates octets s + 4 mt
     const int item = xcd_remap_b(blockIdx.x, gridDim.x);
     const int e = item / WPE, c0 = (item % WPE) * 32;
     const bool act = e < a.ne;
@@ -328,56 +330,69 @@ __device__ inline void step_slots(BatchArgs &a)
     // loads its own operands, so the FP64 sums add no register pressure
     if (s == 1 && src_here) {
 #pragma unroll
-        for (int g = 0; g < 4; ++g)
-            fsrc[g][lane] = 8 * g < G::D ? b_source<P>(a, ex, 8 * g + 4 * half, c0 + jj) : f32x4{};
+        for (int o = 0; o < NO; ++o)
+            fsrc[o][lane] = 8 * o < G::D ? b_source<P>(a, ex, 8 * o + 4 * half, c0 + jj) : f32x4{};
     }
-    // octet s of the tile: the rows this wave updates
-    const bool upd = act && 8 * s < G::D;
-    const size_t idx = (size_t)e * KB * R * 8 + lofs + (size_t)s * R * 8;
-    f32x4 uh{}, wv{}, cp{};
-    f32x16 acc = f32x16{};
+    const size_t ebase = (size_t)e * KB * R * 8 + lofs;
+    f32x4 uh[MT], wv[MT], cp[MT];
+    f32x16 acc[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        uh[mt] = wv[mt] = cp[mt] = f32x4{};
+        acc[mt] = f32x16{};
+    }
+    // the rows this wave updates: octets s + 4 mt
+    auto upd = [&](int mt) { return act && 8 * (s + 4 * mt) < G::D; };
+    auto load_update_operands = [&]() {
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+            if (upd(mt) && colok) {
+                const size_t idx = ebase + (size_t)(s + 4 * mt) * R * 8;
+                uh[mt] = *reinterpret_cast<const f32x4 *>(a.uhi + idx);
+                wv[mt] = ld4(a.winc + idx);
+                cp[mt] = ld4(a.ucomp + idx);
+            }
+    };
     if (act) {
-        const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * KB * 256);
+        const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * MT * KB * 256);
         if constexpr (!BF) {
             const float *X = a.uhi + (size_t)nb * KB * R * 8 + lofs;
-            f32x4 av[KB], bv[KB];
+            f32x4 av[KB][MT], bv[KB];
 #pragma unroll
-            for (int kb = 0; kb < KB; ++kb) av[kb] = Af[kb * 64 + lane];
+            for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) av[kb][mt] = Af[(mt * KB + kb) * 64 + lane];
 #pragma unroll
             for (int kb = 0; kb < KB; ++kb) {
                 const f32x4 x = *reinterpret_cast<const f32x4 *>(X + (size_t)kb * R * 8);
                 bv[kb] = colok ? x : f32x4{};
             }
-            if (upd && colok) {
-                uh = *reinterpret_cast<const f32x4 *>(a.uhi + idx);
-                wv = ld4(a.winc + idx);
-                cp = ld4(a.ucomp + idx);
-            }
+            load_update_operands();
 #pragma unroll
             for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
-                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kb][t], bv[kb][t], acc, 0, 0, 0);
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                        acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kb][mt][t], bv[kb][t], acc[mt], 0, 0, 0);
         } else {
             // a lane's B operand of K-chunk c is octet 2c + half of its column
             const float *X = a.uhi + (size_t)nb * KB * R * 8 + (c0 + (colok ? jj : 0)) * 8;
-            f32x4 ah[KC], al[KC], xv[KC][2];
+            f32x4 ah[KC][MT], al[KC][MT], xv[KC][2];
 #pragma unroll
-            for (int c = 0; c < KC; ++c) {
-                ah[c] = Af[c * 128 + lane];
-                al[c] = Af[c * 128 + 64 + lane];
-            }
+            for (int c = 0; c < KC; ++c)
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    ah[c][mt] = Af[(mt * KC + c) * 128 + lane];
+                    al[c][mt] = Af[(mt * KC + c) * 128 + 64 + lane];
+                }
 #pragma unroll
             for (int c = 0; c < KC; ++c) {
                 const float *o = X + (size_t)(2 * c + half) * R * 8;
                 xv[c][0] = colok ? *reinterpret_cast<const f32x4 *>(o) : f32x4{};
                 xv[c][1] = colok ? *reinterpret_cast<const f32x4 *>(o + 4) : f32x4{};
             }
-            if (upd && colok) {
-                uh = *reinterpret_cast<const f32x4 *>(a.uhi + idx);
-                wv = ld4(a.winc + idx);
-                cp = ld4(a.ucomp + idx);
-            }
+            load_update_operands();
 #pragma unroll
             for (int c = 0; c < KC; ++c) {
                 bf16x8 xh, xl;
@@ -388,50 +403,60 @@ __device__ inline void step_slots(BatchArgs &a)
                     xh[q] = h;
                     xl[q] = (__bf16)(x - (float)h);
                 }
-                const bf16x8 Ah = __builtin_bit_cast(bf16x8, ah[c]);
-                const bf16x8 Al = __builtin_bit_cast(bf16x8, al[c]);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, xh, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, xl, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, xh, acc, 0, 0, 0);
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    const bf16x8 Ah = __builtin_bit_cast(bf16x8, ah[c][mt]);
+                    const bf16x8 Al = __builtin_bit_cast(bf16x8, al[c][mt]);
+                    acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, xh, acc[mt], 0, 0, 0);
+                    acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, xl, acc[mt], 0, 0, 0);
+                    acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, xh, acc[mt], 0, 0, 0);
+                }
             }
         }
 #pragma unroll
-        for (int g = 0; g < 4; ++g)
-            part[s][g][lane] = f32x4{acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]};
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                part[s][mt * 4 + g][lane] =
+                    f32x4{acc[mt][4 * g], acc[mt][4 * g + 1], acc[mt][4 * g + 2], acc[mt][4 * g + 3]};
     }
     __syncthreads();
-    if (upd) {
-        // B u^n in slot order, then - f (x - 0 is x exactly), then the update
-        f32x4 y = part[0][s][lane];
+    const float sc = act ? a.sc32[e] : 0.f;
+    bool bad = false;
 #pragma unroll
-        for (int k = 1; k < 4; ++k) y += part[k][s][lane];
-        if (src_here) y -= fsrc[s][lane];
-        const float sc = a.sc32[e];
-        bool bad = false;
+    for (int mt = 0; mt < MT; ++mt) {
+        if (!upd(mt)) continue;
+        const int o = s + 4 * mt;
+        // B u^n in slot order, then - f (x - 0 is x exactly), then the update
+        f32x4 y = part[0][o][lane];
+#pragma unroll
+        for (int k = 1; k < 4; ++k) y += part[k][o][lane];
+        if (src_here) y -= fsrc[o][lane];
         f32x4 wn, tn, cn;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const float w = wv[q] + sc * y[q];
-            const float z = w - cp[q];
-            const float t = uh[q] + z;
-            cn[q] = (t - uh[q]) - z;
+            const float w = wv[mt][q] + sc * y[q];
+            const float z = w - cp[mt][q];
+            const float t = uh[mt][q] + z;
+            cn[q] = (t - uh[mt][q]) - z;
             tn[q] = t;
             wn[q] = w;
             bad |= !isfinite(t);
         }
         if (colok) {
+            const size_t idx = ebase + (size_t)o * R * 8;
             st4(a.ucomp + idx, cn);
             st4(a.uhi_out + idx, tn);
             st4(a.winc + idx, wn);
         }
-        if (bad) atomicOr(a.flags, 1u);
-        if (rec_here) part[0][s][lane] = tn;
+        if (rec_here) part[0][o][lane] = tn;
     }
+    if (bad) atomicOr(a.flags, 1u);
     if (!rec_here) return;
     // the four waves share the element's receivers, reading u^{n+1} from LDS
     __syncthreads();
     b_receivers<P, 1, R>(a, ex, s, kBWaves, c0, half, jj, colok,
-                         [&](int, int, int r) { return part[0][r / 4][lane][r % 4]; });
+                         [&](int mt, int, int r) { return part[0][mt * 4 + r / 4][lane][r % 4]; });
 }
 
 template <int P, int R>
@@ -657,7 +682,7 @@ template <int P, int R>
 int step_b32_p(const BatchArgs &a, hipStream_t s)
 {
     const size_t items = (size_t)a.ne * BCols<R>::WPE;  // (element, 32-column tile)
-    if constexpr (BGeom<P>::MT == 1) {
+    if constexpr (BGeom<P>::MT <= 2) {
         hipLaunchKernelGGL((k_step_b32s<P, R>), dim3((unsigned)items), dim3(64 * kBWaves), 0, s, a);
     } else {
         constexpr int PD = BGeom<P>::MT <= 2 ? 1 : 0;
@@ -671,7 +696,7 @@ template <int P, int R>
 int step_b16x3_p(const BatchArgs &a, hipStream_t s)
 {
     const size_t items = (size_t)a.ne * BCols<R>::WPE;
-    if constexpr (BGeom<P>::MT == 1) {
+    if constexpr (BGeom<P>::MT <= 2) {
         hipLaunchKernelGGL((k_step_b16x3s<P, R>), dim3((unsigned)items), dim3(64 * kBWaves), 0, s, a);
     } else {
         constexpr int PD = BGeom<P>::MT <= 2 ? 1 : 0;
