@@ -149,6 +149,21 @@ def run_multi(steps, warmup, n_inst=256):
                          f"launch mode {mode_name} (autotuned)")
 
 
+def batched_accuracy(precision, cols=(0, 21, 42, 63)):
+    """Seismogram error of the 64-column path on the multi-source workload as
+    configured (3000 steps, every receiver every step) against the plain FP64
+    path run source by source for a few columns; the FP64 path itself matches
+    the oracle to 1e-12, so this is the error against the FP64 oracle."""
+    from tspeed_amd import workloads as W
+    _, s = W.run_batched(W.multisource(), 64, precision=precision)
+    errs = {}
+    for c in cols:
+        _, _, s1 = W.run_engine(W.multisource_column(c), "fp64")
+        errs[c] = float(np.linalg.norm(s[:, :, c] - s1) / np.linalg.norm(s1))
+    return dict(max_rel_l2=max(errs.values()), columns=list(cols), steps=3000,
+                per_column={str(c): e for c, e in errs.items()})
+
+
 def run_serving(n_req=32, loads=(0.5, 0.8), sample_frac=0.05, seed=11):
     """Serving: requests from the multi-source distribution (20-40 squares per
     side, p in {3,4,5}, random homogeneous materials, 1-16 sources, 500-5000
@@ -320,6 +335,20 @@ def main():
     if not args.no_batched:
         variants["batched_fp32_64col"] = run_batched(args.steps, args.warmup)
         variants["batched_bf16x3_64col"] = run_batched(args.steps, args.warmup, precision="bf16x3")
+        for key, prec in (("batched_fp32_64col", "fp32c"), ("batched_bf16x3_64col", "bf16x3")):
+            variants[key]["seis_err_vs_fp64"] = batched_accuracy(prec)
+        # the target: batched >= 10x the plain FP64 path at <= 1e-4 seismogram error;
+        # like for like is the FP64 path on the same 3200-triangle p = 4 mesh
+        ratio = dict(target=10.0, fp64_on_c2=value)
+        if "square_fp64" in variants:
+            ratio["fp64_same_mesh"] = variants["square_fp64"]["value"]
+        for key in ("batched_fp32_64col", "batched_bf16x3_64col"):
+            v = variants[key]["value"]
+            ratio[key] = dict(over_fp64_on_c2=v / value,
+                              over_fp64_same_mesh=(v / variants["square_fp64"]["value"]
+                                                   if "square_fp64" in variants else None),
+                              seis_err=variants[key]["seis_err_vs_fp64"]["max_rel_l2"])
+        variants["batched_vs_plain_fp64"] = ratio
     if args.multi > 0:
         variants["multi_fp32c"] = run_multi(min(args.steps, 200), min(args.warmup, 10), args.multi)
     if args.serve > 0:
