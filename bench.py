@@ -37,7 +37,7 @@ def algorithmic_bytes(prob, scalar_bytes, vec_bytes_per_dof):
     return nblk * d * d * scalar_bytes + prob.ne * d * vec_bytes_per_dof + prob.ne * (8 + 16)
 
 
-def run_variant(prob, precision, steps, warmup):
+def run_variant(prob, precision, steps, warmup, keep_u=False):
     from tspeed_amd import capi
     eng = capi.Engine(precision)
     t0 = time.time()
@@ -59,8 +59,9 @@ def run_variant(prob, precision, steps, warmup):
     wall = time.perf_counter() - t1
     status = eng.status()
     dev_bytes = eng.device_bytes()
+    u = eng.get_u() if keep_u else None   # after the timed region
     eng.close()
-    return dict(ms_total=ms, wall_s=wall, setup_s=setup_s, status=status, device_bytes=dev_bytes)
+    return dict(ms_total=ms, wall_s=wall, setup_s=setup_s, status=status, device_bytes=dev_bytes, u=u)
 
 
 def run_batched(steps, warmup, ncols=64, precision="fp32c"):
@@ -305,7 +306,7 @@ def main():
         sys.exit(1)
 
     prob = W.layered_ricker(nsteps=args.steps + args.warmup)
-    r64 = run_variant(prob, "fp64", args.steps, args.warmup)
+    r64 = run_variant(prob, "fp64", args.steps, args.warmup, keep_u=not args.no_fp32c)
     ms_step = r64["ms_total"] / args.steps
     value = prob.ndof * args.steps / (r64["ms_total"] / 1e3)
     bytes64 = algorithmic_bytes(prob, 8, 24)
@@ -313,11 +314,15 @@ def main():
 
     variants = {}
     if not args.no_fp32c:
-        r32 = run_variant(prob, "fp32c", args.steps, args.warmup)
+        r32 = run_variant(prob, "fp32c", args.steps, args.warmup, keep_u=True)
         ms32 = r32["ms_total"] / args.steps
         b32 = algorithmic_bytes(prob, 4, 24)
+        # both legs took the same steps on the same mesh: the FP32c displacement
+        # against the FP64 one (the FP64 path is the oracle to 1e-12)
+        u_err = float(np.linalg.norm(r32["u"] - r64["u"]) / np.linalg.norm(r64["u"]))
         variants["fp32c"] = dict(value=prob.ndof * args.steps / (r32["ms_total"] / 1e3), ms_per_step=ms32,
                                  achieved_GBs=b32 / (ms32 / 1e3) / 1e9, algorithmic_bytes=b32,
+                                 u_rel_l2_vs_fp64=u_err, u_err_steps=args.steps + args.warmup + 1,
                                  status=r32["status"])
 
     if not args.no_square:
