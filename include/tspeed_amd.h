@@ -60,7 +60,9 @@ int tsp_device_memory(int device, long long *free_bytes, long long *total_bytes)
  * one mesh (one point source each), stepped together as a block SpMM on the
  * MFMA cores (TSP_FP32C or TSP_BF16X3).  The MFMA tile is 32 columns wide, so
  * 8 and 16 cost the operator stream and products of 32 but carry (and move)
- * only their own columns' state. */
+ * only their own columns' state.  Every column is summed on its own and every
+ * width of a degree runs the same kernel, so a source's results are bitwise
+ * the same at every width. */
 int  tsp_create(int device, int precision, int ncols, tsp_context **out);
 void tsp_destroy(tsp_context *ctx);
 const char *tsp_last_error(const tsp_context *ctx);
