@@ -358,6 +358,11 @@ void operator_ready(tsp_context *c)
     hipcheck(c, hipStreamSynchronize(c->stream), "operator");
 }
 
+// FP64 operators larger than this (the 256 MB Infinity Cache less room for
+// the state) take the store-through-L2 step; smaller ones stay cache-resident
+// between steps and do better with plain stores.
+constexpr double kCacheResidentBytes = 128.0 * (1 << 20);
+
 StepArgs base_args(tsp_context *c)
 {
     StepArgs a{};
@@ -377,6 +382,7 @@ StepArgs base_args(tsp_context *c)
     a.blk64 = c->blk64;
     a.invm = c->invm;
     a.minv = c->minv_after() ? c->d_minv : nullptr;
+    a.wt = (double)c->ne * 4 * c->D * c->D * sizeof(double) > kCacheResidentBytes;
     a.blk32 = c->blk32;
     return a;
 }

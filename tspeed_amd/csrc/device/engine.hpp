@@ -104,6 +104,8 @@ struct StepArgs {
     const double *minv;       // non-orthonormal basis: base inverse mass (nln x nln),
                               // applied after the block product as the reference does
                               // (M^-1 (B u - f), M^-1 = invm_e * minv); null otherwise
+    int wt;                   // 1: FP64 state stored through L2 (operator larger than
+                              // the Infinity Cache); 0: plain stores
     // FP32 compensated
     const float *blk32;
     const float *sc32;        // per element (float)(invm * coef)

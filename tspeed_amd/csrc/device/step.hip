@@ -53,7 +53,9 @@ struct StepShape {
     static constexpr int WAVES = 4 * H;
 };
 
-template <int P, bool FIRST>
+// WT: the new state is stored through L2 (see st_state); taken for operators
+// larger than the Infinity Cache.
+template <int P, bool FIRST, bool WT>
 __global__ __launch_bounds__(64 * StepShape<P>::WAVES) void k_step64(StepArgs a)
 {
     using G = Geom<P>;
@@ -142,7 +144,8 @@ __global__ __launch_bounds__(64 * StepShape<P>::WAVES) void k_step64(StepArgs a)
         } else {
             un = 2 * xs[0][row] - up[q] + iw[q];
         }
-        a.uprev[(size_t)e * G::D + row] = un;
+        if constexpr (WT) st_state(a.uprev + (size_t)e * G::D + row, un);
+        else a.uprev[(size_t)e * G::D + row] = un;
         unew[q] = un;
         bad |= !isfinite(un);
     }
@@ -343,8 +346,14 @@ int step64_p(bool first, const StepArgs &a, hipStream_t s)
 {
     // one element per work-group
     const dim3 block(64 * StepShape<P>::WAVES);
-    if (first) hipLaunchKernelGGL((k_step64<P, true>), dim3((unsigned)a.ne), block, 0, s, a);
-    else       hipLaunchKernelGGL((k_step64<P, false>), dim3((unsigned)a.ne), block, 0, s, a);
+    const dim3 grid((unsigned)a.ne);
+    if (a.wt) {
+        if (first) hipLaunchKernelGGL((k_step64<P, true, true>), grid, block, 0, s, a);
+        else       hipLaunchKernelGGL((k_step64<P, false, true>), grid, block, 0, s, a);
+    } else {
+        if (first) hipLaunchKernelGGL((k_step64<P, true, false>), grid, block, 0, s, a);
+        else       hipLaunchKernelGGL((k_step64<P, false, false>), grid, block, 0, s, a);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 template <int P>

@@ -66,6 +66,18 @@ __device__ inline void record(const StepArgs &a, int4 ex, const V *unew, const i
     }
 }
 
+// Store of the next state written through L2 at agent scope (a relaxed atomic
+// store of the same value), so the step leaves no dirty L2 lines behind for the
+// write-back at the end of the kernel.  On the 160 000-triangle p = 6 FP64 step
+// (16 GB operator): 2.35 -> 2.29 ms (a copy without the store: 2.19 ms); on a
+// 3200-triangle p = 4 one, whose operator stays in the Infinity Cache, plain
+// stores are faster (14.3 against 15.1 us), and the FP32c steps gained nothing.
+template <typename T>
+__device__ inline void st_state(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Gather u^n of one element (nb < 0: zeros) into an LDS slice.
 template <typename G, typename T>
 __device__ inline void gather_one(T *x, const T *u, int nb, int lane)
