@@ -383,7 +383,7 @@ def main():
                    "ne": prob.ne, "p": prob.N, "dof": prob.ndof, "dt": prob.dt, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_step64<6,false>", "algorithmic_bytes": bytes64},
+                     "kernel": "k_step64<6,false,true>", "algorithmic_bytes": bytes64},
         "variants": variants,
         "setup_s": r64["setup_s"],
         "status_flags": r64["status"],
