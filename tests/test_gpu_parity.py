@@ -197,6 +197,28 @@ def test_fp64_trajectory_matches_oracle(N):
     assert eng.status() == 0
 
 
+def test_fp64_large_operator_step_matches_oracle():
+    """FP64 operators above 128 MB (here 1400 triangles at p = 6: 140 MB) take the
+    step whose store of u^{n+1} goes through L2; it stores the same values, so it
+    matches the oracle to the same 1e-12 as the plain-store step of small meshes."""
+    p = W.layered_ricker(nx=35, ny=20, N=6, nsteps=100, nrec=16)
+    p.t0 = np.array([10 * p.dt])
+    p.f0 = np.array([1.0 / (8 * p.dt)])
+    # receivers on a line through the source, which 100 steps reach
+    p.rec_xy = np.stack([np.linspace(1600.0, 2400.0, 16), np.full(16, 1750.0)], 1)
+    p.__post_init__()
+    d = 2 * p.nln
+    assert p.ne * 4 * d * d * 8 > 128 * 2 ** 20
+    say(f"large-operator FP64 ne={p.ne} operator {p.ne * 4 * d * d * 8 / 2**20:.0f} MiB steps={p.nsteps}")
+    eng, u, s = W.run_engine(p, "fp64")
+    assert eng.status() == 0
+    uref, sref = oracle_run(p)
+    e_u, (e_s, e_r) = rel_l2(u, uref), seis_err(s, sref)
+    say(f"  u {e_u:.2e}  seismograms {e_s:.2e} (worst receiver {e_r:.2e})")
+    assert e_u <= 1e-12 and e_s <= 1e-12 and e_r <= 1e-12
+    eng.close()
+
+
 def test_square_ricker_full_parity():
     """The 1 km square at full size (3200 triangles, p=4, 2000 steps, 32 receivers):
     final displacement and every seismogram within 1e-12 of the oracle."""
