@@ -112,9 +112,10 @@ __device__ inline void multi_elem(const MInst &I, int e, int lane, float (*xs)[k
         const float uh = xv[0][row];
         const float y = w - cp[q];
         const float t = uh + y;
-        I.ucomp[idx] = (t - uh) - y;
-        uout[idx] = t;
-        I.winc[idx] = w;
+        // through L2 (st_state): the 256-mesh set steps ~1.3% faster; same values
+        st_state(I.ucomp + idx, (t - uh) - y);
+        st_state(uout + idx, t);
+        st_state(I.winc + idx, w);
         unew[q] = t;
         bad |= !isfinite(t);
     }
@@ -230,9 +231,10 @@ __device__ inline void multi_elem_slots(const MInst &I, int e, int lane, int s, 
         const float uh = xv[0][row];
         const float y = w - cp[q];
         const float t = uh + y;
-        I.ucomp[idx] = (t - uh) - y;
-        uout[idx] = t;
-        I.winc[idx] = w;
+        // through L2 (st_state): the 256-mesh set steps ~1.3% faster; same values
+        st_state(I.ucomp + idx, (t - uh) - y);
+        st_state(uout + idx, t);
+        st_state(I.winc + idx, w);
         unew[q] = t;
         bad |= !isfinite(t);
     }
