@@ -242,6 +242,142 @@ def test_penalty_scales_only_the_jump_part():
     assert np.linalg.eigvalsh(S).min() > -1e-8 * np.abs(S).max()
 
 
+# ------------------------------------------------------------------ SIPG penalty block, closed form
+def _two_triangles():
+    """Two counter-clockwise triangles sharing the edge (100,0)-(0,100)."""
+    pts = np.array([[0.0, 0.0], [100.0, 0.0], [0.0, 100.0], [130.0, 120.0]])
+    tri = np.array([[0, 1, 2], [1, 3, 2]], np.int32)
+    return pts, tri
+
+
+def _face_matrix(lam_e, lam_n, mu_e, mu_n, n, mean="harmonic", two_mu=True):
+    """The penalty tensor of one face (Matrices_imp.hpp:167-173) from the face
+    averages of Parameters::avg_p (Parameters.cpp:37-45).  `mean` and `two_mu`
+    switch in the plausible mistakes the test must be able to see."""
+    def avg(a, b):
+        if mean == "arithmetic":
+            return 0.5 * (a + b)
+        return 0.0 if a + b == 0 else 2 * a * b / (a + b)
+    al, am = avg(lam_e, lam_n), avg(mu_e, mu_n)
+    k = 2.0 if two_mu else 1.0
+    c00 = (al + k * am) * n[0] ** 2 + k * am * n[1] ** 2
+    c11 = (al + k * am) * n[1] ** 2 + k * am * n[0] ** 2
+    c01 = al * n[0] * n[1]
+    return np.array([[c00, c01], [c01, c11]])
+
+
+def _pw_linear_basis(verts):
+    """psi_{e,c,m}: component c of element e equal to 1, (x - xc)/h or (y - yc)/h
+    on e and zero elsewhere; returns (centroid, h) per element."""
+    v = verts.reshape(-1, 3, 2)
+    a, b = v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]
+    return v.mean(1), np.sqrt(np.abs(a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]))
+
+
+def _penalty_closed_form(verts, nb, lam, mu, **mutation):
+    """Q[a, b] = sum over interior faces of (1/|e|) int_e [psi_a]^T C [psi_b] ds
+    for the piecewise-linear basis psi of _pw_linear_basis, integrated exactly
+    (a product of two linear functions on a segment: |e|/6 (2 f_A g_A + f_A g_B
+    + f_B g_A + 2 f_B g_B)).  Only vertex coordinates and materials enter: no
+    quadrature rule and no basis table of the oracle."""
+    v = verts.reshape(-1, 3, 2)
+    ne = v.shape[0]
+    cen, h = _pw_linear_basis(verts)
+    reverse = mutation.pop("unreversed", False)
+    Q = np.zeros((6 * ne, 6 * ne))
+
+    def values(e, X):  # [6 basis functions of element e] x [2 components] at point X
+        m = np.array([1.0, (X[0] - cen[e, 0]) / h[e], (X[1] - cen[e, 1]) / h[e]])
+        out = np.zeros((6, 2))
+        for c in range(2):
+            out[3 * c:3 * c + 3, c] = m
+        return out
+
+    for e in range(ne):
+        for f in range(3):
+            o = nb[e, f]
+            if o < e:          # boundary (-1) or already counted from the other side
+                continue
+            A, B = v[e, (f + 1) % 3], v[e, (f + 2) % 3]
+            L = np.hypot(*(B - A))
+            n = np.array([B[1] - A[1], A[0] - B[0]]) / L
+            C = _face_matrix(lam[e], lam[o], mu[e], mu[o], n, **mutation)
+            idx = np.r_[6 * e:6 * e + 6, 6 * o:6 * o + 6]
+            # jump [psi] = psi|_e - psi|_o at the two ends of the face
+            jA = np.vstack([values(e, A), -values(o, B if reverse else A)])
+            jB = np.vstack([values(e, B), -values(o, A if reverse else B)])
+            Q[np.ix_(idx, idx)] += (2 * jA @ C @ jA.T + jA @ C @ jB.T + jB @ C @ jA.T + 2 * jB @ C @ jB.T) / 6
+    return Q
+
+
+def _pw_linear_coefficients(N, verts):
+    """Columns: the oracle-space coefficients of every psi_{e,c,m} (exact
+    L2 projection, FESpace::transform; exact since N >= 1)."""
+    ne = verts.shape[0]
+    cen, h = _pw_linear_basis(verts)
+    nl = O.nln(N)
+    Phi = np.zeros((2 * ne * nl, 6 * ne))
+    for e in range(ne):
+        for c in range(2):
+            for m in range(3):
+                def fun(x, y, e=e, c=c, m=m):
+                    val = [np.ones_like(x), (x - cen[e, 0]) / h[e], (y - cen[e, 1]) / h[e]][m]
+                    return (val if c == 0 else 0 * x, val if c == 1 else 0 * x)
+                u = capi.project(N, verts, fun)
+                mask = np.zeros(2 * ne * nl, bool)
+                for cc in range(2):
+                    mask[cc * ne * nl + e * nl:cc * ne * nl + (e + 1) * nl] = True
+                Phi[mask, 6 * e + 3 * c + m] = u[mask]
+    return Phi
+
+
+@pytest.mark.parametrize("N,mesh", [(1, "two"), (2, "two"), (1, "grid"), (3, "grid")])
+def test_penalty_block_matches_face_integral_closed_form(N, mesh):
+    """Pins Matrices::stability (Matrices_imp.hpp:142-182) and the harmonic face
+    averages (Parameters::avg_p, Parameters.cpp:37-45) on meshes whose elements
+    all have different lambda != 0 (one lambda = 0) and mu.
+
+    S = (B(pen) - B(0)) / (pen N^2) satisfies, for discontinuous fields u, v,
+    v^T S u = sum_faces (1/|e|) int_e [v]^T C [u] ds with the jump
+    [u] = u_e - u_nb and C the face tensor built from the harmonic means.  For
+    every piecewise-linear pair this is integrated exactly from the vertex
+    coordinates alone.  At N = 1 the piecewise-linear fields span the whole
+    space, so the whole S is pinned; at N = 2, 3 its restriction to them.
+    The same closed form with the arithmetic mean, with mu-bar in place of
+    2 mu-bar, or with the neighbour's trace not reversed (kk = k instead of
+    nq - 1 - k) is far outside the tolerance, so each of these mistakes in the
+    oracle would fail the test (measured: the oracle within 0.6-3e-15 of the
+    closed form; the three mistakes 0.33, 0.42 and 0.08-0.14 away)."""
+    if mesh == "two":
+        pts, tri = _two_triangles()
+        lam, mu = np.array([7.0e9, 2.5e9]), np.array([3.0e9, 9.0e9])
+    else:
+        pts, tri = capi.split_square(3, 2, 0.0, 0.0, 300.0, 200.0, 0.2, 17)
+        ne = tri.shape[0]
+        rng = np.random.default_rng(23)
+        lam, mu = rng.uniform(1e9, 9e9, ne), rng.uniform(1e9, 9e9, ne)
+        lam[3] = 0.0   # avg_p's harmonic mean of 0 and x is 0 (an arithmetic mean is not)
+    verts = pts[tri].reshape(-1, 6)
+    nb, ng = O.neighbors(tri)
+    ne = tri.shape[0]
+    rho = np.full(ne, 2000.0)
+    pen = 10.0
+    op1 = O.Operator(N, verts, nb, ng, lam, mu, rho, pen)
+    op0 = O.Operator(N, verts, nb, ng, lam, mu, rho, 0.0)
+    S = (_full(op1, nb) - _full(op0, nb)) / (pen * N * N)
+    Phi = _pw_linear_coefficients(N, verts)
+    got = Phi.T @ S @ Phi
+    want = _penalty_closed_form(verts, nb, lam, mu)
+    scale = np.abs(want).max()
+    assert scale > 0
+    err = np.abs(got - want).max() / scale
+    assert err <= 1e-12, err
+    for mutation in (dict(mean="arithmetic"), dict(two_mu=False), dict(unreversed=True)):
+        wrong = _penalty_closed_form(verts, nb, lam, mu, **mutation)
+        dist = np.abs(got - wrong).max() / scale
+        assert dist > 1e-3, mutation
+
+
 # ------------------------------------------------------------------ leap-frog
 def test_energy_conserved_without_force():
     """E^{n+1/2} = 1/2 v^T M v + 1/2 u^{n+1}^T B u^n with v = (u^{n+1}-u^n)/dt is
