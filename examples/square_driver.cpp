@@ -5,8 +5,9 @@
 //
 //   build:  g++ -O2 -std=c++17 -I../tspeed_amd/csrc/host square_driver.cpp \
 //               -L../tspeed_amd -ltspeed_amd -Wl,-rpath,$ORIGIN/../tspeed_amd -o square_driver
-//   run:    ./square_driver [steps] [out_prefix] [precision 0|1] [basis 0|1]
-//           basis 1 = FESpace<N, Dunavant<2N>, BoundaryAdapted<N>> (as Lamb.cpp:27)
+//   run:    ./square_driver [steps] [out_prefix] [precision 0|1] [basis 0|1] [digits]
+//           basis 1 = FESpace<N, Dunavant<2N>, BoundaryAdapted<N>> (as Lamb.cpp:27);
+//           digits 0 (default) writes the reference's 6-digit files, 17 every bit
 #include "tspeed.hpp"
 
 #include <cmath>
@@ -22,6 +23,7 @@ int main(int argc, char **argv)
     const std::string out = argc > 2 ? argv[2] : "square";
     const int precision = argc > 3 ? std::atoi(argv[3]) : 0;
     const int basis = argc > 4 ? std::atoi(argv[4]) : 0;
+    const int digits = argc > 5 ? std::atoi(argv[5]) : 0;
     const int N = 4;
 
     Mesh_ptr Th = Mesh::split_square(40, 40, 0.0, 0.0, 1000.0, 1000.0, 0.2, 1);
@@ -59,7 +61,7 @@ int main(int argc, char **argv)
         ++step;
         TA.eval_receivers();
     }
-    TA.write_receivers(out);
+    TA.write_receivers(out, digits);
     const auto &u = TA.get_uh();
     double nrm = 0;
     for (double v : u) nrm += v * v;

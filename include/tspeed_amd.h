@@ -199,6 +199,23 @@ void tsp_host_free(void *p);
  * dt = factor * 2/sqrt(lambda_max), the leap-frog limit scaled by `factor`. */
 int tsp_cfl(tsp_context *ctx, double factor, int iters, double *dt_out, double *lambda_max_out);
 
+/* The operator plugin below LeapFrog, on one vector in the reference's
+ * component-major layout (2*ne*nln doubles), for callers that apply the
+ * operators themselves (an energy check, another time integrator):
+ *   tsp_apply_operator:  y = B x     (MyMatMultiDim<MyMat>::operator*, MyMat.hpp:330-335,
+ *                                     the FP64 B = penalty*N^2*S + I + A)
+ *   tsp_apply_invmass:   y = invM x  (MyMatMultiDimBlockDiag<MyMat>::operator*, MyMat.hpp:337-342,
+ *                                     invM_e = 1/rho_e * 1/detJ_e * base_invmass)
+ * device_ptrs = 0: x and y are host arrays and the call returns with y filled;
+ * device_ptrs != 0: x and y are device arrays on the context's device, the work
+ * is only queued on the context's stream, and the call allocates nothing.  x and
+ * y must not overlap.  tsp_apply_operator needs the FP64 operator
+ * (TSP_ERR_STATE after tsp_release_fp64, and in a TSP_FP32C or multi-column
+ * context of a non-orthonormal basis, whose operator is stored premultiplied by
+ * base_invmass). */
+int tsp_apply_operator(tsp_context *ctx, const double *x, double *y, int device_ptrs);
+int tsp_apply_invmass(tsp_context *ctx, const double *x, double *y, int device_ptrs);
+
 /* OR of TSP_FLAG_* seen on the current displacement. */
 int tsp_status(tsp_context *ctx, unsigned *flags);
 
@@ -212,7 +229,13 @@ long long tsp_device_bytes(const tsp_context *ctx);
  * second assembly, no second copy) and has its own sources, receivers and
  * state: several simulations on one Mesh/Parameters, as a driver would build
  * several LeapFrog objects from one FESpace (TimeAdvance_imp.hpp:30).  The
- * owner must outlive every context that shares its operator. */
+ * owner keeps a list of its borrowers: while it has any, tsp_set_mesh and
+ * tsp_upload_blocks on it return TSP_ERR_STATE; tsp_release_fp64 on it drops
+ * the FP64 copy for the borrowers too; destroying it first leaves each borrower
+ * without an operator (its calls then return TSP_ERR_STATE).  A borrower that
+ * sets its own mesh stops borrowing.  A borrower handed to a tsp_multi set needs
+ * its owner in the same set (which removes owners last) or alive until the set
+ * is destroyed. */
 int tsp_share_operator(tsp_context *owner, tsp_context **out);
 
 /* Free the FP64 operator of a TSP_FP32C context once its FP32 copy exists

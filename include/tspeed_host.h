@@ -86,6 +86,13 @@ int tsh_tables_basis(int N, int basis, double *B, double *G, double *Bedge, doub
 int tsh_base_invmass(int N, int basis, double *out);
 int tsh_basis_eval(int N, int basis, int n, const double *xi, double *phi, double *grad);
 
+/* Receivers::write (Receivers.cpp:32-46): <fn>_rcv_<r>.out for r < nrec, one
+ * "x y" line per sample from seis[(sample*nrec + r)*2 + c].  digits <= 0 writes
+ * the stream's default formatting (6 significant digits), the reference's bytes;
+ * digits > 0 that many significant digits (17 keeps every double).  Returns 0,
+ * or -1 when a file cannot be opened. */
+int tsh_write_receivers(const char *fn, int nrec, int nsamples, const double *seis, int digits);
+
 #ifdef __cplusplus
 }
 #endif

@@ -751,6 +751,9 @@ static void apply_invM(const orc_op *op, const double *x, double *out)
     }
 }
 
+/* MyMatMultiDimBlockDiag<MyMat>::operator* (MyMat.hpp:337-342) */
+void orc_apply_invM(const orc_op *op, const double *x, double *out) { apply_invM(op, x, out); }
+
 double orc_ricker(double f0, double t0, double t)
 {
     const double f0sq = f0 * f0;

@@ -69,6 +69,8 @@ void   orc_export_blocks(const orc_op *op, double *out);
 void   orc_export_invm(const orc_op *op, double *out);
 /* out = B*x (component-major vectors of length 2*ne*nln) */
 void   orc_apply_B(const orc_op *op, const double *x, double *out);
+/* y = invM x with the dense per-element blocks (MyMat.hpp:337-342) */
+void   orc_apply_invM(const orc_op *op, const double *x, double *out);
 
 /* ---- leap-frog (TimeAdvance.cpp:24-79) ---- */
 typedef struct {

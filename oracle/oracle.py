@@ -64,6 +64,7 @@ def lib(omp=False):
     L.orc_export_blocks.argtypes = [ctypes.c_void_p, _dp]
     L.orc_export_invm.argtypes = [ctypes.c_void_p, _dp]
     L.orc_apply_B.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.orc_apply_invM.argtypes = [ctypes.c_void_p, _dp, _dp]
     L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.POINTER(Source),
                           ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, _dp, _dp]
     L.orc_ricker.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
@@ -205,6 +206,12 @@ class Operator:
         x = np.ascontiguousarray(x, dtype=np.float64)
         out = np.zeros_like(x)
         self._L.orc_apply_B(self._h, x, out)
+        return out
+
+    def apply_invM(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros_like(x)
+        self._L.orc_apply_invM(self._h, x, out)
         return out
 
     def lambda_max(self, iters=200, seed=7):

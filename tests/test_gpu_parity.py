@@ -481,7 +481,8 @@ def test_cpp_facade_driver_matches_oracle(tmp_path, basis):
     assert os.path.exists(exe), "build the example with python -m tspeed_amd.build"
     steps = 150
     out = str(tmp_path / "c1")
-    r = subprocess.run([exe, str(steps), out, "0", str(basis)], capture_output=True, text=True, timeout=240)
+    # 17 digits: the files carry every bit (the default is the reference's 6)
+    r = subprocess.run([exe, str(steps), out, "0", str(basis), "17"], capture_output=True, text=True, timeout=240)
     say(r.stdout.strip(), r.stderr.strip()[-500:])
     assert r.returncode == 0
     from tspeed_amd import io

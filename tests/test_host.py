@@ -145,13 +145,57 @@ def test_msh_reader_and_neighbors_on_reference_meshes(name):
 
 
 def test_config_receivers_file_format_roundtrip(tmp_path):
-    """Receivers::write format (Receivers.cpp:32-46): one 'x y' line per sample."""
+    """Receivers::write format (Receivers.cpp:32-46): one 'x y' line per sample;
+    with 17 digits the file carries every double exactly."""
     from tspeed_amd import io
-    s = np.arange(24, dtype=np.float64).reshape(4, 3, 2) * 1.5e-7
-    io.write_receivers(str(tmp_path / "run"), s)
-    for r in range(3):
-        got = np.loadtxt(tmp_path / f"run_rcv_{r}.out")
-        np.testing.assert_array_equal(got, s[:, r, :])
+    s = np.arange(24, dtype=np.float64).reshape(4, 3, 2) * 1.5e-7 + np.pi * 1e-9
+    io.write_receivers(str(tmp_path / "run"), s, digits=17)
+    np.testing.assert_array_equal(io.read_receivers(str(tmp_path / "run"), 3), s)
+
+
+# Values whose 6-digit forms exercise %g's switches: fixed / exponent notation,
+# trailing-zero removal, rounding up to a new decade, signed zero, large exponents.
+_SEIS_VALUES = [0.0, -0.0, 1.0, -2.5, 1e-5, 1.234567e-5, 9.9999996e-5, 123456.7, 1234567.0, 0.1 + 0.2,
+                -3.14159265358979e-12, 6.02214076e23, 1e100, 5e-324, 2.0 / 3.0, 999999.5, 1e-4, 100000.0]
+
+
+def _ostream_reference(tmp_path, values):
+    """The bytes `outf << x << " " << y << std::endl` writes with a default
+    std::ofstream (Receivers.cpp:40-43), from a program compiled here."""
+    import subprocess
+    src = tmp_path / "ref.cpp"
+    src.write_text('#include <fstream>\n#include <cstdio>\nint main(int c, char **v) {\n'
+                   '  std::ofstream f(v[1]); double x, y;\n'
+                   '  while (std::scanf("%la %la", &x, &y) == 2) f << x << " " << y << std::endl;\n}\n')
+    exe = tmp_path / "ref"
+    subprocess.run(["g++", "-O0", "-o", str(exe), str(src)], check=True)
+    out = tmp_path / "ref.out"
+    pairs = "\n".join(f"{float(a).hex()} {float(b).hex()}" for a, b in zip(values[0::2], values[1::2]))
+    subprocess.run([str(exe), str(out)], input=pairs, text=True, check=True)
+    return out.read_bytes()
+
+
+def test_receiver_files_are_byte_identical_to_the_reference_format(tmp_path):
+    """io.write_receivers and the C++ Receivers::write (through
+    tsh_write_receivers) write, by default, exactly the bytes of the
+    reference's default-formatted std::ostream output."""
+    import ctypes
+    from tspeed_amd import io
+    vals = np.array(_SEIS_VALUES)
+    seis = vals.reshape(-1, 1, 2)                  # [sample][receiver 0][x|y]
+    want = _ostream_reference(tmp_path, vals)
+    io.write_receivers(str(tmp_path / "py"), seis)
+    assert (tmp_path / "py_rcv_0.out").read_bytes() == want
+    L = capi.lib()
+    flat = np.ascontiguousarray(seis.reshape(-1))
+    assert L.tsh_write_receivers(str(tmp_path / "cxx").encode(), 1, seis.shape[0], flat, 0) == 0
+    assert (tmp_path / "cxx_rcv_0.out").read_bytes() == want
+    # 17 digits: both writers agree and the values come back exactly
+    assert L.tsh_write_receivers(str(tmp_path / "c17").encode(), 1, seis.shape[0], flat, 17) == 0
+    io.write_receivers(str(tmp_path / "p17"), seis, digits=17)
+    assert (tmp_path / "c17_rcv_0.out").read_bytes() == (tmp_path / "p17_rcv_0.out").read_bytes()
+    np.testing.assert_array_equal(io.read_receivers(str(tmp_path / "c17"), 1), seis)
+    assert L.tsh_write_receivers(str(tmp_path / "no" / "such" / "dir").encode(), 1, 1, flat, 0) == -1
 
 
 # ------------------------------------------------------------------ plot output and error norm

@@ -52,6 +52,8 @@ SIGNATURES = {
     "tsp_get_seismograms": (_i, [_vp, _vp, _i, ctypes.POINTER(_i)]),
     "tsp_cfl": (_i, [_vp, _d, _i, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
     "tsp_status": (_i, [_vp, ctypes.POINTER(ctypes.c_uint)]),
+    "tsp_apply_operator": (_i, [_vp, _vp, _vp, _i]),
+    "tsp_apply_invmass": (_i, [_vp, _vp, _vp, _i]),
     "tsp_synchronize": (_i, [_vp]),
     "tsp_device_bytes": (ctypes.c_longlong, [_vp]),
     "tsp_share_operator": (_i, [_vp, ctypes.POINTER(_vp)]),
@@ -108,6 +110,7 @@ SIGNATURES = {
     "tsh_msh_read": (_i, [ctypes.c_char_p, _dp, _ip, _ip, _ip]),
     "tsh_locate": (_i, [_i, _dp, _i, _ip, _i, _dp, _dp, _i, _ip, _dp, _dp]),
     "tsh_ricker_table": (_i, [_i, _dp, _dp, _dp, _dp, _d, _i, _dp]),
+    "tsh_write_receivers": (_i, [ctypes.c_char_p, _i, _i, _dp, _i]),
 }
 
 
@@ -432,6 +435,20 @@ class Engine:
         dt, lm = _d(), _d()
         self._ck(self._L.tsp_cfl(self._h, float(factor), int(iters), ctypes.byref(dt), ctypes.byref(lm)))
         return dt.value, lm.value
+
+    def apply_operator(self, x):
+        """y = B x on a component-major host vector (tsp_apply_operator)."""
+        x = _c64(x)
+        y = np.zeros_like(x)
+        self._ck(self._L.tsp_apply_operator(self._h, x.ctypes.data, y.ctypes.data, 0))
+        return y
+
+    def apply_invmass(self, x):
+        """y = invM x on a component-major host vector (tsp_apply_invmass)."""
+        x = _c64(x)
+        y = np.zeros_like(x)
+        self._ck(self._L.tsp_apply_invmass(self._h, x.ctypes.data, y.ctypes.data, 0))
+        return y
 
     def status(self):
         f = ctypes.c_uint()

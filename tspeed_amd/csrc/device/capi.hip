@@ -79,6 +79,12 @@ struct tsp_context {
     double dt = 0;
     int steps = 0;        // completed step() calls
     bool first_done = false;
+
+    // tsp_share_operator: the context whose operator this one borrows, and the
+    // contexts borrowing this one's (kept so that neither side is left holding
+    // pointers into buffers the other has freed)
+    tsp_context *lender = nullptr;
+    std::vector<tsp_context *> borrowers;
 };
 
 namespace {
@@ -135,6 +141,31 @@ void dfree(tsp_context *c, T *&p)
         (void)hipFree(p);
     }
     p = nullptr;
+}
+
+// A borrower leaves its lender's list (it keeps nothing it has to free).
+void detach_lender(tsp_context *c)
+{
+    if (!c->lender) return;
+    auto &b = c->lender->borrowers;
+    b.erase(std::remove(b.begin(), b.end(), c), b.end());
+    c->lender = nullptr;
+}
+
+// The lender's buffers are about to be freed: every borrower drops its copies of
+// those pointers, so later calls on it report TSP_ERR_STATE instead of reading
+// freed memory.
+void orphan_borrowers(tsp_context *c)
+{
+    for (tsp_context *b : c->borrowers) {
+        b->lender = nullptr;
+        b->blk64 = nullptr;
+        b->blk32 = nullptr;
+        b->blkA = nullptr;
+        b->invm = nullptr;
+        b->d_minv = nullptr;
+    }
+    c->borrowers.clear();
 }
 
 template <typename T>
@@ -496,6 +527,10 @@ void tsp_destroy(tsp_context *c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (tsp_context *b : c->borrowers)   // their queued steps may read this operator
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+    orphan_borrowers(c);
+    detach_lender(c);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -577,6 +612,10 @@ static void check_mesh_args(tsp_context *c, int ne, const int *neigh, const int 
 
 static void reset_mesh(tsp_context *c, int ne)
 {
+    if (!c->borrowers.empty())
+        fail(c, TSP_ERR_STATE, "%zu context(s) borrow this context's operator (tsp_share_operator): destroy them first",
+             c->borrowers.size());
+    detach_lender(c);   // a borrower that sets its own mesh stops borrowing
     drop_graph(c);
     dfree(c, c->meta); dfree(c, c->blk64); dfree(c, c->invm); dfree(c, c->blk32); dfree(c, c->blkA);
     dfree(c, c->sc_first); dfree(c, c->sc_step);
@@ -1081,6 +1120,52 @@ int tsp_cfl(tsp_context *c, double factor, int iters, double *dt_out, double *lm
     });
 }
 
+// y = B x (mode 0) or y = invM x (mode 1) on component-major vectors.
+static void apply_plugin(tsp_context *c, int mode, const double *x, double *y, int device_ptrs)
+{
+    require_mesh(c);
+    if (!x || !y) fail(c, TSP_ERR_ARG, "null vector");
+    if (mode == 0) {
+        require_fp64_operator(c);
+        if (c->d_minv && !c->minv_after())
+            fail(c, TSP_ERR_STATE,
+                 "this context stores the operator premultiplied by the inverse mass (non-orthonormal basis "
+                 "in a TSP_FP32C or multi-column context): apply B from a TSP_FP64 context");
+    }
+    const size_t n = (size_t)c->ne * c->D;
+    const double *minv = c->d_minv;   // invM_e = invm_e * base_invmass (identity when null)
+    if (device_ptrs) {
+        const uintptr_t xa = (uintptr_t)x, ya = (uintptr_t)y, nb = n * sizeof(double);
+        if (xa < ya + nb && ya < xa + nb) fail(c, TSP_ERR_ARG, "x and y overlap");
+        kcheck(c, launch_apply_cm(c->P, mode, c->ne, c->meta, c->blk64, c->invm, minv, x, y, c->stream), "apply");
+        return;
+    }
+    double *dx = dalloc<double>(c, n, "apply"), *dy = nullptr;
+    try {
+        dy = dalloc<double>(c, n, "apply");
+        h2d(c, dx, x, n, "apply");
+        kcheck(c, launch_apply_cm(c->P, mode, c->ne, c->meta, c->blk64, c->invm, minv, dx, dy, c->stream), "apply");
+        d2h(c, y, dy, n, "apply");
+    } catch (...) {
+        (void)hipStreamSynchronize(c->stream);
+        dfree(c, dx);
+        dfree(c, dy);
+        throw;
+    }
+    dfree(c, dx);
+    dfree(c, dy);
+}
+
+int tsp_apply_operator(tsp_context *c, const double *x, double *y, int device_ptrs)
+{
+    return guarded(c, [&] { apply_plugin(c, 0, x, y, device_ptrs); });
+}
+
+int tsp_apply_invmass(tsp_context *c, const double *x, double *y, int device_ptrs)
+{
+    return guarded(c, [&] { apply_plugin(c, 1, x, y, device_ptrs); });
+}
+
 int tsp_status(tsp_context *c, unsigned *flags)
 {
     return guarded(c, [&] {
@@ -1126,6 +1211,8 @@ int tsp_share_operator(tsp_context *owner, tsp_context **out)
         hipcheck(c, hipMemsetAsync(c->u64[1], 0, sizeof(double) * c->state_len(), c->stream), "memset");
         c->have_state = true;
         rebuild_extra(c);
+        c->lender = owner;
+        owner->borrowers.push_back(c);
     });
     if (rc != TSP_OK) {
         if (owner) owner->err = c->err;
@@ -1142,6 +1229,13 @@ int tsp_release_fp64(tsp_context *c)
         require_mesh(c);
         if (c->precision == TSP_FP64) fail(c, TSP_ERR_STATE, "the FP64 context steps with its FP64 operator");
         hipcheck(c, hipStreamSynchronize(c->stream), "synchronize");
+        // borrowers share this FP64 copy: they lose it too (their steps use the FP32 one)
+        for (tsp_context *b : c->borrowers) {
+            if (b->blk64 == c->blk64) {
+                hipcheck(c, hipStreamSynchronize(b->stream), "synchronize");
+                b->blk64 = nullptr;
+            }
+        }
         dfree(c, c->blk64);
     });
 }

@@ -129,6 +129,9 @@ int launch_sample32(int P, const StepArgs &a, const float *hi, const float *comp
 int launch_apply_minvB(int P, int ne, const int4 *meta, const double *blk, const double *invm,
                        const double *minv, const double *x, double *y, hipStream_t s);
 int launch_dot(size_t n, const double *x, const double *y, double *partial, int nparts, hipStream_t s);
+// Operator plugin on component-major vectors: mode 0 y = B x, mode 1 y = invM x.
+int launch_apply_cm(int P, int mode, int ne, const int4 *meta, const double *blk, const double *invm,
+                    const double *minv, const double *x, double *y, hipStream_t s);
 
 // Many right-hand sides (spmm.hip): R = ncols columns, state in row octets
 // u[((e*KB + k/8)*R + j)*8 + k%8], KB = KP/8.

@@ -299,6 +299,63 @@ __global__ __launch_bounds__(kThreads) void k_apply_minvB(int ne, const int4 *me
     }
 }
 
+// The operator plugin on vectors in the reference's component-major layout
+// x[c*ne*nln + e*nln + i]:
+//   MODE 0: y = B x        (MyMatMultiDim<MyMat>::operator*, MyMat.hpp:330-335)
+//   MODE 1: y = invM x     (MyMatMultiDimBlockDiag<MyMat>::operator*, MyMat.hpp:337-342;
+//                           invM_e = invm_e * base_invmass, Matrices_imp.hpp:187)
+// One wave per element; the element's (MODE 0: and its neighbours') entries are
+// gathered from the two component halves into LDS, so no layout conversion or
+// staging buffer is needed.
+template <int P, int MODE>
+__global__ __launch_bounds__(kThreads) void k_apply_cm(int ne, const int4 *meta, const double *blk,
+                                                       const double *invm, const double *minv,
+                                                       const double *x, double *y)
+{
+    using G = Geom<P>;
+    __shared__ __attribute__((aligned(16))) double xs_all[kWaves][4][G::DP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int e = xcd_remap(blockIdx.x, gridDim.x) * kWaves + wave;
+    if (e >= ne) return;   // no block-wide barrier below: each wave works alone
+    const size_t half = (size_t)ne * G::NLN;
+    const int4 m = meta[e];
+    const int nbs[4] = {e, m.x, m.y, m.z};
+    double(*xs)[G::DP] = xs_all[wave];
+    constexpr int NS = MODE == 0 ? 4 : 1;
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            const int idx = lane + 64 * q;
+            if (idx < G::D)
+                xs[s][idx] = nbs[s] >= 0 ? x[(idx / G::NLN) * half + (size_t)nbs[s] * G::NLN + idx % G::NLN] : 0.0;
+        }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if constexpr (MODE == 0) {
+        double acc[G::RPL];
+        spmv<G, double, 2>(blk, e, xs, lane, acc);
+        const int g = lane / G::RW, r = lane % G::RW;
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            const int row = r + 64 * q;
+            if (g == 0 && row < G::D) y[(row / G::NLN) * half + (size_t)e * G::NLN + row % G::NLN] = acc[q];
+        }
+    } else {
+        const double iv = invm[e];
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            const int row = lane + 64 * q;
+            if (row < G::D) {
+                // orthonormal basis: the dense block is invm_e * I, whose row sum is exactly invm_e * x_i
+                const double v = minv ? apply_minv_row<G>(minv, iv, xs[0], row) : iv * xs[0][row];
+                y[(row / G::NLN) * half + (size_t)e * G::NLN + row % G::NLN] = v;
+            }
+        }
+    }
+}
+
 __global__ void k_dot(size_t n, const double *x, const double *y, double *partial)
 {
     __shared__ double red[256];
@@ -382,6 +439,16 @@ int minvb_p(int ne, const int4 *meta, const double *blk, const double *invm, con
     hipLaunchKernelGGL(k_apply_minvB<P>, dim3(nwg_for(ne)), dim3(kThreads), 0, s, ne, meta, blk, invm, minv, x, y);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+template <int P>
+int apply_cm_p(int mode, int ne, const int4 *meta, const double *blk, const double *invm, const double *minv,
+               const double *x, double *y, hipStream_t s)
+{
+    if (mode == 0)
+        hipLaunchKernelGGL((k_apply_cm<P, 0>), dim3(nwg_for(ne)), dim3(kThreads), 0, s, ne, meta, blk, invm, minv, x, y);
+    else
+        hipLaunchKernelGGL((k_apply_cm<P, 1>), dim3(nwg_for(ne)), dim3(kThreads), 0, s, ne, meta, blk, invm, minv, x, y);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 }  // namespace
 
 #define TSP_DISPATCH_P(P, CALL)                                   \
@@ -421,6 +488,13 @@ int launch_apply_minvB(int P, int ne, const int4 *meta, const double *blk, const
                        const double *minv, const double *x, double *y, hipStream_t s)
 {
 #define CALL(p) minvb_p<p>(ne, meta, blk, invm, minv, x, y, s)
+    TSP_DISPATCH_P(P, CALL)
+#undef CALL
+}
+int launch_apply_cm(int P, int mode, int ne, const int4 *meta, const double *blk, const double *invm,
+                    const double *minv, const double *x, double *y, hipStream_t s)
+{
+#define CALL(p) apply_cm_p<p>(mode, ne, meta, blk, invm, minv, x, y, s)
     TSP_DISPATCH_P(P, CALL)
 #undef CALL
 }

@@ -215,4 +215,15 @@ int tsh_l2_error(int N, int basis, int ne, const double *verts, const double *uh
     return 0;
 }
 
+int tsh_write_receivers(const char *fn, int nrec, int nsamples, const double *seis, int digits)
+{
+    if (!fn || nrec < 0 || nsamples < 0 || (nrec && nsamples && !seis)) return -1;
+    try {
+        Receivers::write(fn, nrec, nsamples, std::vector<double>(seis, seis + (size_t)nrec * nsamples * 2), digits);
+        return 0;
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
 }  // extern "C"

@@ -777,13 +777,15 @@ Receivers::Receivers(FESpace_ptr Xh, const std::vector<Geo::Point> &pts)
     for (const auto &p : pts) add_point(*Xh, p);
 }
 
-void Receivers::write(const std::string &fn, int nrec, int nsamples, const std::vector<double> &seis)
+void Receivers::write(const std::string &fn, int nrec, int nsamples, const std::vector<double> &seis, int digits)
 {
     for (int r = 0; r < nrec; ++r) {
         const std::string name = fn + "_rcv_" + std::to_string(r) + ".out";
         std::ofstream out(name);
         if (!out.is_open()) throw std::runtime_error("File " + name + " may not be opened");
-        out.precision(17);
+        // digits <= 0 keeps the stream's default formatting (6 significant
+        // digits), which is what the reference writes
+        if (digits > 0) out.precision(digits);
         for (int s = 0; s < nsamples; ++s)
             out << seis[((size_t)s * nrec + r) * 2] << " " << seis[((size_t)s * nrec + r) * 2 + 1] << "\n";
     }
@@ -921,13 +923,13 @@ void LeapFrog::run(int n, int stride)
 
 void LeapFrog::eval_receivers() { check(tsp_eval_receivers(ctx_)); }
 
-void LeapFrog::write_receivers(const std::string &fn) const
+void LeapFrog::write_receivers(const std::string &fn, int digits) const
 {
     int ns = 0;
     check(tsp_get_seismograms(ctx_, nullptr, 0, &ns));
     std::vector<double> s((size_t)ns * nrec_ * 2);
     check(tsp_get_seismograms(ctx_, s.data(), ns, &ns));
-    Receivers::write(fn, nrec_, ns, s);
+    Receivers::write(fn, nrec_, ns, s, digits);
 }
 
 const std::vector<double> &LeapFrog::get_uh() const

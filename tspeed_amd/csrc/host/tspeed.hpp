@@ -276,7 +276,11 @@ public:
     Receivers(FESpace_ptr Xh, const Geo::Point &p);        // :54-60
     Receivers(FESpace_ptr Xh, const std::vector<Geo::Point> &pts);
     // Write <fn>_rcv_<i>.out, one "x y" line per sample (Receivers.cpp:32-46).
-    static void write(const std::string &fn, int nrec, int nsamples, const std::vector<double> &seis);
+    // digits <= 0: the stream's default formatting (6 significant digits), byte
+    // for byte the reference's files; digits > 0 sets that precision (17 keeps
+    // every double exactly).
+    static void write(const std::string &fn, int nrec, int nsamples, const std::vector<double> &seis,
+                      int digits = 0);
 };
 
 class Force {
@@ -335,7 +339,7 @@ public:
     void run(int n, int stride);
     bool is_running() const { return !completed_; }
     void eval_receivers();
-    void write_receivers(const std::string &fn) const;
+    void write_receivers(const std::string &fn, int digits = 0) const;   // digits as Receivers::write
     const std::vector<double> &get_uh() const;
     const std::vector<double> &u() const { return get_uh(); }
     int steps_done() const;
