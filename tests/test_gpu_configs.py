@@ -1,0 +1,143 @@
+"""The uneven batch and the large single run at the sizes they are measured on.
+
+- Uneven batch: seven instances of the 256-instance set exactly as the bench
+  draws them (``workloads.uneven_instance``: split squares of 612-18084
+  triangles, one of every degree 2..8, their own materials, source, receivers
+  and 590-4894 steps).  Stepped to completion in every launch mode they are bit
+  for bit their single-context FP32c runs; with the autotuned launch their
+  displacements agree with those within 1e-4; and the same instances, with the source moved
+  into the first steps and receivers added around it, stay within 1e-4 of the
+  FP64 oracle.
+- Large single run: the 160 000-triangle layered model at p = 6 with a source
+  that fires inside the run (t0 = 20 dt) and receivers within 30 m of it,
+  1000 steps: FP32c against the FP64 path (itself the oracle to 1e-12) at
+  1e-4 in the final displacement and in the seismograms.
+
+Error measure as in test_gpu_parity: relative L2 over the whole array;
+per-receiver errors normalised by the whole seismogram norm."""
+import copy
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tspeed_amd import capi, workloads as W
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+# instances of the uneven batch (seed 4): degrees 2..8; instance 7 has 18 084 triangles
+UNEVEN_KS = (12, 47, 7, 29, 33, 17, 4)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def seis_err(s, sref):
+    nrm = np.linalg.norm(sref)
+    per_rec = np.linalg.norm((s - sref).transpose(1, 0, 2).reshape(s.shape[1], -1), axis=1) / nrm
+    return rel_l2(s, sref), float(per_rec.max())
+
+
+def _ring(x, y, radii, n=4):
+    ang = np.arange(n) * 2 * np.pi / n + 0.3
+    return np.array([[x + r * np.cos(a), y + r * np.sin(a)] for r in radii for a in ang])
+
+
+def _early(p, nsteps, radii):
+    """p with the Ricker centred at t0 = 20 dt (f0 = 1/(16 dt)) and receivers on
+    rings around the source added to its own."""
+    q = copy.copy(p)
+    q.nsteps = nsteps
+    q.t0 = np.array([20 * p.dt])
+    q.f0 = np.array([1.0 / (16 * p.dt)])
+    q.rec_xy = np.vstack([p.rec_xy, _ring(*p.src_xy[0], radii)])
+    q.__post_init__()
+    return q
+
+
+def _run_set(probs, mode=None, autotune=0, chunk=32):
+    m = capi.Multi(chunk=chunk)
+    ids = [m.add(W.prepare_instance(p), p.nsteps) for p in probs]
+    if autotune:
+        m.autotune(autotune)
+    if mode is not None:
+        m.set_launch(mode)
+    while m.run(64):
+        pass
+    out = []
+    for i, p in zip(ids, probs):
+        done, n = m.info(i)
+        assert done == n == p.nsteps
+        e = m.engine(i)
+        assert e.status() == 0
+        out.append((e.seismograms(), e.get_u()))
+    info = dict(launch=m.launch(), variants={N: m.variant(N) for N in sorted({p.N for p in probs})})
+    m.close()
+    return out, info
+
+
+def test_uneven_batch_as_configured():
+    probs = [W.uneven_instance(k) for k in UNEVEN_KS]
+    assert sorted(p.N for p in probs) == list(range(2, 9))
+    assert max(p.ne for p in probs) > 15000
+    say("instances: " + ", ".join(f"p{p.N}/ne{p.ne}/{p.nsteps} steps" for p in probs))
+    alone = []
+    for p in probs:
+        _, u, s = W.run_engine(p, "fp32c")
+        alone.append((s, u))
+    for mode in (capi.Multi.PER_DEGREE, capi.Multi.FUSED, capi.Multi.FUSED_PERSISTENT):
+        res, info = _run_set(probs, mode=mode)
+        say(f"  launch mode {mode}: bitwise check of {len(probs)} instances")
+        for (s, u), (s1, u1), p in zip(res, alone, probs):
+            np.testing.assert_array_equal(s, s1, err_msg=p.name)
+            np.testing.assert_array_equal(u, u1, err_msg=p.name)
+    # The autotuner times its candidates on real steps of these instances, so
+    # some of their steps ran other lane mappings (same arithmetic, another FP32
+    # summation order): the displacements agree to the FP32c bar, not bit for
+    # bit.  The seismograms are not compared here: these runs end before most
+    # wavelets arrive (t0 = 1.2/f0 = 0.06-0.24 s against runs of 0.01-0.3 s),
+    # so the top receivers hold precursors at roundoff level, where a change of
+    # summation order is a large relative change (up to 0.6 on instance 7 for
+    # every non-default variant; its displacement agrees to 7e-6).  Arrivals
+    # are checked against the oracle in the next test.
+    res, info = _run_set(probs, autotune=3)
+    say(f"  autotuned: launch {info['launch']}, variants {info['variants']}")
+    for (s, u), (s1, u1), p in zip(res, alone, probs):
+        e_u, e_s = rel_l2(u, u1), rel_l2(s, s1)
+        say(f"    {p.name} p={p.N}: vs single context u {e_u:.2e} (seismograms {e_s:.2e})")
+        assert e_u <= 1e-4
+
+
+def test_uneven_batch_instances_match_the_oracle():
+    """The same seven instances (same meshes, materials, degrees and dt) with
+    the source in the first steps and receivers 0.3-2 elements from it: after
+    200 steps of the autotuned set, each is within 1e-4 of the FP64 oracle
+    (the multi-threaded build, bitwise the single-threaded one)."""
+    probs = [_early(W.uneven_instance(k), 200, (3.0, 10.0, 20.0)) for k in UNEVEN_KS]
+    res, info = _run_set(probs, autotune=3, chunk=16)
+    say(f"  autotuned: launch {info['launch']}, variants {info['variants']}")
+    for (s, u), p in zip(res, probs):
+        op = O.Operator(p.N, p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty, omp=True)
+        uref, sref = op.run(p.dt, p.nsteps, src=p.source_dict(0), rec_elem=p.rec_elem, rec_phi=p.rec_phi,
+                            stride=p.stride)
+        e_s, e_r = seis_err(s, sref)
+        e_u = rel_l2(u, uref)
+        say(f"    {p.name} p={p.N} ne={p.ne}: u {e_u:.2e} seismograms {e_s:.2e} worst receiver {e_r:.2e}")
+        assert e_s <= 1e-4 and e_r <= 1e-4 and e_u <= 1e-4
+
+
+def test_layered_fp32c_accuracy_against_fp64():
+    p = _early(W.layered_ricker(nsteps=1000), 1000, (5.0, 10.0, 20.0, 30.0))
+    say(f"layered ne={p.ne} p={p.N} dt={p.dt:.3e} steps={p.nsteps} receivers={len(p.rec_elem)}")
+    e64, u64, s64 = W.run_engine(p, "fp64")
+    e64.close()
+    e32, u32, s32 = W.run_engine(p, "fp32c")
+    assert e32.status() == 0
+    e32.close()
+    e_u = rel_l2(u32, u64)
+    e_s, e_r = seis_err(s32, s64)
+    say(f"  FP32c vs FP64 after {p.nsteps} steps: u {e_u:.2e} seismograms {e_s:.2e} worst receiver {e_r:.2e}")
+    assert e_u <= 1e-4 and e_s <= 1e-4 and e_r <= 1e-4

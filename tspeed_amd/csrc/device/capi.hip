@@ -1073,50 +1073,62 @@ void tsp_host_free(void *p)
     if (p) (void)hipHostFree(p);
 }
 
+// Power iteration for lambda_max(invM B) on caller-provided device vectors
+// x, y (ne*D doubles each) and kCflParts partial sums; every buffer is fully
+// written before it is read, so none needs clearing.  tsp_cfl and
+// tsp_cfl_noalloc run exactly this, so their results are bitwise the same.
+constexpr int kCflParts = 256;
+
+static void cfl_core(tsp_context *c, double factor, int iters, double *x, double *y, double *part, double *dt_out,
+                     double *lmax_out)
+{
+    require_fp64_operator(c);
+    if (iters < 1) fail(c, TSP_ERR_ARG, "iters must be >= 1");
+    const size_t n = (size_t)c->ne * c->D;
+    double hp[kCflParts];
+    kcheck(c, launch_fill_start(n, x, c->stream), "cfl start");
+    auto norm = [&](const double *v) {
+        kcheck(c, launch_dot(n, v, v, part, kCflParts, c->stream), "dot");
+        d2h(c, hp, part, kCflParts, "dot");
+        double s = 0;
+        for (double p : hp) s += p;
+        return std::sqrt(s);
+    };
+    double lam = 0, nx = norm(x);
+    for (int it = 0; it < iters; ++it) {
+        kcheck(c, launch_apply_minvB(c->P, c->ne, c->meta, c->blk64, c->invm, c->minv_after() ? c->d_minv : nullptr,
+                                     x, y, c->stream), "apply");
+        const double ny = norm(y);
+        lam = ny / nx;
+        std::swap(x, y);
+        nx = ny;
+        if (!(nx > 0) || !std::isfinite(nx)) break;
+        if (nx > 1e100 || nx < 1e-100) {
+            // rescale to keep the iterate in range
+            kcheck(c, launch_scale(n, x, 1.0 / nx, c->stream), "cfl scale");
+            nx = norm(x);
+        }
+    }
+    if (lmax_out) *lmax_out = lam;
+    if (dt_out) *dt_out = lam > 0 ? factor * 2.0 / std::sqrt(lam) : 0.0;
+}
+
 int tsp_cfl(tsp_context *c, double factor, int iters, double *dt_out, double *lmax_out)
 {
     return guarded(c, [&] {
         require_fp64_operator(c);
-        if (iters < 1) fail(c, TSP_ERR_ARG, "iters must be >= 1");
         const size_t n = (size_t)c->ne * c->D;
-        double *x = dalloc<double>(c, n, "cfl"), *y = dalloc<double>(c, n, "cfl");
-        const int nparts = 256;
-        double *part = dalloc<double>(c, nparts, "cfl");
-        std::vector<double> hx(n), hp(nparts);
-        uint64_t st = 88172645463325252ull;
-        for (size_t q = 0; q < n; ++q) {
-            st ^= st << 13; st ^= st >> 7; st ^= st << 17;
-            hx[q] = (double)(st >> 11) / 9007199254740992.0 - 0.5;
-        }
-        h2d(c, x, hx.data(), n, "cfl");
-        auto norm = [&](const double *v) {
-            kcheck(c, launch_dot(n, v, v, part, nparts, c->stream), "dot");
-            d2h(c, hp.data(), part, nparts, "dot");
-            double s = 0;
-            for (double p : hp) s += p;
-            return std::sqrt(s);
-        };
-        double lam = 0, nx = norm(x);
-        for (int it = 0; it < iters; ++it) {
-            kcheck(c, launch_apply_minvB(c->P, c->ne, c->meta, c->blk64, c->invm,
-                                         c->minv_after() ? c->d_minv : nullptr, x, y, c->stream), "apply");
-            const double ny = norm(y);
-            lam = ny / nx;
-            std::swap(x, y);
-            nx = ny;
-            if (!(nx > 0) || !std::isfinite(nx)) break;
-            if (nx > 1e100 || nx < 1e-100) {
-                // rescale to keep the iterate in range
-                std::vector<double> t(n);
-                d2h(c, t.data(), x, n, "cfl");
-                for (auto &v : t) v /= nx;
-                h2d(c, x, t.data(), n, "cfl");
-                nx = 1.0;
-            }
+        double *x = dalloc<double>(c, n, "cfl"), *y = nullptr, *part = nullptr;
+        try {
+            y = dalloc<double>(c, n, "cfl");
+            part = dalloc<double>(c, kCflParts, "cfl");
+            cfl_core(c, factor, iters, x, y, part, dt_out, lmax_out);
+        } catch (...) {
+            (void)hipStreamSynchronize(c->stream);
+            dfree(c, x); dfree(c, y); dfree(c, part);
+            throw;
         }
         dfree(c, x); dfree(c, y); dfree(c, part);
-        if (lmax_out) *lmax_out = lam;
-        if (dt_out) *dt_out = lam > 0 ? factor * 2.0 / std::sqrt(lam) : 0.0;
     });
 }
 

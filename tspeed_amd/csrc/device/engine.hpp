@@ -129,6 +129,8 @@ int launch_sample32(int P, const StepArgs &a, const float *hi, const float *comp
 int launch_apply_minvB(int P, int ne, const int4 *meta, const double *blk, const double *invm,
                        const double *minv, const double *x, double *y, hipStream_t s);
 int launch_dot(size_t n, const double *x, const double *y, double *partial, int nparts, hipStream_t s);
+int launch_fill_start(size_t n, double *x, hipStream_t s);   // power-iteration start vector
+int launch_scale(size_t n, double *x, double v, hipStream_t s);
 // Operator plugin on component-major vectors: mode 0 y = B x, mode 1 y = invM x.
 int launch_apply_cm(int P, int mode, int ne, const int4 *meta, const double *blk, const double *invm,
                     const double *minv, const double *x, double *y, hipStream_t s);

@@ -356,6 +356,25 @@ __global__ __launch_bounds__(kThreads) void k_apply_cm(int ne, const int4 *meta,
     }
 }
 
+// Start vector of the power iteration: x[q] uniform in [-0.5, 0.5) from a
+// counter-based hash of q (splitmix64's finaliser), so no host buffer is needed.
+__global__ void k_fill_start(size_t n, double *x)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    unsigned long long z = (unsigned long long)q + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    x[q] = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+}
+
+__global__ void k_scale(size_t n, double *x, double s)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) x[q] *= s;
+}
+
 __global__ void k_dot(size_t n, const double *x, const double *y, double *partial)
 {
     __shared__ double red[256];
@@ -512,6 +531,16 @@ int launch_join_state(size_t n, const float *hi, const float *comp, double *u64,
 int launch_scale_demote(size_t n, const double *src, double coef, float *dst, hipStream_t s)
 {
     hipLaunchKernelGGL(k_scale_demote, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, src, coef, dst);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int launch_fill_start(size_t n, double *x, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_start, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, x);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int launch_scale(size_t n, double *x, double v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, x, v);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int launch_dot(size_t n, const double *x, const double *y, double *partial, int nparts, hipStream_t s)
