@@ -216,6 +216,17 @@ int tsp_cfl(tsp_context *ctx, double factor, int iters, double *dt_out, double *
 int tsp_apply_operator(tsp_context *ctx, const double *x, double *y, int device_ptrs);
 int tsp_apply_invmass(tsp_context *ctx, const double *x, double *y, int device_ptrs);
 
+/* tsp_cfl without any device allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory, such as code under HIP graph
+ * capture or a serving loop that must not call hipMalloc.  The power-iteration
+ * vectors and partial sums live in a fixed 32 MB array per device, built into
+ * the library, so meshes with more than 2,000,000 coefficients per vector
+ * (ne*2*nln) are refused with TSP_ERR_ARG.  The results are bitwise those of
+ * tsp_cfl.  Every call on one device shares that array: calls that overlap on
+ * one device (from several host threads) race and must be serialised by the
+ * caller. */
+int tsp_cfl_noalloc(tsp_context *ctx, double factor, int iters, double *dt_out, double *lambda_max_out);
+
 /* OR of TSP_FLAG_* seen on the current displacement. */
 int tsp_status(tsp_context *ctx, unsigned *flags);
 

@@ -51,6 +51,7 @@ SIGNATURES = {
     "tsp_get_u": (_i, [_vp, _i, _dp]),
     "tsp_get_seismograms": (_i, [_vp, _vp, _i, ctypes.POINTER(_i)]),
     "tsp_cfl": (_i, [_vp, _d, _i, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
+    "tsp_cfl_noalloc": (_i, [_vp, _d, _i, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
     "tsp_status": (_i, [_vp, ctypes.POINTER(ctypes.c_uint)]),
     "tsp_apply_operator": (_i, [_vp, _vp, _vp, _i]),
     "tsp_apply_invmass": (_i, [_vp, _vp, _vp, _i]),

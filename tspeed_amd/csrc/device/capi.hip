@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1129,6 +1130,47 @@ int tsp_cfl(tsp_context *c, double factor, int iters, double *dt_out, double *lm
             throw;
         }
         dfree(c, x); dfree(c, y); dfree(c, part);
+    });
+}
+
+// ---------------------------------------------------------------- no-alloc CFL
+// tsp_cfl_noalloc runs cfl_core on a file-scope device array instead of
+// buffers it allocates.  Each device holds its own copy of `scratch` (from the
+// first load of this code object on that device until the process ends); its
+// address on a device comes from hipGetSymbolAddress with that device current,
+// taken once per device on the entry point's first call there and kept in
+// g_scratch_ptr.  Host code never passes `scratch` itself to a kernel: in host
+// code the name stands for a host-side shadow, not a device address.
+constexpr size_t kNoAllocMaxDof = 2000000;   // ne*2*nln per vector: 16 MB
+constexpr size_t kNoAllocDoubles = 2 * kNoAllocMaxDof + kCflParts;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];   // 32 MB per device
+
+constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static double *scratch_ptr(tsp_context *c)
+{
+    if (c->device < 0 || c->device >= kMaxDevices) fail(c, TSP_ERR_ARG, "device %d out of range", c->device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[c->device]) {
+        void *p = nullptr;
+        hipcheck(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[c->device] = static_cast<double *>(p);
+    }
+    return g_scratch_ptr[c->device];
+}
+
+int tsp_cfl_noalloc(tsp_context *c, double factor, int iters, double *dt_out, double *lmax_out)
+{
+    return guarded(c, [&] {
+        require_fp64_operator(c);
+        const size_t n = (size_t)c->ne * c->D;
+        if (n > kNoAllocMaxDof)
+            fail(c, TSP_ERR_ARG, "tsp_cfl_noalloc: %zu coefficients per vector exceed its scratch of %zu (use tsp_cfl)",
+                 n, kNoAllocMaxDof);
+        double *base = scratch_ptr(c);   // set_device(c) made c's device current
+        cfl_core(c, factor, iters, base, base + kNoAllocMaxDof, base + 2 * kNoAllocMaxDof, dt_out, lmax_out);
     });
 }
 
