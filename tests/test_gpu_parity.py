@@ -634,14 +634,15 @@ def test_batched_every_degree(N, ncols, precision):
 
 @pytest.mark.parametrize("precision", ["fp32c", "bf16x3"])
 def test_batched_graph_replay_equals_step_launches(precision):
-    """Runs of >= 8 multi-column steps replay a captured graph whose steps read
-    their index from a device counter; the result is bit-identical to the same
-    steps launched one at a time (runs of 3), with a sampling stride of 2, a
-    second replay of the cached graph, and a fall-back after an extra
-    eval_receivers sample."""
+    """Multi-column runs replay a captured graph of 32 steps whose steps read
+    their index from a device counter that the graph itself advances, and
+    launch the rest step by step; the result is bit-identical to the same
+    steps launched one at a time (runs of 3), with a sampling stride of 2:
+    two replays in one run (70 = 2 x 32 + 6), one replay in each of two runs
+    (40 + 40), and a fall-back after an extra eval_receivers sample."""
     p = W.multisource(nsrc=5, nx=6, ny=5, nsteps=100)
     outs = []
-    for chunks in ([40, 40, 20], [3] * 33 + [1]):
+    for chunks in ([70, 10, 20], [40, 40, 20], [3] * 33 + [1]):
         eng = capi.Engine(precision, ncols=32)
         eng.set_tables(p.N)
         eng.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
@@ -653,11 +654,12 @@ def test_batched_graph_replay_equals_step_launches(precision):
             eng.run(n)
         outs.append((eng.seismograms(), eng.get_u(0), eng.get_u(4), eng.steps_done()))
         eng.close()
-    (s0, a0, b0, n0), (s1, a1, b1, n1) = outs
-    assert n0 == n1 == 100
-    np.testing.assert_array_equal(s0, s1)
-    np.testing.assert_array_equal(a0, a1)
-    np.testing.assert_array_equal(b0, b1)
+    (s0, a0, b0, n0) = outs[-1]
+    for (s1, a1, b1, n1) in outs[:-1]:
+        assert n0 == n1 == 100
+        np.testing.assert_array_equal(s0, s1)
+        np.testing.assert_array_equal(a0, a1)
+        np.testing.assert_array_equal(b0, b1)
     # an extra sample makes the slots irregular: the launches take over, samples stay in order
     eng = capi.Engine(precision, ncols=32)
     eng.set_tables(p.N)

@@ -302,11 +302,17 @@ int launch_batched(tsp_context *c, const BatchArgs &b)
                                        : launch_step_b32(c->P, c->ncols, b, c->stream);
 }
 
-// Runs of at least this many multi-column steps replay a captured graph: one
-// launch for the run, each captured step reading its index from a device
-// counter (set before the replay) plus its place in the run.  (Single-column
-// steps are launched directly: replaying them measured no faster.)
-constexpr int kGraphMinSteps = 8;
+// Multi-column runs replay one captured graph of kGraphSteps steps per
+// context: each captured step reads its index from a device counter (set once
+// per tsp_run) plus its place in the graph, and the graph ends by advancing the
+// counter, so replays follow each other with no host work between them.  The
+// rest of a run (and runs shorter than the graph) are launched step by step;
+// the kernels are the same, so the results are too.  The graph is captured on
+// a context's first long run and kept until a set-up call changes what it
+// holds: a running request is never re-captured and no graph is destroyed
+// while others run.  (Single-column steps are launched directly: replaying
+// them measured no faster.)
+constexpr int kGraphSteps = 32;
 
 // samples so far are exactly the stride-driven ones (the graph derives the
 // sample slot of step n as n / stride)
@@ -316,47 +322,51 @@ bool regular_samples(const tsp_context *c)
     return c->nsamples == std::min(c->max_samples, 1 + c->steps / c->stride);
 }
 
-// Capture (once per run length) the graph of `nsteps` multi-column steps.
-void ensure_graph(tsp_context *c, int nsteps)
+// Capture the context's graph of kGraphSteps multi-column steps (once).
+void ensure_graph(tsp_context *c)
 {
+    if (c->graph) return;
     if (!c->d_ctr) c->d_ctr = dalloc<int>(c, 1, "step counter");
-    if (!c->graph || c->graph_len != nsteps) {
-        drop_graph(c);
-        BatchArgs b = batch_args(c);
-        b.sc32 = c->sc_step;
-        b.ctr = c->d_ctr;
-        b.wav = c->wav;
-        b.stride = c->nrec ? c->stride : 0;
-        b.max_samples = c->max_samples;
-        b.uhi_pair[0] = c->uhi[0];
-        b.uhi_pair[1] = c->uhi[1];
-        hipcheck(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "capture");
-        int rc = 0;
-        for (int k = 0; k < nsteps && !rc; ++k) {
-            b.koff = k;  // the step's place in the run: no counter kernel between steps
-            rc = launch_batched(c, b);
-        }
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (rc || e != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            fail(c, TSP_ERR_HIP, "graph capture of %d steps failed", nsteps);
-        }
-        const hipError_t ei = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        hipcheck(c, ei, "graph instantiate");
-        c->graph_len = nsteps;
+    BatchArgs b = batch_args(c);
+    b.sc32 = c->sc_step;
+    b.ctr = c->d_ctr;
+    b.wav = c->wav;
+    b.stride = c->nrec ? c->stride : 0;
+    b.max_samples = c->max_samples;
+    b.uhi_pair[0] = c->uhi[0];
+    b.uhi_pair[1] = c->uhi[1];
+    hipcheck(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "capture");
+    int rc = 0;
+    for (int k = 0; k < kGraphSteps && !rc; ++k) {
+        b.koff = k;  // the step's place in the graph: no counter kernel between steps
+        rc = launch_batched(c, b);
     }
+    if (!rc) rc = launch_ctr_add(c->d_ctr, kGraphSteps, c->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        fail(c, TSP_ERR_HIP, "graph capture of %d steps failed", kGraphSteps);
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    hipcheck(c, ei, "graph instantiate");
+    c->graph = x;
+    c->graph_len = kGraphSteps;
 }
 
-void run_graph(tsp_context *c, int nsteps)
+// Whole graphs of the run; returns the steps left for single launches.
+int run_graph(tsp_context *c, int nsteps)
 {
-    ensure_graph(c, nsteps);
+    ensure_graph(c);
+    const int reps = nsteps / c->graph_len;
     kcheck(c, launch_ctr_set(c->d_ctr, c->steps, c->stream), "step counter");
-    hipcheck(c, hipGraphLaunch(c->graph, c->stream), "graph launch");
-    c->steps += nsteps;
+    for (int r = 0; r < reps; ++r) hipcheck(c, hipGraphLaunch(c->graph, c->stream), "graph launch");
+    c->steps += reps * c->graph_len;
     c->cur = (c->steps + 1) & 1;
     if (c->nrec > 0 && c->stride > 0) c->nsamples = std::min(c->max_samples, 1 + c->steps / c->stride);
+    return nsteps - reps * c->graph_len;
 }
 
 void operator_ready(tsp_context *c)
@@ -910,10 +920,7 @@ int tsp_run(tsp_context *c, int nsteps)
             if (c->nsamples + need > c->max_samples)
                 fail(c, TSP_ERR_ARG, "seismogram buffer of %d samples cannot hold %d more", c->max_samples, need);
         }
-        if (c->batched() && nsteps >= kGraphMinSteps && regular_samples(c)) {
-            run_graph(c, nsteps);
-            return;
-        }
+        if (c->batched() && nsteps >= kGraphSteps && regular_samples(c)) nsteps = run_graph(c, nsteps);
         StepArgs a = base_args(c);
         const double dt = c->dt;
         a.coef = (-dt) * dt;
@@ -954,7 +961,7 @@ int tsp_run_timed(tsp_context *c, int nsteps, float *ms)
     int rc = guarded(c, [&] {
         if (!ms) fail(c, TSP_ERR_ARG, "null output");
         // graph capture is host work: keep it out of the timed span
-        if (c->batched() && c->first_done && nsteps >= kGraphMinSteps && regular_samples(c)) ensure_graph(c, nsteps);
+        if (c->batched() && c->first_done && nsteps >= kGraphSteps && regular_samples(c)) ensure_graph(c);
         hipcheck(c, hipEventCreate(&a), "hipEventCreate");
         hipcheck(c, hipEventCreate(&b), "hipEventCreate");
         hipcheck(c, hipEventRecord(a, c->stream), "hipEventRecord");

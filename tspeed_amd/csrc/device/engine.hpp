@@ -163,6 +163,7 @@ struct BatchArgs {
     float *uhi_pair[2];
 };
 int launch_ctr_set(int *ctr, int value, hipStream_t s);
+int launch_ctr_add(int *ctr, int value, hipStream_t s);
 int batch_kp(int P);                 // padded rows per element (multiple of 32)
 size_t batch_block_floats(int P);    // floats of one element's four packed blocks
 int launch_step_b32(int P, int ncols, const BatchArgs &a, hipStream_t s);

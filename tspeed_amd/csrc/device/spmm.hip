@@ -222,6 +222,7 @@ __device__ inline void resolve_step(BatchArgs &a)
 }
 
 __global__ void k_ctr_set(int *ctr, int value) { *ctr = value; }
+__global__ void k_ctr_add(int *ctr, int value) { *ctr += value; }
 
 // Blocks of three 32-row tiles (p = 7, 8): one wave computes one
 // element's rows for one 32-column tile (R/32 waves share an element when
@@ -777,6 +778,11 @@ int sample_b32_p(const BatchArgs &a, int R, hipStream_t s)
 int launch_ctr_set(int *ctr, int value, hipStream_t s)
 {
     hipLaunchKernelGGL(k_ctr_set, dim3(1), dim3(1), 0, s, ctr, value);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int launch_ctr_add(int *ctr, int value, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ctr_add, dim3(1), dim3(1), 0, s, ctr, value);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
