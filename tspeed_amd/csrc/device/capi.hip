@@ -426,6 +426,9 @@ StepArgs base_args(tsp_context *c)
     a.minv = c->minv_after() ? c->d_minv : nullptr;
     a.wt = (double)c->ne * 4 * c->D * c->D * sizeof(double) > kCacheResidentBytes;
     a.blk32 = c->blk32;
+    // FP32 products drop the element's translation (see gather); the
+    // BoundaryAdapted basis has no single constant mode, so it keeps the plain form
+    a.tfree = c->h_minv.empty() ? 1 : 0;
     return a;
 }
 
@@ -447,6 +450,7 @@ BatchArgs batch_args(tsp_context *c)
     a.sample = -1;
     a.flags = c->flags;
     a.blk = c->blkA;
+    a.tfree = c->h_minv.empty() ? 1 : 0;
     a.ucomp = c->ucomp;
     a.winc = c->winc;
     return a;
@@ -1323,6 +1327,7 @@ bool ctx_view(tsp_context *c, InstView *v)
     v->cur = c->cur;
     v->first_done = c->first_done;
     v->owns_operator = c->blk32 && std::find(c->allocs.begin(), c->allocs.end(), (void *)c->blk32) != c->allocs.end();
+    v->tfree = c->h_minv.empty();
     v->stream = c->stream;
     v->meta = c->meta;
     v->extra = c->extra;

@@ -107,6 +107,7 @@ struct StepArgs {
     int wt;                   // 1: FP64 state stored through L2 (operator larger than
                               // the Infinity Cache); 0: plain stores
     // FP32 compensated
+    int tfree;                // 1: translation-free products (orthonormal basis; see gather)
     const float *blk32;
     const float *sc32;        // per element (float)(invm * coef)
     const float *uhi;         // gathered displacement (hi part)
@@ -148,6 +149,7 @@ struct BatchArgs {
     int sample;
     unsigned *flags;
     const float *blk;         // A-fragment operator
+    int tfree;                // 1: translation-free products (orthonormal basis)
     const float *sc32;
     const float *uhi;
     float *uhi_out, *ucomp, *winc;

@@ -12,6 +12,7 @@ struct InstView {
     int P, ne, nsrc, nrec, ntab, stride, max_samples, nsamples, steps, cur;
     bool first_done;
     bool owns_operator;   // false for a context made by tsp_share_operator
+    bool tfree;           // translation-free products (orthonormal basis)
     hipStream_t stream;
     const int4 *meta, *extra;
     const int *src_sorted, *rec_sorted;

@@ -44,6 +44,7 @@ namespace {
 
 struct MInst {
     int P, variant, ne, nsteps, stride, nrec, nsrc, max_samples;
+    int tfree;  // translation-free products (see gather)
     int steps;  // completed step() calls (advanced on the device)
     int live;   // 0 once removed
     int sel;    // 1 when the instance's elements are in the current item list
@@ -80,7 +81,8 @@ __device__ inline void multi_elem(const MInst &I, int e, int lane, float (*xs)[k
     }
     const float sc = I.sc[e];
     float(*xv)[G::DP] = reinterpret_cast<float(*)[G::DP]>(&xs[0][0]);
-    gather<G, float>(xv, uin, e, m, lane, true);
+    float own[G::RPL];   // u^n of the lane's rows (the update's uh)
+    gather<G, float>(xv, uin, e, m, lane, true, I.tfree != 0, own);
     // the wave reads only its own LDS slice: order its stores before its loads
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -109,7 +111,7 @@ __device__ inline void multi_elem(const MInst &I, int e, int lane, float (*xs)[k
         const size_t idx = (size_t)e * G::D + row;
         const float f = (ex.y > ex.x) ? (float)source_term(a, ex, row, G::NLN) : 0.f;
         const float w = wi[q] + sc * (acc[q] - f);
-        const float uh = xv[0][row];
+        const float uh = own[q];
         const float y = w - cp[q];
         const float t = uh + y;
         // through L2 (st_state): the 256-mesh set steps ~1.3% faster; same values
@@ -188,7 +190,8 @@ __device__ inline void multi_elem_slots(const MInst &I, int e, int lane, int s, 
         cp[q] = rows[q] >= 0 ? I.ucomp[idx] : 0.f;
     }
     const float sc = s == 0 ? I.sc[e] : 0.f;
-    gather_one<G, float>(x, uin, nb, lane);
+    float own[G::RPL];   // wave 0: u^n of the lane's rows (the update's uh)
+    gather_one<G, float>(x, uin, nb, lane, I.tfree ? e : -1, own);
     // the wave reads only its own LDS slice: order its stores before its loads
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -228,7 +231,7 @@ __device__ inline void multi_elem_slots(const MInst &I, int e, int lane, int s, 
         const size_t idx = (size_t)e * G::D + row;
         const float f = (ex.y > ex.x) ? (float)source_term(a, ex, row, G::NLN) : 0.f;
         const float w = wi[q] + sc * (acc[q] - f);
-        const float uh = xv[0][row];
+        const float uh = own[q];
         const float y = w - cp[q];
         const float t = uh + y;
         // through L2 (st_state): the 256-mesh set steps ~1.3% faster; same values
@@ -682,6 +685,7 @@ int tsp_multi_add(tsp_multi *m, tsp_context *c, int nsteps, int *id)
         I.nrec = v.nrec;
         I.nsrc = v.nsrc;
         I.max_samples = v.max_samples;
+        I.tfree = v.tfree ? 1 : 0;
         I.steps = 0;
         I.live = 1;
         I.meta = v.meta;

@@ -73,7 +73,34 @@ struct BGeom {
     static constexpr int MT = (D + 31) / 32;  // 32-row tiles
     static constexpr int KP = 32 * MT;        // padded rows / contraction length
     static constexpr int KB = KP / 8;         // row octets per element
+    // where row NLN (mode 0 of the second component) sits: row octet, the half
+    // of the wave holding it in a 16-byte piece, and its place in the piece /
+    // in the octet
+    static constexpr int KBN = NLN / 8, HN = (NLN % 8) / 4, QN = NLN % 4, Q8N = NLN % 8;
 };
+
+// Translation-free products (the element's mode-0 coefficients, rows 0 and
+// NLN of its own columns, subtracted from those rows of every slot; exact in
+// the algebra because every row of B annihilates a rigid translation -- see
+// gather in step_common.hpp for the accuracy this buys in FP32).  The two
+// values of this lane's column, or 0 where the lane holds neither row; `base`
+// points at the lane's column in the element's first row octet, `piece` is 4
+// (16-byte pieces, lanes of half h hold rows 8 kb + 4 h ..) or 8 (whole octets).
+template <int P, int R, int PIECE>
+__device__ inline void b_translation(const BatchArgs &a, const float *base, int half, bool colok, float &t0,
+                                     float &tn)
+{
+    using G = BGeom<P>;
+    t0 = tn = 0.f;
+    if (!a.tfree || !colok) return;
+    if constexpr (PIECE == 4) {
+        if (half == 0) t0 = base[0];
+        if (half == G::HN) tn = base[(size_t)G::KBN * R * 8 + G::QN];
+    } else {
+        if (half == 0) t0 = base[0];
+        if (half == G::KBN % 2) tn = base[(size_t)G::KBN * R * 8 + G::Q8N];
+    }
+}
 
 // Waves per element and columns per wave: a wave owns one 32-column MFMA tile;
 // a context of fewer than 32 columns fills only the first R columns of it (the
@@ -290,6 +317,9 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
                 }
         }
     };
+    // translation of this lane's column (own element, rows 0 and NLN)
+    float t0 = 0.f, tn = 0.f;
+    b_translation<P, R, 4>(a, a.uhi + ebase, half, colok, t0, tn);
     auto product = [&](int s, int buf) {
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
@@ -298,9 +328,12 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-                    for (int nt = 0; nt < NT; ++nt)
-                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x2f32(
-                            av[buf][kb][mt][t], s == 0 ? own[kb][nt][t] : bv[buf][kb][nt][t], acc[mt][nt], 0, 0, 0);
+                    for (int nt = 0; nt < NT; ++nt) {
+                        float x = s == 0 ? own[kb][nt][t] : bv[buf][kb][nt][t];
+                        if (kb == 0 && t == 0) x = x - t0;
+                        if (kb == G::KBN && t == G::QN) x = x - tn;
+                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[buf][kb][mt][t], x, acc[mt][nt], 0, 0, 0);
+                    }
     };
 #pragma unroll
     for (int s = 0; s < PD && s < 4; ++s) load(s, s % NB);
@@ -396,6 +429,10 @@ __device__ inline void step_slots(BatchArgs &a)
                 const f32x4 x = *reinterpret_cast<const f32x4 *>(X + (size_t)kb * R * 8);
                 bv[kb] = colok ? x : f32x4{};
             }
+            float t0, tn;
+            b_translation<P, R, 4>(a, a.uhi + ebase, half, colok, t0, tn);
+            bv[0][0] = bv[0][0] - t0;
+            bv[G::KBN][G::QN] = bv[G::KBN][G::QN] - tn;
             load_update_operands();
 #pragma unroll
             for (int kb = 0; kb < KB; ++kb)
@@ -421,6 +458,11 @@ __device__ inline void step_slots(BatchArgs &a)
                 xv[c][0] = colok ? *reinterpret_cast<const f32x4 *>(o) : f32x4{};
                 xv[c][1] = colok ? *reinterpret_cast<const f32x4 *>(o + 4) : f32x4{};
             }
+            float t0, tn;
+            b_translation<P, R, 8>(a, a.uhi + (size_t)e * KB * R * 8 + (c0 + (colok ? jj : 0)) * 8, half, colok, t0,
+                                   tn);
+            xv[0][0][0] = xv[0][0][0] - t0;
+            xv[G::KBN / 2][G::Q8N / 4][G::Q8N % 4] = xv[G::KBN / 2][G::Q8N / 4][G::Q8N % 4] - tn;
             load_update_operands();
 #pragma unroll
             for (int c = 0; c < KC; ++c) {
@@ -570,13 +612,17 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3(BatchArgs a)
             }
         }
     };
+    float t0 = 0.f, tn = 0.f;
+    b_translation<P, R, 8>(a, a.uhi + (size_t)e * KB * R * 8 + bofs, half, colok, t0, tn);
     auto product = [&](int buf) {
 #pragma unroll
         for (int c = 0; c < KC; ++c) {
             bf16x8 xh, xl;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const float x = xv[buf][c][q / 4][q % 4];
+                float x = xv[buf][c][q / 4][q % 4];
+                if (c == 0 && q == 0) x = x - t0;
+                if (c == G::KBN / 2 && q == G::Q8N) x = x - tn;
                 const __bf16 h = (__bf16)x;
                 xh[q] = h;
                 xl[q] = (__bf16)(x - (float)h);

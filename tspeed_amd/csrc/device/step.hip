@@ -182,7 +182,8 @@ __global__ __launch_bounds__(kThreads) void k_step32c(StepArgs a)
     }
     const float sc = act ? a.sc32[e] : 0.f;
     float(*xs)[G::DP] = xs_all[wave];
-    gather<G, float>(xs, a.uhi, e, m, lane, act);
+    float own[G::RPL];   // u^n of the lane's rows (the update's uh)
+    gather<G, float>(xs, a.uhi, e, m, lane, act, a.tfree != 0, own);
     __syncthreads();
     if (!act) return;
 
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(kThreads) void k_step32c(StepArgs a)
         const size_t idx = (size_t)e * G::D + row;
         const float f = (ex.y > ex.x) ? (float)source_term(a, ex, row, G::NLN) : 0.f;
         const float w = wi[q] + sc * (acc[q] - f);
-        const float uh = xs[0][row];
+        const float uh = own[q];
         const float y = w - cp[q];
         const float t = uh + y;
         a.ucomp[idx] = (t - uh) - y;
