@@ -433,7 +433,6 @@ __device__ inline void step_slots(BatchArgs &a)
             b_translation<P, R, 4>(a, a.uhi + ebase, half, colok, t0, tn);
             bv[0][0] = bv[0][0] - t0;
             bv[G::KBN][G::QN] = bv[G::KBN][G::QN] - tn;
-            load_update_operands();
 #pragma unroll
             for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
@@ -441,6 +440,10 @@ __device__ inline void step_slots(BatchArgs &a)
 #pragma unroll
                     for (int mt = 0; mt < MT; ++mt)
                         acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kb][mt][t], bv[kb][t], acc[mt], 0, 0, 0);
+            // the update's operands are loaded after the products: their
+            // registers are then not live across the MFMAs (8 waves per SIMD
+            // at one row tile instead of 6)
+            load_update_operands();
         } else {
             // a lane's B operand of K-chunk c is octet 2c + half of its column
             const float *X = a.uhi + (size_t)nb * KB * R * 8 + (c0 + (colok ? jj : 0)) * 8;
@@ -533,8 +536,12 @@ __device__ inline void step_slots(BatchArgs &a)
                          [&](int mt, int, int r) { return part[0][mt * 4 + r / 4][lane][r % 4]; });
 }
 
+// One row tile (p <= 4): 8 waves per SIMD fit without spilling.
+template <int P>
+constexpr int slot_min_waves() { return BGeom<P>::MT == 1 ? 8 : 1; }
+
 template <int P, int R>
-__global__ __launch_bounds__(64 * kBWaves) void k_step_b32s(BatchArgs a)
+__global__ __launch_bounds__(64 * kBWaves, slot_min_waves<P>()) void k_step_b32s(BatchArgs a)
 {
     resolve_step(a);
     step_slots<P, R, false>(a);
