@@ -130,7 +130,7 @@ def run_multi(steps, warmup, n_inst=256):
     # autotuner's trials, these 96 steps, the warm-up and at most 200 timed steps
     # leave every instance running to the end of the timed run)
     modes = {}
-    for mode, name in ((m.PER_DEGREE, "per_degree"), (m.FUSED, "fused"), (m.FUSED_PERSISTENT, "fused_persistent")):
+    for mode, name in ((m.PER_DEGREE, "per_degree"), (m.FUSED, "fused")):
         m.set_launch(mode)
         modes[name] = m.run_timed(32) / 32
     m.set_launch(chosen)
@@ -141,7 +141,7 @@ def run_multi(steps, warmup, n_inst=256):
     variants = {p: m.variant(p) for p in sorted(degrees)}
     m.close()
     gbs = nbytes / (ms / steps / 1e3) / 1e9
-    mode_name = {0: "per_degree", 1: "fused", 2: "fused_persistent"}[chosen]
+    mode_name = {0: "per_degree", 1: "fused"}[chosen]
     return dict(value=ndof * steps / (ms / 1e3), ms_per_step=ms / steps, instances=n_inst, elements=ne_tot,
                 dof=ndof, degrees=degrees, autotuned_variant=variants, launch_mode=mode_name,
                 ms_per_step_by_mode=modes, setup_s=setup,

@@ -283,13 +283,12 @@ int  tsp_multi_run(tsp_multi *m, int nsteps, int *active);
 int  tsp_multi_run_timed(tsp_multi *m, int nsteps, float *ms);
 /* How a step is launched:
  *   TSP_MULTI_PER_DEGREE (default): one kernel per degree present, each with the
- *     register budget of its own degree;
+ *     register budget of its own degree, side by side in the captured graph;
  *   TSP_MULTI_FUSED: one kernel for all degrees, one element per wave, largest
- *     degree first;
- *   TSP_MULTI_FUSED_PERSISTENT: the same kernel with resident waves looping over
- *     the elements.
+ *     degree first (one launch per step, at the occupancy of the largest
+ *     degree's registers).
  * Results are bitwise the same in every mode. */
-enum { TSP_MULTI_PER_DEGREE = 0, TSP_MULTI_FUSED = 1, TSP_MULTI_FUSED_PERSISTENT = 2 };
+enum { TSP_MULTI_PER_DEGREE = 0, TSP_MULTI_FUSED = 1 };
 int  tsp_multi_set_launch(tsp_multi *m, int mode);
 int  tsp_multi_launch(const tsp_multi *m);
 /* Choose, per degree, the fastest launch variant of the element step by timing

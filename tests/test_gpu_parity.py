@@ -769,7 +769,7 @@ def test_multi_graph_and_plain_launches_agree_and_autotune():
     m.autotune(3)
     say("  autotuned variants: " + ", ".join(f"p{N}:{m.variant(N)}" for N in range(1, 9)) +
         f"; launch mode {m.launch()}")
-    assert m.launch() in (capi.Multi.PER_DEGREE, capi.Multi.FUSED, capi.Multi.FUSED_PERSISTENT)
+    assert m.launch() in (capi.Multi.PER_DEGREE, capi.Multi.FUSED)
     while m.run(9):
         pass
     for i, p, s0 in zip(ids, probs, res[0]):
@@ -812,20 +812,20 @@ def test_multi_every_launch_variant():
 
 
 def test_multi_launch_modes_are_bitwise_equal():
-    """The fused kernel (every degree in one launch, one-shot or persistent)
-    runs each element through the same step as the per-degree kernels: the eight
+    """The fused kernel (every degree in one launch) runs each element through
+    the same step as the per-degree kernels: the eight
     mixed instances end bit for bit equal in every launch mode, and switching
     mode mid-run changes nothing either."""
     probs = _mixed_set()
     res = {}
-    for mode in (capi.Multi.PER_DEGREE, capi.Multi.FUSED, capi.Multi.FUSED_PERSISTENT, "switch"):
+    for mode in (capi.Multi.PER_DEGREE, capi.Multi.FUSED, "switch"):
         m = capi.Multi(chunk=8)
         ids = [m.add(W.prepare_instance(p), p.nsteps) for p in probs]
         if mode == "switch":
             k = 0
             while m.run(5):
                 k += 1
-                m.set_launch(k % 3)
+                m.set_launch(k % 2)
         else:
             m.set_launch(mode)
             assert m.launch() == mode
@@ -833,12 +833,12 @@ def test_multi_launch_modes_are_bitwise_equal():
                 pass
         res[mode] = [(m.engine(i).seismograms(), m.engine(i).get_u()) for i in ids]
         m.close()
-    for mode in (capi.Multi.FUSED, capi.Multi.FUSED_PERSISTENT, "switch"):
+    for mode in (capi.Multi.FUSED, "switch"):
         for (s0, u0), (s1, u1) in zip(res[capi.Multi.PER_DEGREE], res[mode]):
             np.testing.assert_array_equal(s0, s1)
             np.testing.assert_array_equal(u0, u1)
     with pytest.raises(capi.EngineError):
-        capi.Multi(chunk=8).set_launch(7)
+        capi.Multi(chunk=8).set_launch(2)     # the persistent fused form was removed
 
 
 def test_multi_shared_operator_and_remove():

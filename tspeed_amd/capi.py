@@ -466,7 +466,7 @@ class Engine:
 class Multi:
     """A set of independent simulations stepped together (tsp_multi_*)."""
 
-    PER_DEGREE, FUSED, FUSED_PERSISTENT = 0, 1, 2
+    PER_DEGREE, FUSED = 0, 1
 
     def __init__(self, device=0, chunk=32):
         self._L = lib()
@@ -526,8 +526,8 @@ class Multi:
         self._ck(self._L.tsp_multi_set_variant(self._h, int(degree), int(variant)))
 
     def set_launch(self, mode):
-        """PER_DEGREE (one kernel per degree), FUSED or FUSED_PERSISTENT (one
-        kernel for every degree)."""
+        """PER_DEGREE (one kernel per degree) or FUSED (one kernel for every
+        degree)."""
         self._ck(self._L.tsp_multi_set_launch(self._h, int(mode)))
 
     def launch(self):

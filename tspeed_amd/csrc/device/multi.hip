@@ -17,15 +17,20 @@
 // own step count; finished ones are dropped from the items at the next chunk.
 //
 // The other launch mode, k_multi_fused, is one kernel for every degree: one
-// item list, largest degree first, each item dispatched on its instance's
-// degree to the same element step.
+// item list, largest degree first, one item per wave, each dispatched on its
+// instance's degree to the same element step.  A kernel is allocated the
+// registers of its largest path, so the small degrees run there at the
+// occupancy of the largest; the per-degree kernels stay the default.  (A
+// persistent form of the fused kernel, resident waves looping over the items,
+// held 169 VGPRs and 3 waves per SIMD and stepped the 256-mesh set in 24.8 ms
+// against 14.1 ms one-shot and 11.5 ms per degree; it was removed.)
 //
 // Autotuning: per degree, tsp_multi_autotune times the launch variants on real
 // steps of that degree's instances and keeps the fastest: persistent loop or
 // one item per wave, and for d <= 32 also the lane mapping (column groups of
 // 16/32 lanes reduced by shuffles, or one row per lane across the wave).  Then
 // it times one graph chunk of the whole set in each launch mode (per-degree
-// kernels, fused one-shot, fused persistent) and keeps the fastest.
+// kernels, fused) and keeps the fastest.
 #include "../../../include/tspeed_amd.h"
 #include "internal.hpp"
 #include "step_common.hpp"
@@ -302,28 +307,18 @@ __device__ inline void fused_dispatch(const MInst &I, int e, int lane, float (*x
     }
 }
 
-template <bool PERSIST>
 __global__ __launch_bounds__(64 * kMWaves) void k_multi_fused(const MInst *insts, const int2 *items, int nitems)
 {
     __shared__ __attribute__((aligned(16))) float xs_all[kMWaves][4][kMaxDP];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     float(*xs)[kMaxDP] = &xs_all[wave][0];
-    if (PERSIST) {
-        for (int it = blockIdx.x * kMWaves + wave; it < nitems; it += gridDim.x * kMWaves) {
-            const int2 t = items[it];
-            const MInst I = insts[t.x];
-            if (!I.live || I.steps >= I.nsteps) continue;
-            fused_dispatch(I, t.y, lane, xs);
-        }
-    } else {
-        const int it = blockIdx.x * kMWaves + wave;
-        if (it >= nitems) return;
-        const int2 t = items[it];
-        const MInst I = insts[t.x];
-        if (!I.live || I.steps >= I.nsteps) return;
-        fused_dispatch(I, t.y, lane, xs);
-    }
+    const int it = blockIdx.x * kMWaves + wave;
+    if (it >= nitems) return;
+    const int2 t = items[it];
+    const MInst I = insts[t.x];
+    if (!I.live || I.steps >= I.nsteps) return;
+    fused_dispatch(I, t.y, lane, xs);
 }
 
 template <int P>
@@ -511,16 +506,8 @@ void rebuild(tsp_multi *m)
         g.variant = 0;
         g.off = 0;
         g.count = m->nitems;
-        const bool persist = m->launch == TSP_MULTI_FUSED_PERSISTENT;
-        g.kern = persist ? reinterpret_cast<const void *>(&k_multi_fused<true>)
-                         : reinterpret_cast<const void *>(&k_multi_fused<false>);
-        const int want = (g.count + kMWaves - 1) / kMWaves;
-        g.grid = want;
-        if (persist) {
-            int per_cu = 0;
-            mhip(m, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, g.kern, 64 * kMWaves, 0), "occupancy");
-            g.grid = std::max(1, std::min(want, std::max(1, per_cu) * cus));
-        }
+        g.kern = reinterpret_cast<const void *>(&k_multi_fused);
+        g.grid = (g.count + kMWaves - 1) / kMWaves;
         m->ranges.push_back(g);
     }
     for (int q = 0; q < m->nitems && !fused;) {
@@ -822,7 +809,7 @@ int tsp_multi_autotune(tsp_multi *m, int steps_per_trial)
             if (enough) {
                 float best = 0;
                 int bestm = TSP_MULTI_PER_DEGREE;
-                for (int mode : {TSP_MULTI_PER_DEGREE, TSP_MULTI_FUSED, TSP_MULTI_FUSED_PERSISTENT}) {
+                for (int mode : {TSP_MULTI_PER_DEGREE, TSP_MULTI_FUSED}) {
                     m->launch = mode;
                     m->dirty = true;
                     rebuild(m);
@@ -854,7 +841,7 @@ int tsp_multi_autotune(tsp_multi *m, int steps_per_trial)
 int tsp_multi_set_launch(tsp_multi *m, int mode)
 {
     return mguard(m, [&] {
-        if (mode != TSP_MULTI_PER_DEGREE && mode != TSP_MULTI_FUSED && mode != TSP_MULTI_FUSED_PERSISTENT)
+        if (mode != TSP_MULTI_PER_DEGREE && mode != TSP_MULTI_FUSED)
             mfail(m, TSP_ERR_ARG, "unknown launch mode %d", mode);
         if (mode != m->launch) {
             sync_progress(m);
