@@ -166,7 +166,7 @@ struct BatchArgs {
 };
 int launch_ctr_set(int *ctr, int value, hipStream_t s);
 int launch_ctr_add(int *ctr, int value, hipStream_t s);
-int batch_kp(int P);                 // padded rows per element (multiple of 32)
+int batch_kp(int P);                 // state rows per element (d padded to a multiple of 16)
 size_t batch_block_floats(int P);    // floats of one element's four packed blocks
 int launch_step_b32(int P, int ncols, const BatchArgs &a, hipStream_t s);
 int launch_pack_mfma(int P, int ne, const int4 *meta, const double *src, float *dst, hipStream_t s);

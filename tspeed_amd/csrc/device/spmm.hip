@@ -14,7 +14,10 @@
 // A[l%32][l/32], B lane l holds B[l/32][l%32], result register r of lane l
 // holds C[8*(r/4) + 4*(l/32) + r%4][l%32].
 //
-// Layouts (rows padded to KP = 32*MT with zeros; KB = KP/8 row octets):
+// Layouts (block rows padded to 32*MT with zeros -- the MFMA output tile --,
+// the contraction and the state rows only to KB = 2*ceil(d/16) octets, the
+// 16-row K-chunk of the BF16 MFMA; e.g. p = 5, d = 42: 64 x 48 blocks, 48 state
+// rows, instead of 64 x 64 and 64):
 //   state     u[((e*KB + i/8)*R + j)*8 + i%8]    row i of element e, column j.
 //             A lane's four result registers 4g..4g+3 are rows 8g+4h..8g+4h+3
 //             (h = lane/32) of one column: 16 contiguous bytes.
@@ -70,9 +73,8 @@ __device__ inline int xcd_remap_b(int bid, int nwg)
 template <int P>
 struct BGeom {
     static constexpr int NLN = Geom<P>::NLN, D = Geom<P>::D;
-    static constexpr int MT = (D + 31) / 32;  // 32-row tiles
-    static constexpr int KP = 32 * MT;        // padded rows / contraction length
-    static constexpr int KB = KP / 8;         // row octets per element
+    static constexpr int MT = (D + 31) / 32;  // 32-row output tiles
+    static constexpr int KB = 2 * ((D + 15) / 16);  // contraction / state row octets (even)
     // where row NLN (mode 0 of the second component) sits: row octet, the half
     // of the wave holding it in a 16-byte piece, and its place in the piece /
     // in the octet
@@ -839,8 +841,12 @@ int launch_ctr_add(int *ctr, int value, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int batch_kp(int P) { return 32 * ((2 * nln_of(P) + 31) / 32); }
-size_t batch_block_floats(int P) { return (size_t)4 * batch_kp(P) / 32 * (batch_kp(P) / 8) * 256; }
+int batch_kp(int P) { return 16 * ((2 * nln_of(P) + 15) / 16); }
+size_t batch_block_floats(int P)
+{
+    const int mt = (2 * nln_of(P) + 31) / 32, kb = batch_kp(P) / 8;
+    return (size_t)4 * mt * kb * 256;
+}
 
 int launch_step_b16x3(int P, int ncols, const BatchArgs &a, hipStream_t s)
 {

@@ -67,9 +67,9 @@ int columns_for(int nsrc) { return nsrc <= 8 ? 8 : nsrc <= 16 ? 16 : nsrc <= 32 
 long long footprint_columns(const Request &r)
 {
     const long long nln = (long long)(r.N + 1) * (r.N + 2) / 2, d = 2 * nln;
-    const long long kp = 32 * ((d + 31) / 32), R = columns_for(r.nsrc);
+    const long long mp = 32 * ((d + 31) / 32), kp = 16 * ((d + 15) / 16), R = columns_for(r.nsrc);
     const long long ns = 1 + r.nsteps / std::max(r.stride, 1);
-    return (long long)r.ne * 4 * kp * kp * 4          // MFMA operator
+    return (long long)r.ne * 4 * mp * kp * 4          // MFMA operator (rows to 32, columns to 16)
            + (long long)r.ne * kp * R * (4 * 4 + 2 * 8)  // FP32 state x4, FP64 staging x2
            + ns * r.nrec * R * 16                     // seismograms of every column
            + (long long)(r.nsteps + 1) * r.nsrc * 16  // force table
