@@ -75,11 +75,39 @@ struct BGeom {
     static constexpr int NLN = Geom<P>::NLN, D = Geom<P>::D;
     static constexpr int MT = (D + 31) / 32;  // 32-row output tiles
     static constexpr int KB = 2 * ((D + 15) / 16);  // contraction / state row octets (even)
+    // Half last tile: when at most 16 rows of the last 32-row tile are real
+    // (p = 1, 2, 5, 7) its operator fragments hold only the lanes of rows
+    // 0..15 (32 of 64 lanes: lane l -> 16 (l / 32) + l % 32); the other lanes
+    // feed zeros from registers.  p = 5: 48 x 48 stored entries per block
+    // instead of 64 x 48.
+    static constexpr bool HL = D - 32 * (MT - 1) <= 16;
+    // operator block size and fragment offsets, in 16-byte units
+    static constexpr int BLK4 = HL ? (MT - 1) * KB * 64 + KB * 32 : MT * KB * 64;
+    static constexpr int BLKF = 4 * BLK4;     // floats per block
+    __host__ __device__ static constexpr bool half(int mt) { return HL && mt == MT - 1; }
+    // FP32 A-fragment (mt, kb)
+    __host__ __device__ static constexpr int frag(int mt, int kb) { return (mt * KB + kb) * 64 - (half(mt) ? kb * 32 : 0); }
+    // BF16 hi/lo fragment pair (mt, K-chunk c)
+    __host__ __device__ static constexpr int frag16(int mt, int c) { return (mt * (KB / 2) + c) * 128 - (half(mt) ? c * 64 : 0); }
     // where row NLN (mode 0 of the second component) sits: row octet, the half
     // of the wave holding it in a 16-byte piece, and its place in the piece /
     // in the octet
     static constexpr int KBN = NLN / 8, HN = (NLN % 8) / 4, QN = NLN % 4, Q8N = NLN % 8;
 };
+
+// Lane l's 16-byte piece of a (possibly half) fragment: full fragments hold all
+// 64 lanes; half ones only lanes with l % 32 < 16, stored compactly, and the
+// other lanes read zeros (their load address is clamped into the fragment).
+template <typename G>
+__device__ inline f32x4 frag_ld(const f32x4 *base, int mt, int lane)
+{
+    if (G::half(mt)) {
+        const int r = lane & 31;
+        const f32x4 v = base[(lane >> 5) * 16 + (r < 16 ? r : 15)];
+        return r < 16 ? v : f32x4{};
+    }
+    return base[lane];
+}
 
 // Translation-free products (the element's mode-0 coefficients, rows 0 and
 // NLN of its own columns, subtracted from those rows of every slot; exact in
@@ -294,12 +322,12 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
     f32x4 wv[KB][NT], cp[KB][NT];
     const size_t ebase = (size_t)e * KB * R * 8 + lofs;
     auto load = [&](int s, int buf) {
-        const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * MT * KB * 256);
+        const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * G::BLKF);
         const float *X = a.uhi + (size_t)nbs[s] * KB * R * 8 + lofs;
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-            for (int mt = 0; mt < MT; ++mt) av[buf][kb][mt] = Af[(mt * KB + kb) * 64 + lane];
+            for (int mt = 0; mt < MT; ++mt) av[buf][kb][mt] = frag_ld<G>(Af + G::frag(mt, kb), mt, lane);
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
@@ -418,14 +446,14 @@ __device__ inline void step_slots(BatchArgs &a)
             }
     };
     if (act) {
-        const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * MT * KB * 256);
+        const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * G::BLKF);
         if constexpr (!BF) {
             const float *X = a.uhi + (size_t)nb * KB * R * 8 + lofs;
             f32x4 av[KB][MT], bv[KB];
 #pragma unroll
             for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-                for (int mt = 0; mt < MT; ++mt) av[kb][mt] = Af[(mt * KB + kb) * 64 + lane];
+                for (int mt = 0; mt < MT; ++mt) av[kb][mt] = frag_ld<G>(Af + G::frag(mt, kb), mt, lane);
 #pragma unroll
             for (int kb = 0; kb < KB; ++kb) {
                 const f32x4 x = *reinterpret_cast<const f32x4 *>(X + (size_t)kb * R * 8);
@@ -454,8 +482,8 @@ __device__ inline void step_slots(BatchArgs &a)
             for (int c = 0; c < KC; ++c)
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt) {
-                    ah[c][mt] = Af[(mt * KC + c) * 128 + lane];
-                    al[c][mt] = Af[(mt * KC + c) * 128 + 64 + lane];
+                    ah[c][mt] = frag_ld<G>(Af + G::frag16(mt, c), mt, lane);
+                    al[c][mt] = frag_ld<G>(Af + G::frag16(mt, c) + (G::half(mt) ? 32 : 64), mt, lane);
                 }
 #pragma unroll
             for (int c = 0; c < KC; ++c) {
@@ -596,14 +624,14 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3(BatchArgs a)
     f32x4 uhv[KB][NT], wv[KB][NT], cp[KB][NT];
     const size_t ebase = (size_t)e * KB * R * 8 + lofs;
     auto load = [&](int s, int buf) {
-        const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * MT * KB * 256);
+        const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * G::BLKF);
         const float *X = a.uhi + (size_t)nbs[s] * KB * R * 8 + bofs;
 #pragma unroll
         for (int c = 0; c < KC; ++c)
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
-                ah[buf][c][mt] = Af[(mt * KC + c) * 128 + lane];
-                al[buf][c][mt] = Af[(mt * KC + c) * 128 + 64 + lane];
+                ah[buf][c][mt] = frag_ld<G>(Af + G::frag16(mt, c), mt, lane);
+                al[buf][c][mt] = frag_ld<G>(Af + G::frag16(mt, c) + (G::half(mt) ? 32 : 64), mt, lane);
             }
 #pragma unroll
         for (int c = 0; c < KC; ++c) {
@@ -666,12 +694,13 @@ __global__ void k_pack_bf16x3(int ne, const int4 *meta, const double *src, __bf1
     using G = BGeom<P>;
     constexpr int D = G::D, MT = G::MT, KC = G::KB / 2;
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;   // one (lane, element-of-8) pair
-    const size_t per = (size_t)MT * KC * 512;
+    const size_t per = (size_t)MT * KC * 512;   // (lane, element) pairs of a full-layout block
     if (q >= (size_t)ne * 4 * per) return;
     const size_t b = q / per;
     const int rem = (int)(q % per);
     const int frag = rem / 512, l = (rem % 512) / 8, i = rem % 8;
     const int mt = frag / KC, c = frag % KC;
+    if (G::half(mt) && l % 32 >= 16) return;    // rows 16..31 of a half tile are not stored
     const int row = mt * 32 + l % 32, col = c * 16 + 8 * (l / 32) + i;
     const int slot = (int)(b % 4);
     const int4 mm = meta[b / 4];
@@ -680,9 +709,12 @@ __global__ void k_pack_bf16x3(int ne, const int4 *meta, const double *src, __bf1
     if (row < D && col < D && nb >= 0) v = src[b * D * D + (size_t)(col / 2) * 2 * D + row * 2 + (col & 1)];
     const __bf16 hi = (__bf16)(float)v;
     const __bf16 lo = (__bf16)(float)(v - (double)(float)hi);
-    __bf16 *f = dst + b * (per * 2) + (size_t)frag * 1024;
-    f[l * 8 + i] = hi;
-    f[512 + l * 8 + i] = lo;
+    // 8 BF16 per 16-byte unit; a pair holds the hi lanes, then the lo lanes
+    const int lanes = G::half(mt) ? 32 : 64;
+    const int lc = G::half(mt) ? (l / 32) * 16 + l % 32 : l;
+    __bf16 *f = dst + b * G::BLKF * 2 + (size_t)G::frag16(mt, c) * 8;
+    f[lc * 8 + i] = hi;
+    f[lanes * 8 + lc * 8 + i] = lo;
 }
 
 // FP64 column-pair operator -> FP32 A-fragment order with zero padding; the
@@ -693,19 +725,21 @@ __global__ void k_pack_mfma(int ne, const int4 *meta, const double *src, float *
     using G = BGeom<P>;
     constexpr int D = G::D, MT = G::MT, KB = G::KB;
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t per = (size_t)MT * KB * 256;
+    const size_t per = (size_t)MT * KB * 256;   // floats of a full-layout block
     if (q >= (size_t)ne * 4 * per) return;
     const size_t b = q / per;
     const int rem = (int)(q % per);
     const int frag = rem / 256, l = (rem % 256) / 4, t = rem % 4;
     const int mt = frag / KB, kb = frag % KB;
+    if (G::half(mt) && l % 32 >= 16) return;    // rows 16..31 of a half tile are not stored
     const int row = mt * 32 + l % 32, col = kb * 8 + 4 * (l / 32) + t;
     const int slot = (int)(b % 4);
     const int4 mm = meta[b / 4];
     const int nb = slot == 0 ? 0 : (slot == 1 ? mm.x : (slot == 2 ? mm.y : mm.z));
     float v = 0.f;
     if (row < D && col < D && nb >= 0) v = (float)src[b * D * D + (size_t)(col / 2) * 2 * D + row * 2 + (col & 1)];
-    dst[q] = v;
+    const int lc = G::half(mt) ? (l / 32) * 16 + l % 32 : l;
+    dst[b * G::BLKF + ((size_t)G::frag(mt, kb) + lc) * 4 + t] = v;
 }
 
 __device__ inline size_t oct_index(int e, int i, int j, int KB, int R)
@@ -844,8 +878,13 @@ int launch_ctr_add(int *ctr, int value, hipStream_t s)
 int batch_kp(int P) { return 16 * ((2 * nln_of(P) + 15) / 16); }
 size_t batch_block_floats(int P)
 {
-    const int mt = (2 * nln_of(P) + 31) / 32, kb = batch_kp(P) / 8;
-    return (size_t)4 * mt * kb * 256;
+    switch (P) {
+    case 1: return (size_t)4 * BGeom<1>::BLKF; case 2: return (size_t)4 * BGeom<2>::BLKF;
+    case 3: return (size_t)4 * BGeom<3>::BLKF; case 4: return (size_t)4 * BGeom<4>::BLKF;
+    case 5: return (size_t)4 * BGeom<5>::BLKF; case 6: return (size_t)4 * BGeom<6>::BLKF;
+    case 7: return (size_t)4 * BGeom<7>::BLKF; case 8: return (size_t)4 * BGeom<8>::BLKF;
+    default: return 0;
+    }
 }
 
 int launch_step_b16x3(int P, int ncols, const BatchArgs &a, hipStream_t s)
