@@ -75,36 +75,38 @@ struct BGeom {
     static constexpr int NLN = Geom<P>::NLN, D = Geom<P>::D;
     static constexpr int MT = (D + 31) / 32;  // 32-row output tiles
     static constexpr int KB = 2 * ((D + 15) / 16);  // contraction / state row octets (even)
-    // Half last tile: when at most 16 rows of the last 32-row tile are real
-    // (p = 1, 2, 5, 7) its operator fragments hold only the lanes of rows
-    // 0..15 (32 of 64 lanes: lane l -> 16 (l / 32) + l % 32); the other lanes
-    // feed zeros from registers.  p = 5: 48 x 48 stored entries per block
-    // instead of 64 x 48.
-    static constexpr bool HL = D - 32 * (MT - 1) <= 16;
+    // Partial last tile: the last 32-row tile stores only LR rows, its real
+    // rows rounded up to a multiple of 8 (lanes l with l % 32 < LR, kept
+    // compactly: lane l -> LR (l / 32) + l % 32); the other lanes feed zeros
+    // from registers.  p = 3: 24 x 32 entries per block instead of 32 x 32;
+    // p = 5: 48 x 48 instead of 64 x 48; p = 6: 56 x 64 instead of 64 x 64.
+    static constexpr int LR = 8 * ((D - 32 * (MT - 1) + 7) / 8);
     // operator block size and fragment offsets, in 16-byte units
-    static constexpr int BLK4 = HL ? (MT - 1) * KB * 64 + KB * 32 : MT * KB * 64;
+    static constexpr int BLK4 = (MT - 1) * KB * 64 + KB * 2 * LR;
     static constexpr int BLKF = 4 * BLK4;     // floats per block
-    __host__ __device__ static constexpr bool half(int mt) { return HL && mt == MT - 1; }
+    __host__ __device__ static constexpr bool part(int mt) { return LR < 32 && mt == MT - 1; }
+    // 16-byte units of one FP32 fragment of row tile mt (one BF16 hi or lo half)
+    __host__ __device__ static constexpr int fsize(int mt) { return mt == MT - 1 ? 2 * LR : 64; }
     // FP32 A-fragment (mt, kb)
-    __host__ __device__ static constexpr int frag(int mt, int kb) { return (mt * KB + kb) * 64 - (half(mt) ? kb * 32 : 0); }
-    // BF16 hi/lo fragment pair (mt, K-chunk c)
-    __host__ __device__ static constexpr int frag16(int mt, int c) { return (mt * (KB / 2) + c) * 128 - (half(mt) ? c * 64 : 0); }
+    __host__ __device__ static constexpr int frag(int mt, int kb) { return mt * KB * 64 + kb * fsize(mt); }
+    // BF16 hi/lo fragment pair (mt, K-chunk c): hi at frag16, lo at frag16 + fsize(mt)
+    __host__ __device__ static constexpr int frag16(int mt, int c) { return mt * KB * 64 + c * 2 * fsize(mt); }
     // where row NLN (mode 0 of the second component) sits: row octet, the half
     // of the wave holding it in a 16-byte piece, and its place in the piece /
     // in the octet
     static constexpr int KBN = NLN / 8, HN = (NLN % 8) / 4, QN = NLN % 4, Q8N = NLN % 8;
 };
 
-// Lane l's 16-byte piece of a (possibly half) fragment: full fragments hold all
-// 64 lanes; half ones only lanes with l % 32 < 16, stored compactly, and the
-// other lanes read zeros (their load address is clamped into the fragment).
+// Lane l's 16-byte piece of a fragment: full fragments hold all 64 lanes; the
+// last tile's only lanes with l % 32 < LR, stored compactly, and the other
+// lanes read zeros (their load address is clamped into the fragment).
 template <typename G>
 __device__ inline f32x4 frag_ld(const f32x4 *base, int mt, int lane)
 {
-    if (G::half(mt)) {
+    if (G::part(mt)) {
         const int r = lane & 31;
-        const f32x4 v = base[(lane >> 5) * 16 + (r < 16 ? r : 15)];
-        return r < 16 ? v : f32x4{};
+        const f32x4 v = base[(lane >> 5) * G::LR + (r < G::LR ? r : G::LR - 1)];
+        return r < G::LR ? v : f32x4{};
     }
     return base[lane];
 }
@@ -483,7 +485,7 @@ __device__ inline void step_slots(BatchArgs &a)
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt) {
                     ah[c][mt] = frag_ld<G>(Af + G::frag16(mt, c), mt, lane);
-                    al[c][mt] = frag_ld<G>(Af + G::frag16(mt, c) + (G::half(mt) ? 32 : 64), mt, lane);
+                    al[c][mt] = frag_ld<G>(Af + G::frag16(mt, c) + G::fsize(mt), mt, lane);
                 }
 #pragma unroll
             for (int c = 0; c < KC; ++c) {
@@ -631,7 +633,7 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3(BatchArgs a)
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 ah[buf][c][mt] = frag_ld<G>(Af + G::frag16(mt, c), mt, lane);
-                al[buf][c][mt] = frag_ld<G>(Af + G::frag16(mt, c) + (G::half(mt) ? 32 : 64), mt, lane);
+                al[buf][c][mt] = frag_ld<G>(Af + G::frag16(mt, c) + G::fsize(mt), mt, lane);
             }
 #pragma unroll
         for (int c = 0; c < KC; ++c) {
@@ -700,7 +702,7 @@ __global__ void k_pack_bf16x3(int ne, const int4 *meta, const double *src, __bf1
     const int rem = (int)(q % per);
     const int frag = rem / 512, l = (rem % 512) / 8, i = rem % 8;
     const int mt = frag / KC, c = frag % KC;
-    if (G::half(mt) && l % 32 >= 16) return;    // rows 16..31 of a half tile are not stored
+    if (G::part(mt) && l % 32 >= G::LR) return;  // rows past LR of the last tile are not stored
     const int row = mt * 32 + l % 32, col = c * 16 + 8 * (l / 32) + i;
     const int slot = (int)(b % 4);
     const int4 mm = meta[b / 4];
@@ -710,8 +712,8 @@ __global__ void k_pack_bf16x3(int ne, const int4 *meta, const double *src, __bf1
     const __bf16 hi = (__bf16)(float)v;
     const __bf16 lo = (__bf16)(float)(v - (double)(float)hi);
     // 8 BF16 per 16-byte unit; a pair holds the hi lanes, then the lo lanes
-    const int lanes = G::half(mt) ? 32 : 64;
-    const int lc = G::half(mt) ? (l / 32) * 16 + l % 32 : l;
+    const int lanes = G::fsize(mt);
+    const int lc = G::part(mt) ? (l / 32) * G::LR + l % 32 : l;
     __bf16 *f = dst + b * G::BLKF * 2 + (size_t)G::frag16(mt, c) * 8;
     f[lc * 8 + i] = hi;
     f[lanes * 8 + lc * 8 + i] = lo;
@@ -731,14 +733,14 @@ __global__ void k_pack_mfma(int ne, const int4 *meta, const double *src, float *
     const int rem = (int)(q % per);
     const int frag = rem / 256, l = (rem % 256) / 4, t = rem % 4;
     const int mt = frag / KB, kb = frag % KB;
-    if (G::half(mt) && l % 32 >= 16) return;    // rows 16..31 of a half tile are not stored
+    if (G::part(mt) && l % 32 >= G::LR) return;  // rows past LR of the last tile are not stored
     const int row = mt * 32 + l % 32, col = kb * 8 + 4 * (l / 32) + t;
     const int slot = (int)(b % 4);
     const int4 mm = meta[b / 4];
     const int nb = slot == 0 ? 0 : (slot == 1 ? mm.x : (slot == 2 ? mm.y : mm.z));
     float v = 0.f;
     if (row < D && col < D && nb >= 0) v = (float)src[b * D * D + (size_t)(col / 2) * 2 * D + row * 2 + (col & 1)];
-    const int lc = G::half(mt) ? (l / 32) * 16 + l % 32 : l;
+    const int lc = G::part(mt) ? (l / 32) * G::LR + l % 32 : l;
     dst[b * G::BLKF + ((size_t)G::frag(mt, kb) + lc) * 4 + t] = v;
 }
 
