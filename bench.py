@@ -282,14 +282,16 @@ def cpu_baseline(sample_steps, runs=3):
     nthr = O.threads(min(ncpu, int(os.environ.get("OMP_NUM_THREADS", ncpu))), omp=True)
     _, vm, rm = timed(True, 3 * sample_steps)
     model = _cpu_model()
-    return dict(value=v1, unit="DOF*steps/s", cores=1, kind="port",
+    return dict(value=v1, unit="DOF*steps/s", cores=1, kind="port", runs=[float(r) for r in r1],
+                spread=[float(min(r1)), float(max(r1))],
                 sample=f"oracle (FP64 C restatement of LeapFrog::step, -O3 -msse4, 1 thread) on the layered workload's "
                        f"model at 40x20 squares ({p.ne} triangles, p=6, {p.ndof} DOF), {sample_steps} steps; "
                        f"median of {runs} runs ({', '.join(f'{r:.3g}' for r in r1)})",
                 cpu_model=model, nproc=ncpu,
-                openmp=dict(value=vm, threads=nthr, runs=[float(r) for r in rm],
+                openmp=dict(value=vm, threads=nthr, runs=[float(r) for r in rm], spread=[float(min(rm)), float(max(rm))],
                             sample=f"same source built with -fopenmp (rows split over {nthr} threads), "
-                                   f"{3 * sample_steps} steps, median of {runs} runs"))
+                                   f"{3 * sample_steps} steps, median of {runs} runs; the host's other "
+                                   f"tenants share these cores, so the spread is part of the figure"))
 
 
 def copy_calibration(gib=2, reps=5):
