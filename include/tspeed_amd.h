@@ -40,7 +40,7 @@ enum tsp_precision {
     TSP_BF16X3 = 2  /* as TSP_FP32C, with each block product split into three BF16
                        products accumulated in FP32 (multi-column contexts only);
                        the split keeps ~16 bits per operand: seismograms drift to
-                       ~2-6e-3 of FP64 over 3000 steps (TSP_FP32C: <= 1e-4)      */
+                       ~4-9e-4 of FP64 over 3000 steps (TSP_FP32C: < 1e-5)       */
 };
 
 /* Status bits reported by tsp_status (no counterpart in the reference, which has
