@@ -14,6 +14,10 @@
   1000 steps: FP32c against the FP64 path (itself the oracle to 1e-12) at
   1e-4 in the final displacement and in the seismograms.
 
+- Request loop: six requests of the serving workload as bench.py draws them
+  (p = 3..5, 1-14 sources, 653-4136 steps), mixed with Poisson arrivals: each
+  bitwise its own run alone; two re-run in FP64 source by source within 1e-4.
+
 Error measure as in test_gpu_parity: relative L2 over the whole array;
 per-receiver errors normalised by the whole seismogram norm."""
 import copy
@@ -145,3 +149,40 @@ def test_layered_fp32c_accuracy_against_fp64():
     e_s, e_r = seis_err(s32, s64)
     say(f"  FP32c vs FP64 after {p.nsteps} steps: u {e_u:.2e} seismograms {e_s:.2e} worst receiver {e_r:.2e}")
     assert e_u <= 1e-4 and e_s <= 1e-4 and e_r <= 1e-4
+
+
+# requests of the serving workload (seed 5) as bench.py draws them: p = 3, 4, 5;
+# 1-15 sources (8- and 16-column contexts); 653-4136 steps; 800-3200 triangles
+SERVE_KS = (0, 1, 2, 3, 4, 7)
+
+
+def test_request_loop_as_configured():
+    """Six requests of the serving workload as configured, submitted with
+    Poisson arrivals (30 per second) into the request loop in its default
+    columns mode: each result is bit for bit the same request served alone,
+    and two of them (1 and 13 sources, p = 3 and 5), re-run source by source
+    on the FP64 path (the oracle to 1e-12), are within 1e-4."""
+    import copy
+    reqs = [W.serving_request(k) for k in SERVE_KS]
+    say("requests: " + ", ".join(f"p{p.N}/ne{p.ne}/{len(p.src_elem)} src/{p.nsteps} steps" for p in reqs))
+    srv, ids, lat, wall = W.serve(reqs, rate=30.0, seed=7)
+    mixed = [srv.result(i)[0] for i in ids]
+    say(f"  mix: latencies {np.round(lat, 3).tolist()} s, wall {wall:.2f} s")
+    srv.close()
+    for p, res in zip(reqs, mixed):
+        srv, ids1, _, _ = W.serve([p], rate=0)
+        alone = srv.result(ids1[0])[0]
+        srv.close()
+        np.testing.assert_array_equal(res, alone, err_msg=p.name)
+    for j in (4, 1):
+        p, res = reqs[SERVE_KS.index(j)], mixed[SERVE_KS.index(j)]
+        errs = []
+        for k in range(len(p.src_elem)):
+            q = copy.copy(p)
+            q.src_xy, q.f0, q.t0, q.dirs = p.src_xy[k:k + 1], p.f0[k:k + 1], p.t0[k:k + 1], p.dirs[k:k + 1]
+            q.__post_init__()
+            eng, _, s64 = W.run_engine(q, "fp64")
+            eng.close()
+            errs.append(rel_l2(res[k], s64))
+        say(f"  {p.name} p={p.N} {len(errs)} sources vs FP64: max {max(errs):.2e}")
+        assert max(errs) <= 1e-4
