@@ -93,7 +93,8 @@ __device__ inline void multi_elem(const MInst &I, int e, int lane, float (*xs)[k
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     float acc[G::RPL];
-    spmv<G, float, 4>(I.blk, e, xv, lane, acc);
+    if constexpr (G::D > 32) spmv_chunked<G>(I.blk, e, xv, lane, acc);
+    else spmv<G, float, 4>(I.blk, e, xv, lane, acc);
 
     StepArgs a{};
     a.nrec = I.nrec;
@@ -169,13 +170,21 @@ __global__ __launch_bounds__(64 * kMWaves, PERSIST ? (G::D > 64 ? 2 : 4) : 1) vo
 // One element per work-group, one slot per wave: wave s gathers slot s's
 // coefficients and streams block (e, s); waves 1..3 leave their partial sums in
 // LDS and wave 0 adds them in slot order and finishes the step as multi_elem
-// does.  Shorter streams per wave, four times the waves; the sums are added in
-// another order than in multi_elem, so this is an autotuner candidate, never
-// the default (the default keeps every instance bitwise equal to its
+// does.  Shorter streams per wave, four times the waves.  For p = 5, 6 a wave
+// sums its slot as the two chunks of Chunks and wave 0 adds all eight in chunk
+// order, bitwise as multi_elem; for the other degrees the slot sums are added
+// in another order than multi_elem's, so there this is an autotuner candidate,
+// never the default (the default keeps every instance bitwise equal to its
 // single-context FP32c run).
+// Chunked slot sums where a lane holds one row (d = 42, 56); with two rows per
+// lane (d = 72, 90) the two halves took 230-256 VGPRs against 140 for one
+// chain per slot, so those degrees keep the slot sums.
+template <typename G>
+constexpr bool kSlotChunks = G::D > 32 && G::RPL == 1;
+
 template <typename G>
 __device__ inline void multi_elem_slots(const MInst &I, int e, int lane, int s, float *x,
-                                        float (*part)[G::RPL][64])
+                                        float (*part)[2][G::RPL][64])
 {
     const int n = I.steps + 1;
     const int cur = (I.steps + 1) & 1;
@@ -203,17 +212,38 @@ __device__ inline void multi_elem_slots(const MInst &I, int e, int lane, int s, 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const float(*xv)[G::DP] = reinterpret_cast<const float(*)[G::DP]>(x);
     float acc[G::RPL];
-    spmv<G, float, 4>(I.blk, e, xv, lane, acc, s, 1);
-    if (s > 0) {
+    if constexpr (kSlotChunks<G>) {
+        float c[2][G::RPL];
 #pragma unroll
-        for (int q = 0; q < G::RPL; ++q) part[s - 1][q][lane] = acc[q];
+        for (int h = 0; h < 2; ++h) {
+            ChunkRegs<G> R;
+            chunk_load<G>(I.blk, e, s, h, lane, R);
+            chunk_fma<G>(R, x, h, c[h]);
+        }
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            if (s > 0) {
+                part[s - 1][0][q][lane] = c[0][q];
+                part[s - 1][1][q][lane] = c[1][q];
+            }
+            acc[q] = c[0][q] + c[1][q];
+        }
+    } else {
+        spmv<G, float, 4>(I.blk, e, xv, lane, acc, s, 1);
+        if (s > 0) {
+#pragma unroll
+            for (int q = 0; q < G::RPL; ++q) part[s - 1][0][q][lane] = acc[q];
+        }
     }
     __syncthreads();
     if (s > 0) return;
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int q = 0; q < G::RPL; ++q) acc[q] += part[k][q][lane];
+        for (int q = 0; q < G::RPL; ++q) {
+            acc[q] += part[k][0][q][lane];
+            if constexpr (kSlotChunks<G>) acc[q] += part[k][1][q][lane];
+        }
 
     StepArgs a{};
     a.nrec = I.nrec;
@@ -255,7 +285,7 @@ __global__ __launch_bounds__(64 * kMWaves) void k_multi32s(const MInst *insts, c
 {
     static_assert(kMWaves == 4, "one wave per slot");
     __shared__ __attribute__((aligned(16))) float xs_all[kMWaves][G::DP];
-    __shared__ float part[kMWaves - 1][G::RPL][64];
+    __shared__ float part[kMWaves - 1][2][G::RPL][64];
     const int lane = threadIdx.x & 63;
     const int s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int step = PERSIST ? gridDim.x : nitems;

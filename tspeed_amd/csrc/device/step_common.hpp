@@ -255,6 +255,99 @@ __device__ inline void spmv_cols(const T *blk, int e, int slot, const T *x, int 
     }
 }
 
+// FP32 blocks one column group wide (d > 32, p >= 5).  The row sums of an
+// element are cut into eight chunks, (slot s, half h) in that order: half 0
+// holds the first half of the block's column quads, half 1 the second half and
+// the trailing pair when d % 4 == 2.  Each chunk is an fma chain from zero and
+// the chunks' sums are added in chunk order.  Every FP32c step of these
+// degrees sums in this order, whether one wave walks the chunks (the
+// multi-instance step) or eight waves take one chunk each (the single-mesh
+// step), so both give bitwise the same state.
+template <typename G>
+struct Chunks {
+    static constexpr int NFULL = G::D / 4, TAIL = G::D % 4;
+    static constexpr int NQH = NFULL / 2;   // column quads of one half
+    static_assert(G::NG == 1 && NFULL % 2 == 0 && (TAIL == 0 || TAIL == 2), "chunked rows: d > 32");
+};
+
+// A chunk's operator entries for the lane's rows (lanes past the last row
+// re-read it; their sums are discarded).
+template <typename G>
+struct ChunkRegs {
+    vec_t<float, 4> q[Chunks<G>::NQH][G::RPL];
+    vec_t<float, 2> t[G::RPL];
+};
+
+template <typename G>
+__device__ inline void chunk_load(const float *blk, int e, int s, int h, int lane, ChunkRegs<G> &R)
+{
+    using C = Chunks<G>;
+    const float *Bs = blk + ((size_t)e * 4 + s) * G::D * G::D;
+#pragma unroll
+    for (int k = 0; k < C::NQH; ++k)
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            const int row = min(lane + 64 * q, G::D - 1);
+            R.q[k][q] = __builtin_nontemporal_load(
+                reinterpret_cast<const vec_t<float, 4> *>(Bs + ((size_t)(h * C::NQH + k) * G::D + row) * 4));
+        }
+    if constexpr (C::TAIL == 2) {
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            const int row = min(lane + 64 * q, G::D - 1);
+            R.t[q] = h == 1 ? __builtin_nontemporal_load(
+                                  reinterpret_cast<const vec_t<float, 2> *>(Bs + (size_t)C::NFULL * G::D * 4 + row * 2))
+                            : vec_t<float, 2>{};
+        }
+    }
+}
+
+// acc = chunk (s, h) of the row sums, x = slot s's gathered coefficients.
+template <typename G>
+__device__ inline void chunk_fma(const ChunkRegs<G> &R, const float *x, int h, float (&acc)[G::RPL])
+{
+    using C = Chunks<G>;
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) acc[q] = 0.f;
+#pragma unroll
+    for (int k = 0; k < C::NQH; ++k) {
+        const vec_t<float, 4> xv = *reinterpret_cast<const vec_t<float, 4> *>(&x[4 * (h * C::NQH + k)]);
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[q] = fma(R.q[k][q][t], xv[t], acc[q]);
+    }
+    if constexpr (C::TAIL == 2) {
+        if (h == 1) {
+            const vec_t<float, 2> xv = *reinterpret_cast<const vec_t<float, 2> *>(&x[4 * C::NFULL]);
+#pragma unroll
+            for (int q = 0; q < G::RPL; ++q) {
+                acc[q] = fma(R.t[q][0], xv[0], acc[q]);
+                acc[q] = fma(R.t[q][1], xv[1], acc[q]);
+            }
+        }
+    }
+}
+
+// One wave walking the eight chunks of element e (xs[s] = slot s's coefficients).
+template <typename G>
+__device__ inline void spmv_chunked(const float *blk, int e, const float (*xs)[G::DP], int lane, float (&acc)[G::RPL])
+{
+    // one slot per iteration, as spmv: the element's loads are not all live at once
+#pragma unroll 1
+    for (int s = 0; s < 4; ++s) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            ChunkRegs<G> R;
+            chunk_load<G>(blk, e, s, h, lane, R);
+            float c[G::RPL];
+            chunk_fma<G>(R, xs[s], h, c);
+#pragma unroll
+            for (int q = 0; q < G::RPL; ++q) acc[q] = (s == 0 && h == 0) ? c[q] : acc[q] + c[q];
+        }
+    }
+}
+
 // y = (invm_e * minv) w on one element's rows, in the reference's order: the
 // entries of the dense inverse mass are formed as invm_e * minv_ij, then each
 // row's sum runs over j from 0 (MyMatMultiDimBlockDiag::vecMult, MyMat.hpp:
