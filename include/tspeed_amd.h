@@ -298,9 +298,11 @@ int  tsp_multi_autotune(tsp_multi *m, int steps_per_trial);
 /* The launch variant in use for a degree: bit 0 = one row per lane across the
  * wave (else column groups of 16/32 lanes, d <= 32 only), bit 1 = one item per
  * wave or work-group (else a persistent loop), bit 2 = one element per
- * work-group with one slot per wave (bit 0 then unused).  The default (2) keeps
- * every instance bitwise equal to its single-context FP32C run; the others
- * agree with it to FP32 rounding. */
+ * work-group with one slot per wave (bit 0 then unused), bit 3 = one element
+ * per work-group of eight waves with one half slot per wave (d > 32, p >= 5
+ * only; bits 0 and 2 then unused).  The default (2) keeps every instance
+ * bitwise equal to its single-context FP32C run, and so do bit 3 and, at
+ * p = 5 and 6, bit 2; the others agree with it to FP32 rounding. */
 int  tsp_multi_variant(const tsp_multi *m, int degree);
 /* Pin the launch variant of a degree (as tsp_multi_autotune would). */
 int  tsp_multi_set_variant(tsp_multi *m, int degree, int variant);

@@ -781,33 +781,44 @@ def test_multi_graph_and_plain_launches_agree_and_autotune():
 def test_multi_every_launch_variant():
     """Every launch variant of every degree (persistent or one item per wave,
     whole-wave rows for d <= 32, one element per work-group with a slot per
-    wave): the default (one item per wave) is bitwise the single-context FP32
-    run, every other variant agrees with it to FP32 rounding."""
+    wave, or with a half slot per wave for d > 32): the default (one item per
+    wave), the half-slot variants and, at p = 5 and 6, the slot variants sum
+    each row in the same chunk order and are bitwise the single-context FP32
+    run; every other variant agrees with it to FP32 rounding."""
     probs = _mixed_set()
     degrees = sorted({p.N for p in probs})
     ref = [W.run_engine(p, "fp32c")[2] for p in probs]
-    for var in range(8):
+    for var in range(16):
         m = capi.Multi(chunk=8)
         ids = [m.add(W.prepare_instance(p), p.nsteps) for p in probs]
+        used = {}
         for N in degrees:
-            narrow = (N + 1) * (N + 2) <= 32
-            v = var if (narrow or not (var & m.ROWS) or (var & m.SLOTS)) else var & ~m.ROWS
+            wide = (N + 1) * (N + 2) > 32
+            v = var if (wide or not (var & m.CHUNKS)) else var & ~m.CHUNKS
+            if wide and (v & m.ROWS) and not (v & (m.SLOTS | m.CHUNKS)):
+                v &= ~m.ROWS
             m.set_variant(N, v)
             assert m.variant(N) == v
+            used[N] = v
         while m.run(9):
             pass
         for i, p, s1 in zip(ids, probs, ref):
             s = m.engine(i).seismograms()
-            if var == m.ONE_SHOT:
-                np.testing.assert_array_equal(s, s1)
+            v = used[p.N]
+            if v == m.ONE_SHOT or (v & m.CHUNKS) or ((v & m.SLOTS) and p.N in (5, 6)):
+                np.testing.assert_array_equal(s, s1, err_msg=f"variant {v}, p = {p.N}")
             else:
-                assert rel_l2(s, s1) <= 1e-5, (var, p.N)
+                assert rel_l2(s, s1) <= 1e-5, (v, p.N)
         m.close()
     m = capi.Multi(chunk=8)
     with pytest.raises(capi.EngineError):
         m.set_variant(6, m.ROWS)          # d = 56: no whole-wave rows
     with pytest.raises(capi.EngineError):
         m.set_variant(3, 9)
+    with pytest.raises(capi.EngineError):
+        m.set_variant(4, m.CHUNKS)        # d = 30: no half-slot chunks
+    with pytest.raises(capi.EngineError):
+        m.set_variant(6, 16)
     m.close()
 
 

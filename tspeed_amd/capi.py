@@ -519,10 +519,10 @@ class Multi:
     def variant(self, degree):
         return self._L.tsp_multi_variant(self._h, int(degree))
 
-    ROWS, ONE_SHOT, SLOTS = 1, 2, 4
+    ROWS, ONE_SHOT, SLOTS, CHUNKS = 1, 2, 4, 8
 
     def set_variant(self, degree, variant):
-        """Pin a degree's launch variant (bits ROWS | ONE_SHOT | SLOTS)."""
+        """Pin a degree's launch variant (bits ROWS | ONE_SHOT | SLOTS | CHUNKS)."""
         self._ck(self._L.tsp_multi_set_variant(self._h, int(degree), int(variant)))
 
     def set_launch(self, mode):

@@ -27,8 +27,9 @@
 //
 // Autotuning: per degree, tsp_multi_autotune times the launch variants on real
 // steps of that degree's instances and keeps the fastest: persistent loop or
-// one item per wave, and for d <= 32 also the lane mapping (column groups of
-// 16/32 lanes reduced by shuffles, or one row per lane across the wave).  Then
+// one item per wave (or work-group), one slot per wave, for d > 32 one half
+// slot per wave, and for d <= 32 also the lane mapping (column groups of 16/32
+// lanes reduced by shuffles, or one row per lane across the wave).  Then
 // it times one graph chunk of the whole set in each launch mode (per-degree
 // kernels, fused) and keeps the fastest.
 #include "../../../include/tspeed_amd.h"
@@ -65,37 +66,15 @@ struct MInst {
 constexpr int kMWaves = 4;
 constexpr int kMaxDP = Geom<8>::DP;
 
+// The end of an element step: source term, compensated update (u^{n+1} into
+// the ping-pong buffer the step writes), receivers.  rows[q] = the lane's q-th
+// row or -1, acc = B u^n of those rows, own = u^n there.
 template <typename G>
-__device__ inline void multi_elem(const MInst &I, int e, int lane, float (*xs)[kMaxDP])
+__device__ inline void finish_elem(const MInst &I, int e, int4 m, const int *rows, const float *wi, const float *cp,
+                                   const float *own, const float *acc, float sc)
 {
     const int n = I.steps + 1;               // 1-based index of this step() call
-    const int cur = (I.steps + 1) & 1;       // after the first step u^n sits in uhi1
-    const float *uin = cur ? I.uhi1 : I.uhi0;
-    float *uout = cur ? I.uhi0 : I.uhi1;
-    const int4 m = I.meta[e];
-    const int g = lane / G::RW, r = lane % G::RW;
-    float wi[G::RPL], cp[G::RPL];
-    int rows[G::RPL];
-#pragma unroll
-    for (int q = 0; q < G::RPL; ++q) {
-        const int row = r + 64 * q;
-        rows[q] = (g == 0 && row < G::D) ? row : -1;
-        const size_t idx = (size_t)e * G::D + row;
-        wi[q] = rows[q] >= 0 ? I.winc[idx] : 0.f;
-        cp[q] = rows[q] >= 0 ? I.ucomp[idx] : 0.f;
-    }
-    const float sc = I.sc[e];
-    float(*xv)[G::DP] = reinterpret_cast<float(*)[G::DP]>(&xs[0][0]);
-    float own[G::RPL];   // u^n of the lane's rows (the update's uh)
-    gather<G, float>(xv, uin, e, m, lane, true, I.tfree != 0, own);
-    // the wave reads only its own LDS slice: order its stores before its loads
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    float acc[G::RPL];
-    if constexpr (G::D > 32) spmv_chunked<G>(I.blk, e, xv, lane, acc);
-    else spmv<G, float, 4>(I.blk, e, xv, lane, acc);
-
+    float *uout = (n & 1) ? I.uhi0 : I.uhi1;
     StepArgs a{};
     a.nrec = I.nrec;
     a.src_sorted = I.src_sorted;
@@ -129,6 +108,37 @@ __device__ inline void multi_elem(const MInst &I, int e, int lane, float (*xs)[k
     }
     if (bad) atomicOr(I.flags, 1u);
     if (a.sample >= 0 && ex.w > ex.z) record<G, float>(a, ex, unew, rows, G::RPL);
+}
+
+template <typename G>
+__device__ inline void multi_elem(const MInst &I, int e, int lane, float (*xs)[kMaxDP])
+{
+    const int cur = (I.steps + 1) & 1;       // after the first step u^n sits in uhi1
+    const float *uin = cur ? I.uhi1 : I.uhi0;
+    const int4 m = I.meta[e];
+    const int g = lane / G::RW, r = lane % G::RW;
+    float wi[G::RPL], cp[G::RPL];
+    int rows[G::RPL];
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const int row = r + 64 * q;
+        rows[q] = (g == 0 && row < G::D) ? row : -1;
+        const size_t idx = (size_t)e * G::D + row;
+        wi[q] = rows[q] >= 0 ? I.winc[idx] : 0.f;
+        cp[q] = rows[q] >= 0 ? I.ucomp[idx] : 0.f;
+    }
+    const float sc = I.sc[e];
+    float(*xv)[G::DP] = reinterpret_cast<float(*)[G::DP]>(&xs[0][0]);
+    float own[G::RPL];   // u^n of the lane's rows (the update's uh)
+    gather<G, float>(xv, uin, e, m, lane, true, I.tfree != 0, own);
+    // the wave reads only its own LDS slice: order its stores before its loads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float acc[G::RPL];
+    if constexpr (G::D > 32) spmv_chunked<G>(I.blk, e, xv, lane, acc);
+    else spmv<G, float, 4>(I.blk, e, xv, lane, acc);
+    finish_elem<G>(I, e, m, rows, wi, cp, own, acc, sc);
     // the next item's gather overwrites this slice
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -142,7 +152,7 @@ __device__ inline void multi_elem(const MInst &I, int e, int lane, float (*xs)[k
 // depends on the degree -- the autotuner decides.
 template <typename G, bool PERSIST>
 __global__ __launch_bounds__(64 * kMWaves, PERSIST ? (G::D > 64 ? 2 : 4) : 1) void k_multi32c(
-    const MInst *insts, const int2 *items, int nitems)
+    const MInst *__restrict__ insts, const int2 *__restrict__ items, int nitems)
 {
     __shared__ __attribute__((aligned(16))) float xs_all[kMWaves][4][G::DP];
     const int lane = threadIdx.x & 63;
@@ -186,10 +196,8 @@ template <typename G>
 __device__ inline void multi_elem_slots(const MInst &I, int e, int lane, int s, float *x,
                                         float (*part)[2][G::RPL][64])
 {
-    const int n = I.steps + 1;
     const int cur = (I.steps + 1) & 1;
     const float *uin = cur ? I.uhi1 : I.uhi0;
-    float *uout = cur ? I.uhi0 : I.uhi1;
     const int4 m = I.meta[e];
     const int nb = s == 0 ? e : (s == 1 ? m.x : (s == 2 ? m.y : m.z));
     const int g = lane / G::RW, r = lane % G::RW;
@@ -244,44 +252,11 @@ __device__ inline void multi_elem_slots(const MInst &I, int e, int lane, int s, 
             acc[q] += part[k][0][q][lane];
             if constexpr (kSlotChunks<G>) acc[q] += part[k][1][q][lane];
         }
-
-    StepArgs a{};
-    a.nrec = I.nrec;
-    a.src_sorted = I.src_sorted;
-    a.rec_sorted = I.rec_sorted;
-    a.src_phi = I.src_phi;
-    a.wav_row = I.nsrc ? I.wav + (size_t)n * I.nsrc * 2 : nullptr;
-    a.rec_phi = I.rec_phi;
-    a.seis = I.seis;
-    const int slot = (I.stride > 0 && n % I.stride == 0) ? n / I.stride : -1;
-    a.sample = (slot >= 0 && slot < I.max_samples) ? slot : -1;
-    const int4 ex = m.w >= 0 ? I.extra[m.w] : make_int4(0, 0, 0, 0);
-    float unew[G::RPL];
-    bool bad = false;
-#pragma unroll
-    for (int q = 0; q < G::RPL; ++q) {
-        unew[q] = 0.f;
-        const int row = rows[q];
-        if (row < 0) continue;
-        const size_t idx = (size_t)e * G::D + row;
-        const float f = (ex.y > ex.x) ? (float)source_term(a, ex, row, G::NLN) : 0.f;
-        const float w = wi[q] + sc * (acc[q] - f);
-        const float uh = own[q];
-        const float y = w - cp[q];
-        const float t = uh + y;
-        // through L2 (st_state): the 256-mesh set steps ~1.3% faster; same values
-        st_state(I.ucomp + idx, (t - uh) - y);
-        st_state(uout + idx, t);
-        st_state(I.winc + idx, w);
-        unew[q] = t;
-        bad |= !isfinite(t);
-    }
-    if (bad) atomicOr(I.flags, 1u);
-    if (a.sample >= 0 && ex.w > ex.z) record<G, float>(a, ex, unew, rows, G::RPL);
+    finish_elem<G>(I, e, m, rows, wi, cp, own, acc, sc);
 }
 
 template <typename G, bool PERSIST>
-__global__ __launch_bounds__(64 * kMWaves) void k_multi32s(const MInst *insts, const int2 *items, int nitems)
+__global__ __launch_bounds__(64 * kMWaves) void k_multi32s(const MInst *__restrict__ insts, const int2 *__restrict__ items, int nitems)
 {
     static_assert(kMWaves == 4, "one wave per slot");
     __shared__ __attribute__((aligned(16))) float xs_all[kMWaves][G::DP];
@@ -299,10 +274,82 @@ __global__ __launch_bounds__(64 * kMWaves) void k_multi32s(const MInst *insts, c
     }
 }
 
+// One element per work-group of eight waves and one chunk of its row sums per
+// wave (Chunks; d > 32): the single-mesh k_step32w on the items of one degree.
+// A wave issues its chunk's block loads before it gathers its slot's
+// coefficients; waves 1..7 leave their chunk sums in LDS and wave 0 adds them
+// in chunk order, bitwise as multi_elem.
+constexpr int kCWaves = 8;
+
+template <typename G>
+__device__ inline void multi_elem_chunks(const MInst &I, int e, int lane, int w, float *x,
+                                         float (*part)[G::RPL][64])
+{
+    const int s = w >> 1, h = w & 1;
+    ChunkRegs<G> R;
+    chunk_load<G>(I.blk, e, s, h, lane, R);
+    // keep the block loads ahead of the gather
+    __builtin_amdgcn_sched_barrier(0);
+    const int cur = (I.steps + 1) & 1;
+    const float *uin = cur ? I.uhi1 : I.uhi0;
+    const int4 m = I.meta[e];
+    const int nb = s == 0 ? e : (s == 1 ? m.x : (s == 2 ? m.y : m.z));
+    float wi[G::RPL], cp[G::RPL];
+    int rows[G::RPL];
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const int row = lane + 64 * q;
+        rows[q] = (w == 0 && row < G::D) ? row : -1;
+        const size_t idx = (size_t)e * G::D + row;
+        wi[q] = rows[q] >= 0 ? I.winc[idx] : 0.f;
+        cp[q] = rows[q] >= 0 ? I.ucomp[idx] : 0.f;
+    }
+    const float sc = w == 0 ? I.sc[e] : 0.f;
+    float own[G::RPL];   // wave 0: u^n of the lane's rows (the update's uh)
+    gather_one<G, float>(x, uin, nb, lane, I.tfree ? e : -1, own);
+    // the wave reads only its own LDS slice: order its stores before its loads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float acc[G::RPL];
+    chunk_fma<G>(R, x, h, acc);
+    if (w > 0) {
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) part[w - 1][q][lane] = acc[q];
+    }
+    __syncthreads();
+    if (w > 0) return;
+#pragma unroll
+    for (int k = 0; k < kCWaves - 1; ++k)
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) acc[q] += part[k][q][lane];
+    finish_elem<G>(I, e, m, rows, wi, cp, own, acc, sc);
+}
+
+template <typename G, bool PERSIST>
+__global__ __launch_bounds__(64 * kCWaves) void k_multi32w(const MInst *__restrict__ insts, const int2 *__restrict__ items, int nitems)
+{
+    __shared__ __attribute__((aligned(16))) float xs_all[kCWaves][G::DP];
+    __shared__ float part[kCWaves - 1][G::RPL][64];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int step = PERSIST ? gridDim.x : nitems;
+    for (int it = blockIdx.x; it < nitems; it += step) {
+        // the whole work-group takes the same item: every branch below is uniform over it
+        const int2 t = items[it];
+        const MInst I = insts[t.x];
+        if (!I.live || I.steps >= I.nsteps) continue;
+        multi_elem_chunks<G>(I, t.y, lane, w, xs_all[w], part);
+        __syncthreads();  // the next item's partial sums overwrite these
+    }
+}
+
 // Launch variants: bit 0 = whole-wave rows (degrees with d <= 32 only),
 // bit 1 = one item per wave / per work-group (else persistent), bit 2 = one
-// element per work-group and one slot per wave (k_multi32s; bit 0 ignored).
-constexpr int kVarRows = 1, kVarOneShot = 2, kVarSlots = 4;
+// element per work-group and one slot per wave (k_multi32s; bit 0 ignored),
+// bit 3 = one element per work-group and one chunk per wave (k_multi32w;
+// d > 32 only, bits 0 and 2 ignored).
+constexpr int kVarRows = 1, kVarOneShot = 2, kVarSlots = 4, kVarChunks = 8;
 
 template <int P>
 __device__ inline void fused_item(const MInst &I, int e, int lane, float (*xs)[kMaxDP])
@@ -337,7 +384,7 @@ __device__ inline void fused_dispatch(const MInst &I, int e, int lane, float (*x
     }
 }
 
-__global__ __launch_bounds__(64 * kMWaves) void k_multi_fused(const MInst *insts, const int2 *items, int nitems)
+__global__ __launch_bounds__(64 * kMWaves) void k_multi_fused(const MInst *__restrict__ insts, const int2 *__restrict__ items, int nitems)
 {
     __shared__ __attribute__((aligned(16))) float xs_all[kMWaves][4][kMaxDP];
     const int lane = threadIdx.x & 63;
@@ -354,6 +401,12 @@ __global__ __launch_bounds__(64 * kMWaves) void k_multi_fused(const MInst *insts
 template <int P>
 const void *kern_p(int v)
 {
+    if constexpr (Geom<P>::D > 32) {
+        if (v & kVarChunks)
+            return (v & kVarOneShot) ? reinterpret_cast<const void *>(&k_multi32w<Geom<P>, false>)
+                                     : reinterpret_cast<const void *>(&k_multi32w<Geom<P>, true>);
+    }
+    if (v & kVarChunks) return nullptr;
     if (v & kVarSlots)
         return (v & kVarOneShot) ? reinterpret_cast<const void *>(&k_multi32s<Geom<P>, false>)
                                  : reinterpret_cast<const void *>(&k_multi32s<Geom<P>, true>);
@@ -410,7 +463,7 @@ struct tsp_multi {
     hipGraphExec_t graph = nullptr;
     int graph_chunk = 0;
     struct Range {
-        int P, variant, off, count, grid;
+        int P, variant, off, count, grid, block;
         const void *kern;
     };
     std::vector<Range> ranges;          // items of one degree: [off, off+count)
@@ -538,6 +591,7 @@ void rebuild(tsp_multi *m)
         g.count = m->nitems;
         g.kern = reinterpret_cast<const void *>(&k_multi_fused);
         g.grid = (g.count + kMWaves - 1) / kMWaves;
+        g.block = 64 * kMWaves;
         m->ranges.push_back(g);
     }
     for (int q = 0; q < m->nitems && !fused;) {
@@ -551,13 +605,16 @@ void rebuild(tsp_multi *m)
         g.count = r - q;
         g.kern = multi_kernel(P, g.variant);
         if (!g.kern) mfail(m, TSP_ERR_UNSUPPORTED, "degree %d not built", P);
-        // items per work-group: one per wave, or one in all with a slot per wave
-        const int want = (g.variant & kVarSlots) ? g.count : (g.count + kMWaves - 1) / kMWaves;
+        // items per work-group: one per wave, or one in all with a slot or a
+        // chunk per wave
+        const bool whole = (g.variant & (kVarSlots | kVarChunks)) != 0;
+        const int want = whole ? g.count : (g.count + kMWaves - 1) / kMWaves;
+        g.block = 64 * ((g.variant & kVarChunks) ? kCWaves : kMWaves);
         if (g.variant & kVarOneShot) {
             g.grid = want;
         } else {
             int per_cu = 0;
-            mhip(m, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, g.kern, 64 * kMWaves, 0), "occupancy");
+            mhip(m, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, g.kern, g.block, 0), "occupancy");
             g.grid = std::max(1, std::min(want, std::max(1, per_cu) * cus));
         }
         m->ranges.push_back(g);
@@ -573,7 +630,7 @@ void launch_steps(tsp_multi *m, int nsteps)
         for (auto &g : m->ranges) {
             const int2 *items = m->d_items + g.off;
             void *args[] = {&m->d_inst, &items, &g.count};
-            mhip(m, hipLaunchKernel(g.kern, dim3(g.grid), dim3(64 * kMWaves), args, 0, m->stream), "multi step");
+            mhip(m, hipLaunchKernel(g.kern, dim3(g.grid), dim3(g.block), args, 0, m->stream), "multi step");
         }
         hipLaunchKernelGGL(k_multi_advance, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->d_inst, n, 0);
     }
@@ -601,7 +658,7 @@ void build_graph(tsp_multi *m)
             hipKernelNodeParams kp{};
             kp.func = const_cast<void *>(r.kern);
             kp.gridDim = dim3(r.grid);
-            kp.blockDim = dim3(64 * kMWaves);
+            kp.blockDim = dim3(r.block);
             kp.sharedMemBytes = 0;
             kp.kernelParams = args;
             kp.extra = nullptr;
@@ -801,6 +858,10 @@ int tsp_multi_autotune(tsp_multi *m, int steps_per_trial)
             std::vector<int> cand = 2 * nln_of(p) <= 32 ? std::vector<int>{0, 1, 2, 3} : std::vector<int>{0, 2};
             cand.push_back(kVarSlots);
             cand.push_back(kVarSlots | kVarOneShot);
+            if (2 * nln_of(p) > 32) {
+                cand.push_back(kVarChunks);
+                cand.push_back(kVarChunks | kVarOneShot);
+            }
             float best = 0;
             int bestv = kVarOneShot;
             for (int var : cand) {
@@ -887,9 +948,12 @@ int tsp_multi_set_variant(tsp_multi *m, int degree, int variant)
 {
     return mguard(m, [&] {
         if (degree < 1 || degree > 8) mfail(m, TSP_ERR_ARG, "degree %d outside 1..8", degree);
-        if (variant < 0 || variant > 7) mfail(m, TSP_ERR_ARG, "unknown variant %d", variant);
-        if ((variant & kVarRows) && !(variant & kVarSlots) && 2 * nln_of(degree) > 32)
+        if (variant < 0 || variant > 15) mfail(m, TSP_ERR_ARG, "unknown variant %d", variant);
+        if ((variant & kVarRows) && !(variant & (kVarSlots | kVarChunks)) && 2 * nln_of(degree) > 32)
             mfail(m, TSP_ERR_ARG, "whole-wave rows need d <= 32 (degree %d has d = %d)", degree,
+                  2 * nln_of(degree));
+        if ((variant & kVarChunks) && 2 * nln_of(degree) <= 32)
+            mfail(m, TSP_ERR_ARG, "chunk per wave needs d > 32 (degree %d has d = %d)", degree,
                   2 * nln_of(degree));
         sync_progress(m);
         m->variant_of_p[degree] = variant;
