@@ -256,15 +256,18 @@ int tsp_release_fp64(tsp_context *ctx);
 /* ------------------------------------------------------------------------
  * Sets of independent simulations advanced together (no counterpart in the
  * reference, which runs one LeapFrog per process).  Instances are ordinary
- * single-column TSP_FP32C contexts, built with the calls above (mesh or
- * tsp_share_operator, sources, receivers with a stride, dt, state) and
- * stopped after tsp_first_step; the set takes ownership of each.  A step of
- * the set covers every element of every running instance, whatever its size
- * and degree, either as one launch per degree present (side by side in the
- * graph) or as one launch of a single kernel that dispatches each element to
- * its degree's step (tsp_multi_set_launch); chunks of steps replay a captured
- * HIP graph.  Results are read through tsp_multi_context + tsp_get_u /
- * tsp_get_seismograms.
+ * TSP_FP32C contexts of any width (1, 8, 16, 32 or 64 columns), built with the
+ * calls above (mesh or tsp_share_operator, sources, receivers with a stride,
+ * dt, state) and stopped after tsp_first_step; the set takes ownership of
+ * each.  A step of the set covers every element of every running instance,
+ * whatever its size, degree and width: the single-column instances as one
+ * launch per degree present (side by side in the graph) or as one launch of a
+ * single kernel that dispatches each element to its degree's step
+ * (tsp_multi_set_launch), the multi-column ones as one launch per (degree,
+ * width) present, each (element, 32-column tile) running its context's own
+ * block-SpMM step.  Chunks of steps replay a captured HIP graph.  Every
+ * instance's result is bitwise that of its context stepped alone.  Results are
+ * read through tsp_multi_context + tsp_get_u / tsp_get_seismograms.
  * ------------------------------------------------------------------------ */
 typedef struct tsp_multi tsp_multi;
 
@@ -291,9 +294,10 @@ int  tsp_multi_run_timed(tsp_multi *m, int nsteps, float *ms);
 enum { TSP_MULTI_PER_DEGREE = 0, TSP_MULTI_FUSED = 1 };
 int  tsp_multi_set_launch(tsp_multi *m, int mode);
 int  tsp_multi_launch(const tsp_multi *m);
-/* Choose, per degree, the fastest launch variant of the element step by timing
- * each on `steps_per_trial` real steps of that degree's instances; then, with
- * a graph chunk set, the fastest launch mode on one chunk of the whole set. */
+/* Choose, per degree, the fastest launch variant of the single-column element
+ * step by timing each on `steps_per_trial` real steps of that degree's
+ * instances; then, with a graph chunk set, the fastest launch mode on one
+ * chunk of the whole set. */
 int  tsp_multi_autotune(tsp_multi *m, int steps_per_trial);
 /* The launch variant in use for a degree: bit 0 = one row per lane across the
  * wave (else column groups of 16/32 lanes, d <= 32 only), bit 1 = one item per
@@ -351,9 +355,12 @@ void   tsp_server_destroy(tsp_server *s);
  *   TSP_SERVE_COLUMNS (default): one 8/16/32/64-column context per request, the
  *     narrowest that holds its sources (sources as
  *     columns of a block SpMM on MFMA), each request's chunk on its own stream;
+ *   TSP_SERVE_BATCHED: the same multi-column context per request, every
+ *     running request in one tsp_multi set (one launch per degree and width
+ *     present per step, chunks of steps from a captured graph);
  *   TSP_SERVE_INSTANCES: one single-column context per source (sharing the
  *     request's operator) in one tsp_multi set. */
-enum { TSP_SERVE_INSTANCES = 0, TSP_SERVE_COLUMNS = 1 };
+enum { TSP_SERVE_INSTANCES = 0, TSP_SERVE_COLUMNS = 1, TSP_SERVE_BATCHED = 2 };
 int    tsp_server_set_mode(tsp_server *s, int mode);
 const char *tsp_server_last_error(const tsp_server *s);
 double tsp_server_now(void);
@@ -367,7 +374,8 @@ int    tsp_server_submit(tsp_server *s, const tsp_request *req, double arrival, 
 /* One pass of the loop: admit, run, retire.  Instances mode runs one chunk
  * of the set and waits for it; columns mode does not wait: it queues the next
  * two chunks on every request whose stream has run dry (and the result
- * download behind a request's last chunk), so call it in a loop.  Reports how many
+ * download behind a request's last chunk), so call it in a loop.  Batched
+ * mode runs one chunk of the set like instances mode.  Reports how many
  * requests finished in this pass (completed, or failed at admission: then
  * tsp_server_result returns TSP_ERR_STATE with the reason in
  * tsp_server_last_error) and how many are running / queued. */

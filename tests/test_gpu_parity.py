@@ -879,21 +879,87 @@ def test_multi_shared_operator_and_remove():
     m.close()
 
 
+def _columns_problem(N, nsrc, nsteps, seed, nx=5, ny=4):
+    """A small mesh with nsrc Ricker sources (one per column of a multi-column context)."""
+    p = small_problem(N, nx=nx, ny=ny, nsteps=nsteps, seed=seed, nrec=5)
+    rng = np.random.default_rng(seed)
+    p.src_xy = np.stack([rng.uniform(100.0, 900.0, nsrc), rng.uniform(100.0, 900.0, nsrc)], 1)
+    p.f0 = np.repeat(p.f0, nsrc)
+    p.t0 = np.repeat(p.t0, nsrc)
+    ang = rng.uniform(0.0, 2 * np.pi, nsrc)
+    p.dirs = np.stack([np.cos(ang), np.sin(ang)], 1)
+    p.__post_init__()
+    return p
+
+
+def test_multi_set_with_multi_column_instances():
+    """Multi-column contexts (8 to 64 columns, p = 3..8, two of the same degree
+    and width) in one set beside single-column ones: each (element, 32-column
+    tile) runs its context's own block-SpMM step, so every instance ends bitwise
+    equal to its context stepped alone (seismograms and every column's
+    displacement), with graph chunks or plain launches and in either launch
+    mode of the single-column instances; a multi-column owner cannot leave the
+    set before its borrower."""
+    cases = [(3, 3, 8, 41), (4, 9, 16, 37), (5, 20, 32, 29), (7, 5, 8, 33), (4, 40, 64, 30), (3, 2, 8, 45),
+             (8, 2, 8, 21), (6, 12, 16, 26)]
+    probs = [_columns_problem(N, ns, st, seed=60 + k) for k, (N, ns, R, st) in enumerate(cases)]
+    widths = [R for (_, _, R, _) in cases]
+    singles = _mixed_set()[:3]
+    alone = []
+    for p, R in zip(probs, widths):
+        eng, s = W.run_batched(p, R)
+        alone.append((s, [eng.get_u(c) for c in range(len(p.src_elem))]))
+        eng.close()
+    alone1 = [W.run_engine(p, "fp32c")[2] for p in singles]
+    for chunk, mode in ((8, capi.Multi.PER_DEGREE), (0, capi.Multi.PER_DEGREE), (8, capi.Multi.FUSED)):
+        say(f"  chunk {chunk}, launch mode {mode}")
+        m = capi.Multi(chunk=chunk)
+        m.set_launch(mode)
+        ids = [m.add(W.prepare_instance(p, ncols=R), p.nsteps) for p, R in zip(probs, widths)]
+        ids1 = [m.add(W.prepare_instance(p), p.nsteps) for p in singles]
+        while m.run(7):
+            pass
+        for i, p, R, (s, us) in zip(ids, probs, widths, alone):
+            e = m.engine(i)
+            assert e.status() == 0
+            done, n = m.info(i)
+            assert done == n == p.nsteps
+            np.testing.assert_array_equal(e.seismograms(), s, err_msg=f"p={p.N} R={R}")
+            for c, u in enumerate(us):
+                np.testing.assert_array_equal(e.get_u(c), u, err_msg=f"p={p.N} R={R} column {c}")
+        for i, s1 in zip(ids1, alone1):
+            np.testing.assert_array_equal(m.engine(i).seismograms(), s1)
+        m.close()
+    owner = W.prepare_instance(probs[0], ncols=8)
+    other = W.prepare_instance(probs[0], share=owner)
+    m = capi.Multi(chunk=4)
+    a, b = m.add(owner, 20), m.add(other, 20)
+    m.run(8)
+    with pytest.raises(capi.EngineError):
+        m.remove(a)               # its operator is still borrowed
+    m.remove(b)
+    m.remove(a)
+    m.close()
+
+
 # ------------------------------------------------------------------ request loop
 def _req(k, nsteps=120, n=(8, 14), nsrc=(1, 4)):
     return W.serving_request(k, n_range=n, nsrc_range=nsrc, steps_range=(nsteps, nsteps + 40), nrec_range=(6, 12))
 
 
-@pytest.mark.parametrize("mode", ["columns", "instances"])
+_MODES = {"columns": capi.Server.COLUMNS, "instances": capi.Server.INSTANCES, "batched": capi.Server.BATCHED}
+
+
+@pytest.mark.parametrize("mode", ["columns", "instances", "batched"])
 def test_server_result_is_the_same_alone_and_in_the_mix(mode):
     """A request's seismograms are identical whether it runs alone or admitted
     among five others with staggered arrivals, and stay within 1e-4 of the FP64
     path source by source.  Instances mode: equal to the single-context FP32
-    path bit for bit; columns mode: equal to the same request run as an
-    8-column context (the width the loop picks for 3 sources), and to FP32
+    path bit for bit; columns and batched modes: equal to the same request run
+    as an 8-column context (the width the loop picks for 3 sources), and to FP32
     rounding to it run as a 32-column one."""
     import copy
-    md = capi.Server.COLUMNS if mode == "columns" else capi.Server.INSTANCES
+    md = _MODES[mode]
     target = _req(0, nsrc=(3, 3))
     srv, ids, lat, _ = W.serve([target], rate=0, mode=md)
     alone, _ = srv.result(ids[0])
@@ -904,7 +970,7 @@ def test_server_result_is_the_same_alone_and_in_the_mix(mode):
     say(f"  {mode}: mix of 6: latencies {np.round(lat, 3).tolist()} s, wall {wall:.2f} s")
     srv.close()
     np.testing.assert_array_equal(mixed, alone)
-    if md == capi.Server.COLUMNS:
+    if md != capi.Server.INSTANCES:
         _, sb = W.run_batched(target, 8)
         np.testing.assert_array_equal(alone, sb[..., :3, :].transpose(2, 0, 1, 3))
         _, sb = W.run_batched(target, 32)   # one slot per wave at full tiles: FP32 rounding apart
@@ -923,7 +989,7 @@ def test_server_result_is_the_same_alone_and_in_the_mix(mode):
         assert err <= 1e-4
 
 
-@pytest.mark.parametrize("mode", ["columns", "instances"])
+@pytest.mark.parametrize("mode", ["columns", "instances", "batched"])
 def test_server_boundary_adapted_request(mode):
     """A request in the BoundaryAdapted basis (tsp_request.basis = 1): the loop
     sets the basis's tables and inverse mass on its contexts, so the result is
@@ -931,14 +997,14 @@ def test_server_boundary_adapted_request(mode):
     each source in its own FP32 context in instances mode), and within the
     basis's FP32 tolerance of the FP64 oracle."""
     import copy
-    md = capi.Server.COLUMNS if mode == "columns" else capi.Server.INSTANCES
+    md = _MODES[mode]
     target = _req(3, nsteps=80, nsrc=(2, 2))
     target.basis = capi.BOUNDARY_ADAPTED
     target.__post_init__()
     srv, ids, lat, _ = W.serve([target], rate=0, mode=md)
     res, _ = srv.result(ids[0])
     srv.close()
-    if md == capi.Server.COLUMNS:
+    if md != capi.Server.INSTANCES:
         _, sb = W.run_batched(target, 8)
         np.testing.assert_array_equal(res, sb[..., :2, :].transpose(2, 0, 1, 3))
     for k in range(2):
@@ -954,14 +1020,14 @@ def test_server_boundary_adapted_request(mode):
         assert err <= 1e-3
 
 
-@pytest.mark.parametrize("mode", ["columns", "instances"])
+@pytest.mark.parametrize("mode", ["columns", "instances", "batched"])
 def test_server_admission_respects_the_memory_budget(mode):
     """With a budget that holds one request at a time, three requests run one
     after the other (each admitted no earlier than the previous completion), and
     all three results equal the request run with an ample budget; a request
     larger than the budget is refused at submission."""
     from tspeed_amd import capi as C
-    md = C.Server.COLUMNS if mode == "columns" else C.Server.INSTANCES
+    md = _MODES[mode]
     _, total = ctypes_mem()
     req = _req(7, nsteps=60, nsrc=(2, 2))
     res, peak = C.Server.footprint(req, mode=md)

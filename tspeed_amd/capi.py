@@ -561,7 +561,7 @@ class Request(ctypes.Structure):
 class Server:
     """The request loop (tsp_server_*)."""
 
-    INSTANCES, COLUMNS = 0, 1
+    INSTANCES, COLUMNS, BATCHED = 0, 1, 2
 
     def __init__(self, device=0, mem_fraction=0.8, chunk=32, mode=COLUMNS):
         self._L = lib()

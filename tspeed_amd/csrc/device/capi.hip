@@ -1326,7 +1326,8 @@ bool ctx_view(tsp_context *c, InstView *v)
     v->steps = c->steps;
     v->cur = c->cur;
     v->first_done = c->first_done;
-    v->owns_operator = c->blk32 && std::find(c->allocs.begin(), c->allocs.end(), (void *)c->blk32) != c->allocs.end();
+    const float *op = c->batched() ? c->blkA : c->blk32;
+    v->owns_operator = op && std::find(c->allocs.begin(), c->allocs.end(), (const void *)op) != c->allocs.end();
     v->tfree = c->h_minv.empty();
     v->stream = c->stream;
     v->meta = c->meta;
@@ -1339,11 +1340,28 @@ bool ctx_view(tsp_context *c, InstView *v)
     v->seis = c->seis;
     v->flags = c->flags;
     v->blk32 = c->blk32;
+    v->blkA = c->blkA;
     v->sc_step = c->sc_step;
     v->uhi[0] = c->uhi[0];
     v->uhi[1] = c->uhi[1];
     v->ucomp = c->ucomp;
     v->winc = c->winc;
+    return true;
+}
+
+bool ctx_batch_args(tsp_context *c, BatchArgs *out)
+{
+    if (!c || !out || !c->batched()) return false;
+    BatchArgs b = batch_args(c);
+    b.sc32 = c->sc_step;
+    b.ctr = nullptr;
+    b.koff = 0;
+    b.wav = c->wav;
+    b.stride = c->nrec ? c->stride : 0;
+    b.max_samples = c->max_samples;
+    b.uhi_pair[0] = c->uhi[0];
+    b.uhi_pair[1] = c->uhi[1];
+    *out = b;
     return true;
 }
 

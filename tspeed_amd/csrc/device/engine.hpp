@@ -164,6 +164,23 @@ struct BatchArgs {
     int stride, max_samples;
     float *uhi_pair[2];
 };
+// One multi-column context inside a multi-instance set (multi.hip): the
+// arguments of its graph-replayed step (ctr null: the step index comes from
+// `steps`) and its progress.
+struct BatchInst {
+    BatchArgs a;
+    int steps, nsteps;        // completed step() calls, steps to run in the set
+    int live, sel;            // 0 once removed / not in the current item list
+    int P, R;
+};
+// The step kernel of every multi-column instance of degree P and width R in one
+// launch, arguments (const BatchInst *, const int2 *items, int nitems), items =
+// (instance, work item); work items per instance = ne * batch_tiles(R), and
+// each work-group of 256 threads takes items_per_wg items.  0, or -2 if the
+// degree or width is not built.
+int batch_multi_kernel(int P, int R, const void **kern, int *items_per_wg);
+int batch_tiles(int R);              // 32-column tiles per element
+
 int launch_ctr_set(int *ctr, int value, hipStream_t s);
 int launch_ctr_add(int *ctr, int value, hipStream_t s);
 int batch_kp(int P);                 // state rows per element (d padded to a multiple of 16)

@@ -19,11 +19,14 @@ struct InstView {
     const double *src_phi, *wav, *rec_phi;
     double *seis;
     unsigned *flags;
-    const float *blk32, *sc_step;
+    const float *blk32, *blkA, *sc_step;   // single-column / multi-column operator
     float *uhi[2], *ucomp, *winc;
 };
 
 bool ctx_view(tsp_context *c, InstView *v);
+// a multi-column context's step arguments for replay from a step counter
+// (BatchArgs::wav, stride, max_samples, uhi_pair set; ctr null)
+bool ctx_batch_args(tsp_context *c, BatchArgs *out);
 // record progress made by kernels launched outside the context
 void ctx_set_progress(tsp_context *c, int steps, int cur, int nsamples);
 

@@ -441,6 +441,15 @@ __global__ void k_multi_advance(MInst *insts, int n, int P)
         insts[i].steps += 1;
 }
 
+// The same for the multi-column instances of degree P and width R (P = 0: all).
+__global__ void k_batch_advance(BatchInst *insts, int n, int P, int R)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (P == 0 || (insts[i].P == P && insts[i].R == R)) && insts[i].sel && insts[i].live &&
+        insts[i].steps < insts[i].nsteps)
+        insts[i].steps += 1;
+}
+
 }  // namespace
 }  // namespace tspeed_dev
 
@@ -451,8 +460,13 @@ struct tsp_multi {
     hipStream_t stream = nullptr;
     std::string err;
     std::vector<tsp_context *> ctx;     // by instance id (nullptr once removed)
-    std::vector<MInst> host;            // host mirror of the instance table
+    // Host mirrors of the two instance tables, both indexed by instance id:
+    // single-column FP32C instances in `host`, multi-column ones in `bhost`
+    // (an id's entry in the other table stays dead: live = 0, R = 0).
+    std::vector<MInst> host;
+    std::vector<BatchInst> bhost;
     MInst *d_inst = nullptr;
+    BatchInst *d_binst = nullptr;
     int d_cap = 0;
     int d_n = 0;                        // entries uploaded by the last rebuild
     bool stale = false;                 // steps launched since the host mirror was read back
@@ -463,10 +477,10 @@ struct tsp_multi {
     hipGraphExec_t graph = nullptr;
     int graph_chunk = 0;
     struct Range {
-        int P, variant, off, count, grid, block;
+        int P, R, variant, off, count, grid, block;   // R = 1: single-column instances
         const void *kern;
     };
-    std::vector<Range> ranges;          // items of one degree: [off, off+count)
+    std::vector<Range> ranges;          // items of one degree (and width): [off, off+count)
     std::map<int, int> variant_of_p;    // autotuned lane mapping per degree
     std::vector<int> only_p;            // restrict the item list (autotuning)
     int launch = TSP_MULTI_PER_DEGREE;  // per-degree kernels or one fused kernel
@@ -522,18 +536,39 @@ void sync_progress(tsp_multi *m)
 {
     if (!m->stale || !m->d_inst || m->d_n == 0) return;
     std::vector<MInst> dev(m->d_n);
+    std::vector<BatchInst> bdev(m->d_n);
     mhip(m, hipMemcpyAsync(dev.data(), m->d_inst, sizeof(MInst) * dev.size(), hipMemcpyDeviceToHost, m->stream),
+         "progress");
+    mhip(m, hipMemcpyAsync(bdev.data(), m->d_binst, sizeof(BatchInst) * bdev.size(), hipMemcpyDeviceToHost,
+                           m->stream),
          "progress");
     mhip(m, hipStreamSynchronize(m->stream), "progress");
     m->stale = false;
     for (size_t i = 0; i < dev.size(); ++i) {
         if (!m->ctx[i]) continue;
-        MInst &h = m->host[i];
-        h.steps = dev[i].steps;
-        const int ns = (h.stride > 0 ? 1 + h.steps / h.stride : 0);
-        ctx_set_progress(m->ctx[i], h.steps, (h.steps + 1) & 1, std::min(ns, h.max_samples));
+        int steps, stride, max_samples;
+        if (m->bhost[i].R > 1) {
+            BatchInst &b = m->bhost[i];
+            b.steps = bdev[i].steps;
+            steps = b.steps;
+            stride = b.a.stride;
+            max_samples = b.a.max_samples;
+        } else {
+            MInst &h = m->host[i];
+            h.steps = dev[i].steps;
+            steps = h.steps;
+            stride = h.stride;
+            max_samples = h.max_samples;
+        }
+        const int ns = (stride > 0 ? 1 + steps / stride : 0);
+        ctx_set_progress(m->ctx[i], steps, (steps + 1) & 1, std::min(ns, max_samples));
     }
 }
+
+// progress of instance i, whichever table holds it
+int inst_steps(const tsp_multi *m, int i) { return m->bhost[i].R > 1 ? m->bhost[i].steps : m->host[i].steps; }
+int inst_nsteps(const tsp_multi *m, int i) { return m->bhost[i].R > 1 ? m->bhost[i].nsteps : m->host[i].nsteps; }
+bool inst_live(const tsp_multi *m, int i) { return m->bhost[i].R > 1 ? m->bhost[i].live : m->host[i].live; }
 
 void rebuild(tsp_multi *m)
 {
@@ -547,14 +582,27 @@ void rebuild(tsp_multi *m)
         h.sel = (h.live && h.steps < h.nsteps &&
                  (m->only_p.empty() || std::find(m->only_p.begin(), m->only_p.end(), h.P) != m->only_p.end()))
                     ? 1 : 0;
+        // multi-column instances have no launch variants: the autotuner's
+        // trials (only_p set) leave them out
+        BatchInst &b = m->bhost[i];
+        b.sel = (b.R > 1 && b.live && b.steps < b.nsteps && m->only_p.empty()) ? 1 : 0;
     }
     if (n > m->d_cap) {
         if (m->d_inst) (void)hipFree(m->d_inst);
+        if (m->d_binst) (void)hipFree(m->d_binst);
+        m->d_inst = nullptr;
+        m->d_binst = nullptr;
         m->d_cap = std::max(n, 2 * m->d_cap);
         mhip(m, hipMalloc(&m->d_inst, sizeof(MInst) * m->d_cap), "instance table");
+        mhip(m, hipMalloc(&m->d_binst, sizeof(BatchInst) * m->d_cap), "instance table");
     }
-    if (n) mhip(m, hipMemcpyAsync(m->d_inst, m->host.data(), sizeof(MInst) * n, hipMemcpyHostToDevice, m->stream),
-                "instance table");
+    if (n) {
+        mhip(m, hipMemcpyAsync(m->d_inst, m->host.data(), sizeof(MInst) * n, hipMemcpyHostToDevice, m->stream),
+             "instance table");
+        mhip(m, hipMemcpyAsync(m->d_binst, m->bhost.data(), sizeof(BatchInst) * n, hipMemcpyHostToDevice,
+                               m->stream),
+             "instance table");
+    }
     m->d_n = n;
     // items of running instances, grouped by degree
     std::vector<int2> items;
@@ -569,6 +617,20 @@ void rebuild(tsp_multi *m)
     });
     for (int i : order)
         for (int e = 0; e < m->host[i].ne; ++e) items.push_back(make_int2(i, e));
+    const int nsingle = (int)items.size();
+    // then the multi-column instances, grouped by (degree, width); their work
+    // items are (element, 32-column tile) pairs
+    std::vector<int> border;
+    for (int i = 0; i < n; ++i)
+        if (m->bhost[i].sel) border.push_back(i);
+    std::stable_sort(border.begin(), border.end(), [&](int a, int b) {
+        const BatchInst &x = m->bhost[a], &y = m->bhost[b];
+        return x.P != y.P ? x.P < y.P : x.R < y.R;
+    });
+    for (int i : border) {
+        const int w = m->bhost[i].a.ne * batch_tiles(m->bhost[i].R);
+        for (int k = 0; k < w; ++k) items.push_back(make_int2(i, k));
+    }
     m->nitems = (int)items.size();
     if (m->nitems > m->items_cap) {
         if (m->d_items) (void)hipFree(m->d_items);
@@ -583,23 +645,25 @@ void rebuild(tsp_multi *m)
     mhip(m, hipGetDeviceProperties(&prop, m->device), "properties");
     const int cus = prop.multiProcessorCount;
     m->ranges.clear();
-    if (fused && m->nitems) {
+    if (fused && nsingle) {
         tsp_multi::Range g;
         g.P = 0;
+        g.R = 1;
         g.variant = 0;
         g.off = 0;
-        g.count = m->nitems;
+        g.count = nsingle;
         g.kern = reinterpret_cast<const void *>(&k_multi_fused);
         g.grid = (g.count + kMWaves - 1) / kMWaves;
         g.block = 64 * kMWaves;
         m->ranges.push_back(g);
     }
-    for (int q = 0; q < m->nitems && !fused;) {
+    for (int q = 0; q < nsingle && !fused;) {
         const int P = m->host[items[q].x].P;
         int r = q;
-        while (r < m->nitems && m->host[items[r].x].P == P) ++r;
+        while (r < nsingle && m->host[items[r].x].P == P) ++r;
         tsp_multi::Range g;
         g.P = P;
+        g.R = 1;
         g.variant = m->host[items[q].x].variant;
         g.off = q;
         g.count = r - q;
@@ -620,6 +684,25 @@ void rebuild(tsp_multi *m)
         m->ranges.push_back(g);
         q = r;
     }
+    for (int q = nsingle; q < m->nitems;) {
+        const BatchInst &b0 = m->bhost[items[q].x];
+        const int P = b0.P, R = b0.R;
+        int r = q;
+        while (r < m->nitems && m->bhost[items[r].x].P == P && m->bhost[items[r].x].R == R) ++r;
+        tsp_multi::Range g;
+        g.P = P;
+        g.R = R;
+        g.variant = 0;
+        g.off = q;
+        g.count = r - q;
+        int per_wg = 1;
+        if (batch_multi_kernel(P, R, &g.kern, &per_wg) != 0)
+            mfail(m, TSP_ERR_UNSUPPORTED, "degree %d at %d columns not built", P, R);
+        g.grid = (g.count + per_wg - 1) / per_wg;
+        g.block = 256;
+        m->ranges.push_back(g);
+        q = r;
+    }
     m->dirty = false;
 }
 
@@ -629,10 +712,12 @@ void launch_steps(tsp_multi *m, int nsteps)
     for (int k = 0; k < nsteps; ++k) {
         for (auto &g : m->ranges) {
             const int2 *items = m->d_items + g.off;
-            void *args[] = {&m->d_inst, &items, &g.count};
+            void *table = g.R > 1 ? (void *)m->d_binst : (void *)m->d_inst;
+            void *args[] = {&table, &items, &g.count};
             mhip(m, hipLaunchKernel(g.kern, dim3(g.grid), dim3(g.block), args, 0, m->stream), "multi step");
         }
         hipLaunchKernelGGL(k_multi_advance, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->d_inst, n, 0);
+        hipLaunchKernelGGL(k_batch_advance, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->d_binst, n, 0, 0);
     }
     mhip(m, hipGetLastError(), "multi step launch");
 }
@@ -650,9 +735,12 @@ void build_graph(tsp_multi *m)
     int n = (int)m->host.size();
     for (auto &r : m->ranges) {
         const int2 *items = m->d_items + r.off;
-        void *args[] = {&m->d_inst, &items, &r.count};
-        int adv_p = r.P;  // 0 for the fused range: every instance
+        void *table = r.R > 1 ? (void *)m->d_binst : (void *)m->d_inst;
+        void *args[] = {&table, &items, &r.count};
+        int adv_p = r.P;  // 0 for the fused range: every single-column instance
+        int adv_r = r.R;
         void *aargs[] = {&m->d_inst, &n, &adv_p};
+        void *bargs[] = {&m->d_binst, &n, &adv_p, &adv_r};
         hipGraphNode_t prev = nullptr;
         for (int k = 0; k < m->chunk; ++k) {
             hipKernelNodeParams kp{};
@@ -665,11 +753,11 @@ void build_graph(tsp_multi *m)
             hipGraphNode_t node;
             mhip(m, hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &kp), "graph node");
             hipKernelNodeParams ap{};
-            ap.func = reinterpret_cast<void *>(&k_multi_advance);
+            ap.func = r.R > 1 ? reinterpret_cast<void *>(&k_batch_advance) : reinterpret_cast<void *>(&k_multi_advance);
             ap.gridDim = dim3((n + 255) / 256);
             ap.blockDim = dim3(256);
             ap.sharedMemBytes = 0;
-            ap.kernelParams = aargs;
+            ap.kernelParams = r.R > 1 ? bargs : aargs;
             ap.extra = nullptr;
             mhip(m, hipGraphAddKernelNode(&prev, g, &node, 1, &ap), "graph node");
         }
@@ -727,6 +815,7 @@ void tsp_multi_destroy(tsp_multi *m)
     for (size_t i = m->ctx.size(); i-- > 0;)
         if (m->ctx[i]) tsp_destroy(m->ctx[i]);
     if (m->d_inst) (void)hipFree(m->d_inst);
+    if (m->d_binst) (void)hipFree(m->d_binst);
     if (m->d_items) (void)hipFree(m->d_items);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
@@ -740,8 +829,8 @@ int tsp_multi_add(tsp_multi *m, tsp_context *c, int nsteps, int *id)
         InstView v{};
         if (!ctx_view(c, &v)) mfail(m, TSP_ERR_ARG, "null context");
         if (v.device != m->device) mfail(m, TSP_ERR_ARG, "context on device %d, set on device %d", v.device, m->device);
-        if (v.precision != TSP_FP32C || v.ncols != 1)
-            mfail(m, TSP_ERR_UNSUPPORTED, "instances must be single-column TSP_FP32C contexts");
+        if (v.precision != TSP_FP32C)
+            mfail(m, TSP_ERR_UNSUPPORTED, "instances must be TSP_FP32C contexts (FP32 products)");
         if (!v.first_done || v.steps != 0) mfail(m, TSP_ERR_STATE, "take the first step, and no other, before adding");
         if (nsteps < 0) mfail(m, TSP_ERR_ARG, "negative step count");
         if (v.nsrc && nsteps >= v.ntab)
@@ -752,7 +841,18 @@ int tsp_multi_add(tsp_multi *m, tsp_context *c, int nsteps, int *id)
         mhip(m, hipStreamSynchronize(v.stream), "context sync");
         sync_progress(m);
         MInst I{};
+        BatchInst B{};
         I.P = v.P;
+        if (v.ncols > 1) {
+            // a multi-column context: its record lives in the batched table, the
+            // single-column one stays dead
+            if (!ctx_batch_args(c, &B.a)) mfail(m, TSP_ERR_ARG, "not a multi-column context");
+            B.steps = 0;
+            B.nsteps = nsteps;
+            B.live = 1;
+            B.P = v.P;
+            B.R = v.ncols;
+        }
         I.ne = v.ne;
         I.nsteps = nsteps;
         I.stride = v.stride;
@@ -761,7 +861,7 @@ int tsp_multi_add(tsp_multi *m, tsp_context *c, int nsteps, int *id)
         I.max_samples = v.max_samples;
         I.tfree = v.tfree ? 1 : 0;
         I.steps = 0;
-        I.live = 1;
+        I.live = v.ncols > 1 ? 0 : 1;
         I.meta = v.meta;
         I.extra = v.extra;
         I.src_sorted = v.src_sorted;
@@ -783,9 +883,11 @@ int tsp_multi_add(tsp_multi *m, tsp_context *c, int nsteps, int *id)
         while (slot < (int)m->ctx.size() && m->ctx[slot]) ++slot;
         if (slot == (int)m->ctx.size()) {
             m->host.push_back(I);
+            m->bhost.push_back(B);
             m->ctx.push_back(c);
         } else {
             m->host[slot] = I;
+            m->bhost[slot] = B;
             m->ctx[slot] = c;
         }
         m->dirty = true;
@@ -808,13 +910,14 @@ int tsp_multi_run(tsp_multi *m, int nsteps, int *active)
         if (nsteps < 0) mfail(m, TSP_ERR_ARG, "negative step count");
         sync_progress(m);
         bool any_done = false;
-        for (const auto &h : m->host)
-            if (h.live && h.steps >= h.nsteps && h.nsteps > 0) any_done = true;
+        for (int i = 0; i < (int)m->ctx.size(); ++i)
+            if (inst_live(m, i) && inst_steps(m, i) >= inst_nsteps(m, i) && inst_nsteps(m, i) > 0) any_done = true;
         if (any_done) m->dirty = true;  // drop finished instances from the items
         advance(m, nsteps);
         sync_progress(m);
         int act = 0;
-        for (const auto &h : m->host) act += (h.live && h.steps < h.nsteps) ? 1 : 0;
+        for (int i = 0; i < (int)m->ctx.size(); ++i)
+            act += (inst_live(m, i) && inst_steps(m, i) < inst_nsteps(m, i)) ? 1 : 0;
         if (active) *active = act;
     });
 }
@@ -847,7 +950,7 @@ int tsp_multi_autotune(tsp_multi *m, int steps_per_trial)
     return mguard(m, [&] {
         if (steps_per_trial < 1) mfail(m, TSP_ERR_ARG, "steps_per_trial must be >= 1");
         sync_progress(m);
-        std::vector<int> degrees;
+        std::vector<int> degrees;   // of the single-column instances (the others have no variants)
         for (const auto &h : m->host)
             if (h.live && h.steps + 4 * steps_per_trial <= h.nsteps) degrees.push_back(h.P);
         std::sort(degrees.begin(), degrees.end());
@@ -973,8 +1076,8 @@ int tsp_multi_info(tsp_multi *m, int id, int *steps_done, int *nsteps)
     return mguard(m, [&] {
         if (id < 0 || id >= (int)m->host.size() || !m->ctx[id]) mfail(m, TSP_ERR_ARG, "no instance %d", id);
         sync_progress(m);
-        if (steps_done) *steps_done = m->host[id].steps;
-        if (nsteps) *nsteps = m->host[id].nsteps;
+        if (steps_done) *steps_done = inst_steps(m, id);
+        if (nsteps) *nsteps = inst_nsteps(m, id);
     });
 }
 
@@ -991,15 +1094,21 @@ int tsp_multi_remove(tsp_multi *m, int id)
         InstView v{};
         ctx_view(m->ctx[id], &v);
         if (v.owns_operator)
-            for (size_t j = 0; j < m->ctx.size(); ++j)
-                if ((int)j != id && m->ctx[j] && m->host[j].blk == v.blk32)
+            for (size_t j = 0; j < m->ctx.size(); ++j) {
+                if ((int)j == id || !m->ctx[j]) continue;
+                const bool uses = v.ncols > 1 ? (m->bhost[j].R > 1 && m->bhost[j].a.blk == v.blkA)
+                                              : (m->bhost[j].R <= 1 && m->host[j].blk == v.blk32);
+                if (uses)
                     mfail(m, TSP_ERR_STATE, "instance %zu still uses the operator of instance %d: remove it first", j,
                           id);
+            }
         mhip(m, hipStreamSynchronize(m->stream), "sync");
         sync_progress(m);
         tsp_destroy(m->ctx[id]);
         m->ctx[id] = nullptr;
         m->host[id].live = 0;
+        m->bhost[id].live = 0;
+        m->bhost[id].R = 0;
         m->dirty = true;
     });
 }

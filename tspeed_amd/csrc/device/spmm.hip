@@ -268,16 +268,21 @@ __device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int h
                               [&](int mt, int nt, int r) { return acc[mt][nt][r]; });
 }
 
-// Per-step values of a graph-replayed step (see BatchArgs::ctr).
-__device__ inline void resolve_step(BatchArgs &a)
+// Per-step values of step n (1-based index of the step() call) from the
+// replay fields of BatchArgs (wav, stride, max_samples, uhi_pair).
+__device__ inline void resolve_n(BatchArgs &a, int n)
 {
-    if (!a.ctr) return;
-    const int n = *a.ctr + 1 + a.koff;
     a.wav_row = a.nsrc ? a.wav + (size_t)n * a.nsrc * 2 : nullptr;
     a.sample = (a.stride > 0 && n % a.stride == 0 && n / a.stride < a.max_samples) ? n / a.stride : -1;
     const bool odd = n & 1;  // selects, not an index: keeps the arguments out of scratch
     a.uhi = odd ? a.uhi_pair[1] : a.uhi_pair[0];
     a.uhi_out = odd ? a.uhi_pair[0] : a.uhi_pair[1];
+}
+
+// Per-step values of a graph-replayed step (see BatchArgs::ctr).
+__device__ inline void resolve_step(BatchArgs &a)
+{
+    if (a.ctr) resolve_n(a, *a.ctr + 1 + a.koff);
 }
 
 __global__ void k_ctr_set(int *ctr, int value) { *ctr = value; }
@@ -290,17 +295,16 @@ __global__ void k_ctr_add(int *ctr, int value) { *ctr += value; }
 // multiplies the element's own columns by zero and the slot loop needs no
 // branch.  Padded rows stay zero: their operator rows and columns are zero and
 // they carry no source.
+// gw = the wave's work item: element gw / WPE, column tile gw % WPE.
 template <int P, int R, int PD>
-__global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
+__device__ inline void step_walk(const BatchArgs &a, int gw)
 {
-    resolve_step(a);
     using G = BGeom<P>;
     constexpr int MT = G::MT, KB = G::KB, NT = 1;
     constexpr int WPE = BCols<R>::WPE, CW = BCols<R>::CW;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int gw = xcd_remap_b(blockIdx.x, gridDim.x) * kBWaves + wave;
+    const int lane = threadIdx.x & 63;
     const int e = gw / WPE, c0 = (gw % WPE) * 32;
-    if (e >= a.ne) return;  // no block-level barrier in this kernel
+    if (e >= a.ne) return;  // no block-level barrier in this step
     const int half = lane >> 5, jj = lane & 31;
     const bool colok = CW == 32 || jj < CW;
     const int4 m = a.meta[e];
@@ -382,6 +386,14 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
     b_finish<P, NT, R>(a, e, m, c0, half, jj, colok, ebase, acc, own, wv, cp);
 }
 
+template <int P, int R, int PD>
+__global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
+{
+    resolve_step(a);
+    const int wave = threadIdx.x >> 6;
+    step_walk<P, R, PD>(a, xcd_remap_b(blockIdx.x, gridDim.x) * kBWaves + wave);
+}
+
 // One slot per wave, for blocks of one or two 32-row tiles (p <= 6), at every
 // width: a work-group of four waves takes one (element, 32-column tile).  Wave
 // s multiplies block (e, s) into its own accumulators (FP32 products, or BF16x3
@@ -395,7 +407,7 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 // 21.2 at 8; at p = 5, 8 columns, 45.6 against 59.6.  Blocks of three row
 // tiles (p = 7, 8) keep the one-wave-per-tile kernels.
 template <int P, int R, bool BF>
-__device__ inline void step_slots(BatchArgs &a)
+__device__ inline void step_slots(const BatchArgs &a, int item)
 {
     using G = BGeom<P>;
     constexpr int MT = G::MT, KB = G::KB, KC = KB / 2, NO = 4 * MT;  // NO row octets per tile column
@@ -408,7 +420,6 @@ __device__ inline void step_slots(BatchArgs &a)
     __shared__ f32x4 fsrc[NO][64];
     const int lane = threadIdx.x & 63;
     const int s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // this wave's slot; it updates octets s + 4 mt
-    const int item = xcd_remap_b(blockIdx.x, gridDim.x);
     const int e = item / WPE, c0 = (item % WPE) * 32;
     const bool act = e < a.ne;
     const int half = lane >> 5, jj = lane & 31;
@@ -576,7 +587,7 @@ template <int P, int R>
 __global__ __launch_bounds__(64 * kBWaves, slot_min_waves<P>()) void k_step_b32s(BatchArgs a)
 {
     resolve_step(a);
-    step_slots<P, R, false>(a);
+    step_slots<P, R, false>(a, xcd_remap_b(blockIdx.x, gridDim.x));
 }
 
 // BF16x3 products (operator layout and split below, at k_step_b16x3).
@@ -584,7 +595,7 @@ template <int P, int R>
 __global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3s(BatchArgs a)
 {
     resolve_step(a);
-    step_slots<P, R, true>(a);
+    step_slots<P, R, true>(a, xcd_remap_b(blockIdx.x, gridDim.x));
 }
 
 // The same step with the block products split into three BF16 products on
@@ -855,6 +866,55 @@ int sample_b32_p(const BatchArgs &a, int R, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Many multi-column contexts in one launch (the multi-instance set,
+// multi.hip): items[k] = (instance, work item of that instance), every instance
+// of the launch of degree P and width R.  Each item runs the same element step
+// as its context's own kernel (bitwise the same results), with the step index
+// taken from the instance's counter.  An instance past its step count is
+// skipped.  Work items are dealt to the XCDs in contiguous ranges as in the
+// single-context kernels.
+template <int P, int R>
+__global__ __launch_bounds__(64 * kBWaves, slot_min_waves<P>()) void k_bmulti_slots(
+    const BatchInst *__restrict__ insts, const int2 *__restrict__ items, int nitems)
+{
+    const int it = xcd_remap_b(blockIdx.x, gridDim.x);   // grid = nitems
+    if (it >= nitems) return;
+    const int2 t = items[it];
+    const BatchInst &I = insts[t.x];
+    if (!I.live || I.steps >= I.nsteps) return;   // uniform over the work-group
+    BatchArgs a = I.a;
+    resolve_n(a, I.steps + 1);
+    step_slots<P, R, false>(a, t.y);
+}
+
+template <int P, int R>
+__global__ __launch_bounds__(64 * kBWaves) void k_bmulti_walk(const BatchInst *__restrict__ insts,
+                                                              const int2 *__restrict__ items, int nitems)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int it = xcd_remap_b(blockIdx.x, gridDim.x) * kBWaves + wave;
+    if (it >= nitems) return;   // no block-level barrier in this step
+    const int2 t = items[it];
+    const BatchInst &I = insts[t.x];
+    if (!I.live || I.steps >= I.nsteps) return;
+    BatchArgs a = I.a;
+    resolve_n(a, I.steps + 1);
+    step_walk<P, R, 0>(a, t.y);
+}
+
+template <int P, int R>
+int bmulti_p(const void **kern, int *per_wg)
+{
+    if constexpr (BGeom<P>::MT <= 2) {
+        *kern = reinterpret_cast<const void *>(&k_bmulti_slots<P, R>);
+        *per_wg = 1;
+    } else {
+        *kern = reinterpret_cast<const void *>(&k_bmulti_walk<P, R>);
+        *per_wg = kBWaves;
+    }
+    return 0;
+}
+
 }  // namespace
 
 #define TSP_DISPATCH_P(P, CALL)                                   \
@@ -878,6 +938,32 @@ int launch_ctr_add(int *ctr, int value, hipStream_t s)
 }
 
 int batch_kp(int P) { return 16 * ((2 * nln_of(P) + 15) / 16); }
+int batch_tiles(int R) { return R >= 32 ? R / 32 : 1; }
+
+int batch_multi_kernel(int P, int R, const void **kern, int *items_per_wg)
+{
+    if (R == 8) {
+#define CALL(p) bmulti_p<p, 8>(kern, items_per_wg)
+        TSP_DISPATCH_P(P, CALL)
+#undef CALL
+    }
+    if (R == 16) {
+#define CALL(p) bmulti_p<p, 16>(kern, items_per_wg)
+        TSP_DISPATCH_P(P, CALL)
+#undef CALL
+    }
+    if (R == 32) {
+#define CALL(p) bmulti_p<p, 32>(kern, items_per_wg)
+        TSP_DISPATCH_P(P, CALL)
+#undef CALL
+    }
+    if (R == 64) {
+#define CALL(p) bmulti_p<p, 64>(kern, items_per_wg)
+        TSP_DISPATCH_P(P, CALL)
+#undef CALL
+    }
+    return -2;
+}
 size_t batch_block_floats(int P)
 {
     switch (P) {

@@ -4,11 +4,13 @@
 // batching: admissions and retirements happen between chunks of steps, while
 // the others keep their fields resident), and leave with their seismograms.
 //
-// A request is one mesh with 1..16 point sources.  Two ways to run it:
+// A request is one mesh with 1..16 point sources.  Three ways to run it:
 //   columns (default): one multi-column context of the narrowest width (8, 16,
 //     32 or 64) that holds the sources, one source per column, stepped as a
 //     block SpMM on the MFMA cores; every running request's chunk of steps is
 //     queued on its own stream, so requests overlap on the device;
+//   batched: the same multi-column context, added to one tsp_multi set: a step
+//     of everything running is one launch per (degree, width) present;
 //   instances: one owner context that assembles the operator and (nsrc-1)
 //     contexts sharing it, all added to one tsp_multi set (one launch per degree
 //     per step for everything running).
@@ -49,11 +51,11 @@ struct Request {
     std::vector<double> verts, lambda, mu, rho, src_phi, wavelet, rec_phi;
     std::vector<int> neigh, neighedge, src_elem, rec_elem;
     long long footprint = 0;
-    std::vector<int> inst;        // multi-set ids, owner first (instances mode)
-    tsp_context *ctx = nullptr;   // the multi-column context (columns mode)
+    std::vector<int> inst;        // multi-set ids, owner first (instances and batched modes)
+    tsp_context *ctx = nullptr;   // the multi-column context (columns mode; batched until added)
     int ncols = 0, done = 0;
     std::vector<double> seis;     // [src][sample][rec][2] (instances mode)
-    double *pinned = nullptr;     // [sample][rec][src][2] (columns mode, page-locked)
+    double *pinned = nullptr;     // [sample][rec][src][2] (columns and batched modes, page-locked)
     int nsamples = 0;
     int state = 0;                // 0 queued, 1 running, 2 done, 3 failed, 4 results in flight
     std::string error;
@@ -78,7 +80,7 @@ long long footprint_columns(const Request &r)
 
 long long footprint_of(const Request &r, int mode)
 {
-    if (mode == TSP_SERVE_COLUMNS) return footprint_columns(r);
+    if (mode != TSP_SERVE_INSTANCES) return footprint_columns(r);
     const long long nln = (long long)(r.N + 1) * (r.N + 2) / 2, d = 2 * nln;
     const long long op32 = (long long)r.ne * 4 * d * d * 4;       // FP32 column-quad operator
     const long long per_col = (long long)r.ne * d * 4 * 4          // uhi x2, comp, increment
@@ -210,11 +212,29 @@ void finish_columns(Request &r)
     r.state = 2;
 }
 
+// batched mode: the request's multi-column context joins the set
+bool admit_batched(tsp_server *s, Request &r)
+{
+    if (!admit_columns(s, r)) return false;
+    int id = -1;
+    if (tsp_multi_add(s->multi, r.ctx, r.nsteps, &id) != TSP_OK) {
+        r.error = std::string("add: ") + tsp_multi_last_error(s->multi);
+        tsp_destroy(r.ctx);
+        r.ctx = nullptr;
+        r.state = 3;
+        return false;
+    }
+    r.ctx = nullptr;   // owned by the set now
+    r.inst.assign(1, id);
+    return true;
+}
+
 // build the contexts of one request and add them to the set; on failure every
 // context made so far is destroyed and the request is marked failed
 bool admit(tsp_server *s, Request &r)
 {
     if (s->mode == TSP_SERVE_COLUMNS) return admit_columns(s, r);
+    if (s->mode == TSP_SERVE_BATCHED) return admit_batched(s, r);
     const Tables &t = tables_for(s, r.N, r.basis);
     const int nln = (r.N + 1) * (r.N + 2) / 2;
     std::vector<tsp_context *> made;
@@ -271,6 +291,22 @@ bool admit(tsp_server *s, Request &r)
     return true;
 }
 
+// batched mode: the set has synchronised its stream (tsp_multi_run), so the
+// request's seismograms are complete; copy them out and drop the instance
+bool retire_batched(tsp_server *s, Request &r)
+{
+    r.nsamples = 1 + r.nsteps / std::max(r.stride, 1);
+    r.pinned = static_cast<double *>(tsp_host_alloc(sizeof(double) * (size_t)r.nsamples * r.nrec * r.nsrc * 2));
+    tsp_context *c = tsp_multi_context(s->multi, r.inst[0]);
+    int ns = 0;
+    const bool ok = r.pinned && tsp_get_seismograms_cols(c, 0, r.nsrc, r.pinned, r.nsamples, &ns, 0) == TSP_OK;
+    if (!ok) r.error = std::string("seismograms: ") + tsp_last_error(c);
+    tsp_multi_remove(s->multi, r.inst[0]);
+    r.inst.clear();
+    r.state = ok ? 2 : 3;
+    return ok;
+}
+
 void retire(tsp_server *s, Request &r)
 {
     r.nsamples = 1 + r.nsteps / std::max(r.stride, 1);
@@ -324,7 +360,8 @@ void tsp_server_destroy(tsp_server *s)
 
 int tsp_server_set_mode(tsp_server *s, int mode)
 {
-    if (!s || (mode != TSP_SERVE_INSTANCES && mode != TSP_SERVE_COLUMNS)) return TSP_ERR_ARG;
+    if (!s || (mode != TSP_SERVE_INSTANCES && mode != TSP_SERVE_COLUMNS && mode != TSP_SERVE_BATCHED))
+        return TSP_ERR_ARG;
     if (!s->req.empty()) {
         s->err = "the mode is fixed once requests were submitted";
         return TSP_ERR_STATE;
@@ -359,7 +396,8 @@ Request sizes_of(const tsp_request *q)
 
 int tsp_server_footprint(const tsp_request *q, int mode, long long *resident, long long *peak)
 {
-    if (!request_ok(q) || (mode != TSP_SERVE_INSTANCES && mode != TSP_SERVE_COLUMNS)) return TSP_ERR_ARG;
+    if (!request_ok(q) || (mode != TSP_SERVE_INSTANCES && mode != TSP_SERVE_COLUMNS && mode != TSP_SERVE_BATCHED))
+        return TSP_ERR_ARG;
     const Request r = sizes_of(q);
     if (resident) *resident = footprint_of(r, mode);
     if (peak) *peak = assembly_peak(r);
@@ -508,8 +546,16 @@ int tsp_server_tick(tsp_server *s, int *completed, int *running, int *queued)
             int steps = 0, n = 0;
             tsp_multi_info(s->multi, r.inst[0], &steps, &n);
             if (steps >= n) {
-                retire(s, r);
+                bool ok = true;
+                if (s->mode == TSP_SERVE_BATCHED) ok = retire_batched(s, r);
+                else retire(s, r);
                 r.completed = t;
+                if (!ok) {
+                    s->used -= r.footprint;
+                    ++done;
+                    it = s->running.erase(it);
+                    continue;
+                }
                 s->used -= r.footprint;
                 s->t_last = t;
                 s->dof_steps += (long long)r.ne * (r.N + 1) * (r.N + 2) * r.nsrc * r.nsteps;

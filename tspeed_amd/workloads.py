@@ -281,23 +281,25 @@ def serve(requests, rate, seed=0, mem_fraction=0.8, chunk=32, stride=1, server=N
     return srv, ids, np.array(lat), srv.now() - t0
 
 
-def prepare_instance(prob: Problem, share=None, stride=None, nsteps=None):
-    """A single-column FP32-compensated Engine for `prob` (operator assembled, or
-    borrowed from `share`), sources/receivers set, first step taken: ready to be
-    added to a Multi set."""
+def prepare_instance(prob: Problem, share=None, stride=None, nsteps=None, ncols=1):
+    """An FP32-compensated Engine for `prob` (operator assembled, or borrowed from
+    `share`), sources/receivers set, first step taken: ready to be added to a
+    Multi set.  ncols > 1: a multi-column context, source k in column k."""
     nsteps = prob.nsteps if nsteps is None else nsteps
     stride = prob.stride if stride is None else stride
     if share is None:
-        eng = capi.Engine("fp32c")
+        eng = capi.Engine("fp32c", ncols=ncols)
         set_space(eng, prob)
         eng.set_mesh(prob.verts, prob.neigh, prob.neighedge, prob.lam, prob.mu, prob.rho, prob.penalty)
         eng.release_fp64()
     else:
         eng = capi.Engine(share=share)
-    eng.set_sources(prob.src_elem, prob.src_phi, prob.wavelet(nsteps + 1))
+    cols = np.arange(len(prob.src_elem)) if ncols > 1 else None
+    eng.set_sources(prob.src_elem, prob.src_phi, prob.wavelet(nsteps + 1), col=cols)
     eng.set_receivers(prob.rec_elem, prob.rec_phi, stride, 1 + nsteps // max(stride, 1))
     eng.set_dt(prob.dt)
-    eng.set_state()
+    for c in range(ncols):
+        eng.set_state(None, None, col=c)
     eng.first_step()
     return eng
 
