@@ -208,12 +208,21 @@ def run_serving(n_req=32, loads=(0.5, 0.8), sample_frac=0.05, seed=11):
     out = dict(requests=n_req, dof_steps=work,
                workload="serving: Poisson arrivals, HBM-footprint admission at 80%, continuous batching "
                         "(chunks of 32 steps); each request one MFMA context of the narrowest width (8/16/32/64 "
-                        "columns) that holds its sources, on its own stream")
+                        "columns) that holds its sources, on its own stream (batched mode: the same contexts in one "
+                        "multi-instance set, one launch per degree and width per step)")
     # the other execution mode (one instance per source in the shared set), saturated, for comparison
     srv, ids, lat, wall = W.serve(reqs, rate=0, mode=capi.Server.INSTANCES)
     st = srv.stats()
     out["instances_mode_saturated"] = dict(value=st["dof_steps"] / st["busy_s"], wall_s=wall,
                                            p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)))
+    srv.close()
+    # the same contexts batched into one set (one launch per degree and width per step)
+    srv, ids, lat, wall = W.serve(reqs, rate=0, mode=capi.Server.BATCHED)
+    st = srv.stats()
+    out["batched_mode_saturated"] = dict(value=st["dof_steps"] / st["busy_s"], wall_s=wall,
+                                         p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                                         host_admit_s=st["admit_s"], host_run_s=st["run_s"],
+                                         host_retire_s=st["retire_s"])
     srv.close()
     srv, ids, lat, wall = W.serve(reqs, rate=0)
     st = srv.stats()
