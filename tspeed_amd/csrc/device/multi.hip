@@ -263,9 +263,19 @@ __global__ __launch_bounds__(64 * kMWaves) void k_multi32s(const MInst *__restri
     __shared__ float part[kMWaves - 1][2][G::RPL][64];
     const int lane = threadIdx.x & 63;
     const int s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int step = PERSIST ? gridDim.x : nitems;
-    for (int it = blockIdx.x; it < nitems; it += step) {
-        // the whole work-group takes the same item: every branch below is uniform over it
+    if (!PERSIST) {
+        // one item per work-group (the whole work-group takes it: every branch
+        // below is uniform over it); without the loop the compiler keeps fewer
+        // registers live (p = 7, 8: 104-124 VGPRs instead of 142-174 in the
+        // half-slot kernel)
+        if ((int)blockIdx.x >= nitems) return;
+        const int2 t = items[blockIdx.x];
+        const MInst I = insts[t.x];
+        if (!I.live || I.steps >= I.nsteps) return;
+        multi_elem_slots<G>(I, t.y, lane, s, xs_all[s], part);
+        return;
+    }
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int2 t = items[it];
         const MInst I = insts[t.x];
         if (!I.live || I.steps >= I.nsteps) continue;
@@ -333,9 +343,16 @@ __global__ __launch_bounds__(64 * kCWaves) void k_multi32w(const MInst *__restri
     __shared__ float part[kCWaves - 1][G::RPL][64];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int step = PERSIST ? gridDim.x : nitems;
-    for (int it = blockIdx.x; it < nitems; it += step) {
-        // the whole work-group takes the same item: every branch below is uniform over it
+    if (!PERSIST) {
+        // one item per work-group, as k_multi32s
+        if ((int)blockIdx.x >= nitems) return;
+        const int2 t = items[blockIdx.x];
+        const MInst I = insts[t.x];
+        if (!I.live || I.steps >= I.nsteps) return;
+        multi_elem_chunks<G>(I, t.y, lane, w, xs_all[w], part);
+        return;
+    }
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int2 t = items[it];
         const MInst I = insts[t.x];
         if (!I.live || I.steps >= I.nsteps) continue;
