@@ -879,7 +879,7 @@ int tsp_first_step(tsp_context *c)
             c->cur = 1;           // u^n = u1 in u64[1]; u^{n-1} = u0 in u64[0]
         } else if (c->batched()) {
             const size_t n = c->state_len();
-            kcheck(c, launch_split_state(n, c->u64[0], c->uhi[0], c->ucomp, c->stream), "split");
+            kcheck(c, launch_split_b(n, c->u64[0], c->uhi[0], c->ucomp, c->stream), "split");
             kcheck(c, launch_scale_demote(c->ne, c->invm, -(dt * dt / 2), c->sc_first, c->stream), "sc");
             kcheck(c, launch_scale_demote(c->ne, c->invm, (-dt) * dt, c->sc_step, c->stream), "sc");
             kcheck(c, launch_scale_demote(n, c->u64[1], dt, c->winc, c->stream), "winc");

@@ -137,7 +137,8 @@ int launch_apply_cm(int P, int mode, int ne, const int4 *meta, const double *blk
                     const double *minv, const double *x, double *y, hipStream_t s);
 
 // Many right-hand sides (spmm.hip): R = ncols columns, state in row octets
-// u[((e*KB + k/8)*R + j)*8 + k%8], KB = KP/8.
+// u[((e*KB + k/8)*R + j)*8 + k%8], KB = KP/8; the compensation array (ucomp)
+// holds BF16 entries at the same indices (2 bytes each).
 struct BatchArgs {
     int ne, nsrc, nrec;
     const int4 *meta;
@@ -190,6 +191,8 @@ int launch_pack_mfma(int P, int ne, const int4 *meta, const double *src, float *
 int launch_step_b16x3(int P, int ncols, const BatchArgs &a, hipStream_t s);
 int launch_pack_bf16x3(int P, int ne, const int4 *meta, const double *src, float *dst, hipStream_t s);
 int launch_sample_b32(int P, int ncols, const BatchArgs &a, hipStream_t s);
+// multi-column state: FP64 staging -> FP32 hi + BF16 compensation (spmm.hip)
+int launch_split_b(size_t n, const double *u64, float *hi, float *comp, hipStream_t s);
 int launch_col_in(int ne, int nln, int KP, int R, int col, const double *cm, double *st, hipStream_t s);
 int launch_col_out(int ne, int nln, int KP, int R, int col, const float *hi, const float *comp, double *cm,
                    hipStream_t s);

@@ -63,6 +63,52 @@ __device__ inline void st4_l2(float *block, int off_bytes, f32x4 v)
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_raw, v), r, off_bytes, 0, kCachePolicySC1);
 }
 
+// The compensation of the multi-column state is kept in BF16 (rounded to
+// nearest even), 2 bytes per entry: the update still runs in FP32 -- the
+// compensation is widened on load and rounded on store, so the displacement
+// carries its FP32 value plus the compensation's leading 8 bits (~32 bits in
+// all).  The step's stores were its costliest part (on the 64-source square, a
+// copy without them stepped in 21.6 us against 31.3); with the compensation in
+// 2 bytes the state moves 20 instead of 24 bytes per entry and step
+// (31.3 -> 28.4 us in the same copy).  A CPU restatement of the update on
+// column 21 of that workload (3000 steps) gives seismograms within 6.5e-6 of
+// FP64 with the BF16 compensation, 5.3e-6 with an FP32 one and 6.0e-6 with none.
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2_raw __attribute__((__vector_size__(2 * sizeof(unsigned int))));
+__device__ inline unsigned short to_bf16(float x)
+{
+    unsigned b = __float_as_uint(x);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return (unsigned short)(b >> 16);
+}
+__device__ inline float from_bf16(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+__device__ inline u16x4 pack_bf16(f32x4 v) { return u16x4{to_bf16(v[0]), to_bf16(v[1]), to_bf16(v[2]), to_bf16(v[3])}; }
+// four compensation entries at entry index idx of a compensation array
+__device__ inline f32x4 ldc4(const float *comp, size_t idx)
+{
+    const u16x4 c = *reinterpret_cast<const u16x4 *>(reinterpret_cast<const unsigned short *>(comp) + idx);
+    return f32x4{from_bf16(c[0]), from_bf16(c[1]), from_bf16(c[2]), from_bf16(c[3])};
+}
+__device__ inline void stc4(float *comp, size_t idx, f32x4 v)
+{
+    *reinterpret_cast<u16x4 *>(reinterpret_cast<unsigned short *>(comp) + idx) = pack_bf16(v);
+}
+__device__ inline float ldc1(const float *comp, size_t idx)
+{
+    return from_bf16(reinterpret_cast<const unsigned short *>(comp)[idx]);
+}
+// stc4 written through L2 as st4_l2: eb = entry index of the element's block,
+// off_bytes = byte offset of the entries in its FP32 layout (NBYTES = bytes of
+// the element's block in FP32)
+template <int NBYTES>
+__device__ inline void stc4_l2(float *comp, size_t eb, int off_bytes, f32x4 v)
+{
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        reinterpret_cast<unsigned short *>(comp) + eb, 0, NBYTES / 2, kRawBufferWord3);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_raw, pack_bf16(v)), r, off_bytes / 2, 0,
+                                          kCachePolicySC1);
+}
+
 __device__ inline int xcd_remap_b(int bid, int nwg)
 {
     const int per = nwg / 8, rem = nwg % 8;
@@ -208,11 +254,11 @@ __device__ inline void b_update(const BatchArgs &a, int e, int c0, int half, int
                         constexpr int NB = G::KB * R * 8 * 4;
                         const size_t eb = (size_t)e * G::KB * R * 8;
                         const int off = (int)(idx - eb) * 4;  // within element e's block
-                        st4_l2<NB>(a.ucomp + eb, off, cn);
+                        stc4_l2<NB>(a.ucomp, eb, off, cn);
                         st4_l2<NB>(a.uhi_out + eb, off, tn);
                         st4_l2<NB>(a.winc + eb, off, wn);
                     } else {
-                        st4(a.ucomp + idx, cn);
+                        stc4(a.ucomp, idx, cn);
                         st4(a.uhi_out + idx, tn);
                         st4(a.winc + idx, wn);
                     }
@@ -349,7 +395,7 @@ __device__ inline void step_walk(const BatchArgs &a, int gw)
                 for (int nt = 0; nt < NT; ++nt) {
                     const size_t idx = ebase + (size_t)kb * R * 8 + nt * 256;
                     wv[kb][nt] = colok ? ld4(a.winc + idx) : f32x4{};
-                    cp[kb][nt] = colok ? ld4(a.ucomp + idx) : f32x4{};
+                    cp[kb][nt] = colok ? ldc4(a.ucomp, idx) : f32x4{};
                 }
         }
     };
@@ -455,7 +501,7 @@ __device__ inline void step_slots(const BatchArgs &a, int item)
                 const size_t idx = ebase + (size_t)(s + 4 * mt) * R * 8;
                 uh[mt] = *reinterpret_cast<const f32x4 *>(a.uhi + idx);
                 wv[mt] = ld4(a.winc + idx);
-                cp[mt] = ld4(a.ucomp + idx);
+                cp[mt] = ldc4(a.ucomp, idx);
             }
     };
     if (act) {
@@ -565,7 +611,7 @@ __device__ inline void step_slots(const BatchArgs &a, int item)
             constexpr int NB = KB * R * 8 * 4;
             const size_t eb = (size_t)e * KB * R * 8;
             const int off = (int)(lofs + (size_t)o * R * 8) * 4;
-            st4_l2<NB>(a.ucomp + eb, off, cn);
+            stc4_l2<NB>(a.ucomp, eb, off, cn);
             st4_l2<NB>(a.uhi_out + eb, off, tn);
             st4_l2<NB>(a.winc + eb, off, wn);
         }
@@ -658,7 +704,7 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3(BatchArgs a)
                 const size_t idx = ebase + (size_t)kb * R * 8;
                 uhv[kb][0] = colok ? *reinterpret_cast<const f32x4 *>(a.uhi + idx) : f32x4{};
                 wv[kb][0] = colok ? ld4(a.winc + idx) : f32x4{};
-                cp[kb][0] = colok ? ld4(a.ucomp + idx) : f32x4{};
+                cp[kb][0] = colok ? ldc4(a.ucomp, idx) : f32x4{};
             }
         }
     };
@@ -772,7 +818,7 @@ __global__ __launch_bounds__(64) void k_sample_b32(BatchArgs a, int R)
         double sx = 0.0, sy = 0.0;
         for (int i = 0; i < G::D; ++i) {
             const size_t idx = oct_index(e, i, j, G::KB, R);
-            const double u = (double)a.uhi[idx] - (double)a.ucomp[idx];
+            const double u = (double)a.uhi[idx] - (double)ldc1(a.ucomp, idx);
             const double v = a.rec_phi[rec * G::NLN + i % G::NLN] * u;
             if (i < G::NLN) sx += v; else sy += v;
         }
@@ -780,6 +826,16 @@ __global__ __launch_bounds__(64) void k_sample_b32(BatchArgs a, int R)
         a.seis[o] = sx;
         a.seis[o + 1] = sy;
     }
+}
+
+// FP64 staging -> FP32 hi + BF16 compensation (true value = hi - comp)
+__global__ void k_split_b(size_t n, const double *u, float *hi, float *comp)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const float h = (float)u[q];
+    hi[q] = h;
+    reinterpret_cast<unsigned short *>(comp)[q] = to_bf16((float)((double)h - u[q]));
 }
 
 // component-major column `col` (FP64) <-> batched staging layout (FP64)
@@ -805,7 +861,7 @@ __global__ void k_col_out(int ne, int nln, int KP, int R, int col, const float *
     const size_t e = rem / nln;
     const int i = (int)(rem % nln);
     const size_t idx = oct_index((int)e, c * nln + i, col, KP / 8, R);
-    cm[q] = (double)hi[idx] - (double)comp[idx];
+    cm[q] = (double)hi[idx] - (double)ldc1(comp, idx);
 }
 
 // Launch shape per degree: one 32-column tile per wave (R/32 waves per
@@ -1044,6 +1100,12 @@ int launch_sample_b32(int P, int ncols, const BatchArgs &a, hipStream_t s)
 #define CALL(p) sample_b32_p<p>(a, ncols, s)
     TSP_DISPATCH_P(P, CALL)
 #undef CALL
+}
+
+int launch_split_b(size_t n, const double *u64, float *hi, float *comp, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_split_b, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, u64, hi, comp);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_col_in(int ne, int nln, int KP, int R, int col, const double *cm, double *st, hipStream_t s)
