@@ -36,7 +36,8 @@ enum tsp_status_code {
 
 enum tsp_precision {
     TSP_FP64 = 0,   /* plain FP64 blocks and update: the correctness path            */
-    TSP_FP32C = 1,  /* FP32 blocks, FP32 increments, Kahan-compensated displacement */
+    TSP_FP32C = 1,  /* FP32 blocks, FP32 increments, Kahan-compensated displacement
+                       (multi-column contexts keep the compensation in BF16) */
     TSP_BF16X3 = 2  /* as TSP_FP32C, with each block product split into three BF16
                        products accumulated in FP32 (multi-column contexts only);
                        the split keeps ~16 bits per operand: seismograms drift to
