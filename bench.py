@@ -93,9 +93,10 @@ def run_batched(steps, warmup, ncols=64, precision="fp32c"):
     nprod = 3 if precision == "bf16x3" else 1
     flops_mfma = 2.0 * 4 * prob.ne * kp * kp * ncols * nprod
     peak = 2500.0 if precision == "bf16x3" else 157.3
-    # operator once (4 B per entry either way) + per DOF and column: u^n read,
-    # u^{n+1} written, compensation and increment read and written (6 x 4 B)
-    nbytes = nblk * d * d * 4 + prob.ne * d * ncols * 24 + prob.ne * (8 + 16)
+    # operator once (4 B per entry either way) + per DOF and column: u^n read and
+    # u^{n+1} written (FP32), the increment read and written (FP32), the
+    # compensation read and written (BF16): 20 B
+    nbytes = nblk * d * d * 4 + prob.ne * d * ncols * 20 + prob.ne * (8 + 16)
     gbs = nbytes / (ms_step / 1e3) / 1e9
     return dict(value=prob.ndof * ncols * steps / (ms / 1e3), ms_per_step=ms_step, columns=ncols,
                 precision=precision,
@@ -257,6 +258,19 @@ def run_serving(n_req=32, loads=(0.5, 0.8), sample_frac=0.05, seed=11):
     return out
 
 
+def run_serving_mode(n_req, mode):
+    """One serving mode alone, all requests at once (capacity): the run to
+    profile a mode by itself."""
+    from tspeed_amd import capi, workloads as W
+    reqs = [W.serving_request(k) for k in range(n_req)]
+    md = {"columns": capi.Server.COLUMNS, "batched": capi.Server.BATCHED, "instances": capi.Server.INSTANCES}[mode]
+    srv, ids, lat, wall = W.serve(reqs, rate=0, mode=md)
+    st = srv.stats()
+    srv.close()
+    return dict(mode=mode, requests=n_req, value=st["dof_steps"] / st["busy_s"], wall_s=wall,
+                p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)))
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -336,6 +350,8 @@ def main():
                     help="skip the error runs against FP64 (profiling the timed legs alone)")
     ap.add_argument("--multi", type=int, default=256, help="instances in the uneven-batch variant (0 = skip)")
     ap.add_argument("--serve", type=int, default=32, help="requests in the serving variant (0 = skip)")
+    ap.add_argument("--serve-mode", default="all", choices=["all", "columns", "batched", "instances"],
+                    help="serving: every mode, loads and FP64 sample (all), or one mode saturated only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -396,7 +412,8 @@ def main():
     if args.multi > 0:
         variants["multi_fp32c"] = run_multi(min(args.steps, 200), min(args.warmup, 10), args.multi)
     if args.serve > 0:
-        variants["serving"] = run_serving(args.serve)
+        variants["serving"] = (run_serving(args.serve) if args.serve_mode == "all"
+                               else run_serving_mode(args.serve, args.serve_mode))
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_step64_layered_400x200_p6.json")
