@@ -38,10 +38,17 @@ enum tsp_precision {
     TSP_FP64 = 0,   /* plain FP64 blocks and update: the correctness path            */
     TSP_FP32C = 1,  /* FP32 blocks, FP32 increments, Kahan-compensated displacement
                        (multi-column contexts keep the compensation in BF16) */
-    TSP_BF16X3 = 2  /* as TSP_FP32C, with each block product split into three BF16
-                       products accumulated in FP32 (multi-column contexts only);
-                       the split keeps ~16 bits per operand: seismograms drift to
-                       ~4-9e-4 of FP64 over 3000 steps (TSP_FP32C: < 1e-5)       */
+    TSP_BF16X3 = 2, /* as TSP_FP32C, with each block product split into three BF16
+                       products accumulated in FP32 (multi-column contexts only):
+                       operator and state each kept to 16 bits (two BF16 parts),
+                       a_h x_h + a_h x_l + a_m x_h; the operator's 16 bits set the
+                       error: seismograms drift to ~4-9e-4 of FP64 over 3000 steps */
+    TSP_BF16X5 = 3  /* five BF16 products on the same MFMA path: the FP32 operator
+                       in three BF16 parts (all 24 bits), the state in two,
+                       a_h x_h + a_h x_l + a_m x_h + a_m x_l + a_l x_h; the BF16
+                       MFMA truncates its internal sums (24 bits below the largest
+                       term), so seismograms end 2-6e-5 from FP64 over 3000 steps
+                       (TSP_FP32C's FP32 products: 4-7e-6), ~6% faster          */
 };
 
 /* Status bits reported by tsp_status (no counterpart in the reference, which has
@@ -59,8 +66,9 @@ int tsp_device_memory(int device, long long *free_bytes, long long *total_bytes)
  * of tsp_precision; `ncols` the number of right-hand-side columns: 1 = the
  * reference's single simulation; 8, 16, 32 or 64 = that many simulations on
  * one mesh (one point source each), stepped together as a block SpMM on the
- * MFMA cores (TSP_FP32C or TSP_BF16X3).  The MFMA tile is 32 columns wide, so
- * 8 and 16 cost the operator stream and products of 32 but carry (and move)
+ * MFMA cores (TSP_FP32C, TSP_BF16X3 or TSP_BF16X5).  The MFMA tile is 32
+ * columns wide, so 8 and 16 cost the operator stream and products of 32 but
+ * carry (and move)
  * only their own columns' state.  Every column is summed on its own and every
  * width of a degree runs the same kernel, so a source's results are bitwise
  * the same at every width. */

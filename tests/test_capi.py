@@ -35,8 +35,9 @@ def test_null_and_bad_arguments_are_rejected_without_a_device():
     L = capi.lib()
     assert L.tsp_create(0, 99, 1, ctypes.byref(ctypes.c_void_p())) == capi.TSP_ERR_UNSUPPORTED
     assert L.tsp_create(0, 0, 1, None) == capi.TSP_ERR_ARG
-    # BF16x3 products exist only for multi-column contexts
-    assert L.tsp_create(0, 2, 1, ctypes.byref(ctypes.c_void_p())) == capi.TSP_ERR_UNSUPPORTED
+    # split BF16 products exist only for multi-column contexts
+    for prec in (capi.TSP_BF16X3, capi.TSP_BF16X5):
+        assert L.tsp_create(0, prec, 1, ctypes.byref(ctypes.c_void_p())) == capi.TSP_ERR_UNSUPPORTED
     assert L.tsp_run(None, 1) == capi.TSP_ERR_ARG
     assert L.tsp_steps_done(None) == -1
     assert L.tsp_last_error(None) == b"null context"

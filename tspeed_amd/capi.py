@@ -15,9 +15,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("TSPEED_AMD_LIB", os.path.join(HERE, "libtspeed_amd.so"))
 
 TSP_OK, TSP_ERR_ARG, TSP_ERR_STATE, TSP_ERR_HIP, TSP_ERR_UNSUPPORTED, TSP_ERR_NOMEM = range(6)
-TSP_FP64, TSP_FP32C, TSP_BF16X3 = 0, 1, 2
+TSP_FP64, TSP_FP32C, TSP_BF16X3, TSP_BF16X5 = 0, 1, 2, 3
 TSP_FLAG_NONFINITE = 1
-PRECISIONS = {"fp64": TSP_FP64, "fp32c": TSP_FP32C, "bf16x3": TSP_BF16X3}
+PRECISIONS = {"fp64": TSP_FP64, "fp32c": TSP_FP32C, "bf16x3": TSP_BF16X3, "bf16x5": TSP_BF16X5}
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

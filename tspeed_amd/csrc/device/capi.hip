@@ -298,8 +298,9 @@ BatchArgs batch_args(tsp_context *c);
 
 int launch_batched(tsp_context *c, const BatchArgs &b)
 {
-    return c->precision == TSP_BF16X3 ? launch_step_b16x3(c->P, c->ncols, b, c->stream)
-                                       : launch_step_b32(c->P, c->ncols, b, c->stream);
+    if (c->precision == TSP_BF16X3) return launch_step_b16(c->P, c->ncols, 3, b, c->stream);
+    if (c->precision == TSP_BF16X5) return launch_step_b16(c->P, c->ncols, 5, b, c->stream);
+    return launch_step_b32(c->P, c->ncols, b, c->stream);
 }
 
 // Multi-column runs replay one captured graph of kGraphSteps steps per
@@ -377,10 +378,8 @@ void operator_ready(tsp_context *c)
     if (c->batched()) {
         c->KP = batch_kp(c->P);
         if (!c->blkA) c->blkA = dalloc<float>(c, (size_t)c->ne * batch_block_floats(c->P), "blocks mfma");
-        if (c->precision == TSP_BF16X3)
-            kcheck(c, launch_pack_bf16x3(c->P, c->ne, c->meta, c->blk64, c->blkA, c->stream), "pack");
-        else
-            kcheck(c, launch_pack_mfma(c->P, c->ne, c->meta, c->blk64, c->blkA, c->stream), "pack");
+        // FP32 and split BF16 products read the same FP32 operator layout
+        kcheck(c, launch_pack_mfma(c->P, c->ne, c->meta, c->blk64, c->blkA, c->stream), "pack");
     } else if (c->precision == TSP_FP32C) {
         const size_t n = (size_t)c->ne * 4 * c->D * c->D;
         if (!c->blk32) c->blk32 = dalloc<float>(c, n, "blocks fp32");
@@ -510,10 +509,11 @@ int tsp_create(int device, int precision, int ncols, tsp_context **out)
 {
     if (!out) return TSP_ERR_ARG;
     *out = nullptr;
-    if (precision != TSP_FP64 && precision != TSP_FP32C && precision != TSP_BF16X3) return TSP_ERR_UNSUPPORTED;
+    const bool split = precision == TSP_BF16X3 || precision == TSP_BF16X5;
+    if (precision != TSP_FP64 && precision != TSP_FP32C && !split) return TSP_ERR_UNSUPPORTED;
     if (ncols != 1 && ncols != 8 && ncols != 16 && ncols != 32 && ncols != 64) return TSP_ERR_UNSUPPORTED;
     if (ncols > 1 && precision == TSP_FP64) return TSP_ERR_UNSUPPORTED;
-    if (ncols == 1 && precision == TSP_BF16X3) return TSP_ERR_UNSUPPORTED;
+    if (ncols == 1 && split) return TSP_ERR_UNSUPPORTED;
     tsp_context *c = new (std::nothrow) tsp_context;
     if (!c) return TSP_ERR_NOMEM;
     c->device = device;

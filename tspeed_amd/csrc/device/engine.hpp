@@ -188,8 +188,8 @@ int batch_kp(int P);                 // state rows per element (d padded to a mu
 size_t batch_block_floats(int P);    // floats of one element's four packed blocks
 int launch_step_b32(int P, int ncols, const BatchArgs &a, hipStream_t s);
 int launch_pack_mfma(int P, int ne, const int4 *meta, const double *src, float *dst, hipStream_t s);
-int launch_step_b16x3(int P, int ncols, const BatchArgs &a, hipStream_t s);
-int launch_pack_bf16x3(int P, int ne, const int4 *meta, const double *src, float *dst, hipStream_t s);
+// split BF16 products on the FP32 operator layout: nprod = 3 or 5
+int launch_step_b16(int P, int ncols, int nprod, const BatchArgs &a, hipStream_t s);
 int launch_sample_b32(int P, int ncols, const BatchArgs &a, hipStream_t s);
 // multi-column state: FP64 staging -> FP32 hi + BF16 compensation (spmm.hip)
 int launch_split_b(size_t n, const double *u64, float *hi, float *comp, hipStream_t s);

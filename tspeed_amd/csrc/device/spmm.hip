@@ -131,16 +131,13 @@ struct BGeom {
     static constexpr int BLK4 = (MT - 1) * KB * 64 + KB * 2 * LR;
     static constexpr int BLKF = 4 * BLK4;     // floats per block
     __host__ __device__ static constexpr bool part(int mt) { return LR < 32 && mt == MT - 1; }
-    // 16-byte units of one FP32 fragment of row tile mt (one BF16 hi or lo half)
+    // 16-byte units of one FP32 fragment of row tile mt
     __host__ __device__ static constexpr int fsize(int mt) { return mt == MT - 1 ? 2 * LR : 64; }
     // FP32 A-fragment (mt, kb)
     __host__ __device__ static constexpr int frag(int mt, int kb) { return mt * KB * 64 + kb * fsize(mt); }
-    // BF16 hi/lo fragment pair (mt, K-chunk c): hi at frag16, lo at frag16 + fsize(mt)
-    __host__ __device__ static constexpr int frag16(int mt, int c) { return mt * KB * 64 + c * 2 * fsize(mt); }
     // where row NLN (mode 0 of the second component) sits: row octet, the half
-    // of the wave holding it in a 16-byte piece, and its place in the piece /
-    // in the octet
-    static constexpr int KBN = NLN / 8, HN = (NLN % 8) / 4, QN = NLN % 4, Q8N = NLN % 8;
+    // of the wave holding it in a 16-byte piece, and its place in the piece
+    static constexpr int KBN = NLN / 8, HN = (NLN % 8) / 4, QN = NLN % 4;
 };
 
 // Lane l's 16-byte piece of a fragment: full fragments hold all 64 lanes; the
@@ -157,27 +154,82 @@ __device__ inline f32x4 frag_ld(const f32x4 *base, int mt, int lane)
     return base[lane];
 }
 
+// Block products on the BF16 MFMA (v_mfma_f32_32x32x16_bf16), accumulating in
+// FP32, from the FP32 operator and state.  Each FP32 operator entry is split
+// into BF16 parts a = a_h + a_m + a_l (exact for normal values: every residual
+// of an FP32 value and its BF16 rounding is again an FP32 value with fewer
+// bits, and three parts carry all 24), each state entry into x = x_h + x_l
+// (16 bits), and, largest terms first,
+//   NP = 3:  a x ~ a_h x_h + a_h x_l + a_m x_h                  (a to 16 bits)
+//   NP = 5:  a x ~ a_h x_h + a_h x_l + a_m x_h + a_m x_l + a_l x_h
+// NP = 3 drops a_l entirely: the operator is then perturbed by ~2^-17 of each
+// entry, a fixed error of B that accumulates as a phase error (64-source
+// square, 3000 steps: seismograms ~9e-4 from FP64).  NP = 5 keeps the operator
+// whole; its remaining error is the MFMA's own summation, measured on this
+// GPU with single-instruction probes: the instruction aligns the accumulator
+// and its 16 exact products to the largest of them and keeps 24 bits below
+// it, truncating each product toward zero and flooring the accumulator
+// (toward -inf) when the products dominate it; v_mfma_f32_32x32x2f32 keeps 23
+// bits but rounds to nearest.  Those one-sided cuts act as a small systematic
+// change of B: NP = 5 ends 1.6e-5 - 6.2e-5 from FP64 on that workload (FP32
+// products 3.6e-6 - 7.2e-6; a CPU restatement with exact sums predicts
+// 6.8e-6).  Largest terms first keeps the accumulator the dominant term of
+// every instruction after the first, so it is seldom floored: 6.2e-5 against
+// 1.25e-4 with the smallest terms first; separate accumulators for the small
+// terms gave 5.2e-5 for 16 more registers.
+//
+// Operands: K-chunk c (16 contraction rows) is FP32 k-octets 2c and 2c + 1.
+// A lane of half h holds, in its 8 BF16 slots, contraction indices
+// 16c + 4h + 0..3 (from octet 2c, the FP32 fragment's four values) and
+// 16c + 8 + 4h + 0..3 (octet 2c + 1); its B operand holds the same rows of
+// its column, the FP32 kernels' two 16-byte pieces.  The MFMA sums over the
+// 16 slots of both halves, i.e. over all 16 rows of the chunk, whatever their
+// order, so the FP32 operator layout serves both product forms.
+template <int NP>
+__device__ inline f32x16 bsplit_mfma(f32x4 a0, f32x4 a1, f32x4 x0, f32x4 x1, f32x16 acc)
+{
+    static_assert(NP == 3 || NP == 5, "three or five split products");
+    bf16x8 ah, am, al, xh, xl;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const float av = q < 4 ? a0[q] : a1[q - 4];
+        const __bf16 h = (__bf16)av;
+        const float r = av - (float)h;
+        const __bf16 m = (__bf16)r;
+        ah[q] = h;
+        am[q] = m;
+        if constexpr (NP == 5) al[q] = (__bf16)(r - (float)m);
+        const float xv = q < 4 ? x0[q] : x1[q - 4];
+        const __bf16 g = (__bf16)xv;
+        xh[q] = g;
+        xl[q] = (__bf16)(xv - (float)g);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, xh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, xl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, xh, acc, 0, 0, 0);
+    if constexpr (NP == 5) {
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, xl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, xh, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
 // Translation-free products (the element's mode-0 coefficients, rows 0 and
 // NLN of its own columns, subtracted from those rows of every slot; exact in
 // the algebra because every row of B annihilates a rigid translation -- see
 // gather in step_common.hpp for the accuracy this buys in FP32).  The two
 // values of this lane's column, or 0 where the lane holds neither row; `base`
-// points at the lane's column in the element's first row octet, `piece` is 4
-// (16-byte pieces, lanes of half h hold rows 8 kb + 4 h ..) or 8 (whole octets).
-template <int P, int R, int PIECE>
+// points at the lane's 16-byte piece of its column in the element's first row
+// octet (lanes of half h hold rows 8 kb + 4 h .. 8 kb + 4 h + 3).
+template <int P, int R>
 __device__ inline void b_translation(const BatchArgs &a, const float *base, int half, bool colok, float &t0,
                                      float &tn)
 {
     using G = BGeom<P>;
     t0 = tn = 0.f;
     if (!a.tfree || !colok) return;
-    if constexpr (PIECE == 4) {
-        if (half == 0) t0 = base[0];
-        if (half == G::HN) tn = base[(size_t)G::KBN * R * 8 + G::QN];
-    } else {
-        if (half == 0) t0 = base[0];
-        if (half == G::KBN % 2) tn = base[(size_t)G::KBN * R * 8 + G::Q8N];
-    }
+    if (half == 0) t0 = base[0];
+    if (half == G::HN) tn = base[(size_t)G::KBN * R * 8 + G::QN];
 }
 
 // Waves per element and columns per wave: a wave owns one 32-column MFMA tile;
@@ -342,7 +394,7 @@ __global__ void k_ctr_add(int *ctr, int value) { *ctr += value; }
 // branch.  Padded rows stay zero: their operator rows and columns are zero and
 // they carry no source.
 // gw = the wave's work item: element gw / WPE, column tile gw % WPE.
-template <int P, int R, int PD>
+template <int P, int R, int PD, int NP>
 __device__ inline void step_walk(const BatchArgs &a, int gw)
 {
     using G = BGeom<P>;
@@ -401,21 +453,39 @@ __device__ inline void step_walk(const BatchArgs &a, int gw)
     };
     // translation of this lane's column (own element, rows 0 and NLN)
     float t0 = 0.f, tn = 0.f;
-    b_translation<P, R, 4>(a, a.uhi + ebase, half, colok, t0, tn);
+    b_translation<P, R>(a, a.uhi + ebase, half, colok, t0, tn);
     auto product = [&](int s, int buf) {
+        // the slot's columns, less the translation (rows 0 and NLN)
+        f32x4 xs[KB][NT];
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int nt = 0; nt < NT; ++nt) {
+                xs[kb][nt] = s == 0 ? own[kb][nt] : bv[buf][kb][nt];
+                if (kb == 0) xs[kb][nt][0] = xs[kb][nt][0] - t0;
+                if (kb == G::KBN) xs[kb][nt][G::QN] = xs[kb][nt][G::QN] - tn;
+            }
+        if constexpr (NP == 0) {
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                        for (int nt = 0; nt < NT; ++nt)
+                            acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[buf][kb][mt][t], xs[kb][nt][t],
+                                                                               acc[mt][nt], 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int c = 0; c < KB / 2; ++c)
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-                    for (int nt = 0; nt < NT; ++nt) {
-                        float x = s == 0 ? own[kb][nt][t] : bv[buf][kb][nt][t];
-                        if (kb == 0 && t == 0) x = x - t0;
-                        if (kb == G::KBN && t == G::QN) x = x - tn;
-                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[buf][kb][mt][t], x, acc[mt][nt], 0, 0, 0);
-                    }
+                    for (int nt = 0; nt < NT; ++nt)
+                        acc[mt][nt] = bsplit_mfma<NP>(av[buf][2 * c][mt], av[buf][2 * c + 1][mt], xs[2 * c][nt],
+                                                      xs[2 * c + 1][nt], acc[mt][nt]);
+        }
     };
 #pragma unroll
     for (int s = 0; s < PD && s < 4; ++s) load(s, s % NB);
@@ -437,14 +507,24 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 {
     resolve_step(a);
     const int wave = threadIdx.x >> 6;
-    step_walk<P, R, PD>(a, xcd_remap_b(blockIdx.x, gridDim.x) * kBWaves + wave);
+    step_walk<P, R, PD, 0>(a, xcd_remap_b(blockIdx.x, gridDim.x) * kBWaves + wave);
+}
+
+// The same step with split BF16 products (bsplit_mfma, NP = 3 or 5).
+template <int P, int R, int PD, int NP>
+__global__ __launch_bounds__(64 * kBWaves) void k_step_b16(BatchArgs a)
+{
+    resolve_step(a);
+    const int wave = threadIdx.x >> 6;
+    step_walk<P, R, PD, NP>(a, xcd_remap_b(blockIdx.x, gridDim.x) * kBWaves + wave);
 }
 
 // One slot per wave, for blocks of one or two 32-row tiles (p <= 6), at every
 // width: a work-group of four waves takes one (element, 32-column tile).  Wave
-// s multiplies block (e, s) into its own accumulators (FP32 products, or BF16x3
-// when BF) and leaves them in LDS; then wave g updates row octets g and g + 4
-// of the tile, adding the four partial sums in slot order, so no wave holds
+// s multiplies block (e, s) into its own accumulators (FP32 products when
+// NP = 0, else NP split BF16 products, bsplit_mfma) and leaves them in LDS;
+// then wave g updates row octets g and g + 4 of the tile, adding the four
+// partial sums in slot order, so no wave holds
 // the whole tile's update operands (p <= 4: 6-7 waves per SIMD instead of 3;
 // p = 5, 6: 2-3 instead of 1).  Wave 1 evaluates the element's source term
 // first; elements with receivers take a second barrier and the four waves
@@ -452,11 +532,11 @@ __global__ __launch_bounds__(64 * kBWaves) void k_step_b32(BatchArgs a)
 // us at 64 columns against 39.5 in the one-wave-per-tile form, 15.2 against
 // 21.2 at 8; at p = 5, 8 columns, 45.6 against 59.6.  Blocks of three row
 // tiles (p = 7, 8) keep the one-wave-per-tile kernels.
-template <int P, int R, bool BF>
+template <int P, int R, int NP>
 __device__ inline void step_slots(const BatchArgs &a, int item)
 {
     using G = BGeom<P>;
-    constexpr int MT = G::MT, KB = G::KB, KC = KB / 2, NO = 4 * MT;  // NO row octets per tile column
+    constexpr int MT = G::MT, KB = G::KB, NO = 4 * MT;  // NO row octets per tile column
     constexpr int WPE = BCols<R>::WPE, CW = BCols<R>::CW;
     static_assert(kBWaves == 4 && MT <= 2, "one wave per slot, MT row octets per wave");
     // part[s][o][lane]: registers 4g..4g+3 of slot s's accumulator of row tile
@@ -506,22 +586,22 @@ __device__ inline void step_slots(const BatchArgs &a, int item)
     };
     if (act) {
         const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * G::BLKF);
-        if constexpr (!BF) {
-            const float *X = a.uhi + (size_t)nb * KB * R * 8 + lofs;
-            f32x4 av[KB][MT], bv[KB];
+        const float *X = a.uhi + (size_t)nb * KB * R * 8 + lofs;
+        f32x4 av[KB][MT], bv[KB];
 #pragma unroll
-            for (int kb = 0; kb < KB; ++kb)
+        for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-                for (int mt = 0; mt < MT; ++mt) av[kb][mt] = frag_ld<G>(Af + G::frag(mt, kb), mt, lane);
+            for (int mt = 0; mt < MT; ++mt) av[kb][mt] = frag_ld<G>(Af + G::frag(mt, kb), mt, lane);
 #pragma unroll
-            for (int kb = 0; kb < KB; ++kb) {
-                const f32x4 x = *reinterpret_cast<const f32x4 *>(X + (size_t)kb * R * 8);
-                bv[kb] = colok ? x : f32x4{};
-            }
-            float t0, tn;
-            b_translation<P, R, 4>(a, a.uhi + ebase, half, colok, t0, tn);
-            bv[0][0] = bv[0][0] - t0;
-            bv[G::KBN][G::QN] = bv[G::KBN][G::QN] - tn;
+        for (int kb = 0; kb < KB; ++kb) {
+            const f32x4 x = *reinterpret_cast<const f32x4 *>(X + (size_t)kb * R * 8);
+            bv[kb] = colok ? x : f32x4{};
+        }
+        float t0, tn;
+        b_translation<P, R>(a, a.uhi + ebase, half, colok, t0, tn);
+        bv[0][0] = bv[0][0] - t0;
+        bv[G::KBN][G::QN] = bv[G::KBN][G::QN] - tn;
+        if constexpr (NP == 0) {
 #pragma unroll
             for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
@@ -529,53 +609,17 @@ __device__ inline void step_slots(const BatchArgs &a, int item)
 #pragma unroll
                     for (int mt = 0; mt < MT; ++mt)
                         acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[kb][mt][t], bv[kb][t], acc[mt], 0, 0, 0);
-            // the update's operands are loaded after the products: their
-            // registers are then not live across the MFMAs (8 waves per SIMD
-            // at one row tile instead of 6)
-            load_update_operands();
         } else {
-            // a lane's B operand of K-chunk c is octet 2c + half of its column
-            const float *X = a.uhi + (size_t)nb * KB * R * 8 + (c0 + (colok ? jj : 0)) * 8;
-            f32x4 ah[KC][MT], al[KC][MT], xv[KC][2];
 #pragma unroll
-            for (int c = 0; c < KC; ++c)
+            for (int c = 0; c < KB / 2; ++c)
 #pragma unroll
-                for (int mt = 0; mt < MT; ++mt) {
-                    ah[c][mt] = frag_ld<G>(Af + G::frag16(mt, c), mt, lane);
-                    al[c][mt] = frag_ld<G>(Af + G::frag16(mt, c) + G::fsize(mt), mt, lane);
-                }
-#pragma unroll
-            for (int c = 0; c < KC; ++c) {
-                const float *o = X + (size_t)(2 * c + half) * R * 8;
-                xv[c][0] = colok ? *reinterpret_cast<const f32x4 *>(o) : f32x4{};
-                xv[c][1] = colok ? *reinterpret_cast<const f32x4 *>(o + 4) : f32x4{};
-            }
-            float t0, tn;
-            b_translation<P, R, 8>(a, a.uhi + (size_t)e * KB * R * 8 + (c0 + (colok ? jj : 0)) * 8, half, colok, t0,
-                                   tn);
-            xv[0][0][0] = xv[0][0][0] - t0;
-            xv[G::KBN / 2][G::Q8N / 4][G::Q8N % 4] = xv[G::KBN / 2][G::Q8N / 4][G::Q8N % 4] - tn;
-            load_update_operands();
-#pragma unroll
-            for (int c = 0; c < KC; ++c) {
-                bf16x8 xh, xl;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const float x = xv[c][q / 4][q % 4];
-                    const __bf16 h = (__bf16)x;
-                    xh[q] = h;
-                    xl[q] = (__bf16)(x - (float)h);
-                }
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt) {
-                    const bf16x8 Ah = __builtin_bit_cast(bf16x8, ah[c][mt]);
-                    const bf16x8 Al = __builtin_bit_cast(bf16x8, al[c][mt]);
-                    acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, xh, acc[mt], 0, 0, 0);
-                    acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, xl, acc[mt], 0, 0, 0);
-                    acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, xh, acc[mt], 0, 0, 0);
-                }
-            }
+                for (int mt = 0; mt < MT; ++mt)
+                    acc[mt] = bsplit_mfma<NP>(av[2 * c][mt], av[2 * c + 1][mt], bv[2 * c], bv[2 * c + 1], acc[mt]);
         }
+        // the update's operands are loaded after the products: their
+        // registers are then not live across the MFMAs (8 waves per SIMD
+        // at one row tile instead of 6 with FP32 products)
+        load_update_operands();
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -633,147 +677,15 @@ template <int P, int R>
 __global__ __launch_bounds__(64 * kBWaves, slot_min_waves<P>()) void k_step_b32s(BatchArgs a)
 {
     resolve_step(a);
-    step_slots<P, R, false>(a, xcd_remap_b(blockIdx.x, gridDim.x));
+    step_slots<P, R, 0>(a, xcd_remap_b(blockIdx.x, gridDim.x));
 }
 
-// BF16x3 products (operator layout and split below, at k_step_b16x3).
-template <int P, int R>
-__global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3s(BatchArgs a)
+// The same step with split BF16 products (bsplit_mfma, NP = 3 or 5).
+template <int P, int R, int NP>
+__global__ __launch_bounds__(64 * kBWaves, slot_min_waves<P>()) void k_step_b16s(BatchArgs a)
 {
     resolve_step(a);
-    step_slots<P, R, true>(a, xcd_remap_b(blockIdx.x, gridDim.x));
-}
-
-// The same step with the block products split into three BF16 products on
-// v_mfma_f32_32x32x16_bf16 (accumulating in FP32): with a = a_hi + a_lo and
-// x = x_hi + x_lo (each part rounded to BF16),
-//   a*x ~ a_lo*x_hi + a_hi*x_lo + a_hi*x_hi      (a_lo*x_lo dropped).
-// Operand layout of v_mfma_f32_32x32x16_bf16 (wave64): A lane l holds
-// A[l%32][8*(l/32) + 0..7], B lane l holds B[8*(l/32) + 0..7][l%32], results as
-// in v_mfma_f32_32x32x2f32.  K-chunk c (16 rows) is row octets 2c and 2c+1, so
-// lane l's B operand is octet 2c + l/32 of its column: 32 contiguous bytes.
-// Operator: per block, row tile mt and k-chunk c, 64 lanes x 8 BF16 of a_hi
-// then 64 lanes x 8 of a_lo (the same bytes as the FP32 operator).
-template <int P, int R, int PD>
-__global__ __launch_bounds__(64 * kBWaves) void k_step_b16x3(BatchArgs a)
-{
-    resolve_step(a);
-    using G = BGeom<P>;
-    constexpr int MT = G::MT, KB = G::KB, KC = KB / 2, NT = 1;
-    constexpr int WPE = BCols<R>::WPE, CW = BCols<R>::CW;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int gw = xcd_remap_b(blockIdx.x, gridDim.x) * kBWaves + wave;
-    const int e = gw / WPE, c0 = (gw % WPE) * 32;
-    if (e >= a.ne) return;  // no block-level barrier in this kernel
-    const int half = lane >> 5, jj = lane & 31;
-    const bool colok = CW == 32 || jj < CW;
-    const int jc = colok ? jj : 0;
-    const int4 m = a.meta[e];
-    const int nbs[4] = {e, m.x >= 0 ? m.x : e, m.y >= 0 ? m.y : e, m.z >= 0 ? m.z : e};
-    const int lofs = (c0 + jc) * 8 + 4 * half;   // update: rows 4h..4h+3 of an octet
-    const int bofs = (c0 + jc) * 8;              // products: a whole octet
-
-    f32x16 acc[MT][NT];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) acc[mt][0] = f32x16{};
-
-    constexpr int NB = PD + 1;
-    f32x4 ah[NB][KC][MT], al[NB][KC][MT];
-    f32x4 xv[NB][KC][2];
-    f32x4 uhv[KB][NT], wv[KB][NT], cp[KB][NT];
-    const size_t ebase = (size_t)e * KB * R * 8 + lofs;
-    auto load = [&](int s, int buf) {
-        const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * G::BLKF);
-        const float *X = a.uhi + (size_t)nbs[s] * KB * R * 8 + bofs;
-#pragma unroll
-        for (int c = 0; c < KC; ++c)
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                ah[buf][c][mt] = frag_ld<G>(Af + G::frag16(mt, c), mt, lane);
-                al[buf][c][mt] = frag_ld<G>(Af + G::frag16(mt, c) + G::fsize(mt), mt, lane);
-            }
-#pragma unroll
-        for (int c = 0; c < KC; ++c) {
-            const float *o = X + (size_t)(2 * c + half) * R * 8;
-            xv[buf][c][0] = colok ? *reinterpret_cast<const f32x4 *>(o) : f32x4{};
-            xv[buf][c][1] = colok ? *reinterpret_cast<const f32x4 *>(o + 4) : f32x4{};
-        }
-        if (s == 3) {
-#pragma unroll
-            for (int kb = 0; kb < KB; ++kb) {
-                const size_t idx = ebase + (size_t)kb * R * 8;
-                uhv[kb][0] = colok ? *reinterpret_cast<const f32x4 *>(a.uhi + idx) : f32x4{};
-                wv[kb][0] = colok ? ld4(a.winc + idx) : f32x4{};
-                cp[kb][0] = colok ? ldc4(a.ucomp, idx) : f32x4{};
-            }
-        }
-    };
-    float t0 = 0.f, tn = 0.f;
-    b_translation<P, R, 8>(a, a.uhi + (size_t)e * KB * R * 8 + bofs, half, colok, t0, tn);
-    auto product = [&](int buf) {
-#pragma unroll
-        for (int c = 0; c < KC; ++c) {
-            bf16x8 xh, xl;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                float x = xv[buf][c][q / 4][q % 4];
-                if (c == 0 && q == 0) x = x - t0;
-                if (c == G::KBN / 2 && q == G::Q8N) x = x - tn;
-                const __bf16 h = (__bf16)x;
-                xh[q] = h;
-                xl[q] = (__bf16)(x - (float)h);
-            }
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                const bf16x8 Ah = __builtin_bit_cast(bf16x8, ah[buf][c][mt]);
-                const bf16x8 Al = __builtin_bit_cast(bf16x8, al[buf][c][mt]);
-                acc[mt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, xh, acc[mt][0], 0, 0, 0);
-                acc[mt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, xl, acc[mt][0], 0, 0, 0);
-                acc[mt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, xh, acc[mt][0], 0, 0, 0);
-            }
-        }
-    };
-#pragma unroll
-    for (int s = 0; s < PD && s < 4; ++s) load(s, s % NB);
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        if (s + PD < 4) load(s + PD, (s + PD) % NB);
-        if constexpr (PD > 0) __builtin_amdgcn_sched_barrier(0);
-        product(s % NB);
-        if constexpr (PD > 0) __builtin_amdgcn_sched_barrier(0);
-    }
-    b_finish<P, NT, R>(a, e, m, c0, half, jj, colok, ebase, acc, uhv, wv, cp);
-}
-
-// FP64 column-pair operator -> BF16 hi/lo fragments (layout above); blocks of
-// missing neighbours are zero.
-template <int P>
-__global__ void k_pack_bf16x3(int ne, const int4 *meta, const double *src, __bf16 *dst)
-{
-    using G = BGeom<P>;
-    constexpr int D = G::D, MT = G::MT, KC = G::KB / 2;
-    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;   // one (lane, element-of-8) pair
-    const size_t per = (size_t)MT * KC * 512;   // (lane, element) pairs of a full-layout block
-    if (q >= (size_t)ne * 4 * per) return;
-    const size_t b = q / per;
-    const int rem = (int)(q % per);
-    const int frag = rem / 512, l = (rem % 512) / 8, i = rem % 8;
-    const int mt = frag / KC, c = frag % KC;
-    if (G::part(mt) && l % 32 >= G::LR) return;  // rows past LR of the last tile are not stored
-    const int row = mt * 32 + l % 32, col = c * 16 + 8 * (l / 32) + i;
-    const int slot = (int)(b % 4);
-    const int4 mm = meta[b / 4];
-    const int nb = slot == 0 ? 0 : (slot == 1 ? mm.x : (slot == 2 ? mm.y : mm.z));
-    double v = 0.0;
-    if (row < D && col < D && nb >= 0) v = src[b * D * D + (size_t)(col / 2) * 2 * D + row * 2 + (col & 1)];
-    const __bf16 hi = (__bf16)(float)v;
-    const __bf16 lo = (__bf16)(float)(v - (double)(float)hi);
-    // 8 BF16 per 16-byte unit; a pair holds the hi lanes, then the lo lanes
-    const int lanes = G::fsize(mt);
-    const int lc = G::part(mt) ? (l / 32) * G::LR + l % 32 : l;
-    __bf16 *f = dst + b * G::BLKF * 2 + (size_t)G::frag16(mt, c) * 8;
-    f[lc * 8 + i] = hi;
-    f[lanes * 8 + lc * 8 + i] = lo;
+    step_slots<P, R, NP>(a, xcd_remap_b(blockIdx.x, gridDim.x));
 }
 
 // FP64 column-pair operator -> FP32 A-fragment order with zero padding; the
@@ -882,27 +794,16 @@ int step_b32_p(const BatchArgs &a, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int P, int R>
-int step_b16x3_p(const BatchArgs &a, hipStream_t s)
+template <int P, int R, int NP>
+int step_b16_p(const BatchArgs &a, hipStream_t s)
 {
     const size_t items = (size_t)a.ne * BCols<R>::WPE;
     if constexpr (BGeom<P>::MT <= 2) {
-        hipLaunchKernelGGL((k_step_b16x3s<P, R>), dim3((unsigned)items), dim3(64 * kBWaves), 0, s, a);
+        hipLaunchKernelGGL((k_step_b16s<P, R, NP>), dim3((unsigned)items), dim3(64 * kBWaves), 0, s, a);
     } else {
-        constexpr int PD = BGeom<P>::MT <= 2 ? 1 : 0;
         const unsigned nwg = (unsigned)((items + kBWaves - 1) / kBWaves);
-        hipLaunchKernelGGL((k_step_b16x3<P, R, PD>), dim3(nwg), dim3(64 * kBWaves), 0, s, a);
+        hipLaunchKernelGGL((k_step_b16<P, R, 0, NP>), dim3(nwg), dim3(64 * kBWaves), 0, s, a);
     }
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-template <int P>
-int pack16_p(int ne, const int4 *meta, const double *src, float *dst, hipStream_t s)
-{
-    using G = BGeom<P>;
-    const size_t n = (size_t)ne * 4 * G::MT * (G::KB / 2) * 512;
-    hipLaunchKernelGGL(k_pack_bf16x3<P>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ne, meta, src,
-                       reinterpret_cast<__bf16 *>(dst));
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -940,7 +841,7 @@ __global__ __launch_bounds__(64 * kBWaves, slot_min_waves<P>()) void k_bmulti_sl
     if (!I.live || I.steps >= I.nsteps) return;   // uniform over the work-group
     BatchArgs a = I.a;
     resolve_n(a, I.steps + 1);
-    step_slots<P, R, false>(a, t.y);
+    step_slots<P, R, 0>(a, t.y);
 }
 
 template <int P, int R>
@@ -955,7 +856,7 @@ __global__ __launch_bounds__(64 * kBWaves) void k_bmulti_walk(const BatchInst *_
     if (!I.live || I.steps >= I.nsteps) return;
     BatchArgs a = I.a;
     resolve_n(a, I.steps + 1);
-    step_walk<P, R, 0>(a, t.y);
+    step_walk<P, R, 0, 0>(a, t.y);
 }
 
 template <int P, int R>
@@ -1031,36 +932,37 @@ size_t batch_block_floats(int P)
     }
 }
 
-int launch_step_b16x3(int P, int ncols, const BatchArgs &a, hipStream_t s)
+template <int NP>
+int launch_step_b16_np(int P, int ncols, const BatchArgs &a, hipStream_t s)
 {
     if (ncols == 8) {
-#define CALL(p) step_b16x3_p<p, 8>(a, s)
+#define CALL(p) step_b16_p<p, 8, NP>(a, s)
         TSP_DISPATCH_P(P, CALL)
 #undef CALL
     }
     if (ncols == 16) {
-#define CALL(p) step_b16x3_p<p, 16>(a, s)
+#define CALL(p) step_b16_p<p, 16, NP>(a, s)
         TSP_DISPATCH_P(P, CALL)
 #undef CALL
     }
     if (ncols == 32) {
-#define CALL(p) step_b16x3_p<p, 32>(a, s)
+#define CALL(p) step_b16_p<p, 32, NP>(a, s)
         TSP_DISPATCH_P(P, CALL)
 #undef CALL
     }
     if (ncols == 64) {
-#define CALL(p) step_b16x3_p<p, 64>(a, s)
+#define CALL(p) step_b16_p<p, 64, NP>(a, s)
         TSP_DISPATCH_P(P, CALL)
 #undef CALL
     }
     return -2;
 }
 
-int launch_pack_bf16x3(int P, int ne, const int4 *meta, const double *src, float *dst, hipStream_t s)
+int launch_step_b16(int P, int ncols, int nprod, const BatchArgs &a, hipStream_t s)
 {
-#define CALL(p) pack16_p<p>(ne, meta, src, dst, s)
-    TSP_DISPATCH_P(P, CALL)
-#undef CALL
+    if (nprod == 3) return launch_step_b16_np<3>(P, ncols, a, s);
+    if (nprod == 5) return launch_step_b16_np<5>(P, ncols, a, s);
+    return -2;
 }
 
 int launch_step_b32(int P, int ncols, const BatchArgs &a, hipStream_t s)
