@@ -94,10 +94,10 @@ def run_batched(steps, warmup, ncols=64, precision="fp32c"):
     nprod = {"fp32c": 1, "bf16x3": 3, "bf16x5": 5}[precision]
     flops_mfma = 2.0 * 4 * prob.ne * kp * kp * ncols * nprod
     peak = 157.3 if precision == "fp32c" else 2500.0
-    # operator once (4 B per entry either way) + per DOF and column: u^n read and
-    # u^{n+1} written (FP32), the increment read and written (FP32), the
-    # compensation read and written (BF16): 20 B
-    nbytes = nblk * d * d * 4 + prob.ne * d * ncols * 20 + prob.ne * (8 + 16)
+    # operator once (4 B per entry for every product form) + per DOF and column
+    # the two time levels of the compensated displacement (FP32 hi + BF16 lo):
+    # u^n and u^{n-1} read, u^{n+1} written over u^{n-1}: 18 B
+    nbytes = nblk * d * d * 4 + prob.ne * d * ncols * 18 + prob.ne * (8 + 16)
     gbs = nbytes / (ms_step / 1e3) / 1e9
     return dict(value=prob.ndof * ncols * steps / (ms / 1e3), ms_per_step=ms_step, columns=ncols,
                 precision=precision,

@@ -39,10 +39,12 @@ struct tsp_context {
     double *blk64 = nullptr, *invm = nullptr;
     float *blk32 = nullptr, *sc_first = nullptr, *sc_step = nullptr;
 
-    // state: FP64 two vectors; FP32c hi ping-pong + compensation + increment
+    // state: FP64 two vectors; single-column FP32c hi ping-pong + compensation +
+    // increment; multi-column hi and BF16 lo, both ping-pong (two time levels)
     double *u64[2] = {nullptr, nullptr};
     int cur = 0;
     float *uhi[2] = {nullptr, nullptr}, *ucomp = nullptr, *winc = nullptr;
+    float *ulo[2] = {nullptr, nullptr};   // 2 bytes per entry (BF16)
     double *scratch64 = nullptr;  // 2*ne*nln staging for layout conversion
     bool have_state = false;
 
@@ -277,8 +279,14 @@ void alloc_state(tsp_context *c)
         const bool fresh = !c->uhi[0];
         for (int k = 0; k < 2; ++k)
             if (!c->uhi[k]) c->uhi[k] = dalloc<float>(c, n, "state hi");
-        if (!c->ucomp) c->ucomp = dalloc<float>(c, n, "state comp");
-        if (!c->winc) c->winc = dalloc<float>(c, n, "state inc");
+        if (c->batched()) {
+            // n is a multiple of 16 (rows padded to KP): n / 2 floats hold n BF16
+            for (int k = 0; k < 2; ++k)
+                if (!c->ulo[k]) c->ulo[k] = dalloc<float>(c, n / 2, "state lo");
+        } else {
+            if (!c->ucomp) c->ucomp = dalloc<float>(c, n, "state comp");
+            if (!c->winc) c->winc = dalloc<float>(c, n, "state inc");
+        }
         if (!c->u64[0]) c->u64[0] = dalloc<double>(c, n, "state in");
         if (!c->u64[1]) c->u64[1] = dalloc<double>(c, n, "state in");
         if (fresh && c->batched()) {
@@ -286,10 +294,9 @@ void alloc_state(tsp_context *c)
             // padded rows must start (and so stay) zero in every buffer
             for (int k = 0; k < 2; ++k) {
                 hipcheck(c, hipMemsetAsync(c->uhi[k], 0, n * sizeof(float), c->stream), "memset");
+                hipcheck(c, hipMemsetAsync(c->ulo[k], 0, n / 2 * sizeof(float), c->stream), "memset");
                 hipcheck(c, hipMemsetAsync(c->u64[k], 0, n * sizeof(double), c->stream), "memset");
             }
-            hipcheck(c, hipMemsetAsync(c->ucomp, 0, n * sizeof(float), c->stream), "memset");
-            hipcheck(c, hipMemsetAsync(c->winc, 0, n * sizeof(float), c->stream), "memset");
         }
     }
 }
@@ -336,6 +343,8 @@ void ensure_graph(tsp_context *c)
     b.max_samples = c->max_samples;
     b.uhi_pair[0] = c->uhi[0];
     b.uhi_pair[1] = c->uhi[1];
+    b.ulo_pair[0] = c->ulo[0];
+    b.ulo_pair[1] = c->ulo[1];
     hipcheck(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "capture");
     int rc = 0;
     for (int k = 0; k < kGraphSteps && !rc; ++k) {
@@ -450,8 +459,10 @@ BatchArgs batch_args(tsp_context *c)
     a.flags = c->flags;
     a.blk = c->blkA;
     a.tfree = c->h_minv.empty() ? 1 : 0;
-    a.ucomp = c->ucomp;
-    a.winc = c->winc;
+    a.uhi = c->uhi[c->cur];
+    a.ulo = c->ulo[c->cur];
+    a.uhi_out = c->uhi[1 - c->cur];
+    a.ulo_out = c->ulo[1 - c->cur];
     return a;
 }
 
@@ -634,7 +645,7 @@ static void reset_mesh(tsp_context *c, int ne)
     drop_graph(c);
     dfree(c, c->meta); dfree(c, c->blk64); dfree(c, c->invm); dfree(c, c->blk32); dfree(c, c->blkA);
     dfree(c, c->sc_first); dfree(c, c->sc_step);
-    for (int k = 0; k < 2; ++k) { dfree(c, c->u64[k]); dfree(c, c->uhi[k]); }
+    for (int k = 0; k < 2; ++k) { dfree(c, c->u64[k]); dfree(c, c->uhi[k]); dfree(c, c->ulo[k]); }
     dfree(c, c->ucomp); dfree(c, c->winc); dfree(c, c->scratch64);
     c->ne = ne;
     c->nsrc = 0; c->h_src_elem.clear(); c->h_src_col.clear(); c->ntab = 0;
@@ -878,17 +889,20 @@ int tsp_first_step(tsp_context *c)
             kcheck(c, launch_step64(c->P, true, a, c->stream), "first step");
             c->cur = 1;           // u^n = u1 in u64[1]; u^{n-1} = u0 in u64[0]
         } else if (c->batched()) {
+            // two time levels: u0 in buffers 0, u^{-1} = u0 - dt v0 (FP64, so the
+            // step's increment u0 - u^{-1} is dt v0) in buffers 1, which the
+            // first step overwrites with u1 (the first-step scale -dt^2/2 invM)
             const size_t n = c->state_len();
-            kcheck(c, launch_split_b(n, c->u64[0], c->uhi[0], c->ucomp, c->stream), "split");
+            kcheck(c, launch_prev_b(n, c->u64[0], c->u64[1], dt, c->stream), "u-1");
+            kcheck(c, launch_split_b(n, c->u64[0], c->uhi[0], c->ulo[0], c->stream), "split");
+            kcheck(c, launch_split_b(n, c->u64[1], c->uhi[1], c->ulo[1], c->stream), "split");
             kcheck(c, launch_scale_demote(c->ne, c->invm, -(dt * dt / 2), c->sc_first, c->stream), "sc");
             kcheck(c, launch_scale_demote(c->ne, c->invm, (-dt) * dt, c->sc_step, c->stream), "sc");
-            kcheck(c, launch_scale_demote(n, c->u64[1], dt, c->winc, c->stream), "winc");
+            c->cur = 0;
             BatchArgs b = batch_args(c);
             b.wav_row = a.wav_row;
             b.sample = a.sample;
             b.sc32 = c->sc_first;
-            b.uhi = c->uhi[0];
-            b.uhi_out = c->uhi[1];
             kcheck(c, launch_batched(c, b), "first step");
             c->cur = 1;
         } else {
@@ -940,12 +954,10 @@ int tsp_run(tsp_context *c, int nsteps)
                 a.uprev = c->u64[1 - c->cur];
                 kcheck(c, launch_step64(c->P, false, a, c->stream), "step");
             } else if (c->batched()) {
-                BatchArgs b = batch_args(c);
+                BatchArgs b = batch_args(c);   // u^n in buffers cur, u^{n-1} in 1 - cur
                 b.wav_row = a.wav_row;
                 b.sample = a.sample;
                 b.sc32 = c->sc_step;
-                b.uhi = c->uhi[c->cur];
-                b.uhi_out = c->uhi[1 - c->cur];
                 kcheck(c, launch_batched(c, b), "step");
             } else {
                 a.uhi = c->uhi[c->cur];
@@ -994,7 +1006,6 @@ int tsp_eval_receivers(tsp_context *c)
         if (c->batched()) {
             BatchArgs b = batch_args(c);
             b.sample = a.sample;
-            b.uhi = c->uhi[c->cur];
             kcheck(c, launch_sample_b32(c->P, c->ncols, b, c->stream), "sample");
         } else if (c->precision == TSP_FP64)
             kcheck(c, launch_sample64(c->P, a, c->u64[c->cur], c->stream), "sample");
@@ -1014,8 +1025,8 @@ int tsp_get_u(tsp_context *c, int col, double *out)
         const size_t n = (size_t)c->ne * c->D;
         if (c->batched()) {
             if (!c->first_done) fail(c, TSP_ERR_STATE, "no displacement yet: call tsp_first_step");
-            kcheck(c, launch_col_out(c->ne, c->nln, c->KP, c->ncols, col, c->uhi[c->cur], c->ucomp, c->scratch64,
-                                     c->stream), "layout");
+            kcheck(c, launch_col_out(c->ne, c->nln, c->KP, c->ncols, col, c->uhi[c->cur], c->ulo[c->cur],
+                                     c->scratch64, c->stream), "layout");
             d2h(c, out, c->scratch64, n, "get_u");
             return;
         }
@@ -1361,6 +1372,8 @@ bool ctx_batch_args(tsp_context *c, BatchArgs *out)
     b.max_samples = c->max_samples;
     b.uhi_pair[0] = c->uhi[0];
     b.uhi_pair[1] = c->uhi[1];
+    b.ulo_pair[0] = c->ulo[0];
+    b.ulo_pair[1] = c->ulo[1];
     *out = b;
     return true;
 }

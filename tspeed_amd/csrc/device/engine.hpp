@@ -152,18 +152,23 @@ struct BatchArgs {
     const float *blk;         // A-fragment operator
     int tfree;                // 1: translation-free products (orthonormal basis)
     const float *sc32;
-    const float *uhi;
-    float *uhi_out, *ucomp, *winc;
+    // Two time levels of the compensated displacement, u = hi - lo (hi FP32,
+    // lo BF16 stored in 2 bytes per entry): u^n in (uhi, ulo); u^{n-1} in
+    // (uhi_out, ulo_out), which the step overwrites with u^{n+1}.  The
+    // increment u^n - u^{n-1} is recomputed from them, not stored.
+    const float *uhi, *ulo;
+    float *uhi_out, *ulo_out;
     // Graph replay: when ctr is set, the step reads its 1-based index
     // n = *ctr + 1 + koff (*ctr = steps done before the replay, koff = the step's
     // place in the captured run) and derives the force row, the sample slot
-    // (n / stride) and the ping-pong buffers (u^n in uhi_pair[n & 1]) on the
-    // device; wav_row, sample, uhi and uhi_out above are then ignored.
+    // (n / stride) and the ping-pong buffers (u^n in uhi_pair[n & 1],
+    // ulo_pair[n & 1]) on the device; wav_row, sample and the four state
+    // pointers above are then ignored.
     const int *ctr;
     int koff;
     const double *wav;
     int stride, max_samples;
-    float *uhi_pair[2];
+    float *uhi_pair[2], *ulo_pair[2];
 };
 // One multi-column context inside a multi-instance set (multi.hip): the
 // arguments of its graph-replayed step (ctr null: the step index comes from
@@ -193,6 +198,8 @@ int launch_step_b16(int P, int ncols, int nprod, const BatchArgs &a, hipStream_t
 int launch_sample_b32(int P, int ncols, const BatchArgs &a, hipStream_t s);
 // multi-column state: FP64 staging -> FP32 hi + BF16 compensation (spmm.hip)
 int launch_split_b(size_t n, const double *u64, float *hi, float *comp, hipStream_t s);
+// u_prev = u0 - dt * v, in place over v (FP64): the multi-column state's u^{-1}
+int launch_prev_b(size_t n, const double *u0, double *v, double dt, hipStream_t s);
 int launch_col_in(int ne, int nln, int KP, int R, int col, const double *cm, double *st, hipStream_t s);
 int launch_col_out(int ne, int nln, int KP, int R, int col, const float *hi, const float *comp, double *cm,
                    hipStream_t s);

@@ -63,16 +63,22 @@ __device__ inline void st4_l2(float *block, int off_bytes, f32x4 v)
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_raw, v), r, off_bytes, 0, kCachePolicySC1);
 }
 
-// The compensation of the multi-column state is kept in BF16 (rounded to
-// nearest even), 2 bytes per entry: the update still runs in FP32 -- the
-// compensation is widened on load and rounded on store, so the displacement
-// carries its FP32 value plus the compensation's leading 8 bits (~32 bits in
-// all).  The step's stores were its costliest part (on the 64-source square, a
-// copy without them stepped in 21.6 us against 31.3); with the compensation in
-// 2 bytes the state moves 20 instead of 24 bytes per entry and step
-// (31.3 -> 28.4 us in the same copy).  A CPU restatement of the update on
-// column 21 of that workload (3000 steps) gives seismograms within 6.5e-6 of
-// FP64 with the BF16 compensation, 5.3e-6 with an FP32 one and 6.0e-6 with none.
+// The multi-column state keeps two time levels of the compensated
+// displacement, u = hi - lo with hi in FP32 and lo in BF16 (rounded to nearest
+// even, 2 bytes per entry; widened to FP32 for the update), so u carries ~32
+// bits; the increment u^n - u^{n-1} is recomputed from them each step instead
+// of being stored.  A step reads u^n (hi, lo) and u^{n-1} (hi, lo) and
+// overwrites u^{n-1} with u^{n+1}: 18 bytes per entry and step, 6 of them
+// stored, against 24 (12 stored) for an FP32 displacement, compensation and
+// increment.  The stores are the step's costliest part (on the 64-source
+// square a copy without them stepped in 21.6 us against 31.3); the
+// compensation in 2 bytes took the step from 24 to 20 bytes (31.3 -> 28.4 us
+// in that copy) and the recomputed increment from 20 to 18.  A CPU
+// restatement of the FP32 update (column 63 of that workload, 3000 steps)
+// gives seismograms within 6.23e-6 of FP64 with the increment recomputed and
+// 6.27e-6 with it stored; on column 21, 6.5e-6 with the BF16 compensation,
+// 5.3e-6 with an FP32 one and 6.0e-6 with none: the operator's FP32 rounding,
+// not the update's, sets the error at these lengths.
 typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2_raw __attribute__((__vector_size__(2 * sizeof(unsigned int))));
 __device__ inline unsigned short to_bf16(float x)
@@ -260,21 +266,55 @@ __device__ inline f32x4 b_source(const BatchArgs &a, int4 ex, int i0, int j)
     return f32x4{(float)fd[0], (float)fd[1], (float)fd[2], (float)fd[3]};
 }
 
+// The compensated update of four rows of one column (one 16-byte piece):
+// u = hi - lo at two time levels, the increment recomputed as their difference,
+//   w = (u^n - u^{n-1}) + sc (B u^n - f),   u^{n+1} = u^n + w
+// with the sum compensated (Fast2Sum on hi, the new error kept in lo); uh, cp:
+// u^n (hi, lo); hp, cq: u^{n-1}; y = B u^n - f.  Returns u^{n+1}'s hi in tn and
+// lo in cn; `bad` collects non-finite results.
+__device__ inline void b_step4(float sc, f32x4 y, f32x4 uh, f32x4 hp, f32x4 cp, f32x4 cq, f32x4 &tn, f32x4 &cn,
+                               bool &bad)
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float w = ((uh[q] - hp[q]) - (cp[q] - cq[q])) + sc * y[q];
+        const float z = w - cp[q];
+        const float t = uh[q] + z;
+        cn[q] = (t - uh[q]) - z;
+        tn[q] = t;
+        bad |= !isfinite(t);
+    }
+}
+
 // Compensated update of the rows and columns a lane holds (four rows of one
 // column per 16-byte access) and the NaN/Inf flag; acc holds B u^n on entry
-// and u^{n+1} on return.  fsrc(mt, nt, g, i0, j) gives the source term of the
-// four rows i0.. of column j.
+// and u^{n+1} on return, uhv u^n's hi.  The rest of the state (u^n's lo,
+// u^{n-1}) is loaded here one 32-row tile at a time, so the four state values
+// of every row are never live at once (p = 8, 64 columns: 170 VGPRs and 2
+// waves per SIMD, against 226 and 1 with all of it loaded up front; 297 against
+// 412 us per step on the 3200-triangle square).  fsrc(mt, nt, g, i0, j) gives
+// the source term of the four rows i0.. of column j.
 template <int P, int NT, int R, class FSrc>
 __device__ inline void b_update(const BatchArgs &a, int e, int c0, int half, int jj, bool colok, size_t ebase,
-                                f32x16 (&acc)[BGeom<P>::MT][NT], const f32x4 (&uhv)[BGeom<P>::KB][NT],
-                                const f32x4 (&wv)[BGeom<P>::KB][NT], const f32x4 (&cp)[BGeom<P>::KB][NT], FSrc fsrc)
+                                f32x16 (&acc)[BGeom<P>::MT][NT], const f32x4 (&uhv)[BGeom<P>::KB][NT], FSrc fsrc)
 {
     using G = BGeom<P>;
     constexpr int MT = G::MT, D = G::D;
     const float sc = a.sc32[e];
     bool bad = false;
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+    for (int mt = 0; mt < MT; ++mt) {
+        f32x4 hp[4][NT], cp[4][NT], cq[4][NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const size_t idx = ebase + (size_t)(mt * 4 + g) * R * 8 + nt * 256;
+                const bool ld = colok && mt * 32 + 8 * g < D;
+                hp[g][nt] = ld ? ld4(a.uhi_out + idx) : f32x4{};
+                cp[g][nt] = ld ? ldc4(a.ulo, idx) : f32x4{};
+                cq[g][nt] = ld ? ldc4(a.ulo_out, idx) : f32x4{};
+            }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
@@ -285,19 +325,12 @@ __device__ inline void b_update(const BatchArgs &a, int e, int c0, int half, int
                 const int kb = mt * 4 + g;
                 const size_t idx = ebase + (size_t)kb * R * 8 + nt * 256;
                 const f32x4 f = fsrc(mt, nt, g, i0, j);
-                const f32x4 uh = uhv[kb][nt];
-                f32x4 wn, tn, cn;
+                f32x4 y, tn, cn;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float w = wv[kb][nt][q] + sc * (acc[mt][nt][4 * g + q] - f[q]);
-                    const float y = w - cp[kb][nt][q];
-                    const float t = uh[q] + y;
-                    cn[q] = (t - uh[q]) - y;
-                    tn[q] = t;
-                    wn[q] = w;
-                    acc[mt][nt][4 * g + q] = t;
-                    bad |= !isfinite(t);
-                }
+                for (int q = 0; q < 4; ++q) y[q] = acc[mt][nt][4 * g + q] - f[q];
+                b_step4(sc, y, uhv[kb][nt], hp[g][nt], cp[g][nt], cq[g][nt], tn, cn, bad);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[mt][nt][4 * g + q] = tn[q];
                 if (colok) {
                     if constexpr (R == 64) {
                         // through L2 at 64 columns: p = 7 measured 393 -> 290 us per
@@ -306,16 +339,17 @@ __device__ inline void b_update(const BatchArgs &a, int e, int c0, int half, int
                         constexpr int NB = G::KB * R * 8 * 4;
                         const size_t eb = (size_t)e * G::KB * R * 8;
                         const int off = (int)(idx - eb) * 4;  // within element e's block
-                        stc4_l2<NB>(a.ucomp, eb, off, cn);
+                        stc4_l2<NB>(a.ulo_out, eb, off, cn);
                         st4_l2<NB>(a.uhi_out + eb, off, tn);
-                        st4_l2<NB>(a.winc + eb, off, wn);
                     } else {
-                        stc4(a.ucomp, idx, cn);
+                        stc4(a.ulo_out, idx, cn);
                         st4(a.uhi_out + idx, tn);
-                        st4(a.winc + idx, wn);
                     }
                 }
             }
+        // one row tile's state at a time: keep the next tile's loads below
+        __builtin_amdgcn_sched_barrier(0);
+    }
     if (bad) atomicOr(a.flags, 1u);
 }
 
@@ -355,11 +389,10 @@ __device__ inline void b_receivers(const BatchArgs &a, int4 ex, int k0, int kste
 // The update, the source term and every receiver of the element, by one wave.
 template <int P, int NT, int R>
 __device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int half, int jj, bool colok, size_t ebase,
-                                f32x16 (&acc)[BGeom<P>::MT][NT], const f32x4 (&uhv)[BGeom<P>::KB][NT],
-                                const f32x4 (&wv)[BGeom<P>::KB][NT], const f32x4 (&cp)[BGeom<P>::KB][NT])
+                                f32x16 (&acc)[BGeom<P>::MT][NT], const f32x4 (&uhv)[BGeom<P>::KB][NT])
 {
     const int4 ex = m.w >= 0 ? a.extra[m.w] : make_int4(0, 0, 0, 0);
-    b_update<P, NT, R>(a, e, c0, half, jj, colok, ebase, acc, uhv, wv, cp,
+    b_update<P, NT, R>(a, e, c0, half, jj, colok, ebase, acc, uhv,
                        [&](int, int, int, int i0, int j) { return b_source<P>(a, ex, i0, j); });
     if (a.sample >= 0)
         b_receivers<P, NT, R>(a, ex, 0, 1, c0, half, jj, colok,
@@ -367,7 +400,7 @@ __device__ inline void b_finish(const BatchArgs &a, int e, int4 m, int c0, int h
 }
 
 // Per-step values of step n (1-based index of the step() call) from the
-// replay fields of BatchArgs (wav, stride, max_samples, uhi_pair).
+// replay fields of BatchArgs (wav, stride, max_samples, uhi_pair, ulo_pair).
 __device__ inline void resolve_n(BatchArgs &a, int n)
 {
     a.wav_row = a.nsrc ? a.wav + (size_t)n * a.nsrc * 2 : nullptr;
@@ -375,6 +408,8 @@ __device__ inline void resolve_n(BatchArgs &a, int n)
     const bool odd = n & 1;  // selects, not an index: keeps the arguments out of scratch
     a.uhi = odd ? a.uhi_pair[1] : a.uhi_pair[0];
     a.uhi_out = odd ? a.uhi_pair[0] : a.uhi_pair[1];
+    a.ulo = odd ? a.ulo_pair[1] : a.ulo_pair[0];
+    a.ulo_out = odd ? a.ulo_pair[0] : a.ulo_pair[1];
 }
 
 // Per-step values of a graph-replayed step (see BatchArgs::ctr).
@@ -420,10 +455,9 @@ __device__ inline void step_walk(const BatchArgs &a, int gw)
     constexpr int NB = PD + 1;
     f32x4 av[NB][KB][MT];
     f32x4 bv[NB][KB][NT];
-    // the element's own columns (slot 0) are also u^n of the update: kept;
-    // the increment and compensation of the update are loaded with the last slot
+    // the element's own columns (slot 0) are also u^n's hi of the update: kept;
+    // the rest of the update's state is loaded by the update (b_update)
     f32x4 own[KB][NT];
-    f32x4 wv[KB][NT], cp[KB][NT];
     const size_t ebase = (size_t)e * KB * R * 8 + lofs;
     auto load = [&](int s, int buf) {
         const f32x4 *Af = reinterpret_cast<const f32x4 *>(a.blk + ((size_t)e * 4 + s) * G::BLKF);
@@ -440,16 +474,6 @@ __device__ inline void step_walk(const BatchArgs &a, int gw)
                 if (!colok) x = f32x4{};
                 if (s == 0) own[kb][nt] = x; else bv[buf][kb][nt] = x;
             }
-        if (s == 3) {
-#pragma unroll
-            for (int kb = 0; kb < KB; ++kb)
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    const size_t idx = ebase + (size_t)kb * R * 8 + nt * 256;
-                    wv[kb][nt] = colok ? ld4(a.winc + idx) : f32x4{};
-                    cp[kb][nt] = colok ? ldc4(a.ucomp, idx) : f32x4{};
-                }
-        }
     };
     // translation of this lane's column (own element, rows 0 and NLN)
     float t0 = 0.f, tn = 0.f;
@@ -499,7 +523,7 @@ __device__ inline void step_walk(const BatchArgs &a, int gw)
         if constexpr (PD > 0) __builtin_amdgcn_sched_barrier(0);
     }
 
-    b_finish<P, NT, R>(a, e, m, c0, half, jj, colok, ebase, acc, own, wv, cp);
+    b_finish<P, NT, R>(a, e, m, c0, half, jj, colok, ebase, acc, own);
 }
 
 template <int P, int R, int PD>
@@ -565,11 +589,11 @@ __device__ inline void step_slots(const BatchArgs &a, int item)
             fsrc[o][lane] = 8 * o < G::D ? b_source<P>(a, ex, 8 * o + 4 * half, c0 + jj) : f32x4{};
     }
     const size_t ebase = (size_t)e * KB * R * 8 + lofs;
-    f32x4 uh[MT], wv[MT], cp[MT];
+    f32x4 uh[MT], hp[MT], cp[MT], cq[MT];
     f32x16 acc[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
-        uh[mt] = wv[mt] = cp[mt] = f32x4{};
+        uh[mt] = hp[mt] = cp[mt] = cq[mt] = f32x4{};
         acc[mt] = f32x16{};
     }
     // the rows this wave updates: octets s + 4 mt
@@ -580,8 +604,9 @@ __device__ inline void step_slots(const BatchArgs &a, int item)
             if (upd(mt) && colok) {
                 const size_t idx = ebase + (size_t)(s + 4 * mt) * R * 8;
                 uh[mt] = *reinterpret_cast<const f32x4 *>(a.uhi + idx);
-                wv[mt] = ld4(a.winc + idx);
-                cp[mt] = ldc4(a.ucomp, idx);
+                hp[mt] = ld4(a.uhi_out + idx);
+                cp[mt] = ldc4(a.ulo, idx);
+                cq[mt] = ldc4(a.ulo_out, idx);
             }
     };
     if (act) {
@@ -639,25 +664,15 @@ __device__ inline void step_slots(const BatchArgs &a, int item)
 #pragma unroll
         for (int k = 1; k < 4; ++k) y += part[k][o][lane];
         if (src_here) y -= fsrc[o][lane];
-        f32x4 wn, tn, cn;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float w = wv[mt][q] + sc * y[q];
-            const float z = w - cp[mt][q];
-            const float t = uh[mt][q] + z;
-            cn[q] = (t - uh[mt][q]) - z;
-            tn[q] = t;
-            wn[q] = w;
-            bad |= !isfinite(t);
-        }
+        f32x4 tn, cn;
+        b_step4(sc, y, uh[mt], hp[mt], cp[mt], cq[mt], tn, cn, bad);
         if (colok) {
             // byte offset within element e's block (lofs < R * 8, o < KB)
             constexpr int NB = KB * R * 8 * 4;
             const size_t eb = (size_t)e * KB * R * 8;
             const int off = (int)(lofs + (size_t)o * R * 8) * 4;
-            stc4_l2<NB>(a.ucomp, eb, off, cn);
+            stc4_l2<NB>(a.ulo_out, eb, off, cn);
             st4_l2<NB>(a.uhi_out + eb, off, tn);
-            st4_l2<NB>(a.winc + eb, off, wn);
         }
         if (rec_here) part[0][o][lane] = tn;
     }
@@ -730,7 +745,7 @@ __global__ __launch_bounds__(64) void k_sample_b32(BatchArgs a, int R)
         double sx = 0.0, sy = 0.0;
         for (int i = 0; i < G::D; ++i) {
             const size_t idx = oct_index(e, i, j, G::KB, R);
-            const double u = (double)a.uhi[idx] - (double)ldc1(a.ucomp, idx);
+            const double u = (double)a.uhi[idx] - (double)ldc1(a.ulo, idx);
             const double v = a.rec_phi[rec * G::NLN + i % G::NLN] * u;
             if (i < G::NLN) sx += v; else sy += v;
         }
@@ -738,6 +753,15 @@ __global__ __launch_bounds__(64) void k_sample_b32(BatchArgs a, int R)
         a.seis[o] = sx;
         a.seis[o + 1] = sy;
     }
+}
+
+// u^{-1} = u0 - dt v0 over the v0 staging (FP64): with it as the earlier time
+// level, the first step's increment u0 - u^{-1} is dt v0
+__global__ void k_prev_b(size_t n, const double *u0, double *v, double dt)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    v[q] = u0[q] - dt * v[q];
 }
 
 // FP64 staging -> FP32 hi + BF16 compensation (true value = hi - comp)
@@ -1002,6 +1026,12 @@ int launch_sample_b32(int P, int ncols, const BatchArgs &a, hipStream_t s)
 #define CALL(p) sample_b32_p<p>(a, ncols, s)
     TSP_DISPATCH_P(P, CALL)
 #undef CALL
+}
+
+int launch_prev_b(size_t n, const double *u0, double *v, double dt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_prev_b, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, u0, v, dt);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_split_b(size_t n, const double *u64, float *hi, float *comp, hipStream_t s)

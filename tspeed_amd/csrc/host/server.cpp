@@ -72,7 +72,7 @@ long long footprint_columns(const Request &r)
     const long long mp = 32 * ((d + 31) / 32), kp = 16 * ((d + 15) / 16), R = columns_for(r.nsrc);
     const long long ns = 1 + r.nsteps / std::max(r.stride, 1);
     return (long long)r.ne * 4 * mp * kp * 4          // MFMA operator (rows to 32, columns to 16)
-           + (long long)r.ne * kp * R * (4 * 4 + 2 * 8)  // FP32 state x4, FP64 staging x2
+           + (long long)r.ne * kp * R * (2 * 4 + 2 * 2 + 2 * 8)  // hi x2, BF16 lo x2, FP64 staging x2
            + ns * r.nrec * R * 16                     // seismograms of every column
            + (long long)(r.nsteps + 1) * r.nsrc * 16  // force table
            + (long long)r.ne * (d * 8 + 64);          // staging vector, per-element data
