@@ -39,11 +39,11 @@ struct tsp_context {
     double *blk64 = nullptr, *invm = nullptr;
     float *blk32 = nullptr, *sc_first = nullptr, *sc_step = nullptr;
 
-    // state: FP64 two vectors; single-column FP32c hi ping-pong + compensation +
-    // increment; multi-column hi and BF16 lo, both ping-pong (two time levels)
+    // state: FP64 two vectors; FP32 (single- and multi-column) two time levels of
+    // u = hi - lo, hi FP32 and lo BF16, both ping-pong
     double *u64[2] = {nullptr, nullptr};
     int cur = 0;
-    float *uhi[2] = {nullptr, nullptr}, *ucomp = nullptr, *winc = nullptr;
+    float *uhi[2] = {nullptr, nullptr};
     float *ulo[2] = {nullptr, nullptr};   // 2 bytes per entry (BF16)
     double *scratch64 = nullptr;  // 2*ne*nln staging for layout conversion
     bool have_state = false;
@@ -277,15 +277,11 @@ void alloc_state(tsp_context *c)
             if (!c->u64[k]) c->u64[k] = dalloc<double>(c, n, "state");
     } else {
         const bool fresh = !c->uhi[0];
-        for (int k = 0; k < 2; ++k)
+        // n is even (d = 2 nln rows per element, or KP, a multiple of 16):
+        // n / 2 floats hold n BF16
+        for (int k = 0; k < 2; ++k) {
             if (!c->uhi[k]) c->uhi[k] = dalloc<float>(c, n, "state hi");
-        if (c->batched()) {
-            // n is a multiple of 16 (rows padded to KP): n / 2 floats hold n BF16
-            for (int k = 0; k < 2; ++k)
-                if (!c->ulo[k]) c->ulo[k] = dalloc<float>(c, n / 2, "state lo");
-        } else {
-            if (!c->ucomp) c->ucomp = dalloc<float>(c, n, "state comp");
-            if (!c->winc) c->winc = dalloc<float>(c, n, "state inc");
+            if (!c->ulo[k]) c->ulo[k] = dalloc<float>(c, n / 2, "state lo");
         }
         if (!c->u64[0]) c->u64[0] = dalloc<double>(c, n, "state in");
         if (!c->u64[1]) c->u64[1] = dalloc<double>(c, n, "state in");
@@ -646,7 +642,7 @@ static void reset_mesh(tsp_context *c, int ne)
     dfree(c, c->meta); dfree(c, c->blk64); dfree(c, c->invm); dfree(c, c->blk32); dfree(c, c->blkA);
     dfree(c, c->sc_first); dfree(c, c->sc_step);
     for (int k = 0; k < 2; ++k) { dfree(c, c->u64[k]); dfree(c, c->uhi[k]); dfree(c, c->ulo[k]); }
-    dfree(c, c->ucomp); dfree(c, c->winc); dfree(c, c->scratch64);
+    dfree(c, c->scratch64);
     c->ne = ne;
     c->nsrc = 0; c->h_src_elem.clear(); c->h_src_col.clear(); c->ntab = 0;
     dfree(c, c->src_phi); dfree(c, c->wav);
@@ -906,17 +902,19 @@ int tsp_first_step(tsp_context *c)
             kcheck(c, launch_batched(c, b), "first step");
             c->cur = 1;
         } else {
-            // hi/comp from u0; the increment starts as (float)(dt*v0)
+            // two time levels as in the multi-column branch: u0 in buffers 0,
+            // u^{-1} = u0 - dt v0 in buffers 1, overwritten with u1
             const size_t n = (size_t)c->ne * c->D;
-            kcheck(c, launch_split_state(n, c->u64[0], c->uhi[0], c->ucomp, c->stream), "split");
+            kcheck(c, launch_prev_b(n, c->u64[0], c->u64[1], dt, c->stream), "u-1");
+            kcheck(c, launch_split_b(n, c->u64[0], c->uhi[0], c->ulo[0], c->stream), "split");
+            kcheck(c, launch_split_b(n, c->u64[1], c->uhi[1], c->ulo[1], c->stream), "split");
             kcheck(c, launch_scale_demote(c->ne, c->invm, -(dt * dt / 2), c->sc_first, c->stream), "sc");
             kcheck(c, launch_scale_demote(c->ne, c->invm, (-dt) * dt, c->sc_step, c->stream), "sc");
-            kcheck(c, launch_scale_demote(n, c->u64[1], dt, c->winc, c->stream), "winc");
             a.sc32 = c->sc_first;
             a.uhi = c->uhi[0];
+            a.ulo = c->ulo[0];
             a.uhi_out = c->uhi[1];
-            a.ucomp = c->ucomp;
-            a.winc = c->winc;
+            a.ulo_out = c->ulo[1];
             kcheck(c, launch_step32c(c->P, true, a, c->stream), "first step");
             c->cur = 1;
         }
@@ -943,8 +941,6 @@ int tsp_run(tsp_context *c, int nsteps)
         const double dt = c->dt;
         a.coef = (-dt) * dt;
         a.sc32 = c->sc_step;
-        a.ucomp = c->ucomp;
-        a.winc = c->winc;
         for (int k = 0; k < nsteps; ++k) {
             const int n = c->steps + 1;  // 1-based index of this step() call
             a.wav_row = c->nsrc ? c->wav + (size_t)n * c->nsrc * 2 : nullptr;
@@ -961,7 +957,9 @@ int tsp_run(tsp_context *c, int nsteps)
                 kcheck(c, launch_batched(c, b), "step");
             } else {
                 a.uhi = c->uhi[c->cur];
+                a.ulo = c->ulo[c->cur];
                 a.uhi_out = c->uhi[1 - c->cur];
+                a.ulo_out = c->ulo[1 - c->cur];
                 kcheck(c, launch_step32c(c->P, false, a, c->stream), "step");
             }
             c->cur = 1 - c->cur;
@@ -1010,7 +1008,7 @@ int tsp_eval_receivers(tsp_context *c)
         } else if (c->precision == TSP_FP64)
             kcheck(c, launch_sample64(c->P, a, c->u64[c->cur], c->stream), "sample");
         else
-            kcheck(c, launch_sample32(c->P, a, c->uhi[c->cur], c->ucomp, c->stream), "sample");
+            kcheck(c, launch_sample32(c->P, a, c->uhi[c->cur], c->ulo[c->cur], c->stream), "sample");
     });
 }
 
@@ -1037,7 +1035,8 @@ int tsp_get_u(tsp_context *c, int col, double *out)
             src = c->u64[c->cur];
         } else {
             // u0/v0 staging is free once the first step has run: join hi - comp there
-            kcheck(c, launch_join_state((size_t)c->ne * c->D, c->uhi[c->cur], c->ucomp, c->u64[0], c->stream), "join");
+            kcheck(c, launch_join_state((size_t)c->ne * c->D, c->uhi[c->cur], c->ulo[c->cur], c->u64[0], c->stream),
+                   "join");
             src = c->u64[0];
         }
         kcheck(c, launch_to_comp_major(c->ne, c->nln, src, c->scratch64, c->stream), "layout");
@@ -1355,8 +1354,8 @@ bool ctx_view(tsp_context *c, InstView *v)
     v->sc_step = c->sc_step;
     v->uhi[0] = c->uhi[0];
     v->uhi[1] = c->uhi[1];
-    v->ucomp = c->ucomp;
-    v->winc = c->winc;
+    v->ulo[0] = c->ulo[0];
+    v->ulo[1] = c->ulo[1];
     return true;
 }
 

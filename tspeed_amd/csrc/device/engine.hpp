@@ -43,6 +43,27 @@ struct Geom {
 
 inline int nln_of(int P) { return (P + 1) * (P + 2) / 2; }
 
+// The FP32 paths keep the displacement as u = hi - lo at two time levels
+// (u^n and u^{n-1}), hi in FP32 and lo, its compensation, in BF16: 2 bytes per
+// entry (float-sized allocations of half the entries), rounded to nearest even
+// on store and widened exactly on load, so u carries ~32 bits.  The increment
+// u^n - u^{n-1} is recomputed from the two levels rather than stored.
+__device__ inline unsigned short lo_bits(float x)
+{
+    unsigned b = __float_as_uint(x);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return (unsigned short)(b >> 16);
+}
+__device__ inline float lo_value(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+__device__ inline float lo_ld(const float *lo, size_t i)
+{
+    return lo_value(reinterpret_cast<const unsigned short *>(lo)[i]);
+}
+__device__ inline void lo_st(float *lo, size_t i, float v) { reinterpret_cast<unsigned short *>(lo)[i] = lo_bits(v); }
+// u^n - u^{n-1} from (hi, lo) of both levels: uh, cp at step n, hp, cq at n-1.
+// Both differences are exact in FP32 for neighbouring time levels.
+__device__ inline float lf_incr(float uh, float hp, float cp, float cq) { return (uh - hp) - (cp - cq); }
+
 // Per-element FP64 geometry produced by the geometry kernel.
 struct ElemGeo {
     double inv[4];   // invJ row-major
@@ -106,25 +127,26 @@ struct StepArgs {
                               // (M^-1 (B u - f), M^-1 = invm_e * minv); null otherwise
     int wt;                   // 1: FP64 state stored through L2 (operator larger than
                               // the Infinity Cache); 0: plain stores
-    // FP32 compensated
+    // FP32 compensated: u^n in (uhi, ulo), u^{n-1} in (uhi_out, ulo_out), which
+    // the step overwrites with u^{n+1}; lo arrays hold BF16 (lo_ld / lo_st)
     int tfree;                // 1: translation-free products (orthonormal basis; see gather)
     const float *blk32;
     const float *sc32;        // per element (float)(invm * coef)
     const float *uhi;         // gathered displacement (hi part)
-    float *uhi_out, *ucomp, *winc;
+    const float *ulo;
+    float *uhi_out, *ulo_out;
 };
 
 int launch_step64(int P, bool first, const StepArgs &a, hipStream_t s);
 int launch_step32c(int P, bool first, const StepArgs &a, hipStream_t s);
 
-// FP32-compensated state helpers
-int launch_split_state(size_t n, const double *u64, float *hi, float *comp, hipStream_t s);
-int launch_join_state(size_t n, const float *hi, const float *comp, double *u64, hipStream_t s);
+// FP32-compensated state helpers (lo in BF16; splitting is launch_split_b)
+int launch_join_state(size_t n, const float *hi, const float *lo, double *u64, hipStream_t s);
 int launch_scale_demote(size_t n, const double *src, double coef, float *dst, hipStream_t s);
 
 // Receiver sample of the current displacement (eval_receivers).
 int launch_sample64(int P, const StepArgs &a, const double *u, hipStream_t s);
-int launch_sample32(int P, const StepArgs &a, const float *hi, const float *comp, hipStream_t s);
+int launch_sample32(int P, const StepArgs &a, const float *hi, const float *lo, hipStream_t s);
 
 // Power iteration pieces for the step-size check: y = invM * B * x (FP64).
 int launch_apply_minvB(int P, int ne, const int4 *meta, const double *blk, const double *invm,
@@ -137,8 +159,8 @@ int launch_apply_cm(int P, int mode, int ne, const int4 *meta, const double *blk
                     const double *minv, const double *x, double *y, hipStream_t s);
 
 // Many right-hand sides (spmm.hip): R = ncols columns, state in row octets
-// u[((e*KB + k/8)*R + j)*8 + k%8], KB = KP/8; the compensation array (ucomp)
-// holds BF16 entries at the same indices (2 bytes each).
+// u[((e*KB + k/8)*R + j)*8 + k%8], KB = KP/8; the lo arrays hold BF16 entries
+// at the same indices (2 bytes each).
 struct BatchArgs {
     int ne, nsrc, nrec;
     const int4 *meta;

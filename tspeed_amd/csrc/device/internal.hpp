@@ -20,7 +20,7 @@ struct InstView {
     double *seis;
     unsigned *flags;
     const float *blk32, *blkA, *sc_step;   // single-column / multi-column operator
-    float *uhi[2], *ucomp, *winc;
+    float *uhi[2], *ulo[2];   // FP32 state: two time levels of u = hi - lo (lo BF16)
 };
 
 bool ctx_view(tsp_context *c, InstView *v);

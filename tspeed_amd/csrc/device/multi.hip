@@ -60,21 +60,22 @@ struct MInst {
     double *seis;
     unsigned *flags;
     const float *blk, *sc;
-    float *uhi0, *uhi1, *ucomp, *winc;
+    float *uhi0, *uhi1, *ulo0, *ulo1;   // the two time levels of u = hi - lo (lo BF16)
 };
 
 constexpr int kMWaves = 4;
 constexpr int kMaxDP = Geom<8>::DP;
 
-// The end of an element step: source term, compensated update (u^{n+1} into
-// the ping-pong buffer the step writes), receivers.  rows[q] = the lane's q-th
-// row or -1, acc = B u^n of those rows, own = u^n there.
+// The end of an element step: source term, compensated update (u^{n+1} over
+// u^{n-1} in the ping-pong buffers the step writes), receivers.  rows[q] = the
+// lane's q-th row or -1, acc = B u^n of those rows, own = u^n's hi there, cp
+// its lo, wi = u^n - u^{n-1} (lf_incr).
 template <typename G>
 __device__ inline void finish_elem(const MInst &I, int e, int4 m, const int *rows, const float *wi, const float *cp,
                                    const float *own, const float *acc, float sc)
 {
     const int n = I.steps + 1;               // 1-based index of this step() call
-    float *uout = (n & 1) ? I.uhi0 : I.uhi1;
+    float *uout = (n & 1) ? I.uhi0 : I.uhi1, *lout = (n & 1) ? I.ulo0 : I.ulo1;
     StepArgs a{};
     a.nrec = I.nrec;
     a.src_sorted = I.src_sorted;
@@ -100,9 +101,8 @@ __device__ inline void finish_elem(const MInst &I, int e, int4 m, const int *row
         const float y = w - cp[q];
         const float t = uh + y;
         // through L2 (st_state): the 256-mesh set steps ~1.3% faster; same values
-        st_state(I.ucomp + idx, (t - uh) - y);
+        st_state(reinterpret_cast<unsigned short *>(lout) + idx, lo_bits((t - uh) - y));
         st_state(uout + idx, t);
-        st_state(I.winc + idx, w);
         unew[q] = t;
         bad |= !isfinite(t);
     }
@@ -114,23 +114,27 @@ template <typename G>
 __device__ inline void multi_elem(const MInst &I, int e, int lane, float (*xs)[kMaxDP])
 {
     const int cur = (I.steps + 1) & 1;       // after the first step u^n sits in uhi1
-    const float *uin = cur ? I.uhi1 : I.uhi0;
+    const float *uin = cur ? I.uhi1 : I.uhi0, *lin = cur ? I.ulo1 : I.ulo0;
+    const float *hout = cur ? I.uhi0 : I.uhi1, *lout = cur ? I.ulo0 : I.ulo1;   // u^{n-1}
     const int4 m = I.meta[e];
     const int g = lane / G::RW, r = lane % G::RW;
-    float wi[G::RPL], cp[G::RPL];
+    float wi[G::RPL], cp[G::RPL], hp[G::RPL], cq[G::RPL];
     int rows[G::RPL];
 #pragma unroll
     for (int q = 0; q < G::RPL; ++q) {
         const int row = r + 64 * q;
         rows[q] = (g == 0 && row < G::D) ? row : -1;
         const size_t idx = (size_t)e * G::D + row;
-        wi[q] = rows[q] >= 0 ? I.winc[idx] : 0.f;
-        cp[q] = rows[q] >= 0 ? I.ucomp[idx] : 0.f;
+        hp[q] = rows[q] >= 0 ? hout[idx] : 0.f;
+        cp[q] = rows[q] >= 0 ? lo_ld(lin, idx) : 0.f;
+        cq[q] = rows[q] >= 0 ? lo_ld(lout, idx) : 0.f;
     }
     const float sc = I.sc[e];
     float(*xv)[G::DP] = reinterpret_cast<float(*)[G::DP]>(&xs[0][0]);
     float own[G::RPL];   // u^n of the lane's rows (the update's uh)
     gather<G, float>(xv, uin, e, m, lane, true, I.tfree != 0, own);
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) wi[q] = lf_incr(own[q], hp[q], cp[q], cq[q]);
     // the wave reads only its own LDS slice: order its stores before its loads
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -197,19 +201,16 @@ __device__ inline void multi_elem_slots(const MInst &I, int e, int lane, int s, 
                                         float (*part)[2][G::RPL][64])
 {
     const int cur = (I.steps + 1) & 1;
-    const float *uin = cur ? I.uhi1 : I.uhi0;
+    const float *uin = cur ? I.uhi1 : I.uhi0, *lin = cur ? I.ulo1 : I.ulo0;
+    const float *hout = cur ? I.uhi0 : I.uhi1, *lout = cur ? I.ulo0 : I.ulo1;   // u^{n-1}
     const int4 m = I.meta[e];
     const int nb = s == 0 ? e : (s == 1 ? m.x : (s == 2 ? m.y : m.z));
     const int g = lane / G::RW, r = lane % G::RW;
-    float wi[G::RPL], cp[G::RPL];
     int rows[G::RPL];
 #pragma unroll
     for (int q = 0; q < G::RPL; ++q) {
         const int row = r + 64 * q;
         rows[q] = (s == 0 && g == 0 && row < G::D) ? row : -1;
-        const size_t idx = (size_t)e * G::D + row;
-        wi[q] = rows[q] >= 0 ? I.winc[idx] : 0.f;
-        cp[q] = rows[q] >= 0 ? I.ucomp[idx] : 0.f;
     }
     const float sc = s == 0 ? I.sc[e] : 0.f;
     float own[G::RPL];   // wave 0: u^n of the lane's rows (the update's uh)
@@ -245,6 +246,18 @@ __device__ inline void multi_elem_slots(const MInst &I, int e, int lane, int s, 
     }
     __syncthreads();
     if (s > 0) return;
+    // wave 0 finishes the element: the rest of its state is loaded only now,
+    // so it adds nothing to the registers of the products (p = 7, 8: 66
+    // instead of 90-94 VGPRs, 7 instead of 5 waves per SIMD)
+    float wi[G::RPL], cp[G::RPL];
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const size_t idx = (size_t)e * G::D + rows[q];
+        const float hp = rows[q] >= 0 ? hout[idx] : 0.f;
+        cp[q] = rows[q] >= 0 ? lo_ld(lin, idx) : 0.f;
+        const float cq = rows[q] >= 0 ? lo_ld(lout, idx) : 0.f;
+        wi[q] = lf_incr(own[q], hp, cp[q], cq);
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -301,22 +314,26 @@ __device__ inline void multi_elem_chunks(const MInst &I, int e, int lane, int w,
     // keep the block loads ahead of the gather
     __builtin_amdgcn_sched_barrier(0);
     const int cur = (I.steps + 1) & 1;
-    const float *uin = cur ? I.uhi1 : I.uhi0;
+    const float *uin = cur ? I.uhi1 : I.uhi0, *lin = cur ? I.ulo1 : I.ulo0;
+    const float *hout = cur ? I.uhi0 : I.uhi1, *lout = cur ? I.ulo0 : I.ulo1;   // u^{n-1}
     const int4 m = I.meta[e];
     const int nb = s == 0 ? e : (s == 1 ? m.x : (s == 2 ? m.y : m.z));
-    float wi[G::RPL], cp[G::RPL];
+    float wi[G::RPL], cp[G::RPL], hp[G::RPL], cq[G::RPL];
     int rows[G::RPL];
 #pragma unroll
     for (int q = 0; q < G::RPL; ++q) {
         const int row = lane + 64 * q;
         rows[q] = (w == 0 && row < G::D) ? row : -1;
         const size_t idx = (size_t)e * G::D + row;
-        wi[q] = rows[q] >= 0 ? I.winc[idx] : 0.f;
-        cp[q] = rows[q] >= 0 ? I.ucomp[idx] : 0.f;
+        hp[q] = rows[q] >= 0 ? hout[idx] : 0.f;
+        cp[q] = rows[q] >= 0 ? lo_ld(lin, idx) : 0.f;
+        cq[q] = rows[q] >= 0 ? lo_ld(lout, idx) : 0.f;
     }
     const float sc = w == 0 ? I.sc[e] : 0.f;
     float own[G::RPL];   // wave 0: u^n of the lane's rows (the update's uh)
     gather_one<G, float>(x, uin, nb, lane, I.tfree ? e : -1, own);
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) wi[q] = lf_incr(own[q], hp[q], cp[q], cq[q]);
     // the wave reads only its own LDS slice: order its stores before its loads
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -892,8 +909,8 @@ int tsp_multi_add(tsp_multi *m, tsp_context *c, int nsteps, int *id)
         I.sc = v.sc_step;
         I.uhi0 = v.uhi[0];
         I.uhi1 = v.uhi[1];
-        I.ucomp = v.ucomp;
-        I.winc = v.winc;
+        I.ulo0 = v.ulo[0];
+        I.ulo1 = v.ulo[1];
         // reuse the slot of a removed instance, so the table stays as long as
         // the most instances ever held at once
         int slot = 0;

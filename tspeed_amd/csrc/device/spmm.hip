@@ -81,13 +81,8 @@ __device__ inline void st4_l2(float *block, int off_bytes, f32x4 v)
 // not the update's, sets the error at these lengths.
 typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2_raw __attribute__((__vector_size__(2 * sizeof(unsigned int))));
-__device__ inline unsigned short to_bf16(float x)
-{
-    unsigned b = __float_as_uint(x);
-    b += 0x7FFFu + ((b >> 16) & 1u);
-    return (unsigned short)(b >> 16);
-}
-__device__ inline float from_bf16(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+__device__ inline unsigned short to_bf16(float x) { return lo_bits(x); }
+__device__ inline float from_bf16(unsigned short h) { return lo_value(h); }
 __device__ inline u16x4 pack_bf16(f32x4 v) { return u16x4{to_bf16(v[0]), to_bf16(v[1]), to_bf16(v[2]), to_bf16(v[3])}; }
 // four compensation entries at entry index idx of a compensation array
 __device__ inline f32x4 ldc4(const float *comp, size_t idx)
@@ -277,7 +272,7 @@ __device__ inline void b_step4(float sc, f32x4 y, f32x4 uh, f32x4 hp, f32x4 cp, 
 {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const float w = ((uh[q] - hp[q]) - (cp[q] - cq[q])) + sc * y[q];
+        const float w = lf_incr(uh[q], hp[q], cp[q], cq[q]) + sc * y[q];
         const float z = w - cp[q];
         const float t = uh[q] + z;
         cn[q] = (t - uh[q]) - z;

@@ -157,10 +157,13 @@ __global__ __launch_bounds__(64 * StepShape<P>::WAVES) void k_step64(StepArgs a)
 // ---------------------------------------------------- FP32 compensated update
 // Velocity form of the same recurrence: w^{n+1/2} = w^{n-1/2} + sc_e*(B u^n - f)
 // with sc_e = invm_e*(-dt^2), and u^{n+1} = u^n + w^{n+1/2} accumulated by Kahan
-// summation (hi part uhi, compensation ucomp, true value = uhi - ucomp).  The
-// block products and the increment are FP32; the compensation keeps the
-// displacement's roundoff from drifting over thousands of steps.  The first step
-// uses the same kernel with winc = dt*v0 and sc_e = invm_e*(-dt^2/2).
+// summation (u = hi - lo, lo in BF16), the increment w^{n-1/2} recomputed as
+// u^n - u^{n-1} from the two stored time levels (lf_incr): a step reads u^n and
+// u^{n-1} and writes u^{n+1} over u^{n-1}, 6 bytes stored per entry instead of
+// 12 (FP32 hi, compensation and increment).  The block products and the
+// increment are FP32; the compensation keeps the displacement's roundoff from
+// drifting over thousands of steps.  The first step uses the same kernel with
+// u^{-1} = u0 - dt v0 (so the increment is dt v0) and sc_e = invm_e*(-dt^2/2).
 template <int P>
 __global__ __launch_bounds__(kThreads) void k_step32c(StepArgs a)
 {
@@ -171,20 +174,23 @@ __global__ __launch_bounds__(kThreads) void k_step32c(StepArgs a)
     const bool act = e < a.ne;
     const int4 m = act ? a.meta[e] : make_int4(-1, -1, -1, -1);
     const int g = lane / G::RW, r = lane % G::RW;
-    float wi[G::RPL], cp[G::RPL];
+    float wi[G::RPL], cp[G::RPL], hp[G::RPL], cq[G::RPL];
     int rows[G::RPL];
 #pragma unroll
     for (int q = 0; q < G::RPL; ++q) {
         const int row = r + 64 * q;
         rows[q] = (act && g == 0 && row < G::D) ? row : -1;
         const size_t idx = (size_t)e * G::D + row;
-        wi[q] = rows[q] >= 0 ? a.winc[idx] : 0.f;
-        cp[q] = rows[q] >= 0 ? a.ucomp[idx] : 0.f;
+        hp[q] = rows[q] >= 0 ? a.uhi_out[idx] : 0.f;
+        cp[q] = rows[q] >= 0 ? lo_ld(a.ulo, idx) : 0.f;
+        cq[q] = rows[q] >= 0 ? lo_ld(a.ulo_out, idx) : 0.f;
     }
     const float sc = act ? a.sc32[e] : 0.f;
     float(*xs)[G::DP] = xs_all[wave];
     float own[G::RPL];   // u^n of the lane's rows (the update's uh)
     gather<G, float>(xs, a.uhi, e, m, lane, act, a.tfree != 0, own);
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) wi[q] = lf_incr(own[q], hp[q], cp[q], cq[q]);
     __syncthreads();
     if (!act) return;
 
@@ -205,9 +211,8 @@ __global__ __launch_bounds__(kThreads) void k_step32c(StepArgs a)
         const float uh = own[q];
         const float y = w - cp[q];
         const float t = uh + y;
-        a.ucomp[idx] = (t - uh) - y;
+        lo_st(a.ulo_out, idx, (t - uh) - y);
         a.uhi_out[idx] = t;
-        a.winc[idx] = w;
         unew[q] = t;
         bad |= !isfinite(t);
     }
@@ -241,19 +246,22 @@ __global__ __launch_bounds__(64 * 8) void k_step32w(StepArgs a)
     __builtin_amdgcn_sched_barrier(0);
     const int4 m = a.meta[e];
     const int nb = s == 0 ? e : (s == 1 ? m.x : (s == 2 ? m.y : m.z));
-    float wi[G::RPL], cp[G::RPL];
+    float wi[G::RPL], cp[G::RPL], hp[G::RPL], cq[G::RPL];
     int rows[G::RPL];
 #pragma unroll
     for (int q = 0; q < G::RPL; ++q) {
         const int row = lane + 64 * q;
         rows[q] = (w == 0 && row < G::D) ? row : -1;
         const size_t idx = (size_t)e * G::D + row;
-        wi[q] = rows[q] >= 0 ? a.winc[idx] : 0.f;
-        cp[q] = rows[q] >= 0 ? a.ucomp[idx] : 0.f;
+        hp[q] = rows[q] >= 0 ? a.uhi_out[idx] : 0.f;
+        cp[q] = rows[q] >= 0 ? lo_ld(a.ulo, idx) : 0.f;
+        cq[q] = rows[q] >= 0 ? lo_ld(a.ulo_out, idx) : 0.f;
     }
     const float sc = w == 0 ? a.sc32[e] : 0.f;
     float own[G::RPL];   // wave 0: u^n of the lane's rows (the update's uh)
     gather_one<G, float>(xs_all[w], a.uhi, nb, lane, a.tfree ? e : -1, own);
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) wi[q] = lf_incr(own[q], hp[q], cp[q], cq[q]);
     // the wave reads only its own LDS slice: order its stores before its loads
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -285,9 +293,8 @@ __global__ __launch_bounds__(64 * 8) void k_step32w(StepArgs a)
         const float uh = own[q];
         const float y = wn - cp[q];
         const float t = uh + y;
-        st_state(a.ucomp + idx, (t - uh) - y);
+        st_state(reinterpret_cast<unsigned short *>(a.ulo_out) + idx, lo_bits((t - uh) - y));
         st_state(a.uhi_out + idx, t);
-        st_state(a.winc + idx, wn);
         unew[q] = t;
         bad |= !isfinite(t);
     }
@@ -296,9 +303,9 @@ __global__ __launch_bounds__(64 * 8) void k_step32w(StepArgs a)
 }
 
 // Receiver sample of a stored state, outside a step (eval_receivers):
-// one wave per receiver.
+// one wave per receiver; lo (FP32 state: its BF16 compensation) or null.
 template <int P, typename T>
-__global__ __launch_bounds__(64) void k_sample(StepArgs a, const T *u, const T *comp)
+__global__ __launch_bounds__(64) void k_sample(StepArgs a, const T *u, const float *lo)
 {
     using G = Geom<P>;
     const int rec = blockIdx.x;
@@ -309,7 +316,7 @@ __global__ __launch_bounds__(64) void k_sample(StepArgs a, const T *u, const T *
     for (int row = lane; row < G::D; row += 64) {
         const int c = row / G::NLN, i = row % G::NLN;
         double uv = (double)u[(size_t)re * G::D + row];
-        if (comp) uv -= (double)comp[(size_t)re * G::D + row];
+        if (lo) uv -= (double)lo_ld(lo, (size_t)re * G::D + row);
         const double v = a.rec_phi[rec * G::NLN + i] * uv;
         if (c == 0) sx += v; else sy += v;
     }
@@ -321,19 +328,10 @@ __global__ __launch_bounds__(64) void k_sample(StepArgs a, const T *u, const T *
     }
 }
 
-__global__ void k_split(size_t n, const double *u, float *hi, float *comp)
+__global__ void k_join(size_t n, const float *hi, const float *lo, double *u)
 {
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const float h = (float)u[q];
-    hi[q] = h;
-    comp[q] = (float)((double)h - u[q]);
-}
-
-__global__ void k_join(size_t n, const float *hi, const float *comp, double *u)
-{
-    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) u[q] = (double)hi[q] - (double)comp[q];
+    if (q < n) u[q] = (double)hi[q] - (double)lo_ld(lo, q);
 }
 
 __global__ void k_scale_demote(size_t n, const double *src, double coef, float *dst)
@@ -527,13 +525,13 @@ int step32c_p(bool first, const StepArgs &a, hipStream_t s)
 template <int P>
 int sample64_p(const StepArgs &a, const double *u, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_sample<P, double>), dim3(a.nrec), dim3(64), 0, s, a, u, (const double *)nullptr);
+    hipLaunchKernelGGL((k_sample<P, double>), dim3(a.nrec), dim3(64), 0, s, a, u, (const float *)nullptr);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 template <int P>
-int sample32_p(const StepArgs &a, const float *hi, const float *comp, hipStream_t s)
+int sample32_p(const StepArgs &a, const float *hi, const float *lo, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_sample<P, float>), dim3(a.nrec), dim3(64), 0, s, a, hi, comp);
+    hipLaunchKernelGGL((k_sample<P, float>), dim3(a.nrec), dim3(64), 0, s, a, hi, lo);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 template <int P>
@@ -582,9 +580,9 @@ int launch_sample64(int P, const StepArgs &a, const double *u, hipStream_t s)
     TSP_DISPATCH_P(P, CALL)
 #undef CALL
 }
-int launch_sample32(int P, const StepArgs &a, const float *hi, const float *comp, hipStream_t s)
+int launch_sample32(int P, const StepArgs &a, const float *hi, const float *lo, hipStream_t s)
 {
-#define CALL(p) sample32_p<p>(a, hi, comp, s)
+#define CALL(p) sample32_p<p>(a, hi, lo, s)
     TSP_DISPATCH_P(P, CALL)
 #undef CALL
 }
@@ -603,14 +601,9 @@ int launch_apply_cm(int P, int mode, int ne, const int4 *meta, const double *blk
 #undef CALL
 }
 
-int launch_split_state(size_t n, const double *u64, float *hi, float *comp, hipStream_t s)
+int launch_join_state(size_t n, const float *hi, const float *lo, double *u64, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_split, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, u64, hi, comp);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-int launch_join_state(size_t n, const float *hi, const float *comp, double *u64, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_join, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, hi, comp, u64);
+    hipLaunchKernelGGL(k_join, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, hi, lo, u64);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 int launch_scale_demote(size_t n, const double *src, double coef, float *dst, hipStream_t s)

@@ -83,7 +83,7 @@ long long footprint_of(const Request &r, int mode)
     if (mode != TSP_SERVE_INSTANCES) return footprint_columns(r);
     const long long nln = (long long)(r.N + 1) * (r.N + 2) / 2, d = 2 * nln;
     const long long op32 = (long long)r.ne * 4 * d * d * 4;       // FP32 column-quad operator
-    const long long per_col = (long long)r.ne * d * 4 * 4          // uhi x2, comp, increment
+    const long long per_col = (long long)r.ne * d * (2 * 4 + 2 * 2)  // hi x2, BF16 lo x2
                               + (long long)r.ne * d * 8 * 3        // staging + layout scratch
                               + (long long)(r.nsteps + 1) * 16     // force rows
                               + (long long)(1 + r.nsteps / std::max(r.stride, 1)) * r.nrec * 16;
