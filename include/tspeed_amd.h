@@ -36,8 +36,9 @@ enum tsp_status_code {
 
 enum tsp_precision {
     TSP_FP64 = 0,   /* plain FP64 blocks and update: the correctness path            */
-    TSP_FP32C = 1,  /* FP32 blocks, FP32 increments, Kahan-compensated displacement
-                       (multi-column contexts keep the compensation in BF16) */
+    TSP_FP32C = 1,  /* FP32 blocks and products, Kahan-compensated displacement kept
+                       at two time levels as FP32 hi + BF16 compensation (the
+                       increment u^n - u^{n-1} recomputed, not stored)         */
     TSP_BF16X3 = 2, /* as TSP_FP32C, with each block product split into three BF16
                        products accumulated in FP32 (multi-column contexts only):
                        operator and state each kept to 16 bits (two BF16 parts),
