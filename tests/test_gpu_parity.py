@@ -684,7 +684,9 @@ def test_batched_graph_replay_equals_step_launches(precision):
 
 def test_batched_state_roundtrip_and_eval_receivers():
     """set_state / get_u per column, eval_receivers, and linearity across columns:
-    column 1 carries twice column 0's source and initial state."""
+    column 1 carries twice column 0's source and initial state (u0 and v0, so
+    the first step's earlier time level u0 - dt v0 is exercised); column 0
+    against the FP64 oracle from the same u0 and v0."""
     p = W.multisource(nsrc=2, nx=6, ny=5, nsteps=40)
     p.src_xy[1] = p.src_xy[0]
     p.f0[1], p.t0[1] = p.f0[0], p.t0[0]
@@ -692,6 +694,7 @@ def test_batched_state_roundtrip_and_eval_receivers():
     p.__post_init__()
     rng = np.random.default_rng(2)
     u0 = rng.standard_normal(p.ndof) * 1e-6
+    v0 = rng.standard_normal(p.ndof) * 1e-3
     eng = capi.Engine("fp32c", ncols=32)
     eng.set_tables(p.N)
     eng.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
@@ -699,7 +702,8 @@ def test_batched_state_roundtrip_and_eval_receivers():
     eng.set_receivers(p.rec_elem, p.rec_phi, 0, 2)
     eng.set_dt(p.dt)
     for c in range(32):
-        eng.set_state(u0 if c == 0 else (2 * u0 if c == 1 else None), None, col=c)
+        eng.set_state(u0 if c == 0 else (2 * u0 if c == 1 else None),
+                      v0 if c == 0 else (2 * v0 if c == 1 else None), col=c)
     eng.first_step()
     eng.eval_receivers()
     eng.run(p.nsteps)
@@ -713,7 +717,8 @@ def test_batched_state_roundtrip_and_eval_receivers():
     q = W.multisource_column(0, nx=6, ny=5, nsteps=p.nsteps, nrec=p.rec_xy.shape[0])
     q.src_xy, q.f0, q.t0, q.dirs = p.src_xy[:1], p.f0[:1], p.t0[:1], p.dirs[:1]
     q.__post_init__()
-    uref, _ = oracle_run(q, u0=u0)
+    uref, _ = oracle_run(q, u0=u0, v0=v0)
+    say(f"  column 0 from u0, v0 after {p.nsteps} steps: u vs oracle {rel_l2(u_0, uref):.2e}")
     assert rel_l2(u_0, uref) <= 1e-4
 
 

@@ -81,22 +81,16 @@ __device__ inline void st4_l2(float *block, int off_bytes, f32x4 v)
 // not the update's, sets the error at these lengths.
 typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2_raw __attribute__((__vector_size__(2 * sizeof(unsigned int))));
-__device__ inline unsigned short to_bf16(float x) { return lo_bits(x); }
-__device__ inline float from_bf16(unsigned short h) { return lo_value(h); }
-__device__ inline u16x4 pack_bf16(f32x4 v) { return u16x4{to_bf16(v[0]), to_bf16(v[1]), to_bf16(v[2]), to_bf16(v[3])}; }
+__device__ inline u16x4 pack_bf16(f32x4 v) { return u16x4{lo_bits(v[0]), lo_bits(v[1]), lo_bits(v[2]), lo_bits(v[3])}; }
 // four compensation entries at entry index idx of a compensation array
 __device__ inline f32x4 ldc4(const float *comp, size_t idx)
 {
     const u16x4 c = *reinterpret_cast<const u16x4 *>(reinterpret_cast<const unsigned short *>(comp) + idx);
-    return f32x4{from_bf16(c[0]), from_bf16(c[1]), from_bf16(c[2]), from_bf16(c[3])};
+    return f32x4{lo_value(c[0]), lo_value(c[1]), lo_value(c[2]), lo_value(c[3])};
 }
 __device__ inline void stc4(float *comp, size_t idx, f32x4 v)
 {
     *reinterpret_cast<u16x4 *>(reinterpret_cast<unsigned short *>(comp) + idx) = pack_bf16(v);
-}
-__device__ inline float ldc1(const float *comp, size_t idx)
-{
-    return from_bf16(reinterpret_cast<const unsigned short *>(comp)[idx]);
 }
 // stc4 written through L2 as st4_l2: eb = entry index of the element's block,
 // off_bytes = byte offset of the entries in its FP32 layout (NBYTES = bytes of
@@ -740,7 +734,7 @@ __global__ __launch_bounds__(64) void k_sample_b32(BatchArgs a, int R)
         double sx = 0.0, sy = 0.0;
         for (int i = 0; i < G::D; ++i) {
             const size_t idx = oct_index(e, i, j, G::KB, R);
-            const double u = (double)a.uhi[idx] - (double)ldc1(a.ulo, idx);
+            const double u = (double)a.uhi[idx] - (double)lo_ld(a.ulo, idx);
             const double v = a.rec_phi[rec * G::NLN + i % G::NLN] * u;
             if (i < G::NLN) sx += v; else sy += v;
         }
@@ -766,7 +760,7 @@ __global__ void k_split_b(size_t n, const double *u, float *hi, float *comp)
     if (q >= n) return;
     const float h = (float)u[q];
     hi[q] = h;
-    reinterpret_cast<unsigned short *>(comp)[q] = to_bf16((float)((double)h - u[q]));
+    reinterpret_cast<unsigned short *>(comp)[q] = lo_bits((float)((double)h - u[q]));
 }
 
 // component-major column `col` (FP64) <-> batched staging layout (FP64)
@@ -792,7 +786,7 @@ __global__ void k_col_out(int ne, int nln, int KP, int R, int col, const float *
     const size_t e = rem / nln;
     const int i = (int)(rem % nln);
     const size_t idx = oct_index((int)e, c * nln + i, col, KP / 8, R);
-    cm[q] = (double)hi[idx] - (double)ldc1(comp, idx);
+    cm[q] = (double)hi[idx] - (double)lo_ld(comp, idx);
 }
 
 // Launch shape per degree: one 32-column tile per wave (R/32 waves per
