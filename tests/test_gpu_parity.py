@@ -124,6 +124,36 @@ def test_boundary_adapted_on_the_batched_and_multi_paths(N):
     eng.close()
 
 
+@pytest.mark.parametrize("N", [2, 4, 5])
+def test_boundary_adapted_split_products(N):
+    """The split BF16 products on the folded BoundaryAdapted operator: three
+    sources in an 8-column context, 50 steps, with five products within 1e-4
+    of the FP64 oracle, as the FP32 products are (measured 1e-5 - 3.5e-5 and
+    2e-6 - 3.3e-5); three products (operator to 16 bits, 1e-4 - 1e-3) are
+    reported beside them."""
+    import copy
+    p = _ba_problem(N, nsteps=50)
+    p.src_xy = np.array([[300.0, 700.0], [650.0, 420.0], [820.0, 160.0]])
+    p.f0 = np.repeat(p.f0, 3)
+    p.t0 = np.repeat(p.t0, 3)
+    p.dirs = np.array([[0.0, -1.0], [0.6, 0.8], [1.0, 0.0]])
+    p.__post_init__()
+    runs = {}
+    for precision in ("fp32c", "bf16x5", "bf16x3"):
+        eng, s = W.run_batched(p, 8, precision=precision)
+        assert eng.status() == 0
+        eng.close()
+        runs[precision] = s
+    for k in range(3):
+        q = copy.copy(p)
+        q.src_xy, q.f0, q.t0, q.dirs = p.src_xy[k:k + 1], p.f0[k:k + 1], p.t0[k:k + 1], p.dirs[k:k + 1]
+        q.__post_init__()
+        _, sref = oracle_run(q)
+        errs = {prec: rel_l2(s[:, :, k], sref) for prec, s in runs.items()}
+        say(f"  BA p={N} column {k} vs oracle: " + ", ".join(f"{a} {b:.2e}" for a, b in errs.items()))
+        assert errs["fp32c"] <= 1e-4 and errs["bf16x5"] <= 1e-4
+
+
 @pytest.mark.parametrize("N", range(1, 9))
 def test_boundary_adapted_operator_and_fp64_trajectory(N):
     """Non-orthonormal basis (dense inverse mass, Matrices_imp.hpp:183-190): an
