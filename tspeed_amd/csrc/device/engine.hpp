@@ -61,7 +61,9 @@ __device__ inline float lo_ld(const float *lo, size_t i)
 }
 __device__ inline void lo_st(float *lo, size_t i, float v) { reinterpret_cast<unsigned short *>(lo)[i] = lo_bits(v); }
 // u^n - u^{n-1} from (hi, lo) of both levels: uh, cp at step n, hp, cq at n-1.
-// Both differences are exact in FP32 for neighbouring time levels.
+// Each difference is exact in FP32 when its two values are within a factor 2
+// of each other (Sterbenz), as neighbouring time levels are except near a zero
+// crossing, where the values, and so the rounding, are small.
 __device__ inline float lf_incr(float uh, float hp, float cp, float cq) { return (uh - hp) - (cp - cq); }
 
 // Per-element FP64 geometry produced by the geometry kernel.
