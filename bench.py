@@ -1,16 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark of the leap-frog hot path on one MI355X.
 
-A "step" is one LeapFrog::step over the whole mesh: block-ELL SpMV + inverse
-mass + leap-frog update + point source + receiver sampling, fused in one
-kernel launch.  The workload is layered_ricker (4000 m x 2000 m, 160k triangles,
-p = 6, 8.96M DOF, three layers with lognormal moduli, one Ricker source,
-128 receivers), the largest single-GPU configuration; synthetic inputs.
+A "step" is one LeapFrog::step over the whole mesh: block SpMV + inverse mass
++ leap-frog update + point source + receiver sampling.  The FP64 step runs in
+its face form: a face pass applies each interior face's block (read once) to
+both sides, then the element step adds its own block's product and fuses the
+rest.  The workload is layered_ricker (4000 m x 2000 m, 160k triangles, p = 6,
+8.96M DOF, three layers with lognormal moduli, one Ricker source, 128
+receivers), the largest single-GPU configuration; synthetic inputs.
 
 Prints ONE JSON line: value = DOF*steps/s of the plain FP64 path (whole job,
-inputs resident in HBM), plus the FP32-compensated variant, a roofline entry
-for the dominant kernel and a CPU baseline (the FP64 oracle, single thread,
-on a bounded sample of the same workload).
+inputs resident in HBM), plus the same step reading every stored block, the
+FP32-compensated variant, a roofline entry for the step's kernels and a CPU
+baseline (the FP64 oracle, single thread, on a bounded sample of the same
+workload).
 """
 from __future__ import annotations
 
@@ -42,9 +45,23 @@ def algorithmic_bytes(prob, scalar_bytes, vec_bytes_per_dof):
     return nblk * d * d * scalar_bytes + prob.ne * d * vec_bytes_per_dof + prob.ne * (8 + 16)
 
 
-def run_variant(prob, precision, steps, warmup, keep_u=False):
+def face_form_bytes(prob):
+    """The same for the FP64 face form (one step = the face pass + the element
+    step): each element's own block and each interior face's block once
+    (ne + #faces blocks instead of ne + #half-faces), the state (24 B per DOF),
+    the face products written and read once (16 B per row of each half-face),
+    the inverse-mass scalar and metadata, and the face list (16 B per face)."""
+    d = 2 * prob.nln
+    nhalf = int((prob.neigh >= 0).sum())
+    nfaces = nhalf // 2
+    return ((prob.ne + nfaces) * d * d * 8 + prob.ne * d * 24 + nhalf * d * 16 + prob.ne * (8 + 16)
+            + nfaces * 16)
+
+
+def run_variant(prob, precision, steps, warmup, keep_u=False, face_form=True):
     from tspeed_amd import capi
     eng = capi.Engine(precision)
+    eng.set_face_form(face_form)
     t0 = time.time()
     eng.set_tables(prob.N)
     eng.set_mesh(prob.verts, prob.neigh, prob.neighedge, prob.lam, prob.mu, prob.rho, prob.penalty)
@@ -65,8 +82,10 @@ def run_variant(prob, precision, steps, warmup, keep_u=False):
     status = eng.status()
     dev_bytes = eng.device_bytes()
     u = eng.get_u() if keep_u else None   # after the timed region
+    faces = eng.set_face_form()
     eng.close()
-    return dict(ms_total=ms, wall_s=wall, setup_s=setup_s, status=status, device_bytes=dev_bytes, u=u)
+    return dict(ms_total=ms, wall_s=wall, setup_s=setup_s, status=status, device_bytes=dev_bytes, u=u,
+                face_form=faces)
 
 
 def run_batched(steps, warmup, ncols=64, precision="fp32c"):
@@ -352,6 +371,8 @@ def main():
     ap.add_argument("--no-fp32c", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-square", action="store_true", help="skip the 3200-triangle FP64 leg")
+    ap.add_argument("--no-block-ell", action="store_true",
+                    help="skip the FP64 leg that reads every stored block (no face form)")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the error runs against FP64 (profiling the timed legs alone)")
     ap.add_argument("--multi", type=int, default=256, help="instances in the uneven-batch variant (0 = skip)")
@@ -373,10 +394,18 @@ def main():
     r64 = run_variant(prob, "fp64", args.steps, args.warmup)
     ms_step = r64["ms_total"] / args.steps
     value = prob.ndof * args.steps / (r64["ms_total"] / 1e3)
-    bytes64 = algorithmic_bytes(prob, 8, 24)
+    bytes64 = face_form_bytes(prob) if r64["face_form"] else algorithmic_bytes(prob, 8, 24)
     achieved = bytes64 / (ms_step / 1e3) / 1e9
 
     variants = {}
+    if not args.no_block_ell:
+        # the same FP64 step reading every stored block (both copies of each face block)
+        re = run_variant(prob, "fp64", args.steps, args.warmup, face_form=False)
+        ms_e = re["ms_total"] / args.steps
+        be = algorithmic_bytes(prob, 8, 24)
+        variants["fp64_block_ell"] = dict(value=prob.ndof * args.steps / (re["ms_total"] / 1e3), ms_per_step=ms_e,
+                                          achieved_GBs=be / (ms_e / 1e3) / 1e9, algorithmic_bytes=be,
+                                          status=re["status"])
     if not args.no_fp32c:
         r32 = run_variant(prob, "fp32c", args.steps, args.warmup)
         ms32 = r32["ms_total"] / args.steps
@@ -426,9 +455,9 @@ def main():
                                else run_serving_mode(args.serve, args.serve_mode))
 
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_step64_layered_400x200_p6.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("bytes_per_launch")
+    tf = os.path.join(ROOT, "profiles", "traffic_fp64_step_faces_layered_400x200_p6.json")
+    if r64["face_form"] and os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_step")
 
     out = {
         "metric": "DOF-steps/s batched FP32 vs plain FP64 @<=1e-4 seismogram err; p99 latency",
@@ -446,10 +475,13 @@ def main():
         "config": {"workload": "layered_ricker: 4000x2000 m, 400x200 split squares (160000 triangles), p=6, "
                                "8.96M DOF, 3 layers + lognormal moduli, 1 Ricker source, 128 receivers; "
                                "plain FP64 leap-frog",
-                   "ne": prob.ne, "p": prob.N, "dof": prob.ndof, "dt": prob.dt, "precision": "fp64"},
+                   "ne": prob.ne, "p": prob.N, "dof": prob.ndof, "dt": prob.dt, "precision": "fp64",
+                   "face_form": r64["face_form"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_step64<6,false,true>", "algorithmic_bytes": bytes64},
+                     "kernel": ("k_face64<6> + k_step64<6,false,true,true> (one step: the face pass and "
+                                "the element step)" if r64["face_form"] else "k_step64<6,false,true>"),
+                     "algorithmic_bytes": bytes64},
         "variants": variants,
         "setup_s": r64["setup_s"],
         "status_flags": r64["status"],

@@ -249,6 +249,116 @@ def test_fp64_large_operator_step_matches_oracle():
     eng.close()
 
 
+@pytest.mark.parametrize("basis", [O.DUBINER, O.BOUNDARY_ADAPTED])
+@pytest.mark.parametrize("N", [5, 6, 7, 8])
+def test_fp64_face_form_against_block_ell_and_oracle(N, basis):
+    """The FP64 face form (each interior face's block read once, from the
+    lower-numbered element, and applied to both sides) against the step that
+    reads every stored block, and both against the oracle: within 1e-12 of the
+    oracle and of each other (they differ by the assembled blocks' rounding
+    asymmetry and the order of the slot sums: ~1e-13 on the BoundaryAdapted
+    basis, whose runs are themselves ~1e-13 from the oracle)."""
+    p = small_problem(N, nsteps=150)
+    if basis != O.DUBINER:
+        p.basis = basis
+        p.__post_init__()
+    rng = np.random.default_rng(10 + N)
+    u0 = rng.standard_normal(p.ndof) * 1e-6
+    v0 = rng.standard_normal(p.ndof) * 1e-4
+    runs = {}
+    for on in (True, False):
+        eng = capi.Engine("fp64")
+        eng.set_face_form(on)
+        eng, u, s = W.run_engine(p, engine=eng, u0=u0, v0=v0)
+        assert eng.set_face_form() is on
+        assert eng.status() == 0
+        runs[on] = (u, s)
+        eng.close()
+    uref, sref = oracle_run(p, u0=u0, v0=v0)
+    (uf, sf), (ue, se) = runs[True], runs[False]
+    say(f"fp64 face form N={N} basis={basis}: vs oracle u {rel_l2(uf, uref):.2e} seis {rel_l2(sf, sref):.2e}; "
+        f"vs block-ELL u {rel_l2(uf, ue):.2e} seis {rel_l2(sf, se):.2e}")
+    assert rel_l2(uf, uref) <= 1e-12 and max(seis_err(sf, sref)) <= 1e-12
+    assert rel_l2(ue, uref) <= 1e-12 and max(seis_err(se, sref)) <= 1e-12
+    assert rel_l2(uf, ue) <= 1e-12 and rel_l2(sf, se) <= 1e-12
+
+
+def test_fp64_face_form_availability_and_uploaded_blocks():
+    """The face form is taken for assembled operators at N >= 5 only, and for
+    uploaded blocks only when they are symmetric bit for bit; on such blocks
+    it reads the same entries as the block-ELL step (they differ in the order
+    of the slot sums only)."""
+    for N in (4, 5):
+        p = small_problem(N, nsteps=5)
+        eng = capi.Engine("fp64")
+        W.set_space(eng, p)
+        eng.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+        assert eng.set_face_form() is (N >= 5)
+        assert eng.set_face_form(False) is False and eng.set_face_form(True) is (N >= 5)
+        eng.close()
+    p = small_problem(6, nsteps=60)
+    op = oracle_op(p)
+    blocks = op.blocks().copy()
+    # the assembled blocks are symmetric to rounding only: uploaded as they are,
+    # every stored block is read
+    eng = capi.Engine("fp64")
+    eng.set_tables(p.N)
+    eng.upload_blocks(p.neigh, blocks, op.invm())
+    assert eng.set_face_form() is False
+    eng.close()
+    # made symmetric bit for bit (B_{b,a} := B_{a,b}^T for a < b)
+    nb = np.asarray(p.neigh).reshape(-1, 3)
+    for a in range(p.ne):
+        for s in range(3):
+            b = nb[a, s]
+            if b > a:
+                sb = list(nb[b]).index(a)
+                blocks[b, 1 + sb] = blocks[a, 1 + s].T
+    out = {}
+    for on in (True, False):
+        eng = capi.Engine("fp64")
+        eng.set_tables(p.N)
+        eng.set_face_form(on)
+        eng.upload_blocks(p.neigh, blocks, op.invm())
+        assert eng.set_face_form() is on
+        eng.set_sources(p.src_elem, p.src_phi, p.wavelet())
+        eng.set_receivers(p.rec_elem, p.rec_phi, 1, p.nsamples)
+        eng.set_dt(p.dt)
+        eng.set_state()
+        eng.first_step()
+        eng.run(p.nsteps)
+        out[on] = (eng.get_u(), eng.seismograms())
+        assert eng.status() == 0
+        eng.close()
+    say(f"uploaded symmetric blocks N=6: face form vs block-ELL u {rel_l2(out[True][0], out[False][0]):.2e}")
+    assert rel_l2(out[True][0], out[False][0]) <= 1e-12 and rel_l2(out[True][1], out[False][1]) <= 1e-12
+
+
+def test_fp64_face_form_shared_operator_and_toggle_mid_run():
+    """A context borrowing the operator (tsp_share_operator) builds its own face
+    list and products; switching the face form off and on between runs keeps
+    the trajectory (to 1e-12 of a run that never switches)."""
+    p = small_problem(7, nsteps=80)
+    e1, u1, s1 = W.run_engine(p, "fp64")
+    b = capi.Engine("fp64", share=e1)
+    assert b.set_face_form() is True
+    b.set_sources(p.src_elem, p.src_phi, p.wavelet())
+    b.set_receivers(p.rec_elem, p.rec_phi, 1, p.nsamples)
+    b.set_dt(p.dt)
+    b.set_state()
+    b.first_step()
+    b.run(30)
+    assert b.set_face_form(False) is False
+    b.run(20)
+    assert b.set_face_form(True) is True
+    b.run(p.nsteps - 50)
+    u2, s2 = b.get_u(), b.seismograms()
+    say(f"shared operator, face form toggled: u {rel_l2(u2, u1):.2e} seis {rel_l2(s2, s1):.2e}")
+    assert rel_l2(u2, u1) <= 1e-12 and rel_l2(s2, s1) <= 1e-12
+    b.close()
+    e1.close()
+
+
 def test_square_ricker_full_parity():
     """The 1 km square at full size (3200 triangles, p=4, 2000 steps, 32 receivers):
     final displacement and every seismogram within 1e-12 of the oracle."""

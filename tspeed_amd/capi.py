@@ -42,6 +42,7 @@ SIGNATURES = {
     "tsp_set_wavelet": (_i, [_vp, _i, _i, _dp]),
     "tsp_set_receivers": (_i, [_vp, _i, _ip, _dp, _i, _i]),
     "tsp_set_stride": (_i, [_vp, _i]),
+    "tsp_set_face_form": (_i, [_vp, _i, ctypes.POINTER(_i)]),
     "tsp_set_dt": (_i, [_vp, _d]),
     "tsp_first_step": (_i, [_vp]),
     "tsp_run": (_i, [_vp, _i]),
@@ -397,6 +398,13 @@ class Engine:
 
     def set_stride(self, stride):
         self._ck(self._L.tsp_set_stride(self._h, int(stride)))
+
+    def set_face_form(self, on=None):
+        """FP64 face form (each interior face's block read once for both sides):
+        on True/False sets it, None only queries; returns whether steps use it."""
+        act = _i(0)
+        self._ck(self._L.tsp_set_face_form(self._h, -1 if on is None else int(bool(on)), ctypes.byref(act)))
+        return bool(act.value)
 
     def set_dt(self, dt):
         self._ck(self._L.tsp_set_dt(self._h, float(dt)))

@@ -38,6 +38,14 @@ struct tsp_context {
     int4 *meta = nullptr;
     double *blk64 = nullptr, *invm = nullptr;
     float *blk32 = nullptr, *sc_first = nullptr, *sc_step = nullptr;
+    // FP64 face form (StepArgs::faces): taken when the operator is symmetric
+    // (assembled by tsp_set_mesh, or uploaded blocks symmetric bit for bit),
+    // the degree has it, every neighbour relation is mutual and the caller has
+    // not turned it off (tsp_set_face_form)
+    bool symmetric = false, face_form = true;
+    int4 *faces = nullptr;
+    int nfaces = 0;
+    double *zf = nullptr;
 
     // state: FP64 two vectors; FP32 (single- and multi-column) two time levels of
     // u = hi - lo, hi FP32 and lo BF16, both ping-pong
@@ -115,6 +123,7 @@ void hipcheck(tsp_context *c, hipError_t e, const char *what)
 void kcheck(tsp_context *c, int rc, const char *what)
 {
     if (rc == -2) fail(c, TSP_ERR_UNSUPPORTED, "%s: degree %d not built", what, c->P);
+    if (rc == -3) fail(c, TSP_ERR_STATE, "%s: no face form at degree %d", what, c->P);
     if (rc != 0) fail(c, TSP_ERR_HIP, "%s: launch failed: %s", what, hipGetErrorString(hipGetLastError()));
 }
 
@@ -375,6 +384,65 @@ int run_graph(tsp_context *c, int nsteps)
     return nsteps - reps * c->graph_len;
 }
 
+// Face list of the FP64 face form: one entry per interior face, from the
+// lower-numbered element's side.  Left empty (the block-ELL step) unless every
+// neighbour relation is mutual and single, so that each (element, slot) with a
+// neighbour is written by exactly one face.
+void build_faces(tsp_context *c)
+{
+    drop_graph(c);
+    dfree(c, c->faces);
+    dfree(c, c->zf);
+    c->nfaces = 0;
+    if (c->precision != TSP_FP64 || c->batched() || !c->symmetric || !c->face_form || !face_form_degree(c->P) ||
+        !c->ne)
+        return;
+    const std::vector<int> &nb = c->h_neigh;
+    auto count = [&](int e, int x) {
+        return (nb[3 * e] == x) + (nb[3 * e + 1] == x) + (nb[3 * e + 2] == x);
+    };
+    std::vector<int4> f;
+    f.reserve((size_t)c->ne * 3 / 2 + 1);
+    for (int e = 0; e < c->ne; ++e)
+        for (int s = 0; s < 3; ++s) {
+            const int x = nb[3 * e + s];
+            if (x < 0) continue;
+            if (x == e || x >= c->ne || count(e, x) != 1 || count(x, e) != 1) return;
+            if (x < e) continue;
+            const int sx = nb[3 * x] == e ? 0 : (nb[3 * x + 1] == e ? 1 : 2);
+            f.push_back(make_int4(e, 1 + s, x, 1 + sx));
+        }
+    c->zf = dalloc<double>(c, (size_t)c->ne * 3 * c->D, "face products");
+    // boundary slots are never written: they read zero
+    hipcheck(c, hipMemsetAsync(c->zf, 0, sizeof(double) * (size_t)c->ne * 3 * c->D, c->stream), "memset");
+    c->faces = dalloc<int4>(c, f.size(), "faces");
+    if (!f.empty()) h2d(c, c->faces, f.data(), f.size(), "faces");
+    c->nfaces = (int)f.size();
+    hipcheck(c, hipStreamSynchronize(c->stream), "faces");
+}
+
+// Blocks in the canonical form (tsp_upload_blocks) with B_{b,a} = B_{a,b}^T bit
+// for bit on every face whose neighbour relation is mutual.
+bool blocks_symmetric(int ne, int d, const int *neigh, const double *blocks)
+{
+    const size_t bs = (size_t)d * d;
+    for (int a = 0; a < ne; ++a)
+        for (int s = 0; s < 3; ++s) {
+            const int b = neigh[3 * a + s];
+            if (b <= a || b >= ne) continue;
+            int sb = -1;
+            for (int k = 0; k < 3; ++k)
+                if (neigh[3 * b + k] == a) sb = k;
+            if (sb < 0) return false;
+            const double *F = blocks + ((size_t)a * 4 + 1 + s) * bs;
+            const double *Fb = blocks + ((size_t)b * 4 + 1 + sb) * bs;
+            for (int i = 0; i < d; ++i)
+                for (int j = 0; j < d; ++j)
+                    if (F[(size_t)i * d + j] != Fb[(size_t)j * d + i]) return false;
+        }
+    return true;
+}
+
 void operator_ready(tsp_context *c)
 {
     // neighbour table on the device first: the multi-column packing reads it
@@ -402,6 +470,7 @@ void operator_ready(tsp_context *c)
     c->first_done = false;
     c->nsamples = 0;
     hipcheck(c, hipStreamSynchronize(c->stream), "operator");
+    build_faces(c);
 }
 
 // FP64 operators larger than this (the 256 MB Infinity Cache less room for
@@ -429,6 +498,9 @@ StepArgs base_args(tsp_context *c)
     a.invm = c->invm;
     a.minv = c->minv_after() ? c->d_minv : nullptr;
     a.wt = (double)c->ne * 4 * c->D * c->D * sizeof(double) > kCacheResidentBytes;
+    a.faces = c->faces;
+    a.nfaces = c->nfaces;
+    a.zf = c->zf;
     a.blk32 = c->blk32;
     // FP32 products drop the element's translation (see gather); the
     // BoundaryAdapted basis has no single constant mode, so it keeps the plain form
@@ -641,6 +713,9 @@ static void reset_mesh(tsp_context *c, int ne)
     drop_graph(c);
     dfree(c, c->meta); dfree(c, c->blk64); dfree(c, c->invm); dfree(c, c->blk32); dfree(c, c->blkA);
     dfree(c, c->sc_first); dfree(c, c->sc_step);
+    dfree(c, c->faces); dfree(c, c->zf);
+    c->nfaces = 0;
+    c->symmetric = false;
     for (int k = 0; k < 2; ++k) { dfree(c, c->u64[k]); dfree(c, c->uhi[k]); dfree(c, c->ulo[k]); }
     dfree(c, c->scratch64);
     c->ne = ne;
@@ -695,6 +770,8 @@ int tsp_set_mesh(tsp_context *c, int ne, const double *verts, const int *neigh, 
             kcheck(c, launch_premul_invmass(c->P, ne, c->d_minv, c->blk64, c->stream), "invmass");
         hipcheck(c, hipStreamSynchronize(c->stream), "assemble");
         dfree(c, d_verts); dfree(c, d_nb); dfree(c, d_ng); dfree(c, d_mat); dfree(c, d_geo); dfree(c, d_face);
+        // SIPG: symmetric by construction (to the rounding of assembly)
+        c->symmetric = true;
         operator_ready(c);
     });
 }
@@ -720,6 +797,7 @@ int tsp_upload_blocks(tsp_context *c, int ne, const int *neigh, const double *bl
         hipcheck(c, hipStreamSynchronize(c->stream), "upload");
         dfree(c, canon);
         dfree(c, d_nb);
+        c->symmetric = blocks_symmetric(ne, c->D, neigh, blocks);
         operator_ready(c);
     });
 }
@@ -854,6 +932,22 @@ int tsp_set_stride(tsp_context *c, int stride)
         if (stride < 0) fail(c, TSP_ERR_ARG, "negative stride");
         if (stride != c->stride) drop_graph(c);
         c->stride = stride;
+    });
+}
+
+int tsp_set_face_form(tsp_context *c, int on, int *active)
+{
+    return guarded(c, [&] {
+        if (on < -1 || on > 1) fail(c, TSP_ERR_ARG, "on must be 1, 0 or -1 (query)");
+        if (on >= 0 && (on != 0) != c->face_form) {
+            c->face_form = on != 0;
+            if (c->ne) {
+                // queued steps may still read the face buffers
+                hipcheck(c, hipStreamSynchronize(c->stream), "synchronize");
+                build_faces(c);
+            }
+        }
+        if (active) *active = c->faces != nullptr;
     });
 }
 
@@ -1286,6 +1380,9 @@ int tsp_share_operator(tsp_context *owner, tsp_context **out)
         hipcheck(c, hipMemsetAsync(c->u64[1], 0, sizeof(double) * c->state_len(), c->stream), "memset");
         c->have_state = true;
         rebuild_extra(c);
+        c->symmetric = owner->symmetric;
+        c->face_form = owner->face_form;
+        build_faces(c);
         c->lender = owner;
         owner->borrowers.push_back(c);
     });

@@ -42,6 +42,9 @@ struct Geom {
 };
 
 inline int nln_of(int P) { return (P + 1) * (P + 2) / 2; }
+// Degrees whose FP64 step has the face form: blocks one column group wide
+// (d > 32), where the face pass gives each half of a block's columns a wave.
+inline bool face_form_degree(int P) { return P >= 5 && P <= kMaxDegree; }
 
 // The FP32 paths keep the displacement as u = hi - lo at two time levels
 // (u^n and u^{n-1}), hi in FP32 and lo, its compensation, in BF16: 2 bytes per
@@ -129,6 +132,14 @@ struct StepArgs {
                               // (M^-1 (B u - f), M^-1 = invm_e * minv); null otherwise
     int wt;                   // 1: FP64 state stored through L2 (operator larger than
                               // the Infinity Cache); 0: plain stores
+    // Face form of the FP64 product (face_form_degree, symmetric operator):
+    // the face pass (k_face64) applies each interior face's block, read once
+    // from the lower-numbered element's copy, to both sides and leaves the
+    // products in zf; the element step adds them to its own block's product.
+    // faces null: every block is read by the element step (block-ELL).
+    const int4 *faces;        // {ea, slot of eb in ea, eb, slot of ea in eb}, ea < eb, slots 1..3
+    int nfaces;
+    double *zf;               // zf[(e*3 + slot-1)*d + row]; boundary slots stay zero
     // FP32 compensated: u^n in (uhi, ulo), u^{n-1} in (uhi_out, ulo_out), which
     // the step overwrites with u^{n+1}; lo arrays hold BF16 (lo_ld / lo_st)
     int tfree;                // 1: translation-free products (orthonormal basis; see gather)
@@ -139,6 +150,7 @@ struct StepArgs {
     float *uhi_out, *ulo_out;
 };
 
+// FP64 step: with a.faces set, the face pass and then the element step.
 int launch_step64(int P, bool first, const StepArgs &a, hipStream_t s);
 int launch_step32c(int P, bool first, const StepArgs &a, hipStream_t s);
 

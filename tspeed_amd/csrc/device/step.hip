@@ -48,19 +48,22 @@ namespace tspeed_dev {
 // four blocks: on the 160 000-triangle p = 6 mesh 2.38 ms per step (85% of HBM)
 // against 2.42 with one wave per slot and 2.48 with one wave per element; on
 // 3200 triangles at p = 4, 14.4 us against 17.2.
-template <int P>
+// With the face form (FACES) the work-group holds only slot 0's waves and
+// adds the three face products the face pass left in zf, in slot order.
+template <int P, bool FACES = false>
 struct StepShape {
     static constexpr int H = Geom<P>::NG == 1 ? 2 : 1;  // waves per slot
-    static constexpr int WAVES = 4 * H;
+    static constexpr int WAVES = FACES ? H : 4 * H;
 };
 
 // WT: the new state is stored through L2 (see st_state); taken for operators
 // larger than the Infinity Cache.
-template <int P, bool FIRST, bool WT>
-__global__ __launch_bounds__(64 * StepShape<P>::WAVES) void k_step64(StepArgs a)
+template <int P, bool FIRST, bool WT, bool FACES = false>
+__global__ __launch_bounds__((64 * StepShape<P, FACES>::WAVES)) void k_step64(StepArgs a)
 {
     using G = Geom<P>;
-    constexpr int H = StepShape<P>::H, NW = StepShape<P>::WAVES;
+    constexpr int H = StepShape<P>::H, NW = StepShape<P, FACES>::WAVES;
+    static_assert(!FACES || G::NG == 1, "face form: d > 32");
     __shared__ __attribute__((aligned(16))) double xs_all[NW][G::DP];
     __shared__ double part[NW - 1][G::RPL][64];
     __shared__ double wrow[G::DP];   // dense inverse mass: the element's w rows
@@ -82,6 +85,15 @@ __global__ __launch_bounds__(64 * StepShape<P>::WAVES) void k_step64(StepArgs a)
         up[q] = rows[q] >= 0 ? a.uprev[(size_t)e * G::D + row] : 0.0;
     }
     const double invm = (act && w == 0) ? a.invm[e] : 0.0;
+    // face products of slots 1..3 for the rows of wave 0, fetched before the barrier
+    double zs[3][G::RPL];
+    if constexpr (FACES) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int q = 0; q < G::RPL; ++q)
+                zs[k][q] = rows[q] >= 0 ? a.zf[((size_t)e * 3 + k) * G::D + rows[q]] : 0.0;
+    }
     double(*xs)[G::DP] = &xs_all[w];
     gather_one<G, double>(xs[0], a.ucur, nb, lane);
     // the wave reads only its own LDS slice: order its stores before its loads
@@ -106,6 +118,12 @@ __global__ __launch_bounds__(64 * StepShape<P>::WAVES) void k_step64(StepArgs a)
     for (int k = 0; k < NW - 1; ++k)
 #pragma unroll
         for (int q = 0; q < G::RPL; ++q) acc[q] += part[k][q][lane];
+    if constexpr (FACES) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int q = 0; q < G::RPL; ++q) acc[q] += zs[k][q];
+    }
 
     const int4 ex = m.w >= 0 ? a.extra[m.w] : make_int4(0, 0, 0, 0);
     double wv[G::RPL];
@@ -152,6 +170,212 @@ __global__ __launch_bounds__(64 * StepShape<P>::WAVES) void k_step64(StepArgs a)
     }
     if (bad) atomicOr(a.flags, 1u);
     if (a.sample >= 0 && ex.w > ex.z) record<G, double>(a, ex, unew, rows, G::RPL);
+}
+
+// ------------------------------------------------------- FP64 face pass
+// B is symmetric: the consistency term enters as I + I^T and the penalty block
+// is the same form seen from either side, so B_{eb,ea} = B_{ea,eb}^T (to the
+// rounding of assembly, ~2e-16 of the largest entry).  The face form reads the
+// lower-numbered element's copy F = B_{ea,eb} of each interior face once and
+// applies it to both sides:
+//   zf(ea, slot of eb) = F x_eb     lane = row, as the block-ELL SpMV
+//   zf(eb, slot of ea) = F^T x_ea   each lane forms F(row, j) x_ea(row) and a
+//                                   reduce-scatter butterfly sums each column
+//                                   over the lanes
+// The operator stream per step falls from ne + #half-faces blocks to ne +
+// #faces (3.99 -> 2.50 blocks per element on a 160 000-triangle mesh).  Two waves per
+// face, one per half of the block's column pairs, as the element step splits
+// its blocks: the transposed sums of a half are complete within its wave, the
+// row sums of the two halves are added in LDS.  Two faces per work-group.
+
+// Lane exchanges of the transposed sums.  DPP moves (quad permutations, the
+// half-row mirror, the 8-lane rotation of a 16-lane row) stay in the VALU;
+// the exchanges across 16-lane rows go through the LDS crossbar (ds_swizzle
+// within 32 lanes, ds_bpermute across the halves).
+template <int CTRL>
+__device__ inline double dpp64(double x)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]: lane ^ 1
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]: lane ^ 2
+constexpr int kDppHalfMirror = 0x141; // lane ^ 7 within 8 lanes
+constexpr int kDppRor8 = 0x128;       // lane ^ 8 within 16 lanes
+__device__ inline double swz_xor16(double x)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    // bitmask mode: and 0x1F, or 0, xor 0x10 (lane ^ 16 within 32 lanes)
+    const int lo = __builtin_amdgcn_ds_swizzle((int)(unsigned)u, 0x401F);
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(unsigned)(u >> 32), 0x401F);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// One halving step of the reduce-scatter: the lane whose bit is set keeps the
+// upper half of its n values and adds its partner's copy of that half.
+template <int CTRL, int N>
+__device__ inline void halve(double (&v)[8], bool hi)
+{
+#pragma unroll
+    for (int k = 0; k < N / 2; ++k) {
+        const double keep = hi ? v[k + N / 2] : v[k];
+        const double send = hi ? v[k] : v[k + N / 2];
+        v[k] = keep + dpp64<CTRL>(send);
+    }
+}
+
+// Sum over the wave's 64 lanes of NV values per lane (NV = 2, 4, 8).
+// Halving steps on lane bit 2 (partner lane ^ 7), bit 1 (lane ^ 2) and bit 0
+// (lane ^ 1), as many as NV needs, then plain butterfly steps over the other
+// bits.  Returns the full sum of value k: k = lane & 7 (NV = 8),
+// (lane >> 1) & 3 (NV = 4), (lane >> 2) & 1 (NV = 2).  The order of the
+// additions depends only on the lane bits.
+template <int NV>
+__device__ inline double fold_lanes(const double (&v0)[8], int lane)
+{
+    static_assert(NV == 2 || NV == 4 || NV == 8, "NV = 2, 4 or 8");
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = v0[k];
+    halve<kDppHalfMirror, NV>(v, (lane & 4) != 0);
+    if constexpr (NV >= 4) halve<kDppXor2, NV / 2>(v, (lane & 2) != 0);
+    if constexpr (NV >= 8) halve<kDppXor1, NV / 4>(v, (lane & 1) != 0);
+    double t = v[0];
+    if constexpr (NV <= 4) t += dpp64<kDppXor1>(t);
+    if constexpr (NV <= 2) t += dpp64<kDppXor2>(t);
+    t += dpp64<kDppRor8>(t);
+    t += swz_xor16(t);
+    t += __shfl_xor(t, 32);
+    return t;
+}
+
+// Chunk CI (up to four column pairs) of the NPH column pairs a wave holds:
+// the transposed sums of its columns.  bl holds the wave's block entries.
+template <typename G, int NPH, int CI>
+__device__ inline void face_chunk(const vec_t<double, 2> (&bl)[NPH][G::RPL], const double (&xa)[G::RPL], int lane,
+                                  double (&t)[(NPH + 3) / 4])
+{
+    if constexpr (4 * CI < NPH) {
+        constexpr int NP = NPH - 4 * CI < 4 ? NPH - 4 * CI : 4;
+        constexpr int NV = NP == 1 ? 2 : (NP == 2 ? 4 : 8);
+        double pv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) pv[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NP; ++k)
+#pragma unroll
+            for (int q = 0; q < G::RPL; ++q) {
+                // lanes past the last row hold xa = 0
+                pv[2 * k] = fma(bl[4 * CI + k][q][0], xa[q], pv[2 * k]);
+                pv[2 * k + 1] = fma(bl[4 * CI + k][q][1], xa[q], pv[2 * k + 1]);
+            }
+        t[CI] = fold_lanes<NV>(pv, lane);
+        face_chunk<G, NPH, CI + 1>(bl, xa, lane, t);
+    }
+}
+
+// Which transposed sum lane `lane` stores: chunk CI = lane >> 3, and within it
+// one lane per column (the lanes of a chunk's 8 hold its up to 8 columns).
+template <int NPH, int CI>
+__device__ inline void face_pick(const double (&t)[(NPH + 3) / 4], int lane, double &val, int &col)
+{
+    if constexpr (4 * CI < NPH) {
+        constexpr int NP = NPH - 4 * CI < 4 ? NPH - 4 * CI : 4;
+        constexpr int NV = NP == 1 ? 2 : (NP == 2 ? 4 : 8);
+        const int k = NV == 8 ? (lane & 7) : (NV == 4 ? (lane >> 1) & 3 : (lane >> 2) & 1);
+        const bool writer = NV == 8 ? true : (NV == 4 ? (lane & 1) == 0 : (lane & 3) == 0);
+        if ((lane >> 3) == CI && writer && k < 2 * NP) {
+            val = t[CI];
+            col = 8 * CI + k;
+        }
+        face_pick<NPH, CI + 1>(t, lane, val, col);
+    }
+}
+
+// Column pairs [C0, C1) of a face block: all of the wave's loads are issued
+// first; then the row sums of F x_eb into acc (fma chain in column order, as
+// spmv_cols) and the column sums of F^T x_ea, stored to zb[col] with one
+// store per lane when `store` (the face is real).
+template <typename G, int C0, int C1>
+__device__ inline void face_half(const double *F, const double *xb, const double (&xa)[G::RPL], int lane,
+                                 double (&acc)[G::RPL], double *zb, bool store)
+{
+    constexpr int NPH = C1 - C0, NCH = (NPH + 3) / 4;
+    static_assert(NCH <= 8, "at most 8 chunks per wave");
+    vec_t<double, 2> bl[NPH][G::RPL];
+#pragma unroll
+    for (int k = 0; k < NPH; ++k)
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            const int row = min(lane + 64 * q, G::D - 1);
+            bl[k][q] = __builtin_nontemporal_load(
+                reinterpret_cast<const vec_t<double, 2> *>(F + ((size_t)(C0 + k) * G::D + row) * 2));
+        }
+#pragma unroll
+    for (int k = 0; k < NPH; ++k) {
+        const vec_t<double, 2> xv = *reinterpret_cast<const vec_t<double, 2> *>(&xb[2 * (C0 + k)]);
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            acc[q] = fma(bl[k][q][0], xv[0], acc[q]);
+            acc[q] = fma(bl[k][q][1], xv[1], acc[q]);
+        }
+    }
+    double t[NCH];
+    face_chunk<G, NPH, 0>(bl, xa, lane, t);
+    double val = 0.0;
+    int col = -1;
+    face_pick<NPH, 0>(t, lane, val, col);
+    if (store && col >= 0) zb[2 * C0 + col] = val;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_face64(StepArgs a)
+{
+    using G = Geom<P>;
+    static_assert(G::NG == 1 && G::D % 2 == 0, "face form: d > 32");
+    constexpr int NIT = G::DH, HALF = NIT / 2;
+    __shared__ __attribute__((aligned(16))) double xs_all[4][G::DP];
+    __shared__ double part[2][G::RPL][64];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int fl = w >> 1, h = w & 1;   // face of the work-group, half of the column pairs
+    const int fraw = xcd_remap(blockIdx.x, gridDim.x) * 2 + fl;
+    const bool act = fraw < a.nfaces;
+    // a spare wave pair of the last work-group recomputes the last face and stores nothing
+    const int4 fc = a.faces[act ? fraw : a.nfaces - 1];
+    double *xb = xs_all[w];
+    gather_one<G, double>(xb, a.ucur, fc.z, lane);
+    double xa[G::RPL];
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const int idx = lane + 64 * q;
+        xa[q] = idx < G::D ? a.ucur[(size_t)fc.x * G::D + idx] : 0.0;
+    }
+    // the wave reads only its own LDS slice: order its stores before its loads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const double *F = a.blk64 + ((size_t)fc.x * 4 + fc.y) * G::D * G::D;
+    double *zb = a.zf + ((size_t)fc.z * 3 + fc.w - 1) * G::D;
+    double acc[G::RPL];
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) acc[q] = 0.0;
+    if (h == 0) face_half<G, 0, HALF>(F, xb, xa, lane, acc, zb, act);
+    else        face_half<G, HALF, NIT>(F, xb, xa, lane, acc, zb, act);
+    if (h == 1) {
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) part[fl][q][lane] = acc[q];
+    }
+    __syncthreads();
+    if (h == 1 || !act) return;
+    double *za = a.zf + ((size_t)fc.x * 3 + fc.y - 1) * G::D;
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const int row = lane + 64 * q;
+        if (row < G::D) za[row] = acc[q] + part[fl][q][lane];
+    }
 }
 
 // ---------------------------------------------------- FP32 compensated update
@@ -501,6 +725,22 @@ template <int P>
 int step64_p(bool first, const StepArgs &a, hipStream_t s)
 {
     // one element per work-group
+    if constexpr (Geom<P>::NG == 1) {
+        if (a.faces) {
+            if (a.nfaces > 0)
+                hipLaunchKernelGGL(k_face64<P>, dim3((unsigned)((a.nfaces + 1) / 2)), dim3(256), 0, s, a);
+            const dim3 block(64 * StepShape<P, true>::WAVES), grid((unsigned)a.ne);
+            if (a.wt) {
+                if (first) hipLaunchKernelGGL((k_step64<P, true, true, true>), grid, block, 0, s, a);
+                else       hipLaunchKernelGGL((k_step64<P, false, true, true>), grid, block, 0, s, a);
+            } else {
+                if (first) hipLaunchKernelGGL((k_step64<P, true, false, true>), grid, block, 0, s, a);
+                else       hipLaunchKernelGGL((k_step64<P, false, false, true>), grid, block, 0, s, a);
+            }
+            return hipGetLastError() == hipSuccess ? 0 : -1;
+        }
+    }
+    if (a.faces) return -3;   // no face form at this degree
     const dim3 block(64 * StepShape<P>::WAVES);
     const dim3 grid((unsigned)a.ne);
     if (a.wt) {
