@@ -412,9 +412,10 @@ void build_faces(tsp_context *c)
             const int sx = nb[3 * x] == e ? 0 : (nb[3 * x + 1] == e ? 1 : 2);
             f.push_back(make_int4(e, 1 + s, x, 1 + sx));
         }
-    c->zf = dalloc<double>(c, (size_t)c->ne * 3 * c->D, "face products");
+    const size_t nz = (size_t)c->ne * 3 * face_zstride(c->D);
+    c->zf = dalloc<double>(c, nz, "face products");
     // boundary slots are never written: they read zero
-    hipcheck(c, hipMemsetAsync(c->zf, 0, sizeof(double) * (size_t)c->ne * 3 * c->D, c->stream), "memset");
+    hipcheck(c, hipMemsetAsync(c->zf, 0, sizeof(double) * nz, c->stream), "memset");
     c->faces = dalloc<int4>(c, f.size(), "faces");
     if (!f.empty()) h2d(c, c->faces, f.data(), f.size(), "faces");
     c->nfaces = (int)f.size();

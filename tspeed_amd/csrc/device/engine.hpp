@@ -39,12 +39,14 @@ struct Geom {
     static constexpr int NG = 64 / RW;                  // column groups per wave
     static constexpr int RPL = (D + 63) / 64;           // rows per lane when D > 64
     static constexpr int DP = (D + 3) / 4 * 4;          // LDS row stride (16-byte aligned groups)
+    static constexpr int ZS = (D + 15) / 16 * 16;       // face-product row stride: whole 128-byte lines
 };
 
 inline int nln_of(int P) { return (P + 1) * (P + 2) / 2; }
 // Degrees whose FP64 step has the face form: blocks one column group wide
 // (d > 32), where the face pass gives each half of a block's columns a wave.
 inline bool face_form_degree(int P) { return P >= 5 && P <= kMaxDegree; }
+inline int face_zstride(int d) { return (d + 15) / 16 * 16; }
 
 // The FP32 paths keep the displacement as u = hi - lo at two time levels
 // (u^n and u^{n-1}), hi in FP32 and lo, its compensation, in BF16: 2 bytes per
@@ -139,7 +141,8 @@ struct StepArgs {
     // faces null: every block is read by the element step (block-ELL).
     const int4 *faces;        // {ea, slot of eb in ea, eb, slot of ea in eb}, ea < eb, slots 1..3
     int nfaces;
-    double *zf;               // zf[(e*3 + slot-1)*d + row]; boundary slots stay zero
+    double *zf;               // zf[(e*3 + slot-1)*zs + row], zs = face_zstride(d): rows padded
+                              // to whole 128-byte lines, written whole; boundary slots stay zero
     // FP32 compensated: u^n in (uhi, ulo), u^{n-1} in (uhi_out, ulo_out), which
     // the step overwrites with u^{n+1}; lo arrays hold BF16 (lo_ld / lo_st)
     int tfree;                // 1: translation-free products (orthonormal basis; see gather)

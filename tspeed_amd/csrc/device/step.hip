@@ -56,10 +56,51 @@ struct StepShape {
     static constexpr int WAVES = FACES ? H : 4 * H;
 };
 
+// Columns [C0, C1) of block `slot` of element e (one column group, no tail),
+// with the block loads issued before the gather of x (u of element nb into the
+// wave's LDS slice), so the gather's latency hides behind the stream; sums as
+// spmv_cols.
+template <typename G, int C0, int C1>
+__device__ inline void spmv_cols_loads_first(const double *blk, int e, int slot, const double *u, int nb, double *x,
+                                             int lane, double (&acc)[G::RPL])
+{
+    static_assert(G::NG == 1 && G::D % 2 == 0, "one column group, no tail");
+    const double *Bs = blk + ((size_t)e * 4 + slot) * G::D * G::D;
+    vec_t<double, 2> bl[C1 - C0][G::RPL];
+#pragma unroll
+    for (int k = 0; k < C1 - C0; ++k)
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            const int row = min(lane + 64 * q, G::D - 1);
+            bl[k][q] = __builtin_nontemporal_load(
+                reinterpret_cast<const vec_t<double, 2> *>(Bs + ((size_t)(C0 + k) * G::D + row) * 2));
+        }
+    gather_one<G, double>(x, u, nb, lane);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) acc[q] = 0.0;
+#pragma unroll
+    for (int k = 0; k < C1 - C0; ++k) {
+        const vec_t<double, 2> xv = *reinterpret_cast<const vec_t<double, 2> *>(&x[2 * (C0 + k)]);
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) {
+            acc[q] = fma(bl[k][q][0], xv[0], acc[q]);
+            acc[q] = fma(bl[k][q][1], xv[1], acc[q]);
+        }
+    }
+}
+
 // WT: the new state is stored through L2 (see st_state); taken for operators
-// larger than the Infinity Cache.
+// larger than the Infinity Cache.  The face form's element step issues its
+// block loads before the gather where a lane holds one row (d <= 64), and then
+// asks for four waves per SIMD (at most 128 VGPRs, no spill).
+template <int P, bool FACES>
+constexpr bool elem_loads_first() { return FACES && Geom<P>::RPL == 1; }
 template <int P, bool FIRST, bool WT, bool FACES = false>
-__global__ __launch_bounds__((64 * StepShape<P, FACES>::WAVES)) void k_step64(StepArgs a)
+__global__ __launch_bounds__((64 * StepShape<P, FACES>::WAVES), (elem_loads_first<P, FACES>() ? 4 : 1))
+void k_step64(StepArgs a)
 {
     using G = Geom<P>;
     constexpr int H = StepShape<P>::H, NW = StepShape<P, FACES>::WAVES;
@@ -85,28 +126,35 @@ __global__ __launch_bounds__((64 * StepShape<P, FACES>::WAVES)) void k_step64(St
         up[q] = rows[q] >= 0 ? a.uprev[(size_t)e * G::D + row] : 0.0;
     }
     const double invm = (act && w == 0) ? a.invm[e] : 0.0;
-    // face products of slots 1..3 for the rows of wave 0, fetched before the barrier
+    // face products of slots 1..3 for the rows of wave 0 (read once), fetched before the barrier
     double zs[3][G::RPL];
     if constexpr (FACES) {
 #pragma unroll
         for (int k = 0; k < 3; ++k)
 #pragma unroll
             for (int q = 0; q < G::RPL; ++q)
-                zs[k][q] = rows[q] >= 0 ? a.zf[((size_t)e * 3 + k) * G::D + rows[q]] : 0.0;
+                zs[k][q] = rows[q] >= 0 ? __builtin_nontemporal_load(a.zf + ((size_t)e * 3 + k) * G::ZS + rows[q])
+                                        : 0.0;
     }
     double(*xs)[G::DP] = &xs_all[w];
-    gather_one<G, double>(xs[0], a.ucur, nb, lane);
-    // the wave reads only its own LDS slice: order its stores before its loads
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     double acc[G::RPL];
-    if constexpr (H == 1) {
-        spmv<G, double, 2>(a.blk64, e, xs, lane, acc, s, 1);
-    } else {
+    if constexpr (elem_loads_first<P, FACES>()) {
         constexpr int NIT = G::D / 2, HALF = NIT / 2;
-        if (h == 0) spmv_cols<G, double, 2, 0, HALF>(a.blk64, e, s, xs[0], lane, acc);
-        else        spmv_cols<G, double, 2, HALF, NIT>(a.blk64, e, s, xs[0], lane, acc);
+        if (h == 0) spmv_cols_loads_first<G, 0, HALF>(a.blk64, e, 0, a.ucur, nb, xs[0], lane, acc);
+        else        spmv_cols_loads_first<G, HALF, NIT>(a.blk64, e, 0, a.ucur, nb, xs[0], lane, acc);
+    } else {
+        gather_one<G, double>(xs[0], a.ucur, nb, lane);
+        // the wave reads only its own LDS slice: order its stores before its loads
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if constexpr (H == 1) {
+            spmv<G, double, 2>(a.blk64, e, xs, lane, acc, s, 1);
+        } else {
+            constexpr int NIT = G::D / 2, HALF = NIT / 2;
+            if (h == 0) spmv_cols<G, double, 2, 0, HALF>(a.blk64, e, s, xs[0], lane, acc);
+            else        spmv_cols<G, double, 2, HALF, NIT>(a.blk64, e, s, xs[0], lane, acc);
+        }
     }
     if (w > 0) {
 #pragma unroll
@@ -183,10 +231,11 @@ __global__ __launch_bounds__((64 * StepShape<P, FACES>::WAVES)) void k_step64(St
 //                                   reduce-scatter butterfly sums each column
 //                                   over the lanes
 // The operator stream per step falls from ne + #half-faces blocks to ne +
-// #faces (3.99 -> 2.50 blocks per element on a 160 000-triangle mesh).  Two waves per
-// face, one per half of the block's column pairs, as the element step splits
-// its blocks: the transposed sums of a half are complete within its wave, the
-// row sums of the two halves are added in LDS.  Two faces per work-group.
+// #faces (3.99 -> 2.50 blocks per element on a 160 000-triangle mesh).  Each
+// wave of a face takes a contiguous range of the block's column pairs (2, 4 or
+// 6 waves by degree, so that a wave's block entries stay in registers): the
+// transposed sums of its columns are complete within the wave, the row sums of
+// the ranges are added in LDS in wave order.
 
 // Lane exchanges of the transposed sums.  DPP moves (quad permutations, the
 // half-row mirror, the 8-lane rotation of a 16-lane row) stay in the VALU;
@@ -294,13 +343,15 @@ __device__ inline void face_pick(const double (&t)[(NPH + 3) / 4], int lane, dou
     }
 }
 
-// Column pairs [C0, C1) of a face block: all of the wave's loads are issued
-// first; then the row sums of F x_eb into acc (fma chain in column order, as
-// spmv_cols) and the column sums of F^T x_ea, stored to zb[col] with one
-// store per lane when `store` (the face is real).
+// Column pairs [C0, C1) of the face block F of face fc: all of the wave's
+// block loads are issued first, then the gathers (x_eb into the wave's LDS
+// slice xb, x_ea of the lane's rows into registers), so their latency hides
+// behind the stream; then the row sums of F x_eb into acc (fma chain in column
+// order, as spmv_cols) and the column sums of F^T x_ea, written to the face's
+// LDS row zrow[col] (one write per lane).
 template <typename G, int C0, int C1>
-__device__ inline void face_half(const double *F, const double *xb, const double (&xa)[G::RPL], int lane,
-                                 double (&acc)[G::RPL], double *zb, bool store)
+__device__ inline void face_half(const double *F, const double *u, int4 fc, double *xb, int lane,
+                                 double (&acc)[G::RPL], double *zrow)
 {
     constexpr int NPH = C1 - C0, NCH = (NPH + 3) / 4;
     static_assert(NCH <= 8, "at most 8 chunks per wave");
@@ -313,6 +364,17 @@ __device__ inline void face_half(const double *F, const double *xb, const double
             bl[k][q] = __builtin_nontemporal_load(
                 reinterpret_cast<const vec_t<double, 2> *>(F + ((size_t)(C0 + k) * G::D + row) * 2));
         }
+    gather_one<G, double>(xb, u, fc.z, lane);
+    double xa[G::RPL];
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const int idx = lane + 64 * q;
+        xa[q] = idx < G::D ? u[(size_t)fc.x * G::D + idx] : 0.0;
+    }
+    // the wave reads only its own LDS slice: order its stores before its loads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
     for (int k = 0; k < NPH; ++k) {
         const vec_t<double, 2> xv = *reinterpret_cast<const vec_t<double, 2> *>(&xb[2 * (C0 + k)]);
@@ -327,54 +389,82 @@ __device__ inline void face_half(const double *F, const double *xb, const double
     double val = 0.0;
     int col = -1;
     face_pick<NPH, 0>(t, lane, val, col);
-    if (store && col >= 0) zb[2 * C0 + col] = val;
+    if (col >= 0) zrow[2 * C0 + col] = val;
+}
+
+// Waves per face (each takes a contiguous range of the block's column pairs,
+// sized so that its loads stay in registers) and faces per work-group.
+template <int P>
+struct FaceShape {
+    static constexpr int HF = P <= 5 ? 2 : (P <= 7 ? 4 : 6);
+    static constexpr int FPW = HF == 2 ? 2 : 1;
+    static constexpr int THREADS = 64 * HF * FPW;
+};
+
+// Wave h of a face: column pairs [NIT*h/HF, NIT*(h+1)/HF).
+template <typename G, int HF, int H>
+__device__ inline void face_wave(int h, const double *F, const double *u, int4 fc, double *xb, int lane,
+                                 double (&acc)[G::RPL], double *zrow)
+{
+    if constexpr (H < HF) {
+        constexpr int NIT = G::DH;
+        if (h == H) face_half<G, NIT * H / HF, NIT * (H + 1) / HF>(F, u, fc, xb, lane, acc, zrow);
+        else        face_wave<G, HF, H + 1>(h, F, u, fc, xb, lane, acc, zrow);
+    }
 }
 
 template <int P>
-__global__ __launch_bounds__(256) void k_face64(StepArgs a)
+__global__ __launch_bounds__(FaceShape<P>::THREADS) void k_face64(StepArgs a)
 {
     using G = Geom<P>;
+    using S = FaceShape<P>;
     static_assert(G::NG == 1 && G::D % 2 == 0, "face form: d > 32");
-    constexpr int NIT = G::DH, HALF = NIT / 2;
-    __shared__ __attribute__((aligned(16))) double xs_all[4][G::DP];
-    __shared__ double part[2][G::RPL][64];
+    __shared__ __attribute__((aligned(16))) double xs_all[S::HF * S::FPW][G::DP];
+    __shared__ double part[S::FPW][S::HF - 1][G::RPL][64];
+    __shared__ double zrow[S::FPW][G::ZS];   // the face's transposed sums, assembled from its waves
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int fl = w >> 1, h = w & 1;   // face of the work-group, half of the column pairs
-    const int fraw = xcd_remap(blockIdx.x, gridDim.x) * 2 + fl;
+    const int fl = w / S::HF, h = w % S::HF;   // face of the work-group, column range of the wave
+    const int fraw = xcd_remap(blockIdx.x, gridDim.x) * S::FPW + fl;
     const bool act = fraw < a.nfaces;
-    // a spare wave pair of the last work-group recomputes the last face and stores nothing
+    // a spare wave group of the last work-group recomputes the last face and stores nothing
     const int4 fc = a.faces[act ? fraw : a.nfaces - 1];
-    double *xb = xs_all[w];
-    gather_one<G, double>(xb, a.ucur, fc.z, lane);
-    double xa[G::RPL];
-#pragma unroll
-    for (int q = 0; q < G::RPL; ++q) {
-        const int idx = lane + 64 * q;
-        xa[q] = idx < G::D ? a.ucur[(size_t)fc.x * G::D + idx] : 0.0;
-    }
-    // the wave reads only its own LDS slice: order its stores before its loads
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const double *F = a.blk64 + ((size_t)fc.x * 4 + fc.y) * G::D * G::D;
-    double *zb = a.zf + ((size_t)fc.z * 3 + fc.w - 1) * G::D;
     double acc[G::RPL];
 #pragma unroll
     for (int q = 0; q < G::RPL; ++q) acc[q] = 0.0;
-    if (h == 0) face_half<G, 0, HALF>(F, xb, xa, lane, acc, zb, act);
-    else        face_half<G, HALF, NIT>(F, xb, xa, lane, acc, zb, act);
-    if (h == 1) {
+    if (h == 0 && lane < G::ZS - G::D) zrow[fl][G::D + lane] = 0.0;   // the row's padding
+    face_wave<G, S::HF, 0>(h, F, a.ucur, fc, xs_all[w], lane, acc, zrow[fl]);
+    if (h > 0) {
 #pragma unroll
-        for (int q = 0; q < G::RPL; ++q) part[fl][q][lane] = acc[q];
+        for (int q = 0; q < G::RPL; ++q) part[fl][h - 1][q][lane] = acc[q];
     }
     __syncthreads();
-    if (h == 1 || !act) return;
-    double *za = a.zf + ((size_t)fc.x * 3 + fc.y - 1) * G::D;
+    if (h > 1 || !act) return;
+    // each product row is written whole, padding included, with non-temporal
+    // stores: wave 0 the row sums, wave 1 the transposed sums.  Writes mixed
+    // into the block stream cost far more than their bytes (a block stream
+    // with 896 bytes written per 25 KB block: 7.15 -> 5.85 TB/s of blocks), and
+    // the non-temporal form is the cheapest measured here (p = 6: 1.85 ->
+    // 1.78 ms per step; stores through L2 1.82)
+    if (h == 1) {
+        double *zb = a.zf + ((size_t)fc.z * 3 + fc.w - 1) * G::ZS;
+#pragma unroll
+        for (int q = 0; q < (G::ZS + 63) / 64; ++q) {
+            const int row = lane + 64 * q;
+            if (row < G::ZS) __builtin_nontemporal_store(zrow[fl][row], zb + row);
+        }
+        return;
+    }
+    double *za = a.zf + ((size_t)fc.x * 3 + fc.y - 1) * G::ZS;
+#pragma unroll
+    for (int k = 0; k < S::HF - 1; ++k)
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) acc[q] += part[fl][k][q][lane];
 #pragma unroll
     for (int q = 0; q < G::RPL; ++q) {
         const int row = lane + 64 * q;
-        if (row < G::D) za[row] = acc[q] + part[fl][q][lane];
+        if (row < G::ZS) __builtin_nontemporal_store(row < G::D ? acc[q] : 0.0, za + row);
     }
 }
 
@@ -728,7 +818,8 @@ int step64_p(bool first, const StepArgs &a, hipStream_t s)
     if constexpr (Geom<P>::NG == 1) {
         if (a.faces) {
             if (a.nfaces > 0)
-                hipLaunchKernelGGL(k_face64<P>, dim3((unsigned)((a.nfaces + 1) / 2)), dim3(256), 0, s, a);
+                hipLaunchKernelGGL(k_face64<P>, dim3((unsigned)((a.nfaces + FaceShape<P>::FPW - 1) / FaceShape<P>::FPW)),
+                                   dim3(FaceShape<P>::THREADS), 0, s, a);
             const dim3 block(64 * StepShape<P, true>::WAVES), grid((unsigned)a.ne);
             if (a.wt) {
                 if (first) hipLaunchKernelGGL((k_step64<P, true, true, true>), grid, block, 0, s, a);
