@@ -481,7 +481,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": ("k_face64<6> + k_step64<6,false,true,true> (one step: the face pass and "
                                 "the element step)" if r64["face_form"] else "k_step64<6,false,true>"),
-                     "algorithmic_bytes": bytes64},
+                     "algorithmic_bytes": bytes64,
+                     # the face form moves fewer bytes than the block-ELL step (every stored
+                     # block read): the same step time on those bytes, and their floor at peak
+                     "block_ell_bytes": algorithmic_bytes(prob, 8, 24),
+                     "achieved_on_block_ell_bytes": algorithmic_bytes(prob, 8, 24) / (ms_step / 1e3) / 1e9,
+                     "block_ell_floor_ms": algorithmic_bytes(prob, 8, 24) / (HBM_PEAK_GBS * 1e9) * 1e3},
         "variants": variants,
         "setup_s": r64["setup_s"],
         "status_flags": r64["status"],
