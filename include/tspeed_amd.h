@@ -162,12 +162,14 @@ int tsp_set_stride(tsp_context *ctx, int stride);
  * symmetric, so the two off-diagonal blocks of an interior face carry the same
  * entries (transposed); the face form reads only the lower-numbered element's
  * copy and applies it to both sides, streaming ne + #faces blocks per step
- * instead of ne + 2 #faces.  It is taken for degrees N >= 5 when the operator
- * comes from tsp_set_mesh, or from tsp_upload_blocks with B_{b,a} = B_{a,b}^T
- * bit for bit, and every neighbour relation is mutual; it changes the result by
- * the rounding asymmetry of the assembled blocks (~2e-16 of the largest entry).
- * on = 1 use it where possible (the default), 0 never (every block read as
- * stored), -1 leave the setting; *active (may be NULL) receives whether the
+ * instead of ne + 2 #faces.  It is possible when the operator comes from
+ * tsp_set_mesh, or from tsp_upload_blocks with B_{b,a} = B_{a,b}^T bit for bit,
+ * and every neighbour relation is mutual; it changes the result by the
+ * rounding asymmetry of the assembled blocks (~2e-16 of the largest entry).
+ * on = 1 (the default): use it where possible for operators above 128 MB (a
+ * smaller one stays in the Infinity Cache, where one launch reading every
+ * block is faster); 2: wherever possible; 0: never (every block read as
+ * stored); -1: leave the setting.  *active (may be NULL) receives whether the
  * steps use it. */
 int tsp_set_face_form(tsp_context *ctx, int on, int *active);
 

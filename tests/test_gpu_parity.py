@@ -250,7 +250,7 @@ def test_fp64_large_operator_step_matches_oracle():
 
 
 @pytest.mark.parametrize("basis", [O.DUBINER, O.BOUNDARY_ADAPTED])
-@pytest.mark.parametrize("N", [5, 6, 7, 8])
+@pytest.mark.parametrize("N", range(1, 9))
 def test_fp64_face_form_against_block_ell_and_oracle(N, basis):
     """The FP64 face form (each interior face's block read once, from the
     lower-numbered element, and applied to both sides) against the step that
@@ -268,7 +268,7 @@ def test_fp64_face_form_against_block_ell_and_oracle(N, basis):
     runs = {}
     for on in (True, False):
         eng = capi.Engine("fp64")
-        eng.set_face_form(on)
+        eng.set_face_form("always" if on else False)   # a small operator: forced
         eng, u, s = W.run_engine(p, engine=eng, u0=u0, v0=v0)
         assert eng.set_face_form() is on
         assert eng.status() == 0
@@ -284,24 +284,40 @@ def test_fp64_face_form_against_block_ell_and_oracle(N, basis):
 
 
 def test_fp64_face_form_availability_and_uploaded_blocks():
-    """The face form is taken for assembled operators at N >= 5 only, and for
-    uploaded blocks only when they are symmetric bit for bit; on such blocks
-    it reads the same entries as the block-ELL step (they differ in the order
-    of the slot sums only)."""
-    for N in (4, 5):
+    """The face form is taken by default for assembled FP64 operators above
+    the cache-resident size (128 MB) at every degree, on request ("always") for
+    smaller ones, never for other precisions, and for uploaded blocks only
+    when they are symmetric bit for bit; on such blocks it reads the same
+    entries as the block-ELL step (they differ in the order of the slot sums
+    only)."""
+    for N in (2, 4, 5):
         p = small_problem(N, nsteps=5)
         eng = capi.Engine("fp64")
         W.set_space(eng, p)
         eng.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
-        assert eng.set_face_form() is (N >= 5)
-        assert eng.set_face_form(False) is False and eng.set_face_form(True) is (N >= 5)
+        assert eng.set_face_form() is False          # a small operator
+        assert eng.set_face_form("always") is True
+        assert eng.set_face_form(False) is False and eng.set_face_form("always") is True
         eng.close()
+    big = W.layered_ricker(nx=35, ny=20, N=6, nsteps=5)   # 140 MB operator
+    eng = capi.Engine("fp64")
+    W.set_space(eng, big)
+    eng.set_mesh(big.verts, big.neigh, big.neighedge, big.lam, big.mu, big.rho, big.penalty)
+    assert eng.set_face_form() is True
+    eng.close()
+    f = capi.Engine("fp32c")
+    p = small_problem(5, nsteps=5)
+    W.set_space(f, p)
+    f.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+    assert f.set_face_form("always") is False
+    f.close()
     p = small_problem(6, nsteps=60)
     op = oracle_op(p)
     blocks = op.blocks().copy()
     # the assembled blocks are symmetric to rounding only: uploaded as they are,
     # every stored block is read
     eng = capi.Engine("fp64")
+    eng.set_face_form("always")
     eng.set_tables(p.N)
     eng.upload_blocks(p.neigh, blocks, op.invm())
     assert eng.set_face_form() is False
@@ -318,7 +334,7 @@ def test_fp64_face_form_availability_and_uploaded_blocks():
     for on in (True, False):
         eng = capi.Engine("fp64")
         eng.set_tables(p.N)
-        eng.set_face_form(on)
+        eng.set_face_form("always" if on else False)
         eng.upload_blocks(p.neigh, blocks, op.invm())
         assert eng.set_face_form() is on
         eng.set_sources(p.src_elem, p.src_phi, p.wavelet())
@@ -339,7 +355,9 @@ def test_fp64_face_form_shared_operator_and_toggle_mid_run():
     list and products; switching the face form off and on between runs keeps
     the trajectory (to 1e-12 of a run that never switches)."""
     p = small_problem(7, nsteps=80)
-    e1, u1, s1 = W.run_engine(p, "fp64")
+    e1 = capi.Engine("fp64")
+    e1.set_face_form("always")
+    e1, u1, s1 = W.run_engine(p, engine=e1)
     b = capi.Engine("fp64", share=e1)
     assert b.set_face_form() is True
     b.set_sources(p.src_elem, p.src_phi, p.wavelet())
@@ -350,7 +368,7 @@ def test_fp64_face_form_shared_operator_and_toggle_mid_run():
     b.run(30)
     assert b.set_face_form(False) is False
     b.run(20)
-    assert b.set_face_form(True) is True
+    assert b.set_face_form("always") is True
     b.run(p.nsteps - 50)
     u2, s2 = b.get_u(), b.seismograms()
     say(f"shared operator, face form toggled: u {rel_l2(u2, u1):.2e} seis {rel_l2(s2, s1):.2e}")

@@ -401,9 +401,12 @@ class Engine:
 
     def set_face_form(self, on=None):
         """FP64 face form (each interior face's block read once for both sides):
-        on True/False sets it, None only queries; returns whether steps use it."""
+        True = for operators above the cache-resident size (the default),
+        "always" = wherever possible, False = never, None = only ask; returns
+        whether the steps use it."""
+        code = -1 if on is None else (2 if on == "always" else int(bool(on)))
         act = _i(0)
-        self._ck(self._L.tsp_set_face_form(self._h, -1 if on is None else int(bool(on)), ctypes.byref(act)))
+        self._ck(self._L.tsp_set_face_form(self._h, code, ctypes.byref(act)))
         return bool(act.value)
 
     def set_dt(self, dt):

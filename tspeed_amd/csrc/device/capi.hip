@@ -42,7 +42,8 @@ struct tsp_context {
     // (assembled by tsp_set_mesh, or uploaded blocks symmetric bit for bit),
     // the degree has it, every neighbour relation is mutual and the caller has
     // not turned it off (tsp_set_face_form)
-    bool symmetric = false, face_form = true;
+    bool symmetric = false;
+    int face_form = 1;   // tsp_set_face_form: 0 never, 1 above the cache-resident size, 2 always
     int4 *faces = nullptr;
     int nfaces = 0;
     double *zf = nullptr;
@@ -384,6 +385,14 @@ int run_graph(tsp_context *c, int nsteps)
     return nsteps - reps * c->graph_len;
 }
 
+// FP64 operators larger than this (the 256 MB Infinity Cache less room for
+// the state) take the store-through-L2 step and, by default, the face form;
+// smaller ones stay cache-resident between steps and do better with plain
+// stores and with every block read by one launch (3200 triangles at p = 4:
+// 14.5 us per step against 15.9 with the face form's two launches; 1600 at
+// p = 6, a 160 MB operator: 27.7 against 20.7).
+constexpr double kCacheResidentBytes = 128.0 * (1 << 20);
+
 // Face list of the FP64 face form: one entry per interior face, from the
 // lower-numbered element's side.  Left empty (the block-ELL step) unless every
 // neighbour relation is mutual and single, so that each (element, slot) with a
@@ -394,7 +403,9 @@ void build_faces(tsp_context *c)
     dfree(c, c->faces);
     dfree(c, c->zf);
     c->nfaces = 0;
-    if (c->precision != TSP_FP64 || c->batched() || !c->symmetric || !c->face_form || !face_form_degree(c->P) ||
+    const double ell_bytes = (double)c->ne * 4 * c->D * c->D * sizeof(double);
+    if (c->face_form == 0 || (c->face_form == 1 && ell_bytes <= kCacheResidentBytes)) return;
+    if (c->precision != TSP_FP64 || c->batched() || !c->symmetric || !face_form_degree(c->P) ||
         !c->ne)
         return;
     const std::vector<int> &nb = c->h_neigh;
@@ -474,10 +485,6 @@ void operator_ready(tsp_context *c)
     build_faces(c);
 }
 
-// FP64 operators larger than this (the 256 MB Infinity Cache less room for
-// the state) take the store-through-L2 step; smaller ones stay cache-resident
-// between steps and do better with plain stores.
-constexpr double kCacheResidentBytes = 128.0 * (1 << 20);
 
 StepArgs base_args(tsp_context *c)
 {
@@ -939,9 +946,9 @@ int tsp_set_stride(tsp_context *c, int stride)
 int tsp_set_face_form(tsp_context *c, int on, int *active)
 {
     return guarded(c, [&] {
-        if (on < -1 || on > 1) fail(c, TSP_ERR_ARG, "on must be 1, 0 or -1 (query)");
-        if (on >= 0 && (on != 0) != c->face_form) {
-            c->face_form = on != 0;
+        if (on < -1 || on > 2) fail(c, TSP_ERR_ARG, "on must be 1, 2, 0 or -1 (query)");
+        if (on >= 0 && on != c->face_form) {
+            c->face_form = on;
             if (c->ne) {
                 // queued steps may still read the face buffers
                 hipcheck(c, hipStreamSynchronize(c->stream), "synchronize");

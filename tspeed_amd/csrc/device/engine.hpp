@@ -43,9 +43,10 @@ struct Geom {
 };
 
 inline int nln_of(int P) { return (P + 1) * (P + 2) / 2; }
-// Degrees whose FP64 step has the face form: blocks one column group wide
-// (d > 32), where the face pass gives each half of a block's columns a wave.
-inline bool face_form_degree(int P) { return P >= 5 && P <= kMaxDegree; }
+// Degrees whose FP64 step has the face form: all of them (the face pass splits
+// a block's columns over waves for d > 32 and gives a lane group one face for
+// d <= 32).
+inline bool face_form_degree(int P) { return P >= 1 && P <= kMaxDegree; }
 inline int face_zstride(int d) { return (d + 15) / 16 * 16; }
 
 // The FP32 paths keep the displacement as u = hi - lo at two time levels

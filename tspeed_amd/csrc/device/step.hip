@@ -97,16 +97,15 @@ __device__ inline void spmv_cols_loads_first(const double *blk, int e, int slot,
 // block loads before the gather where a lane holds one row (d <= 64), and then
 // asks for four waves per SIMD (at most 128 VGPRs, no spill).
 template <int P, bool FACES>
-constexpr bool elem_loads_first() { return FACES && Geom<P>::RPL == 1; }
+constexpr bool elem_loads_first() { return FACES && Geom<P>::NG == 1 && Geom<P>::RPL == 1; }
 template <int P, bool FIRST, bool WT, bool FACES = false>
 __global__ __launch_bounds__((64 * StepShape<P, FACES>::WAVES), (elem_loads_first<P, FACES>() ? 4 : 1))
 void k_step64(StepArgs a)
 {
     using G = Geom<P>;
     constexpr int H = StepShape<P>::H, NW = StepShape<P, FACES>::WAVES;
-    static_assert(!FACES || G::NG == 1, "face form: d > 32");
     __shared__ __attribute__((aligned(16))) double xs_all[NW][G::DP];
-    __shared__ double part[NW - 1][G::RPL][64];
+    __shared__ double part[NW > 1 ? NW - 1 : 1][G::RPL][64];
     __shared__ double wrow[G::DP];   // dense inverse mass: the element's w rows
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -275,15 +274,16 @@ __device__ inline void halve(double (&v)[8], bool hi)
     }
 }
 
-// Sum over the wave's 64 lanes of NV values per lane (NV = 2, 4, 8).
-// Halving steps on lane bit 2 (partner lane ^ 7), bit 1 (lane ^ 2) and bit 0
-// (lane ^ 1), as many as NV needs, then plain butterfly steps over the other
-// bits.  Returns the full sum of value k: k = lane & 7 (NV = 8),
-// (lane >> 1) & 3 (NV = 4), (lane >> 2) & 1 (NV = 2).  The order of the
-// additions depends only on the lane bits.
-template <int NV>
+// Sum over an aligned group of GL lanes (16, 32 or the wave's 64) of NV values
+// per lane (NV = 2, 4, 8).  Halving steps on lane bit 2 (partner lane ^ 7),
+// bit 1 (lane ^ 2) and bit 0 (lane ^ 1), as many as NV needs, then plain
+// butterfly steps over the group's other bits.  Returns the full sum of value
+// k: k = lane & 7 (NV = 8), (lane >> 1) & 3 (NV = 4), (lane >> 2) & 1
+// (NV = 2).  The order of the additions depends only on the lane bits.
+template <int NV, int GL = 64>
 __device__ inline double fold_lanes(const double (&v0)[8], int lane)
 {
+    static_assert(GL == 16 || GL == 32 || GL == 64, "groups of 16, 32 or 64 lanes");
     static_assert(NV == 2 || NV == 4 || NV == 8, "NV = 2, 4 or 8");
     double v[8];
 #pragma unroll
@@ -295,14 +295,14 @@ __device__ inline double fold_lanes(const double (&v0)[8], int lane)
     if constexpr (NV <= 4) t += dpp64<kDppXor1>(t);
     if constexpr (NV <= 2) t += dpp64<kDppXor2>(t);
     t += dpp64<kDppRor8>(t);
-    t += swz_xor16(t);
-    t += __shfl_xor(t, 32);
+    if constexpr (GL >= 32) t += swz_xor16(t);
+    if constexpr (GL == 64) t += __shfl_xor(t, 32);
     return t;
 }
 
 // Chunk CI (up to four column pairs) of the NPH column pairs a wave holds:
 // the transposed sums of its columns.  bl holds the wave's block entries.
-template <typename G, int NPH, int CI>
+template <typename G, int NPH, int CI, int GL = 64>
 __device__ inline void face_chunk(const vec_t<double, 2> (&bl)[NPH][G::RPL], const double (&xa)[G::RPL], int lane,
                                   double (&t)[(NPH + 3) / 4])
 {
@@ -320,8 +320,8 @@ __device__ inline void face_chunk(const vec_t<double, 2> (&bl)[NPH][G::RPL], con
                 pv[2 * k] = fma(bl[4 * CI + k][q][0], xa[q], pv[2 * k]);
                 pv[2 * k + 1] = fma(bl[4 * CI + k][q][1], xa[q], pv[2 * k + 1]);
             }
-        t[CI] = fold_lanes<NV>(pv, lane);
-        face_chunk<G, NPH, CI + 1>(bl, xa, lane, t);
+        t[CI] = fold_lanes<NV, GL>(pv, lane);
+        face_chunk<G, NPH, CI + 1, GL>(bl, xa, lane, t);
     }
 }
 
@@ -466,6 +466,65 @@ __global__ __launch_bounds__(FaceShape<P>::THREADS) void k_face64(StepArgs a)
         const int row = lane + 64 * q;
         if (row < G::ZS) __builtin_nontemporal_store(row < G::D ? acc[q] : 0.0, za + row);
     }
+}
+
+// The face pass for blocks of at most 32 rows (d <= 32, p <= 4): each group
+// of RW = 16 or 32 lanes (lane = row) takes one face whole: its block entries
+// in registers (loads first), the row sums, the transposed sums folded over
+// the group's lanes, and the two product rows written whole by the group.
+template <int P>
+__global__ __launch_bounds__(256) void k_face64n(StepArgs a)
+{
+    using G = Geom<P>;
+    static_assert(G::NG > 1 && G::RPL == 1 && G::D % 2 == 0, "narrow face pass: d <= 32");
+    constexpr int NF = 4 * G::NG, NPH = G::DH, NCH = (NPH + 3) / 4;
+    static_assert(NCH <= G::RW / 8 && G::ZS <= G::RW, "a group holds its chunks and its product rows");
+    __shared__ __attribute__((aligned(16))) double xs_all[NF][G::DP];
+    __shared__ double zrow[NF][G::ZS];
+    const int lane = threadIdx.x & 63;
+    const int g = lane / G::RW, r = lane % G::RW;
+    const int fl = (threadIdx.x >> 6) * G::NG + g;   // face of the work-group
+    const int fraw = xcd_remap(blockIdx.x, gridDim.x) * NF + fl;
+    const bool act = fraw < a.nfaces;
+    // a spare group of the last work-group recomputes the last face and stores nothing
+    const int4 fc = a.faces[act ? fraw : a.nfaces - 1];
+    const double *F = a.blk64 + ((size_t)fc.x * 4 + fc.y) * G::D * G::D;
+    const int row = min(r, G::D - 1);
+    vec_t<double, 2> bl[NPH][1];
+#pragma unroll
+    for (int k = 0; k < NPH; ++k)
+        bl[k][0] = __builtin_nontemporal_load(
+            reinterpret_cast<const vec_t<double, 2> *>(F + ((size_t)k * G::D + row) * 2));
+    double *xb = xs_all[fl];
+    const double xbv = r < G::D ? a.ucur[(size_t)fc.z * G::D + r] : 0.0;
+    double xa[1] = {r < G::D ? a.ucur[(size_t)fc.x * G::D + r] : 0.0};
+    if (r < G::D) xb[r] = xbv;
+    if (r < G::ZS - G::D) zrow[fl][G::D + r] = 0.0;   // the row's padding
+    // the group reads only its own LDS slices: order its stores before its loads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < NPH; ++k) {
+        const vec_t<double, 2> xv = *reinterpret_cast<const vec_t<double, 2> *>(&xb[2 * k]);
+        acc = fma(bl[k][0][0], xv[0], acc);
+        acc = fma(bl[k][0][1], xv[1], acc);
+    }
+    double t[NCH];
+    face_chunk<G, NPH, 0, G::RW>(bl, xa, lane, t);
+    double val = 0.0;
+    int col = -1;
+    face_pick<NPH, 0>(t, r, val, col);
+    if (col >= 0) zrow[fl][col] = val;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (!act || r >= G::ZS) return;
+    double *za = a.zf + ((size_t)fc.x * 3 + fc.y - 1) * G::ZS;
+    double *zb = a.zf + ((size_t)fc.z * 3 + fc.w - 1) * G::ZS;
+    __builtin_nontemporal_store(r < G::D ? acc : 0.0, za + r);
+    __builtin_nontemporal_store(zrow[fl][r], zb + r);
 }
 
 // ---------------------------------------------------- FP32 compensated update
@@ -815,11 +874,18 @@ template <int P>
 int step64_p(bool first, const StepArgs &a, hipStream_t s)
 {
     // one element per work-group
-    if constexpr (Geom<P>::NG == 1) {
+    {
         if (a.faces) {
-            if (a.nfaces > 0)
-                hipLaunchKernelGGL(k_face64<P>, dim3((unsigned)((a.nfaces + FaceShape<P>::FPW - 1) / FaceShape<P>::FPW)),
-                                   dim3(FaceShape<P>::THREADS), 0, s, a);
+            if (a.nfaces > 0) {
+                if constexpr (Geom<P>::NG == 1)
+                    hipLaunchKernelGGL(k_face64<P>,
+                                       dim3((unsigned)((a.nfaces + FaceShape<P>::FPW - 1) / FaceShape<P>::FPW)),
+                                       dim3(FaceShape<P>::THREADS), 0, s, a);
+                else
+                    hipLaunchKernelGGL(k_face64n<P>,
+                                       dim3((unsigned)((a.nfaces + 4 * Geom<P>::NG - 1) / (4 * Geom<P>::NG))),
+                                       dim3(256), 0, s, a);
+            }
             const dim3 block(64 * StepShape<P, true>::WAVES), grid((unsigned)a.ne);
             if (a.wt) {
                 if (first) hipLaunchKernelGGL((k_step64<P, true, true, true>), grid, block, 0, s, a);
@@ -831,7 +897,6 @@ int step64_p(bool first, const StepArgs &a, hipStream_t s)
             return hipGetLastError() == hipSuccess ? 0 : -1;
         }
     }
-    if (a.faces) return -3;   // no face form at this degree
     const dim3 block(64 * StepShape<P>::WAVES);
     const dim3 grid((unsigned)a.ne);
     if (a.wt) {
