@@ -197,6 +197,33 @@ def test_operator_symmetric_and_psd():
     assert (ev < 1e-10 * ev[-1]).sum() == 3          # exactly the three rigid-body modes
 
 
+@pytest.mark.parametrize("basis", [O.DUBINER, O.BOUNDARY_ADAPTED])
+@pytest.mark.parametrize("N", range(1, 9))
+def test_face_blocks_are_transposes_of_each_other(N, basis):
+    """B_{b,a} = B_{a,b}^T on every interior face, to the rounding of assembly
+    (the consistency term enters as I + I^T and the penalty block is the same
+    form seen from either side), at every degree, in both bases, with
+    per-element materials: the premise of the FP64 face form, which reads one
+    copy of each face block for both sides.  The element's own blocks are
+    symmetric too."""
+    pts, tri, verts = _mesh(4, 3, seed=5 + N)
+    nb, ng = O.neighbors(tri)
+    ne = tri.shape[0]
+    rng = np.random.default_rng(N)
+    op = O.Operator(N, verts, nb, ng, 6e9 * rng.uniform(0.5, 1.5, ne), 6e9 * rng.uniform(0.5, 1.5, ne),
+                    2000.0 * rng.uniform(0.8, 1.2, ne), 10.0, basis=basis)
+    B = op.blocks()
+    scale = np.abs(B).max()
+    worst = 0.0
+    for a in range(ne):
+        worst = max(worst, np.abs(B[a, 0] - B[a, 0].T).max())
+        for f in range(3):
+            b = nb[a, f]
+            if b >= 0:
+                worst = max(worst, np.abs(B[a, 1 + f] - B[b, 1 + ng[a, f]].T).max())
+    assert worst <= 1e-15 * scale
+
+
 @pytest.mark.parametrize("N", [1, 2, 4])
 def test_rigid_body_modes_in_kernel(N):
     """All boundaries are traction-free (Matrices_imp.hpp:107,155 skip boundary

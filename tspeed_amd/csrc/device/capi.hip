@@ -805,7 +805,8 @@ int tsp_upload_blocks(tsp_context *c, int ne, const int *neigh, const double *bl
         hipcheck(c, hipStreamSynchronize(c->stream), "upload");
         dfree(c, canon);
         dfree(c, d_nb);
-        c->symmetric = blocks_symmetric(ne, c->D, neigh, blocks);
+        // only an FP64 single-column context can take the face form
+        c->symmetric = c->precision == TSP_FP64 && !c->batched() && blocks_symmetric(ne, c->D, neigh, blocks);
         operator_ready(c);
     });
 }
