@@ -1263,7 +1263,7 @@ def _wedge_initial_fields(dt, x0=1480.0):
     return u0, v0
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+@pytest.mark.parametrize("precision", ["fp64", "fp64_faces", "fp32c"])
 def test_reference_wedge_example(precision):
     """The reference's own Wedge example input (Examples/Meshes/wedge.msh and
     Examples/Receivers/wedge.rcv, kept as fixtures in tests/golden): an
@@ -1272,7 +1272,9 @@ def test_reference_wedge_example(precision):
     (L2-projected as FESpace::transform does) started next to the first
     receiver, 100 receivers on the surface sampled every second step
     (Wedge.cpp:93-146).  FP64 matches the oracle to 1e-12 in the final
-    displacement and every seismogram; FP32c to 1e-4."""
+    displacement and every seismogram, in the block-ELL step and in the face
+    form (forced on this small operator: an unstructured mesh's face slots);
+    FP32c to 1e-4."""
     N, dt, nsteps, stride = 5, 5e-5, 300, 2
     pts, tri, reg, _ = capi.read_msh(os.path.join(GOLDEN, "wedge.msh"))
     nb, ng = capi.neighbors(tri)
@@ -1290,9 +1292,11 @@ def test_reference_wedge_example(precision):
     op = O.Operator(N, verts, nb, ng, lam, mu, rho, 2.0)
     assert dt < 2 / np.sqrt(op.lambda_max(300))          # the example's step is stable
     uref, sref = op.run(dt, nsteps, rec_elem=rel, rec_phi=rphi, stride=stride, u0=u0, v0=v0)
-    e = capi.Engine(precision)
+    e = capi.Engine("fp64" if precision == "fp64_faces" else precision)
+    e.set_face_form("always" if precision == "fp64_faces" else True)
     e.set_tables(N)
     e.set_mesh(verts, nb, ng, lam, mu, rho, 2.0)
+    assert e.set_face_form() is (precision == "fp64_faces")
     e.set_receivers(rel, rphi, stride, 1 + nsteps // stride)
     e.set_dt(dt)
     e.set_state(u0, v0)
@@ -1302,14 +1306,14 @@ def test_reference_wedge_example(precision):
     assert e.status() == 0
     e.close()
     assert np.abs(sref).max() > 1e-3 * np.abs(uref).max()   # the receivers record the pulse
-    tol = 1e-12 if precision == "fp64" else 1e-4
+    tol = 1e-4 if precision == "fp32c" else 1e-12
     eu, es = rel_l2(u, uref), seis_err(s, sref)
     say(f"  wedge {precision}: u {eu:.2e}, seismograms {es[0]:.2e} (worst receiver {es[1]:.2e})")
     assert s.shape == sref.shape
     assert eu <= tol and es[0] <= tol and es[1] <= tol
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+@pytest.mark.parametrize("precision", ["fp64", "fp64_faces", "fp32c"])
 def test_reference_lamb_example(precision):
     """The reference's own Lamb example input (Examples/Meshes/
     Lamb_fullyunstruct3.msh and Examples/Receivers/lamb.rcv, fixtures in
@@ -1319,7 +1323,8 @@ def test_reference_lamb_example(precision):
     (1500, 1950).  Its two receivers are 700-1200 m away, beyond a short run,
     so a third receiver right above the source is added; 1500 steps, samples
     every second step.  FP64 (inverse mass applied after the block product, as
-    the reference does) matches the oracle's BoundaryAdapted run to 1e-12;
+    the reference does) matches the oracle's BoundaryAdapted run to 1e-12, in
+    the block-ELL step and in the face form (forced on this small operator);
     FP32c, which folds the inverse mass into the operator, to 1e-3 (the basis's
     mass matrix is ill-conditioned in FP32)."""
     N, dt, nsteps, stride = 5, 5e-5, 1500, 2
@@ -1342,10 +1347,12 @@ def test_reference_lamb_example(precision):
     uref, sref = op.run(dt, nsteps, src=dict(elem=int(sel[0]), phi=sphi, f0=f0, t0=t0, dirx=0.0, diry=-1.0),
                         rec_elem=rel, rec_phi=rphi, stride=stride)
     assert np.abs(sref[:, 2]).max() > 1e-3 * np.abs(uref).max()   # the added receiver records the source
-    e = capi.Engine(precision)
+    e = capi.Engine("fp64" if precision == "fp64_faces" else precision)
+    e.set_face_form("always" if precision == "fp64_faces" else True)
     e.set_tables(N, capi.tables(N, BA))
     e.set_mass(capi.base_invmass(N, BA))
     e.set_mesh(verts, nb, ng, lam, mu, rho, 2.0)
+    assert e.set_face_form() is (precision == "fp64_faces")
     wav = capi.ricker_table(f0, t0, 0.0, -1.0, dt, nsteps + 1)
     e.set_sources(sel, sphi[None, :], wav)
     e.set_receivers(rel, rphi, stride, 1 + nsteps // stride)
@@ -1356,7 +1363,7 @@ def test_reference_lamb_example(precision):
     u, s = e.get_u(), e.seismograms()
     assert e.status() == 0
     e.close()
-    tol = 1e-12 if precision == "fp64" else 1e-3
+    tol = 1e-3 if precision == "fp32c" else 1e-12
     eu, es = rel_l2(u, uref), seis_err(s, sref)
     say(f"  lamb {precision}: u {eu:.2e}, seismograms {es[0]:.2e} (worst receiver {es[1]:.2e})")
     assert s.shape == sref.shape
