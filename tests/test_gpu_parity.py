@@ -633,6 +633,44 @@ def test_layered_full_size_blocks_and_linearity():
     assert eng.status() == 0
 
 
+def test_layered_full_size_face_form_against_block_ell():
+    """The layered workload at full size (160k triangles, p = 6, 16 GB of FP64
+    blocks), whose operator takes the face form by default: 300 steps with the
+    source firing early (t0 = 20 dt, f0 = 1/(16 dt)) and 16 receivers 5-30 m
+    from it, against the same context stepping every stored block (the
+    block-ELL step, itself the oracle to 1e-12 on the meshes the oracle runs):
+    within 1e-12 in u and in every seismogram, the wave having reached the
+    receivers."""
+    p = W.layered_ricker(nsteps=300)
+    p.t0 = np.array([20 * p.dt])
+    p.f0 = np.array([1.0 / (16 * p.dt)])
+    ang = np.arange(4) * np.pi / 2 + 0.3
+    p.rec_xy = np.array([[p.src_xy[0, 0] + r * np.cos(a), p.src_xy[0, 1] + r * np.sin(a)]
+                         for r in (5.0, 10.0, 20.0, 30.0) for a in ang])
+    p.__post_init__()
+    eng = capi.Engine("fp64")
+    W.set_space(eng, p)
+    eng.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+    eng.set_sources(p.src_elem, p.src_phi, p.wavelet())
+    eng.set_receivers(p.rec_elem, p.rec_phi, 1, p.nsamples)
+    eng.set_dt(p.dt)
+    out = {}
+    for on in (True, False):
+        assert eng.set_face_form(on) is on
+        eng.set_state()
+        eng.first_step()
+        eng.run(p.nsteps)
+        out[on] = (eng.get_u(), eng.seismograms())
+        assert eng.status() == 0
+    eng.close()
+    (uf, sf), (ue, se) = out[True], out[False]
+    eu, es = rel_l2(uf, ue), seis_err(sf, se)
+    say(f"  layered full size, face form vs block-ELL: u {eu:.2e}, seismograms {es[0]:.2e} "
+        f"(worst receiver {es[1]:.2e}), |s|max {np.abs(se).max():.2e}")
+    assert np.abs(se).max() > 1e-3 * np.abs(ue).max()
+    assert eu <= 1e-12 and es[0] <= 1e-12 and es[1] <= 1e-12
+
+
 @pytest.mark.parametrize("basis", [0, 1])
 def test_cpp_facade_driver_matches_oracle(tmp_path, basis):
     """examples/square_driver (the C++ LeapFrog facade, reference-style loop with
