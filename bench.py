@@ -45,17 +45,18 @@ def algorithmic_bytes(prob, scalar_bytes, vec_bytes_per_dof):
     return nblk * d * d * scalar_bytes + prob.ne * d * vec_bytes_per_dof + prob.ne * (8 + 16)
 
 
-def face_form_bytes(prob):
-    """The same for the FP64 face form (one step = the face pass + the element
+def face_form_bytes(prob, scalar_bytes=8, vec_bytes_per_dof=24):
+    """The same for the face form (one step = the face pass + the element
     step): each element's own block and each interior face's block once
-    (ne + #faces blocks instead of ne + #half-faces), the state (24 B per DOF),
-    the face products written and read once (16 B per row of each half-face),
-    the inverse-mass scalar and metadata, and the face list (16 B per face)."""
+    (ne + #faces blocks instead of ne + #half-faces), the state, the face
+    products written and read once (2 x scalar_bytes per row of each
+    half-face), the per-element scalar and metadata, and the face list (16 B
+    per face)."""
     d = 2 * prob.nln
     nhalf = int((prob.neigh >= 0).sum())
     nfaces = nhalf // 2
-    return ((prob.ne + nfaces) * d * d * 8 + prob.ne * d * 24 + nhalf * d * 16 + prob.ne * (8 + 16)
-            + nfaces * 16)
+    return ((prob.ne + nfaces) * d * d * scalar_bytes + prob.ne * d * vec_bytes_per_dof
+            + nhalf * d * 2 * scalar_bytes + prob.ne * (8 + 16) + nfaces * 16)
 
 
 def run_variant(prob, precision, steps, warmup, keep_u=False, face_form=True):
@@ -191,7 +192,7 @@ def batched_accuracy(precision, cols=(0, 21, 42, 63)):
                 per_column={str(c): e for c, e in errs.items()})
 
 
-def fp32c_accuracy(nsteps=1000):
+def fp32c_accuracy(nsteps=1000, face_form=True):
     """FP32c against the FP64 path (the oracle to 1e-12) on the full layered
     mesh with the Ricker source firing inside the run (t0 = 20 dt,
     f0 = 1/(16 dt)) and 16 receivers 5-30 m from it beside the 128 surface ones:
@@ -207,9 +208,13 @@ def fp32c_accuracy(nsteps=1000):
                      for r in (5.0, 10.0, 20.0, 30.0) for a in ang])
     p.rec_xy = np.vstack([p.rec_xy, ring])
     p.__post_init__()
+    from tspeed_amd import capi
     out = {}
     for prec in ("fp64", "fp32c"):
-        eng, u, s = W.run_engine(p, prec)
+        eng = capi.Engine(prec)
+        if prec == "fp32c":
+            eng.set_face_form(face_form)
+        eng, u, s = W.run_engine(p, engine=eng)
         eng.close()
         out[prec] = (u, s)
     (u64, s64), (u32, s32) = out["fp64"], out["fp32c"]
@@ -407,13 +412,23 @@ def main():
                                           achieved_GBs=be / (ms_e / 1e3) / 1e9, algorithmic_bytes=be,
                                           status=re["status"])
     if not args.no_fp32c:
-        r32 = run_variant(prob, "fp32c", args.steps, args.warmup)
+        # the FP32c step in its face form (opted in: an FP32c context's default
+        # is the block-ELL step, bitwise its run inside a multi-instance set)
+        r32 = run_variant(prob, "fp32c", args.steps, args.warmup, face_form=True)
         ms32 = r32["ms_total"] / args.steps
-        b32 = algorithmic_bytes(prob, 4, FP32_STATE_BYTES)
+        b32 = face_form_bytes(prob, 4, FP32_STATE_BYTES) if r32["face_form"] else algorithmic_bytes(prob, 4, FP32_STATE_BYTES)
         variants["fp32c"] = dict(value=prob.ndof * args.steps / (r32["ms_total"] / 1e3), ms_per_step=ms32,
                                  achieved_GBs=b32 / (ms32 / 1e3) / 1e9, algorithmic_bytes=b32,
-                                 accuracy_vs_fp64=None if args.no_accuracy else fp32c_accuracy(),
+                                 face_form=r32["face_form"],
+                                 accuracy_vs_fp64=None if args.no_accuracy else fp32c_accuracy(face_form=True),
                                  status=r32["status"])
+        if not args.no_block_ell:
+            re32 = run_variant(prob, "fp32c", args.steps, args.warmup, face_form=False)
+            mse = re32["ms_total"] / args.steps
+            be32 = algorithmic_bytes(prob, 4, FP32_STATE_BYTES)
+            variants["fp32c_block_ell"] = dict(value=prob.ndof * args.steps / (re32["ms_total"] / 1e3),
+                                               ms_per_step=mse, achieved_GBs=be32 / (mse / 1e3) / 1e9,
+                                               algorithmic_bytes=be32, status=re32["status"])
 
     if not args.no_square:
         # the reference's CPU-sized case at its own length: 2000 steps of one 2000-step run

@@ -140,17 +140,26 @@ def test_uneven_batch_instances_match_the_oracle():
         assert e_s <= 1e-4 and e_r <= 1e-4 and e_u <= 1e-4
 
 
-def test_layered_fp32c_accuracy_against_fp64():
+@pytest.mark.parametrize("face_form", [False, True])
+def test_layered_fp32c_accuracy_against_fp64(face_form):
+    """The full layered workload (160k triangles, p = 6) with the source firing
+    early and receivers 5-30 m from it: FP32c within 1e-4 of the FP64 path
+    (itself in its face form here) after 1000 steps, with the FP32c step
+    reading every stored block and in its own face form (opted in)."""
     p = _early(W.layered_ricker(nsteps=1000), 1000, (5.0, 10.0, 20.0, 30.0))
     say(f"layered ne={p.ne} p={p.N} dt={p.dt:.3e} steps={p.nsteps} receivers={len(p.rec_elem)}")
     e64, u64, s64 = W.run_engine(p, "fp64")
     e64.close()
-    e32, u32, s32 = W.run_engine(p, "fp32c")
+    e32 = capi.Engine("fp32c")
+    e32.set_face_form(face_form)
+    e32, u32, s32 = W.run_engine(p, engine=e32)
+    assert e32.set_face_form() is face_form
     assert e32.status() == 0
     e32.close()
     e_u = rel_l2(u32, u64)
     e_s, e_r = seis_err(s32, s64)
-    say(f"  FP32c vs FP64 after {p.nsteps} steps: u {e_u:.2e} seismograms {e_s:.2e} worst receiver {e_r:.2e}")
+    say(f"  FP32c (face form {face_form}) vs FP64 after {p.nsteps} steps: u {e_u:.2e} seismograms {e_s:.2e} "
+        f"worst receiver {e_r:.2e}")
     assert e_u <= 1e-4 and e_s <= 1e-4 and e_r <= 1e-4
 
 

@@ -286,7 +286,7 @@ def test_fp64_face_form_against_block_ell_and_oracle(N, basis):
 def test_fp64_face_form_availability_and_uploaded_blocks():
     """The face form is taken by default for assembled FP64 operators above
     the cache-resident size (128 MB) at every degree, on request ("always") for
-    smaller ones, never for other precisions, and for uploaded blocks only
+    smaller ones, by FP32c contexts only on request, and for uploaded blocks only
     when they are symmetric bit for bit; on such blocks it reads the same
     entries as the block-ELL step (they differ in the order of the slot sums
     only)."""
@@ -309,7 +309,7 @@ def test_fp64_face_form_availability_and_uploaded_blocks():
     p = small_problem(5, nsteps=5)
     W.set_space(f, p)
     f.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
-    assert f.set_face_form("always") is False
+    assert f.set_face_form() is False            # FP32c: only when asked
     f.close()
     p = small_problem(6, nsteps=60)
     op = oracle_op(p)
@@ -422,6 +422,49 @@ def test_fp32c_small_many_degrees(N):
     e_s, _ = seis_err(s, sref)
     say(f"fp32c N={N}: u rel L2 {rel_l2(u, uref):.3e} seis rel L2 {e_s:.3e}")
     assert e_s <= 1e-4 and rel_l2(u, uref) <= 1e-4
+
+
+@pytest.mark.parametrize("N", [5, 6, 7, 8])
+def test_fp32c_face_form_against_oracle_and_block_ell(N):
+    """The FP32c face form (each interior face's FP32 block read once, both
+    sides' translation-free products from it), forced on a small mesh, against
+    the oracle over 400 steps (the FP32c bar, 1e-4) and against the FP32c step
+    reading every stored block (FP32 rounding of different sums)."""
+    p = small_problem(N, nsteps=400)
+    runs = {}
+    for on in (True, False):
+        eng = capi.Engine("fp32c")
+        assert eng.set_face_form() is False            # FP32c: off unless asked
+        eng.set_face_form("always" if on else False)
+        eng, u, s = W.run_engine(p, engine=eng)
+        assert eng.set_face_form() is on
+        assert eng.status() == 0
+        runs[on] = (u, s)
+        eng.close()
+    uref, sref = oracle_run(p)
+    (uf, sf), (ue, se) = runs[True], runs[False]
+    say(f"fp32c face form N={N}: vs oracle u {rel_l2(uf, uref):.2e} seis {rel_l2(sf, sref):.2e}; "
+        f"block-ELL vs oracle u {rel_l2(ue, uref):.2e} seis {rel_l2(se, sref):.2e}; "
+        f"face vs block-ELL seis {rel_l2(sf, se):.2e}")
+    assert rel_l2(uf, uref) <= 1e-4 and max(seis_err(sf, sref)) <= 1e-4
+    assert rel_l2(sf, se) <= 1e-4
+
+
+def test_fp32c_face_form_availability():
+    """FP32c contexts take the face form only when asked, at N >= 5, on the
+    orthonormal basis (a BoundaryAdapted FP32 operator has the dense inverse
+    mass folded into its rows and is not symmetric)."""
+    for N, basis, want in ((4, O.DUBINER, False), (5, O.DUBINER, True), (5, O.BOUNDARY_ADAPTED, False)):
+        p = small_problem(N, nsteps=5)
+        if basis != O.DUBINER:
+            p.basis = basis
+            p.__post_init__()
+        eng = capi.Engine("fp32c")
+        W.set_space(eng, p)
+        eng.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+        assert eng.set_face_form() is False
+        assert eng.set_face_form("always") is want, (N, basis)
+        eng.close()
 
 
 # ------------------------------------------------------------------ API paths

@@ -30,8 +30,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 
 # Step kernels run every time step: a register spill there costs bandwidth (and
 # one spilling variant was measured wrong), so the build refuses any.
-HOT_KERNELS = ("k_step64", "k_step32c", "k_step32w", "k_multi32c", "k_multi32s", "k_multi32w", "k_multi_fused",
-               "k_step_b32", "k_step_b16", "k_bmulti")
+HOT_KERNELS = ("k_step64", "k_step32c", "k_step32w", "k_face64", "k_face32", "k_multi32c", "k_multi32s",
+               "k_multi32w", "k_multi_fused", "k_step_b32", "k_step_b16", "k_bmulti")
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
 
 

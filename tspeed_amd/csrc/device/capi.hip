@@ -47,6 +47,7 @@ struct tsp_context {
     int4 *faces = nullptr;
     int nfaces = 0;
     double *zf = nullptr;
+    float *zf32 = nullptr;   // the FP32c face form's product rows
 
     // state: FP64 two vectors; FP32 (single- and multi-column) two time levels of
     // u = hi - lo, hi FP32 and lo BF16, both ping-pong
@@ -402,12 +403,17 @@ void build_faces(tsp_context *c)
     drop_graph(c);
     dfree(c, c->faces);
     dfree(c, c->zf);
+    dfree(c, c->zf32);
     c->nfaces = 0;
-    const double ell_bytes = (double)c->ne * 4 * c->D * c->D * sizeof(double);
-    if (c->face_form == 0 || (c->face_form == 1 && ell_bytes <= kCacheResidentBytes)) return;
-    if (c->precision != TSP_FP64 || c->batched() || !c->symmetric || !face_form_degree(c->P) ||
-        !c->ne)
+    // FP64, or FP32c on the orthonormal basis (its FP32 operator is the FP64
+    // one rounded; with a dense inverse mass folded in it is not symmetric)
+    const bool fp32 = c->precision == TSP_FP32C;
+    if (c->batched() || !c->symmetric || !c->ne) return;
+    if (!(c->precision == TSP_FP64 && face_form_degree(c->P)) &&
+        !(fp32 && c->h_minv.empty() && c->blk32 && face_form_degree32(c->P)))
         return;
+    const double ell_bytes = (double)c->ne * 4 * c->D * c->D * (fp32 ? sizeof(float) : sizeof(double));
+    if (c->face_form == 0 || (c->face_form == 1 && ell_bytes <= kCacheResidentBytes)) return;
     const std::vector<int> &nb = c->h_neigh;
     auto count = [&](int e, int x) {
         return (nb[3 * e] == x) + (nb[3 * e + 1] == x) + (nb[3 * e + 2] == x);
@@ -423,10 +429,16 @@ void build_faces(tsp_context *c)
             const int sx = nb[3 * x] == e ? 0 : (nb[3 * x + 1] == e ? 1 : 2);
             f.push_back(make_int4(e, 1 + s, x, 1 + sx));
         }
-    const size_t nz = (size_t)c->ne * 3 * face_zstride(c->D);
-    c->zf = dalloc<double>(c, nz, "face products");
     // boundary slots are never written: they read zero
-    hipcheck(c, hipMemsetAsync(c->zf, 0, sizeof(double) * nz, c->stream), "memset");
+    if (fp32) {
+        const size_t nz = (size_t)c->ne * 3 * face_zstride32(c->D);
+        c->zf32 = dalloc<float>(c, nz, "face products");
+        hipcheck(c, hipMemsetAsync(c->zf32, 0, sizeof(float) * nz, c->stream), "memset");
+    } else {
+        const size_t nz = (size_t)c->ne * 3 * face_zstride(c->D);
+        c->zf = dalloc<double>(c, nz, "face products");
+        hipcheck(c, hipMemsetAsync(c->zf, 0, sizeof(double) * nz, c->stream), "memset");
+    }
     c->faces = dalloc<int4>(c, f.size(), "faces");
     if (!f.empty()) h2d(c, c->faces, f.data(), f.size(), "faces");
     c->nfaces = (int)f.size();
@@ -509,6 +521,7 @@ StepArgs base_args(tsp_context *c)
     a.faces = c->faces;
     a.nfaces = c->nfaces;
     a.zf = c->zf;
+    a.zf32 = c->zf32;
     a.blk32 = c->blk32;
     // FP32 products drop the element's translation (see gather); the
     // BoundaryAdapted basis has no single constant mode, so it keeps the plain form
@@ -606,6 +619,9 @@ int tsp_create(int device, int precision, int ncols, tsp_context **out)
     c->device = device;
     c->precision = precision;
     c->ncols = ncols;
+    // the face form by default for FP64 only: an FP32c context steps bitwise as
+    // it does inside a multi-instance set (tsp_multi_add) unless asked otherwise
+    c->face_form = precision == TSP_FP64 ? 1 : 0;
     int rc = guarded(c, [&] {
         int n = 0;
         hipcheck(c, hipGetDeviceCount(&n), "hipGetDeviceCount");
@@ -721,7 +737,7 @@ static void reset_mesh(tsp_context *c, int ne)
     drop_graph(c);
     dfree(c, c->meta); dfree(c, c->blk64); dfree(c, c->invm); dfree(c, c->blk32); dfree(c, c->blkA);
     dfree(c, c->sc_first); dfree(c, c->sc_step);
-    dfree(c, c->faces); dfree(c, c->zf);
+    dfree(c, c->faces); dfree(c, c->zf); dfree(c, c->zf32);
     c->nfaces = 0;
     c->symmetric = false;
     for (int k = 0; k < 2; ++k) { dfree(c, c->u64[k]); dfree(c, c->uhi[k]); dfree(c, c->ulo[k]); }
@@ -805,8 +821,9 @@ int tsp_upload_blocks(tsp_context *c, int ne, const int *neigh, const double *bl
         hipcheck(c, hipStreamSynchronize(c->stream), "upload");
         dfree(c, canon);
         dfree(c, d_nb);
-        // only an FP64 single-column context can take the face form
-        c->symmetric = c->precision == TSP_FP64 && !c->batched() && blocks_symmetric(ne, c->D, neigh, blocks);
+        // only a single-column FP64 or FP32c context can take the face form
+        c->symmetric = (c->precision == TSP_FP64 || c->precision == TSP_FP32C) && !c->batched() &&
+                       blocks_symmetric(ne, c->D, neigh, blocks);
         operator_ready(c);
     });
 }
@@ -956,7 +973,7 @@ int tsp_set_face_form(tsp_context *c, int on, int *active)
                 build_faces(c);
             }
         }
-        if (active) *active = c->faces != nullptr;
+        if (active) *active = c->zf != nullptr || c->zf32 != nullptr;
     });
 }
 

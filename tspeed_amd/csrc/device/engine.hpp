@@ -40,6 +40,7 @@ struct Geom {
     static constexpr int RPL = (D + 63) / 64;           // rows per lane when D > 64
     static constexpr int DP = (D + 3) / 4 * 4;          // LDS row stride (16-byte aligned groups)
     static constexpr int ZS = (D + 15) / 16 * 16;       // face-product row stride: whole 128-byte lines
+    static constexpr int ZS32 = (D + 31) / 32 * 32;     // the same for FP32 rows
 };
 
 inline int nln_of(int P) { return (P + 1) * (P + 2) / 2; }
@@ -48,6 +49,10 @@ inline int nln_of(int P) { return (P + 1) * (P + 2) / 2; }
 // d <= 32).
 inline bool face_form_degree(int P) { return P >= 1 && P <= kMaxDegree; }
 inline int face_zstride(int d) { return (d + 15) / 16 * 16; }
+inline int face_zstride32(int d) { return (d + 31) / 32 * 32; }
+// Degrees whose FP32c step has the face form: blocks one column group wide
+// (d > 32), each half of a block's column quads to a wave as in k_step32w.
+inline bool face_form_degree32(int P) { return P >= 5 && P <= kMaxDegree; }
 
 // The FP32 paths keep the displacement as u = hi - lo at two time levels
 // (u^n and u^{n-1}), hi in FP32 and lo, its compensation, in BF16: 2 bytes per
@@ -144,6 +149,8 @@ struct StepArgs {
     int nfaces;
     double *zf;               // zf[(e*3 + slot-1)*zs + row], zs = face_zstride(d): rows padded
                               // to whole 128-byte lines, written whole; boundary slots stay zero
+    float *zf32;              // FP32c face form (translation-free products): the same rows in
+                              // FP32, zs = face_zstride32(d)
     // FP32 compensated: u^n in (uhi, ulo), u^{n-1} in (uhi_out, ulo_out), which
     // the step overwrites with u^{n+1}; lo arrays hold BF16 (lo_ld / lo_st)
     int tfree;                // 1: translation-free products (orthonormal basis; see gather)

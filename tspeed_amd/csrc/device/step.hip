@@ -241,12 +241,17 @@ void k_step64(StepArgs a)
 // the exchanges across 16-lane rows go through the LDS crossbar (ds_swizzle
 // within 32 lanes, ds_bpermute across the halves).
 template <int CTRL>
-__device__ inline double dpp64(double x)
+__device__ inline double lane_dpp(double x)
 {
     const unsigned long long u = (unsigned long long)__double_as_longlong(x);
     const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
     const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <int CTRL>
+__device__ inline float lane_dpp(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
 }
 constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]: lane ^ 1
 constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]: lane ^ 2
@@ -260,17 +265,21 @@ __device__ inline double swz_xor16(double x)
     const int hi = __builtin_amdgcn_ds_swizzle((int)(unsigned)(u >> 32), 0x401F);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
+__device__ inline float swz_xor16(float x)
+{
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), 0x401F));
+}
 
 // One halving step of the reduce-scatter: the lane whose bit is set keeps the
 // upper half of its n values and adds its partner's copy of that half.
-template <int CTRL, int N>
-__device__ inline void halve(double (&v)[8], bool hi)
+template <int CTRL, int N, typename T>
+__device__ inline void halve(T (&v)[8], bool hi)
 {
 #pragma unroll
     for (int k = 0; k < N / 2; ++k) {
-        const double keep = hi ? v[k + N / 2] : v[k];
-        const double send = hi ? v[k] : v[k + N / 2];
-        v[k] = keep + dpp64<CTRL>(send);
+        const T keep = hi ? v[k + N / 2] : v[k];
+        const T send = hi ? v[k] : v[k + N / 2];
+        v[k] = keep + lane_dpp<CTRL>(send);
     }
 }
 
@@ -280,21 +289,21 @@ __device__ inline void halve(double (&v)[8], bool hi)
 // butterfly steps over the group's other bits.  Returns the full sum of value
 // k: k = lane & 7 (NV = 8), (lane >> 1) & 3 (NV = 4), (lane >> 2) & 1
 // (NV = 2).  The order of the additions depends only on the lane bits.
-template <int NV, int GL = 64>
-__device__ inline double fold_lanes(const double (&v0)[8], int lane)
+template <int NV, int GL = 64, typename T>
+__device__ inline T fold_lanes(const T (&v0)[8], int lane)
 {
     static_assert(GL == 16 || GL == 32 || GL == 64, "groups of 16, 32 or 64 lanes");
     static_assert(NV == 2 || NV == 4 || NV == 8, "NV = 2, 4 or 8");
-    double v[8];
+    T v[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = v0[k];
     halve<kDppHalfMirror, NV>(v, (lane & 4) != 0);
     if constexpr (NV >= 4) halve<kDppXor2, NV / 2>(v, (lane & 2) != 0);
     if constexpr (NV >= 8) halve<kDppXor1, NV / 4>(v, (lane & 1) != 0);
-    double t = v[0];
-    if constexpr (NV <= 4) t += dpp64<kDppXor1>(t);
-    if constexpr (NV <= 2) t += dpp64<kDppXor2>(t);
-    t += dpp64<kDppRor8>(t);
+    T t = v[0];
+    if constexpr (NV <= 4) t += lane_dpp<kDppXor1>(t);
+    if constexpr (NV <= 2) t += lane_dpp<kDppXor2>(t);
+    t += lane_dpp<kDppRor8>(t);
     if constexpr (GL >= 32) t += swz_xor16(t);
     if constexpr (GL == 64) t += __shfl_xor(t, 32);
     return t;
@@ -602,11 +611,14 @@ __global__ __launch_bounds__(kThreads) void k_step32c(StepArgs a)
 // through L2.  On the 160 000-triangle p = 6 mesh (synthetic operator, same
 // shapes): 1.246 ms per step against 1.323 with one wave walking the element,
 // 1.274 with the gather first, 1.293 with one wave per slot.
-template <int P>
-__global__ __launch_bounds__(64 * 8) void k_step32w(StepArgs a)
+//
+// FACES: the FP32c face form's element step: slot 0's two chunks only, then
+// the three face products of its slots (zf32) added in slot order.
+template <int P, bool FACES = false>
+__global__ __launch_bounds__(64 * (FACES ? 2 : 8)) void k_step32w(StepArgs a)
 {
     using G = Geom<P>;
-    constexpr int NW = 8;
+    constexpr int NW = FACES ? 2 : 8;
     __shared__ __attribute__((aligned(16))) float xs_all[NW][G::DP];
     __shared__ float part[NW - 1][G::RPL][64];
     const int lane = threadIdx.x & 63;
@@ -631,6 +643,15 @@ __global__ __launch_bounds__(64 * 8) void k_step32w(StepArgs a)
         cq[q] = rows[q] >= 0 ? lo_ld(a.ulo_out, idx) : 0.f;
     }
     const float sc = w == 0 ? a.sc32[e] : 0.f;
+    float zs[3][G::RPL];   // face form: the products of slots 1..3 (wave 0's rows, read once)
+    if constexpr (FACES) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int q = 0; q < G::RPL; ++q)
+                zs[k][q] = rows[q] >= 0 ? __builtin_nontemporal_load(a.zf32 + ((size_t)e * 3 + k) * G::ZS32 + rows[q])
+                                        : 0.f;
+    }
     float own[G::RPL];   // wave 0: u^n of the lane's rows (the update's uh)
     gather_one<G, float>(xs_all[w], a.uhi, nb, lane, a.tfree ? e : -1, own);
 #pragma unroll
@@ -651,6 +672,12 @@ __global__ __launch_bounds__(64 * 8) void k_step32w(StepArgs a)
     for (int k = 0; k < NW - 1; ++k)
 #pragma unroll
         for (int q = 0; q < G::RPL; ++q) acc[q] += part[k][q][lane];
+    if constexpr (FACES) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int q = 0; q < G::RPL; ++q) acc[q] += zs[k][q];
+    }
 
     const int4 ex = m.w >= 0 ? a.extra[m.w] : make_int4(0, 0, 0, 0);
     float unew[G::RPL];
@@ -673,6 +700,122 @@ __global__ __launch_bounds__(64 * 8) void k_step32w(StepArgs a)
     }
     if (bad) atomicOr(a.flags, 1u);
     if (a.sample >= 0 && ex.w > ex.z) record<G, float>(a, ex, unew, rows, G::RPL);
+}
+
+// ------------------------------------------------------- FP32c face pass
+// The FP64 face form for the FP32c step (orthonormal basis): each interior
+// face's FP32 block, read once from the lower-numbered element's copy, gives
+// both product rows with the translation-free inputs each side's block-ELL
+// step uses, zf32(ea) = F (x_eb - T_ea) and zf32(eb) = F^T (x_ea - T_eb) (T_e:
+// element e's own mode-0 coefficients).  Two waves per face, one per half of
+// the block's column quads (the trailing pair with half 1) as k_step32w splits
+// a block; transposed sums folded over the lanes two quads at a time.
+
+// Transposed sums of the column quads of half h, chunk CI = quads 2CI, 2CI+1
+// (one quad in the last chunk when the half has an odd count); lanes
+// 8CI .. 8CI+7 write the chunk's columns to zrow.
+template <typename G, int CI>
+__device__ inline void face32_quads(const ChunkRegs<G> &R, const float (&xa)[G::RPL], int h, int lane, float *zrow)
+{
+    using C = Chunks<G>;
+    constexpr int NQC = (C::NQH + 1) / 2;
+    static_assert(NQC + 1 <= 8, "at most 8 chunks per wave");
+    if constexpr (CI < NQC) {
+        constexpr int NQ = C::NQH - 2 * CI < 2 ? C::NQH - 2 * CI : 2;
+        constexpr int NV = NQ == 2 ? 8 : 4;
+        float pv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) pv[k] = 0.f;
+#pragma unroll
+        for (int k = 0; k < NQ; ++k)
+#pragma unroll
+            for (int q = 0; q < G::RPL; ++q)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) pv[4 * k + t] = fma(R.q[2 * CI + k][q][t], xa[q], pv[4 * k + t]);
+        const float tsum = fold_lanes<NV, 64>(pv, lane);
+        const int k = NV == 8 ? (lane & 7) : (lane >> 1) & 3;
+        const bool writer = NV == 8 ? true : (lane & 1) == 0;
+        if ((lane >> 3) == CI && writer) zrow[4 * (h * C::NQH + 2 * CI) + k] = tsum;
+        face32_quads<G, CI + 1>(R, xa, h, lane, zrow);
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_face32(StepArgs a)
+{
+    using G = Geom<P>;
+    using C = Chunks<G>;
+    static_assert(G::NG == 1, "FP32 face pass: d > 32");
+    __shared__ __attribute__((aligned(16))) float xs_all[4][G::DP];
+    __shared__ float part[2][G::RPL][64];
+    __shared__ float zrow[2][G::ZS32];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int fl = w >> 1, h = w & 1;   // face of the work-group, half of the column quads
+    const int fraw = xcd_remap(blockIdx.x, gridDim.x) * 2 + fl;
+    const bool act = fraw < a.nfaces;
+    // a spare wave pair of the last work-group recomputes the last face and stores nothing
+    const int4 fc = a.faces[act ? fraw : a.nfaces - 1];
+    ChunkRegs<G> R;
+    chunk_load<G>(a.blk32, fc.x, fc.y, h, lane, R);
+    // keep the block loads ahead of the gathers
+    __builtin_amdgcn_sched_barrier(0);
+    gather_one<G, float>(xs_all[w], a.uhi, fc.z, lane, a.tfree ? fc.x : -1);   // x_eb - T_ea
+    float xa[G::RPL];                                                          // x_ea - T_eb
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const int row = lane + 64 * q;
+        float v = 0.f;
+        if (row < G::D) {
+            v = a.uhi[(size_t)fc.x * G::D + row];
+            if (a.tfree && (row == 0 || row == G::NLN)) v = v - a.uhi[(size_t)fc.z * G::D + row];
+        }
+        xa[q] = v;
+    }
+    if (h == 0 && lane < G::ZS32 - G::D) zrow[fl][G::D + lane] = 0.f;   // the row's padding
+    // the wave reads only its own LDS slice: order its stores before its loads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float acc[G::RPL];
+    chunk_fma<G>(R, xs_all[w], h, acc);
+    face32_quads<G, 0>(R, xa, h, lane, zrow[fl]);
+    if constexpr (C::TAIL == 2) {
+        if (h == 1) {
+            float pv[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) pv[k] = 0.f;
+#pragma unroll
+            for (int q = 0; q < G::RPL; ++q) {
+                pv[0] = fma(R.t[q][0], xa[q], pv[0]);
+                pv[1] = fma(R.t[q][1], xa[q], pv[1]);
+            }
+            const float tsum = fold_lanes<2, 64>(pv, lane);
+            constexpr int CT = (C::NQH + 1) / 2;   // the chunk after the quads
+            if ((lane >> 3) == CT && (lane & 3) == 0) zrow[fl][4 * C::NFULL + ((lane >> 2) & 1)] = tsum;
+        }
+    }
+    if (h == 1) {
+#pragma unroll
+        for (int q = 0; q < G::RPL; ++q) part[fl][q][lane] = acc[q];
+    }
+    __syncthreads();
+    if (!act) return;
+    if (h == 1) {
+        float *zb = a.zf32 + ((size_t)fc.z * 3 + fc.w - 1) * G::ZS32;
+#pragma unroll
+        for (int q = 0; q < (G::ZS32 + 63) / 64; ++q) {
+            const int row = lane + 64 * q;
+            if (row < G::ZS32) __builtin_nontemporal_store(zrow[fl][row], zb + row);
+        }
+        return;
+    }
+    float *za = a.zf32 + ((size_t)fc.x * 3 + fc.y - 1) * G::ZS32;
+#pragma unroll
+    for (int q = 0; q < G::RPL; ++q) {
+        const int row = lane + 64 * q;
+        if (row < G::ZS32) __builtin_nontemporal_store(row < G::D ? acc[q] + part[fl][q][lane] : 0.f, za + row);
+    }
 }
 
 // Receiver sample of a stored state, outside a step (eval_receivers):
@@ -912,6 +1055,15 @@ template <int P>
 int step32c_p(bool first, const StepArgs &a, hipStream_t s)
 {
     (void)first;  // the first step differs only in the data the context prepares
+    if constexpr (Geom<P>::NG == 1) {
+        if (a.zf32) {
+            if (a.nfaces > 0)
+                hipLaunchKernelGGL(k_face32<P>, dim3((unsigned)((a.nfaces + 1) / 2)), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_step32w<P, true>), dim3((unsigned)a.ne), dim3(64 * 2), 0, s, a);
+            return hipGetLastError() == hipSuccess ? 0 : -1;
+        }
+    }
+    if (a.zf32) return -3;   // no FP32 face form at this degree
     if constexpr (Geom<P>::D > 32)
         hipLaunchKernelGGL(k_step32w<P>, dim3((unsigned)a.ne), dim3(64 * 8), 0, s, a);
     else
