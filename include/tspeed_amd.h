@@ -170,7 +170,10 @@ int tsp_set_stride(tsp_context *ctx, int stride);
  * smaller one stays in the Infinity Cache, where one launch reading every
  * block is faster); 2: wherever possible; 0: never (every block read as
  * stored); -1: leave the setting.  *active (may be NULL) receives whether the
- * steps use it. */
+ * steps use it.  A TSP_FP32C context (single column, N >= 5, orthonormal
+ * basis) has the same form with translation-free FP32 products on both sides,
+ * but only on request (its default is 0): by default it steps bitwise as it
+ * does inside a multi-instance set (tsp_multi_add). */
 int tsp_set_face_form(tsp_context *ctx, int on, int *active);
 
 /* Step size (TimeAdvance::set_dt, TimeAdvance.hpp:153). */
@@ -292,7 +295,10 @@ int tsp_release_fp64(tsp_context *ctx);
  * (tsp_multi_set_launch), the multi-column ones as one launch per (degree,
  * width) present, each (element, 32-column tile) running its context's own
  * block-SpMM step.  Chunks of steps replay a captured HIP graph.  Every
- * instance's result is bitwise that of its context stepped alone.  Results are
+ * instance's result is bitwise that of its context stepped alone in its
+ * default form (a single-column FP32c context asked for the face form,
+ * tsp_set_face_form, steps differently alone; in a set it reads every stored
+ * block).  Results are
  * read through tsp_multi_context + tsp_get_u / tsp_get_seismograms.
  * ------------------------------------------------------------------------ */
 typedef struct tsp_multi tsp_multi;
