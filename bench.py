@@ -11,9 +11,9 @@ receivers), the largest single-GPU configuration; synthetic inputs.
 
 Prints ONE JSON line: value = DOF*steps/s of the plain FP64 path (whole job,
 inputs resident in HBM), plus the same step reading every stored block, the
-FP32-compensated variant, a roofline entry for the step's kernels and a CPU
-baseline (the FP64 oracle, single thread, on a bounded sample of the same
-workload).
+FP32-compensated variant (in its face form, and reading every block), a
+roofline entry for the step's kernels and a CPU baseline (the FP64 oracle,
+single thread, on a bounded sample of the same workload).
 """
 from __future__ import annotations
 
@@ -377,7 +377,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-square", action="store_true", help="skip the 3200-triangle FP64 leg")
     ap.add_argument("--no-block-ell", action="store_true",
-                    help="skip the FP64 leg that reads every stored block (no face form)")
+                    help="skip the FP64 and FP32c legs that read every stored block (no face form)")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the error runs against FP64 (profiling the timed legs alone)")
     ap.add_argument("--multi", type=int, default=256, help="instances in the uneven-batch variant (0 = skip)")
