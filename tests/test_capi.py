@@ -39,5 +39,6 @@ def test_null_and_bad_arguments_are_rejected_without_a_device():
     for prec in (capi.TSP_BF16X3, capi.TSP_BF16X5):
         assert L.tsp_create(0, prec, 1, ctypes.byref(ctypes.c_void_p())) == capi.TSP_ERR_UNSUPPORTED
     assert L.tsp_run(None, 1) == capi.TSP_ERR_ARG
+    assert L.tsp_set_face_form(None, 1, None) == capi.TSP_ERR_ARG
     assert L.tsp_steps_done(None) == -1
     assert L.tsp_last_error(None) == b"null context"

@@ -295,6 +295,9 @@ def test_fp64_face_form_availability_and_uploaded_blocks():
         eng = capi.Engine("fp64")
         W.set_space(eng, p)
         eng.set_mesh(p.verts, p.neigh, p.neighedge, p.lam, p.mu, p.rho, p.penalty)
+        # out-of-range settings are refused and leave the setting as it was
+        assert eng._L.tsp_set_face_form(eng._h, 3, None) == capi.TSP_ERR_ARG
+        assert eng._L.tsp_set_face_form(eng._h, -2, None) == capi.TSP_ERR_ARG
         assert eng.set_face_form() is False          # a small operator
         assert eng.set_face_form("always") is True
         assert eng.set_face_form(False) is False and eng.set_face_form("always") is True
