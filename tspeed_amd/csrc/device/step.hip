@@ -353,14 +353,16 @@ __device__ inline void face_pick(const double (&t)[(NPH + 3) / 4], int lane, dou
 }
 
 // Column pairs [C0, C1) of the face block F of face fc: all of the wave's
-// block loads are issued first, then the gathers (x_eb into the wave's LDS
-// slice xb, x_ea of the lane's rows into registers), so their latency hides
-// behind the stream; then the row sums of F x_eb into acc (fma chain in column
+// block loads are issued first, then the face's first wave (gather) puts u of
+// both elements into the face's LDS rows (x_eb at xb, x_ea after it) and the
+// work-group meets at a barrier, so the gathers' latency hides behind the
+// stream and each face gathers once (p = 6: 1.78 -> 1.76 ms per step against a
+// gather per wave); then the row sums of F x_eb into acc (fma chain in column
 // order, as spmv_cols) and the column sums of F^T x_ea, written to the face's
 // LDS row zrow[col] (one write per lane).
 template <typename G, int C0, int C1>
 __device__ inline void face_half(const double *F, const double *u, int4 fc, double *xb, int lane,
-                                 double (&acc)[G::RPL], double *zrow)
+                                 double (&acc)[G::RPL], double *zrow, bool gather)
 {
     constexpr int NPH = C1 - C0, NCH = (NPH + 3) / 4;
     static_assert(NCH <= 8, "at most 8 chunks per wave");
@@ -373,17 +375,17 @@ __device__ inline void face_half(const double *F, const double *u, int4 fc, doub
             bl[k][q] = __builtin_nontemporal_load(
                 reinterpret_cast<const vec_t<double, 2> *>(F + ((size_t)(C0 + k) * G::D + row) * 2));
         }
-    gather_one<G, double>(xb, u, fc.z, lane);
+    if (gather) {
+        gather_one<G, double>(xb, u, fc.z, lane);
+        gather_one<G, double>(xb + G::DP, u, fc.x, lane);
+    }
+    __syncthreads();
     double xa[G::RPL];
 #pragma unroll
     for (int q = 0; q < G::RPL; ++q) {
         const int idx = lane + 64 * q;
-        xa[q] = idx < G::D ? u[(size_t)fc.x * G::D + idx] : 0.0;
+        xa[q] = idx < G::D ? xb[G::DP + idx] : 0.0;
     }
-    // the wave reads only its own LDS slice: order its stores before its loads
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
     for (int k = 0; k < NPH; ++k) {
         const vec_t<double, 2> xv = *reinterpret_cast<const vec_t<double, 2> *>(&xb[2 * (C0 + k)]);
@@ -417,7 +419,7 @@ __device__ inline void face_wave(int h, const double *F, const double *u, int4 f
 {
     if constexpr (H < HF) {
         constexpr int NIT = G::DH;
-        if (h == H) face_half<G, NIT * H / HF, NIT * (H + 1) / HF>(F, u, fc, xb, lane, acc, zrow);
+        if (h == H) face_half<G, NIT * H / HF, NIT * (H + 1) / HF>(F, u, fc, xb, lane, acc, zrow, H == 0);
         else        face_wave<G, HF, H + 1>(h, F, u, fc, xb, lane, acc, zrow);
     }
 }
@@ -443,7 +445,7 @@ __global__ __launch_bounds__(FaceShape<P>::THREADS) void k_face64(StepArgs a)
 #pragma unroll
     for (int q = 0; q < G::RPL; ++q) acc[q] = 0.0;
     if (h == 0 && lane < G::ZS - G::D) zrow[fl][G::D + lane] = 0.0;   // the row's padding
-    face_wave<G, S::HF, 0>(h, F, a.ucur, fc, xs_all[w], lane, acc, zrow[fl]);
+    face_wave<G, S::HF, 0>(h, F, a.ucur, fc, xs_all[fl * S::HF], lane, acc, zrow[fl]);
     if (h > 0) {
 #pragma unroll
         for (int q = 0; q < G::RPL; ++q) part[fl][h - 1][q][lane] = acc[q];
