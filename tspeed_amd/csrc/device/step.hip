@@ -762,25 +762,23 @@ __global__ __launch_bounds__(256) void k_face32(StepArgs a)
     chunk_load<G>(a.blk32, fc.x, fc.y, h, lane, R);
     // keep the block loads ahead of the gathers
     __builtin_amdgcn_sched_barrier(0);
-    gather_one<G, float>(xs_all[w], a.uhi, fc.z, lane, a.tfree ? fc.x : -1);   // x_eb - T_ea
-    float xa[G::RPL];                                                          // x_ea - T_eb
+    // the face's first wave gathers both rows into the face's LDS slices:
+    // x_eb - T_ea, then x_ea - T_eb; the work-group meets before they are read
+    float *xb = xs_all[2 * fl], *xav = xs_all[2 * fl + 1];
+    if (h == 0) {
+        gather_one<G, float>(xb, a.uhi, fc.z, lane, a.tfree ? fc.x : -1);
+        gather_one<G, float>(xav, a.uhi, fc.x, lane, a.tfree ? fc.z : -1);
+        if (lane < G::ZS32 - G::D) zrow[fl][G::D + lane] = 0.f;   // the row's padding
+    }
+    __syncthreads();
+    float xa[G::RPL];
 #pragma unroll
     for (int q = 0; q < G::RPL; ++q) {
         const int row = lane + 64 * q;
-        float v = 0.f;
-        if (row < G::D) {
-            v = a.uhi[(size_t)fc.x * G::D + row];
-            if (a.tfree && (row == 0 || row == G::NLN)) v = v - a.uhi[(size_t)fc.z * G::D + row];
-        }
-        xa[q] = v;
+        xa[q] = row < G::D ? xav[row] : 0.f;
     }
-    if (h == 0 && lane < G::ZS32 - G::D) zrow[fl][G::D + lane] = 0.f;   // the row's padding
-    // the wave reads only its own LDS slice: order its stores before its loads
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     float acc[G::RPL];
-    chunk_fma<G>(R, xs_all[w], h, acc);
+    chunk_fma<G>(R, xb, h, acc);
     face32_quads<G, 0>(R, xa, h, lane, zrow[fl]);
     if constexpr (C::TAIL == 2) {
         if (h == 1) {
