@@ -231,8 +231,8 @@ void k_step64(StepArgs a)
 //                                   over the lanes
 // The operator stream per step falls from ne + #half-faces blocks to ne +
 // #faces (3.99 -> 2.50 blocks per element on a 160 000-triangle mesh).  Each
-// wave of a face takes a contiguous range of the block's column pairs (2, 4 or
-// 6 waves by degree, so that a wave's block entries stay in registers): the
+// wave of a face takes a contiguous range of the block's column pairs (3, 4 or
+// 5 waves by degree, so that a wave's block entries stay in registers): the
 // transposed sums of its columns are complete within the wave, the row sums of
 // the ranges are added in LDS in wave order.
 
@@ -407,7 +407,7 @@ __device__ inline void face_half(const double *F, const double *u, int4 fc, doub
 // sized so that its loads stay in registers) and faces per work-group.
 template <int P>
 struct FaceShape {
-    static constexpr int HF = P <= 5 ? 2 : (P <= 7 ? 4 : 6);
+    static constexpr int HF = P <= 5 ? 3 : (P <= 7 ? 4 : 5);
     static constexpr int FPW = HF == 2 ? 2 : 1;
     static constexpr int THREADS = 64 * HF * FPW;
 };
